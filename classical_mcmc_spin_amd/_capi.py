@@ -1,0 +1,129 @@
+"""ctypes binding of libcsl_hip.so (the C ABI declared in include/csl_gpu.h).
+
+The library is the only compute path: if it is missing or fails to load, every
+call raises :class:`CslError` — there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libcsl_hip.so")
+
+CSL_OK = 0
+CSL_ERR_NULL = -1
+CSL_ERR_ARG = -2
+CSL_ERR_UNSUPPORTED = -3
+CSL_ERR_NOSTATE = -4
+CSL_ERR_HIP = -5
+CSL_ERR_CAPACITY = -6
+CSL_ERR_NONFINITE = -7
+
+TGRID = {"gpu": 0, "batched": 1, "boost": 2}
+
+# (name, restype, argtypes) for every exported entry point of csl_gpu.h
+_vp = C.c_void_p
+_sz = C.c_size_t
+_d = C.c_double
+_i = C.c_int
+_pd = C.POINTER(C.c_double)
+_psz = C.POINTER(C.c_size_t)
+_pi = C.POINTER(C.c_int)
+_cs = C.c_char_p
+
+SIGNATURES = {
+    "csl_last_error": (C.c_char_p, []),
+    "csl_device_count": (_i, [_pi]),
+    "csl_version": (_i, [_pi, _pi]),
+    "csl_create_lattice": (_i, [_sz, _sz, _sz, _sz, _pd, _pd, _pd, _psz, _psz, C.POINTER(_vp)]),
+    "csl_create_lattice_csr": (_i, [_sz, _sz, _sz, _pd, _pd, _psz, _psz, _pd, C.POINTER(_vp)]),
+    "csl_destroy_lattice": (_i, [_vp]),
+    "csl_lattice_info": (_i, [_vp, _psz, _psz, _psz, _psz, _psz, _pi]),
+    "csl_set_pulse": (_i, [_vp, _pd, _d, _d, _d, _d, _d]),
+    "csl_set_damping": (_i, [_vp, _d]),
+    "csl_set_frames": (_i, [_vp, _pd, _pd]),
+    "csl_set_spins": (_i, [_vp, _pd, _sz]),
+    "csl_get_spins": (_i, [_vp, _pd, _sz]),
+    "csl_step": (_i, [_vp, _d, _d, _cs]),
+    "csl_integrate": (_i, [_vp, _d, _d, _d, _sz, _cs, _i, _i, _sz, _pd, _pd, _pd, _pd, _psz]),
+    "csl_energy": (_i, [_vp, _pd]),
+    "csl_normalize": (_i, [_vp, _d]),
+    "csl_magnetization": (_i, [_vp, _pd, _pd]),
+    "csl_local_field": (_i, [_vp, _pd, _sz]),
+    "csl_rhs": (_i, [_vp, _d, _pd, _sz]),
+    "csl_integrate_batched": (_i, [_vp, _sz, _pd, _i, _pd, _pd, _d, _d, _d, _sz, _cs, _i, _i, _sz,
+                                   _pd, _pd, _pd, _pd, _psz]),
+    "csl_count_records": (_i, [_d, _d, _d, _sz, _i, _psz, _psz]),
+    "csl_pump_probe": (_i, [_vp, _pd, _pd, _d, _d, _d, _d, _d, _d, _d, _d, _d, _sz, _sz, _psz, _psz,
+                            _pd, _pd, _pd, _pd, _pd]),
+    "csl_autotune_batched": (_i, [_vp, _sz, _sz, _pi, _pd]),
+    "csl_last_kernel_ms": (_i, [_vp, _pd, _psz]),
+}
+
+
+class CslError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"csl error {code}: {msg}")
+        self.code = code
+
+
+_lib = None
+_load_error = None
+
+
+def lib():
+    """Return the loaded library, raising CslError if it cannot be loaded."""
+    global _lib, _load_error
+    if _lib is not None:
+        return _lib
+    if _load_error is not None:
+        raise CslError(CSL_ERR_HIP, _load_error)
+    if not os.path.exists(LIB_PATH):
+        _load_error = (f"{LIB_PATH} is missing: build it with "
+                       "`python -m classical_mcmc_spin_amd.build` (needs hipcc)")
+        raise CslError(CSL_ERR_HIP, _load_error)
+    try:
+        L = C.CDLL(LIB_PATH)
+    except OSError as e:  # missing HIP runtime etc.
+        _load_error = f"cannot load {LIB_PATH}: {e}"
+        raise CslError(CSL_ERR_HIP, _load_error) from e
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = L
+    return L
+
+
+def check(rc: int) -> None:
+    if rc != CSL_OK:
+        msg = lib().csl_last_error()
+        raise CslError(rc, msg.decode() if msg else "")
+
+
+def dptr(a):
+    """Pointer to a C-contiguous float64 array (None -> NULL)."""
+    if a is None:
+        return None
+    assert a.dtype == np.float64 and a.flags["C_CONTIGUOUS"], "need contiguous float64"
+    return a.ctypes.data_as(_pd)
+
+
+def szptr(a):
+    if a is None:
+        return None
+    assert a.dtype == np.uintp and a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(_psz)
+
+
+def f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def device_count() -> int:
+    n = C.c_int(0)
+    check(lib().csl_device_count(C.byref(n)))
+    return n.value

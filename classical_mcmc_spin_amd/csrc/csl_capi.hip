@@ -1,0 +1,993 @@
+// csl_capi.hip — host implementation of the C ABI declared in include/csl_gpu.h.
+//
+// Owns the device copies of one lattice (handle), converts the reference's
+// table layouts into the device layout described in csl_internal.hpp,
+// computes the integrator time grids on the host and drives the kernels of
+// csl_kernels.hip on one stream per handle.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <limits>
+#include <map>
+#include <stdexcept>
+#include <string>
+#include <tuple>
+#include <vector>
+
+#include "../../include/csl_gpu.h"
+#include "csl_kernels.hpp"
+
+namespace {
+
+thread_local std::string g_err;
+
+struct CslError : std::runtime_error {
+    int code;
+    CslError(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+[[noreturn]] void fail(int code, const std::string& msg) { throw CslError(code, msg); }
+
+void hip_check(hipError_t e, const char* what) {
+    if (e != hipSuccess) fail(CSL_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+template <class F>
+int guarded(F&& f) {
+    try {
+        f();
+        return CSL_OK;
+    } catch (const CslError& e) {
+        g_err = e.what();
+        return e.code;
+    } catch (const std::exception& e) {
+        g_err = e.what();
+        return CSL_ERR_HIP;
+    }
+}
+
+// Device buffer owned by a handle.
+template <class T>
+struct DBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    DBuf() = default;
+    DBuf(const DBuf&) = delete;
+    DBuf& operator=(const DBuf&) = delete;
+    ~DBuf() { release(); }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    void ensure(size_t count) {
+        if (count <= n && p) return;
+        release();
+        hip_check(hipMalloc(&p, std::max<size_t>(count, 1) * sizeof(T)), "hipMalloc");
+        n = count;
+    }
+    void upload(const T* h, size_t count, hipStream_t s) {
+        ensure(count);
+        if (count) hip_check(hipMemcpyAsync(p, h, count * sizeof(T), hipMemcpyHostToDevice, s), "H2D");
+    }
+};
+
+// Time grid of an integration (host side): start time of every step plus the
+// time after the last step.
+std::vector<double> make_grid(double T0, double T1, double dt, int policy) {
+    if (!(dt > 0.0) || !std::isfinite(T0) || !std::isfinite(T1))
+        fail(CSL_ERR_ARG, "time grid needs finite T0, T1 and dt > 0");
+    std::vector<double> g;
+    if (policy == CSL_TGRID_BOOST) {
+        // boost::numeric::odeint integrate_const (stepper_tag): t_k = T0 + k*dt,
+        // step while t_k + dt - T1 <= eps (less_eq_with_sign).
+        size_t k = 0;
+        double t = T0;
+        while ((t + dt) - T1 <= std::numeric_limits<double>::epsilon()) {
+            g.push_back(t);
+            ++k;
+            t = T0 + double(k) * dt;
+        }
+        g.push_back(t);
+    } else if (policy == CSL_TGRID_GPU || policy == CSL_TGRID_BATCHED) {
+        // gpu::ode::integrate (integrator.cuh:470-487) / batched loop
+        // (lattice_gpu.cu:557-565): accumulated t += dt.
+        const double lim = policy == CSL_TGRID_GPU ? T1 : T1 - 1e-12;
+        double t = T0;
+        while (t < lim) {
+            g.push_back(t);
+            t += dt;
+        }
+        g.push_back(t);
+    } else {
+        fail(CSL_ERR_ARG, "unknown time-grid policy");
+    }
+    return g;
+}
+
+size_t records_for(size_t n_steps, size_t si) { return (n_steps + si - 1) / si + 1; }
+
+struct TuneKey {
+    int NP, zmax, uniform, Bclass;
+    bool operator<(const TuneKey& o) const {
+        return std::tie(NP, zmax, uniform, Bclass) < std::tie(o.NP, o.zmax, o.uniform, o.Bclass);
+    }
+};
+struct TuneVal {
+    int block, spt;
+    bool dbuf;
+};
+std::map<TuneKey, TuneVal> g_tune;
+
+}  // namespace
+
+struct csl_lattice {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    size_t N = 0, D = 3, n_atoms = 1;
+    csl::LatDev L{};
+    size_t lds_limit = 65536;
+    int n_cu = 1;
+    // tables (device)
+    DBuf<int> col, cnt, cnt_atom;
+    DBuf<double> mats, field, onsite, frames, afm, fdrive;
+    // host copies of per-atom data
+    std::vector<double> h_frames, h_afm, h_fdrive;
+    double amp = 0.0, width = 1.0, freq = 0.0, t1 = 0.0, t2 = 0.0;
+    // state
+    DBuf<double> state;  // AoS [N*3]
+    bool has_state = false;
+    // workspaces
+    DBuf<double> X, ACC, Y1, Y2, partials, fac, tgrid, rec_mags, rec_energy, rec_states, aux;
+    DBuf<double> bt_init, bt_tp, bt_amp, bt_final;
+    double last_ms = 0.0;
+    size_t last_launches = 0;
+    ~csl_lattice() {
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+};
+
+namespace {
+
+csl_lattice* check_handle(csl_lattice* h) {
+    if (!h) fail(CSL_ERR_NULL, "null lattice handle");
+    hip_check(hipSetDevice(h->device), "hipSetDevice");
+    return h;
+}
+
+// Build device tables from the reference CSR mirror (lattice.h:868-903).
+void build_tables(csl_lattice* h, size_t N, size_t D, size_t n_atoms, const double* field,
+                  const double* onsite, const std::vector<size_t>& off,
+                  const std::vector<size_t>& partner, const double* J) {
+    if (D != 3) fail(CSL_ERR_UNSUPPORTED, "spin_dim != 3 is handled by the mixed-lattice API");
+    if (n_atoms == 0 || N == 0 || N % n_atoms) fail(CSL_ERR_ARG, "lattice_size must be a multiple of N_atoms");
+    if (N > size_t(std::numeric_limits<int>::max() / 4)) fail(CSL_ERR_ARG, "lattice too large");
+    h->N = N;
+    h->D = D;
+    h->n_atoms = n_atoms;
+    const int n_cells = int(N / n_atoms);
+    const int npad = (n_cells + csl::kWave - 1) / csl::kWave * csl::kWave;
+    const int NP = int(n_atoms) * npad;
+    auto dev = [&](size_t i) { return int(i % n_atoms) * npad + int(i / n_atoms); };
+    int zmax = 0;
+    for (size_t i = 0; i < N; ++i) {
+        if (off[i + 1] < off[i]) fail(CSL_ERR_ARG, "bond offsets must be non-decreasing");
+        zmax = std::max<int>(zmax, int(off[i + 1] - off[i]));
+    }
+    if (zmax > 12) fail(CSL_ERR_UNSUPPORTED, "more than 12 bonds per site");
+    for (size_t k = 0; k < off[N]; ++k)
+        if (partner[k] >= N) fail(CSL_ERR_ARG, "bond partner index out of range");
+    for (size_t q = 0; q < N * D * (D + 1) + off[N] * D * D; ++q) {
+        const double v = q < N * D ? field[q] : q < N * D * (D + 1) ? onsite[q - N * D] : J[q - N * D * (D + 1)];
+        if (!std::isfinite(v)) fail(CSL_ERR_NONFINITE, "non-finite Hamiltonian coefficient");
+    }
+    // translation invariance of bond matrices (per sublattice and slot)
+    bool uni = true;
+    for (size_t i = 0; i < N && uni; ++i) {
+        const size_t r = i % n_atoms;  // same sublattice in cell 0
+        const size_t c = off[i + 1] - off[i];
+        if (c != off[r + 1] - off[r]) { uni = false; break; }
+        if (std::memcmp(J + off[i] * 9, J + off[r] * 9, c * 9 * sizeof(double)) != 0) uni = false;
+    }
+    bool site_uni = true;
+    for (size_t i = 0; i < N && site_uni; ++i) {
+        const size_t r = i % n_atoms;
+        if (std::memcmp(field + i * 3, field + r * 3, 3 * sizeof(double)) ||
+            std::memcmp(onsite + i * 9, onsite + r * 9, 9 * sizeof(double)))
+            site_uni = false;
+    }
+    std::vector<int> col(size_t(zmax) * NP, 0), cnt(NP, 0), cnt_atom(n_atoms, 0);
+    std::vector<double> mats;
+    if (uni) mats.assign(n_atoms * size_t(zmax) * 9, 0.0);
+    else mats.assign(size_t(zmax) * NP * 9, 0.0);
+    for (size_t i = 0; i < N; ++i) {
+        const int p = dev(i);
+        const int c = int(off[i + 1] - off[i]);
+        cnt[p] = c;
+        for (int k = 0; k < zmax; ++k) col[size_t(k) * NP + p] = k < c ? dev(partner[off[i] + k]) : p;
+        for (int k = 0; k < c; ++k) {
+            const double* src = J + (off[i] + k) * 9;
+            double* dst = uni ? &mats[((i % n_atoms) * zmax + k) * 9] : &mats[(size_t(k) * NP + p) * 9];
+            std::memcpy(dst, src, 9 * sizeof(double));
+        }
+    }
+    for (size_t a = 0; a < n_atoms; ++a) cnt_atom[a] = int(off[a + 1] - off[a]);
+    std::vector<double> f, o;
+    if (site_uni) {
+        f.assign(field, field + n_atoms * 3);
+        o.assign(onsite, onsite + n_atoms * 9);
+    } else {
+        f.assign(size_t(NP) * 3, 0.0);
+        o.assign(size_t(NP) * 9, 0.0);
+        for (size_t i = 0; i < N; ++i) {
+            std::memcpy(&f[size_t(dev(i)) * 3], field + i * 3, 3 * sizeof(double));
+            std::memcpy(&o[size_t(dev(i)) * 9], onsite + i * 9, 9 * sizeof(double));
+        }
+    }
+    hipStream_t s = h->stream;
+    h->col.upload(col.data(), col.size(), s);
+    h->cnt.upload(cnt.data(), cnt.size(), s);
+    h->cnt_atom.upload(cnt_atom.data(), cnt_atom.size(), s);
+    h->mats.upload(mats.data(), mats.size(), s);
+    h->field.upload(f.data(), f.size(), s);
+    h->onsite.upload(o.data(), o.size(), s);
+    h->h_frames.assign(n_atoms * 9, 0.0);
+    for (size_t a = 0; a < n_atoms; ++a)
+        for (int d = 0; d < 3; ++d) h->h_frames[a * 9 + d * 4] = 1.0;
+    h->h_afm.assign(n_atoms, 1.0);
+    h->h_fdrive.assign(2 * n_atoms * 3, 0.0);
+    h->frames.upload(h->h_frames.data(), h->h_frames.size(), s);
+    h->afm.upload(h->h_afm.data(), h->h_afm.size(), s);
+    h->fdrive.upload(h->h_fdrive.data(), h->h_fdrive.size(), s);
+    hip_check(hipStreamSynchronize(s), "table upload");
+
+    csl::LatDev& L = h->L;
+    L.N = int(N);
+    L.n_atoms = int(n_atoms);
+    L.n_cells = n_cells;
+    L.npad = npad;
+    L.NP = NP;
+    L.zmax = std::max(zmax, 1);
+    L.uniform = uni ? 1 : 0;
+    L.site_uniform = site_uni ? 1 : 0;
+    L.col = h->col.p;
+    L.cnt = h->cnt.p;
+    L.cnt_atom = h->cnt_atom.p;
+    L.mats = h->mats.p;
+    L.field = h->field.p;
+    L.onsite = h->onsite.p;
+    L.frames = h->frames.p;
+    L.afm = h->afm.p;
+    L.fdrive = h->fdrive.p;
+    L.width = 1.0;
+    L.freq = 0.0;
+    L.alpha = 0.0;
+}
+
+csl_lattice* new_handle() {
+    auto* h = new csl_lattice();
+    hip_check(hipGetDevice(&h->device), "hipGetDevice");
+    hip_check(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking), "hipStreamCreate");
+    int lds = 0, ncu = 0;
+    hip_check(hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, h->device),
+              "lds attribute");
+    hip_check(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, h->device),
+              "cu attribute");
+    h->lds_limit = size_t(lds);
+    h->n_cu = ncu;
+    return h;
+}
+
+bool method_is_rk4(const char* m) { return m == nullptr || std::strcmp(m, "rk4") == 0; }
+
+void check_finite(const double* v, size_t n, const char* what) {
+    for (size_t i = 0; i < n; ++i)
+        if (!std::isfinite(v[i])) fail(CSL_ERR_NONFINITE, std::string("non-finite value in ") + what);
+}
+
+// Candidate persistent launch shapes for this lattice (block multiple of 64).
+std::vector<TuneVal> persistent_candidates(const csl_lattice* h) {
+    std::vector<TuneVal> c;
+    const int NP = h->L.NP;
+    for (int spt : {1, 2, 3, 4, 6}) {
+        int block = (NP + spt - 1) / spt;
+        block = (block + 63) / 64 * 64;
+        if (block > csl::persistent_block_limit(spt) || block < 64) continue;
+        if (block * spt - NP >= 64 * spt) continue;  // a whole idle wave per site slot
+        for (bool db : {false, true})
+            if (csl::persistent_lds_bytes(h->L, block, db) <= h->lds_limit) c.push_back({block, spt, db});
+    }
+    return c;
+}
+
+TuneVal pick_persistent(const csl_lattice* h, size_t B, int block_req) {
+    auto cands = persistent_candidates(h);
+    if (cands.empty()) fail(CSL_ERR_UNSUPPORTED, "lattice does not fit the persistent kernel");
+    if (block_req > 0) {
+        for (auto& c : cands)
+            if (c.block == block_req) return c;
+        fail(CSL_ERR_ARG, "requested block size is not a valid persistent launch shape");
+    }
+    const TuneKey key{h->L.NP, h->L.zmax, h->L.uniform, B >= 64 ? 2 : (B > 1 ? 1 : 0)};
+    auto it = g_tune.find(key);
+    if (it != g_tune.end()) return it->second;
+    // heuristic: single LDS buffer with the fewest sites per thread whose
+    // block is <= 1024
+    for (auto& c : cands)
+        if (!c.dbuf) return c;
+    return cands.front();
+}
+
+struct BatchOut {
+    std::vector<double> times;
+    size_t n_steps = 0, n_rec = 0;
+};
+
+// Runs the persistent kernel; device outputs are left in h->rec_mags /
+// h->rec_energy / h->bt_final.
+void run_persistent(csl_lattice* h, size_t B, const double* d_init, size_t init_stride,
+                    const double* d_tp, const double* d_amp, const std::vector<double>& grid,
+                    double dt, size_t si, bool want_energy, bool want_final, const TuneVal& tv,
+                    double* ms_out) {
+    const size_t n_steps = grid.size() - 1;
+    const size_t n_rec = records_for(n_steps, si);
+    if (n_steps > size_t(std::numeric_limits<int>::max())) fail(CSL_ERR_ARG, "too many steps");
+    hipStream_t s = h->stream;
+    h->tgrid.upload(grid.data(), grid.size(), s);
+    h->rec_mags.ensure(n_rec * B * 9);
+    if (want_energy) h->rec_energy.ensure(n_rec * B);
+    if (want_final) h->bt_final.ensure(B * h->N * 3);
+    csl::PersistArgs A{};
+    A.B = int(B);
+    A.init = d_init;
+    A.init_stride = init_stride;
+    A.t_pulse = d_tp;
+    A.amp = d_amp;
+    A.tgrid = h->tgrid.p;
+    A.n_steps = int(n_steps);
+    A.save_interval = int(si);
+    A.n_rec = int(n_rec);
+    A.dt = dt;
+    A.mags = h->rec_mags.p;
+    A.energy = want_energy ? h->rec_energy.p : nullptr;
+    A.final_states = want_final ? h->bt_final.p : nullptr;
+    hipEvent_t e0, e1;
+    hip_check(hipEventCreate(&e0), "event");
+    hip_check(hipEventCreate(&e1), "event");
+    hip_check(hipEventRecord(e0, s), "event record");
+    hip_check(csl::launch_persistent(h->L, A, tv.block, tv.spt, tv.dbuf, s), "k_persistent launch");
+    hip_check(hipEventRecord(e1, s), "event record");
+    hip_check(hipEventSynchronize(e1), "k_persistent");
+    float ms = 0.f;
+    hip_check(hipEventElapsedTime(&ms, e0, e1), "elapsed");
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    h->last_ms = ms;
+    h->last_launches = 1;
+    if (ms_out) *ms_out = ms;
+}
+
+// Plain path: grid-wide stage kernels on SoA buffers, B trajectories in
+// grid.y; X must already hold the SoA initial state(s).
+void run_stages(csl_lattice* h, size_t B, const double* d_tp, const double* d_amp, bool drive_on,
+                const std::vector<double>& grid, double dt, size_t si, bool want_energy,
+                double* d_states_out /* [n_rec][B][N*3] or null */) {
+    const size_t n_steps = grid.size() - 1;
+    const size_t n_rec = records_for(n_steps, si);
+    hipStream_t s = h->stream;
+    const csl::LatDev& L = h->L;
+    const size_t plane = B * 3 * size_t(L.NP);
+    h->ACC.ensure(plane);
+    h->Y1.ensure(plane);
+    h->Y2.ensure(plane);
+    const int nblk = csl::stage_blocks(L);
+    h->partials.ensure(B * size_t(nblk) * 10);
+    h->rec_mags.ensure(n_rec * B * 9);
+    if (want_energy) h->rec_energy.ensure(n_rec * B);
+    h->tgrid.upload(grid.data(), grid.size(), s);
+    if (drive_on) {
+        h->fac.ensure(std::max<size_t>(n_steps * 3 * B * 2, 1));
+        hip_check(csl::launch_drive_table(h->tgrid.p, int(n_steps), dt, int(B), d_tp, d_amp,
+                                          L.width, L.freq, h->fac.p, s),
+                  "drive table");
+    }
+    const double inv_n = 1.0 / double(h->N);
+    hipEvent_t e0, e1;
+    hip_check(hipEventCreate(&e0), "event");
+    hip_check(hipEventCreate(&e1), "event");
+    hip_check(hipEventRecord(e0, s), "event record");
+    size_t launches = 0;
+    for (size_t n = 0; n < n_steps; ++n) {
+        const bool rec = (n % si) == 0;
+        const size_t ri = n / si;
+        if (rec && d_states_out)
+            hip_check(csl::launch_perm_to_aos(L, h->X.p, d_states_out + ri * B * h->N * 3, int(B), s),
+                      "state record");
+        for (int st = 0; st < 4; ++st) {
+            csl::StageArgs A{};
+            A.B = int(B);
+            A.stage = st;
+            A.x = h->X.p;
+            A.acc = h->ACC.p;
+            A.ycur = st == 0 ? h->X.p : (st == 2 ? h->Y2.p : h->Y1.p);
+            A.ynext = st == 0 ? h->Y1.p : (st == 1 ? h->Y2.p : (st == 2 ? h->Y1.p : h->Y2.p));
+            const int stt = st == 0 ? 0 : (st == 3 ? 2 : 1);
+            A.fac = drive_on ? h->fac.p + ((n * 3 + stt) * B) * 2 : nullptr;
+            A.dt = dt;
+            A.rec = (st == 0 && rec) ? 1 : 0;
+            A.partials = h->partials.p;
+            hip_check(csl::launch_stage(L, A, s), "k_stage launch");
+            ++launches;
+        }
+        if (rec)
+            hip_check(csl::launch_reduce_obs(h->partials.p, nblk, int(B), int(ri), inv_n, h->rec_mags.p,
+                                             want_energy ? h->rec_energy.p : nullptr, s),
+                      "reduce");
+    }
+    hip_check(hipEventRecord(e1, s), "event record");
+    hip_check(csl::launch_observe(L, h->X.p, h->partials.p, int(B), s), "observe");
+    hip_check(csl::launch_reduce_obs(h->partials.p, nblk, int(B), int(n_rec - 1), inv_n, h->rec_mags.p,
+                                     want_energy ? h->rec_energy.p : nullptr, s),
+              "reduce");
+    if (d_states_out)
+        hip_check(csl::launch_perm_to_aos(L, h->X.p, d_states_out + (n_rec - 1) * B * h->N * 3, int(B), s),
+                  "state record");
+    hip_check(hipStreamSynchronize(s), "stage integration");
+    float ms = 0.f;
+    hip_check(hipEventElapsedTime(&ms, e0, e1), "elapsed");
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    h->last_ms = launches ? ms / double(launches) : 0.0;
+    h->last_launches = launches;
+}
+
+void copy_back(double* host, const double* dev, size_t n, hipStream_t s) {
+    if (!host || !n) return;
+    hip_check(hipMemcpyAsync(host, dev, n * sizeof(double), hipMemcpyDeviceToHost, s), "D2H");
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* csl_last_error(void) { return g_err.c_str(); }
+
+int csl_version(int* major, int* minor) {
+    return guarded([&] {
+        if (!major || !minor) fail(CSL_ERR_NULL, "null output");
+        *major = 0;
+        *minor = 1;
+    });
+}
+
+int csl_device_count(int* count) {
+    return guarded([&] {
+        if (!count) fail(CSL_ERR_NULL, "null output");
+        int c = 0;
+        hip_check(hipGetDeviceCount(&c), "hipGetDeviceCount");
+        *count = c;
+    });
+}
+
+int csl_create_lattice(size_t N, size_t D, size_t n_atoms, size_t max_bi, const double* field,
+                       const double* onsite, const double* bilinear, const size_t* partners,
+                       const size_t* counts, csl_lattice** out) {
+    return guarded([&] {
+        if (!out || !field || !onsite || !counts || (max_bi && (!bilinear || !partners)))
+            fail(CSL_ERR_NULL, "null table pointer");
+        *out = nullptr;
+        if (D != 3) fail(CSL_ERR_UNSUPPORTED, "spin_dim != 3 is handled by the mixed-lattice API");
+        // padded reference layout -> CSR (lattice.h:5459-5479)
+        std::vector<size_t> off(N + 1, 0), part;
+        std::vector<double> J;
+        for (size_t i = 0; i < N; ++i) {
+            if (counts[i] > max_bi) fail(CSL_ERR_ARG, "bond count exceeds max_bilinear");
+            off[i + 1] = off[i] + counts[i];
+            for (size_t n = 0; n < counts[i]; ++n) {
+                part.push_back(partners[i * max_bi + n]);
+                J.insert(J.end(), bilinear + (i * max_bi + n) * D * D, bilinear + (i * max_bi + n + 1) * D * D);
+            }
+        }
+        csl_lattice* h = new_handle();
+        try {
+            build_tables(h, N, D, n_atoms, field, onsite, off, part, J.data());
+        } catch (...) {
+            delete h;
+            throw;
+        }
+        *out = h;
+    });
+}
+
+int csl_create_lattice_csr(size_t N, size_t D, size_t n_atoms, const double* field,
+                           const double* onsite, const size_t* offset, const size_t* partner,
+                           const double* J, csl_lattice** out) {
+    return guarded([&] {
+        if (!out || !field || !onsite || !offset) fail(CSL_ERR_NULL, "null table pointer");
+        *out = nullptr;
+        std::vector<size_t> off(offset, offset + N + 1);
+        if (off[N] && (!partner || !J)) fail(CSL_ERR_NULL, "null bond table");
+        std::vector<size_t> part(partner, partner + off[N]);
+        csl_lattice* h = new_handle();
+        try {
+            build_tables(h, N, D, n_atoms, field, onsite, off, part, J);
+        } catch (...) {
+            delete h;
+            throw;
+        }
+        *out = h;
+    });
+}
+
+int csl_destroy_lattice(csl_lattice* h) {
+    return guarded([&] {
+        if (!h) fail(CSL_ERR_NULL, "null lattice handle");
+        (void)hipSetDevice(h->device);
+        (void)hipStreamSynchronize(h->stream);
+        delete h;
+    });
+}
+
+int csl_lattice_info(const csl_lattice* h, size_t* N, size_t* D, size_t* n_atoms, size_t* n_padded,
+                     size_t* max_bonds, int* uniform) {
+    return guarded([&] {
+        if (!h) fail(CSL_ERR_NULL, "null lattice handle");
+        if (N) *N = h->N;
+        if (D) *D = h->D;
+        if (n_atoms) *n_atoms = h->n_atoms;
+        if (n_padded) *n_padded = size_t(h->L.NP);
+        if (max_bonds) *max_bonds = size_t(h->L.zmax);
+        if (uniform) *uniform = h->L.uniform;
+    });
+}
+
+int csl_set_pulse(csl_lattice* h, const double* field_drive, double amp, double width, double freq,
+                  double t_pulse_1, double t_pulse_2) {
+    return guarded([&] {
+        check_handle(h);
+        if (!field_drive) fail(CSL_ERR_NULL, "null field_drive");
+        check_finite(field_drive, 2 * h->n_atoms * 3, "field_drive");
+        if (!std::isfinite(amp) || !std::isfinite(width) || !std::isfinite(freq) ||
+            !std::isfinite(t_pulse_1) || !std::isfinite(t_pulse_2))
+            fail(CSL_ERR_NONFINITE, "non-finite pulse parameter");
+        h->h_fdrive.assign(field_drive, field_drive + 2 * h->n_atoms * 3);
+        h->fdrive.upload(h->h_fdrive.data(), h->h_fdrive.size(), h->stream);
+        hip_check(hipStreamSynchronize(h->stream), "pulse upload");
+        h->amp = amp;
+        h->width = width;
+        h->freq = freq;
+        h->t1 = t_pulse_1;
+        h->t2 = t_pulse_2;
+        h->L.width = width;
+        h->L.freq = freq;
+    });
+}
+
+int csl_set_damping(csl_lattice* h, double alpha) {
+    return guarded([&] {
+        check_handle(h);
+        if (!std::isfinite(alpha) || alpha < 0.0) fail(CSL_ERR_ARG, "alpha must be finite and >= 0");
+        h->L.alpha = alpha;
+    });
+}
+
+int csl_set_frames(csl_lattice* h, const double* frames, const double* afm_signs) {
+    return guarded([&] {
+        check_handle(h);
+        if (frames) {
+            check_finite(frames, h->n_atoms * 9, "frames");
+            h->h_frames.assign(frames, frames + h->n_atoms * 9);
+        }
+        if (afm_signs) {
+            check_finite(afm_signs, h->n_atoms, "afm_signs");
+            h->h_afm.assign(afm_signs, afm_signs + h->n_atoms);
+        }
+        h->frames.upload(h->h_frames.data(), h->h_frames.size(), h->stream);
+        h->afm.upload(h->h_afm.data(), h->h_afm.size(), h->stream);
+        hip_check(hipStreamSynchronize(h->stream), "frames upload");
+    });
+}
+
+int csl_set_spins(csl_lattice* h, const double* spins, size_t n) {
+    return guarded([&] {
+        check_handle(h);
+        if (!spins) fail(CSL_ERR_NULL, "null spins");
+        if (n != h->N * h->D) fail(CSL_ERR_ARG, "spin array size must be lattice_size*spin_dim");
+        check_finite(spins, n, "spins");
+        h->state.upload(spins, n, h->stream);
+        hip_check(hipStreamSynchronize(h->stream), "spin upload");
+        h->has_state = true;
+    });
+}
+
+int csl_get_spins(csl_lattice* h, double* spins, size_t n) {
+    return guarded([&] {
+        check_handle(h);
+        if (!spins) fail(CSL_ERR_NULL, "null spins");
+        if (!h->has_state) fail(CSL_ERR_NOSTATE, "spins were never set");
+        if (n != h->N * h->D) fail(CSL_ERR_ARG, "spin array size must be lattice_size*spin_dim");
+        copy_back(spins, h->state.p, n, h->stream);
+        hip_check(hipStreamSynchronize(h->stream), "spin download");
+    });
+}
+
+int csl_count_records(double T0, double T1, double dt, size_t save_interval, int tgrid,
+                      size_t* n_steps, size_t* n_rec) {
+    return guarded([&] {
+        if (save_interval == 0) fail(CSL_ERR_ARG, "save_interval must be >= 1");
+        auto g = make_grid(T0, T1, dt, tgrid);
+        if (n_steps) *n_steps = g.size() - 1;
+        if (n_rec) *n_rec = records_for(g.size() - 1, save_interval);
+    });
+}
+
+int csl_integrate(csl_lattice* h, double T0, double T1, double dt, size_t save_interval,
+                  const char* method, int tgrid, int engine, size_t max_rec, double* times,
+                  double* states, double* mags, double* energy, size_t* n_rec_out) {
+    return guarded([&] {
+        check_handle(h);
+        if (!h->has_state) fail(CSL_ERR_NOSTATE, "spins were never set");
+        if (!method_is_rk4(method)) fail(CSL_ERR_UNSUPPORTED, "only method \"rk4\" is implemented");
+        if (save_interval == 0) fail(CSL_ERR_ARG, "save_interval must be >= 1");
+        auto grid = make_grid(T0, T1, dt, tgrid);
+        const size_t n_steps = grid.size() - 1;
+        const size_t n_rec = records_for(n_steps, save_interval);
+        if (n_rec_out) *n_rec_out = n_rec;
+        if ((times || states || mags || energy) && max_rec < n_rec)
+            fail(CSL_ERR_CAPACITY, "max_rec is smaller than the number of records");
+        hipStream_t s = h->stream;
+        const csl::LatDev& L = h->L;
+        const bool drive_on = h->amp != 0.0;
+        const double tp[2] = {h->t1, h->t2};
+        const double am[2] = {h->amp, h->amp};
+        h->bt_tp.upload(tp, 2, s);
+        h->bt_amp.upload(am, 2, s);
+        bool persistent = false;
+        if (engine == 2) persistent = true;
+        else if (engine == 0) persistent = !states && !persistent_candidates(h).empty();
+        else if (engine != 1) fail(CSL_ERR_ARG, "engine must be 0, 1 or 2");
+        if (persistent && states) fail(CSL_ERR_ARG, "the persistent engine does not record full states");
+        if (persistent) {
+            TuneVal tv = pick_persistent(h, 1, 0);
+            run_persistent(h, 1, h->state.p, 0, h->bt_tp.p, h->bt_amp.p, grid, dt, save_interval,
+                           energy != nullptr, true, tv, nullptr);
+            hip_check(hipMemcpyAsync(h->state.p, h->bt_final.p, h->N * 3 * sizeof(double),
+                                     hipMemcpyDeviceToDevice, s),
+                      "final state");
+        } else {
+            h->X.ensure(3 * size_t(L.NP));
+            hip_check(csl::launch_perm_to_soa(L, h->state.p, 0, h->X.p, 1, s), "perm");
+            if (states) h->rec_states.ensure(n_rec * h->N * 3);
+            run_stages(h, 1, h->bt_tp.p, h->bt_amp.p, drive_on, grid, dt, save_interval, energy != nullptr,
+                       states ? h->rec_states.p : nullptr);
+            hip_check(csl::launch_perm_to_aos(L, h->X.p, h->state.p, 1, s), "perm");
+        }
+        copy_back(mags, h->rec_mags.p, n_rec * 9, s);
+        copy_back(energy, h->rec_energy.p, n_rec, s);
+        copy_back(states, h->rec_states.p, n_rec * h->N * 3, s);
+        hip_check(hipStreamSynchronize(s), "integrate");
+        if (times) {
+            size_t r = 0;
+            for (size_t n = 0; n < n_steps; n += save_interval) times[r++] = grid[n];
+            times[r] = grid[n_steps];
+        }
+    });
+}
+
+int csl_step(csl_lattice* h, double t, double dt, const char* method) {
+    return guarded([&] {
+        check_handle(h);
+        if (!h->has_state) fail(CSL_ERR_NOSTATE, "spins were never set");
+        if (!method_is_rk4(method)) fail(CSL_ERR_UNSUPPORTED, "only method \"rk4\" is implemented");
+        if (!(dt > 0.0) || !std::isfinite(t)) fail(CSL_ERR_ARG, "need finite t and dt > 0");
+        hipStream_t s = h->stream;
+        const csl::LatDev& L = h->L;
+        std::vector<double> grid = {t, t + dt};
+        const double tp[2] = {h->t1, h->t2};
+        const double am[2] = {h->amp, h->amp};
+        h->bt_tp.upload(tp, 2, s);
+        h->bt_amp.upload(am, 2, s);
+        h->X.ensure(3 * size_t(L.NP));
+        hip_check(csl::launch_perm_to_soa(L, h->state.p, 0, h->X.p, 1, s), "perm");
+        run_stages(h, 1, h->bt_tp.p, h->bt_amp.p, h->amp != 0.0, grid, dt, 1, false, nullptr);
+        hip_check(csl::launch_perm_to_aos(L, h->X.p, h->state.p, 1, s), "perm");
+        hip_check(hipStreamSynchronize(s), "step");
+    });
+}
+
+int csl_local_field(csl_lattice* h, double* H, size_t n) {
+    return guarded([&] {
+        check_handle(h);
+        if (!h->has_state) fail(CSL_ERR_NOSTATE, "spins were never set");
+        if (!H || n != h->N * 3) fail(CSL_ERR_ARG, "output must hold lattice_size*spin_dim doubles");
+        hipStream_t s = h->stream;
+        const csl::LatDev& L = h->L;
+        h->X.ensure(3 * size_t(L.NP));
+        h->aux.ensure(3 * size_t(L.NP));
+        hip_check(hipMemsetAsync(h->aux.p, 0, 3 * size_t(L.NP) * sizeof(double), s), "memset");
+        hip_check(csl::launch_perm_to_soa(L, h->state.p, 0, h->X.p, 1, s), "perm");
+        hip_check(csl::launch_eval(L, h->X.p, h->aux.p, nullptr, 0.0, 0.0, 0, s), "eval");
+        h->Y1.ensure(h->N * 3);
+        hip_check(csl::launch_perm_to_aos(L, h->aux.p, h->Y1.p, 1, s), "perm");
+        copy_back(H, h->Y1.p, n, s);
+        hip_check(hipStreamSynchronize(s), "local field");
+    });
+}
+
+int csl_rhs(csl_lattice* h, double t, double* dsdt, size_t n) {
+    return guarded([&] {
+        check_handle(h);
+        if (!h->has_state) fail(CSL_ERR_NOSTATE, "spins were never set");
+        if (!dsdt || n != h->N * 3) fail(CSL_ERR_ARG, "output must hold lattice_size*spin_dim doubles");
+        hipStream_t s = h->stream;
+        const csl::LatDev& L = h->L;
+        // drive factors evaluated exactly as the reference host does
+        // (lattice_md.cpp:45-56)
+        double f1 = 0.0, f2 = 0.0;
+        const bool drive = h->amp != 0.0;
+        if (drive) {
+            const double d1 = t - h->t1, d2 = t - h->t2;
+            f1 = h->amp * std::exp(-std::pow(d1 / (2.0 * h->width), 2)) * std::cos(h->freq * d1);
+            f2 = h->amp * std::exp(-std::pow(d2 / (2.0 * h->width), 2)) * std::cos(h->freq * d2);
+        }
+        h->X.ensure(3 * size_t(L.NP));
+        h->aux.ensure(3 * size_t(L.NP));
+        hip_check(hipMemsetAsync(h->aux.p, 0, 3 * size_t(L.NP) * sizeof(double), s), "memset");
+        hip_check(csl::launch_perm_to_soa(L, h->state.p, 0, h->X.p, 1, s), "perm");
+        hip_check(csl::launch_eval(L, h->X.p, nullptr, h->aux.p, f1, f2, drive ? 1 : 0, s), "eval");
+        h->Y1.ensure(h->N * 3);
+        hip_check(csl::launch_perm_to_aos(L, h->aux.p, h->Y1.p, 1, s), "perm");
+        copy_back(dsdt, h->Y1.p, n, s);
+        hip_check(hipStreamSynchronize(s), "rhs");
+    });
+}
+
+int csl_energy(csl_lattice* h, double* E) {
+    return guarded([&] {
+        check_handle(h);
+        if (!E) fail(CSL_ERR_NULL, "null output");
+        if (!h->has_state) fail(CSL_ERR_NOSTATE, "spins were never set");
+        hipStream_t s = h->stream;
+        const csl::LatDev& L = h->L;
+        h->X.ensure(3 * size_t(L.NP));
+        const int nblk = csl::stage_blocks(L);
+        h->partials.ensure(size_t(nblk) * 10);
+        h->rec_mags.ensure(9);
+        h->rec_energy.ensure(1);
+        hip_check(csl::launch_perm_to_soa(L, h->state.p, 0, h->X.p, 1, s), "perm");
+        hip_check(csl::launch_observe(L, h->X.p, h->partials.p, 1, s), "observe");
+        hip_check(csl::launch_reduce_obs(h->partials.p, nblk, 1, 0, 1.0, h->rec_mags.p, h->rec_energy.p, s),
+                  "reduce");
+        copy_back(E, h->rec_energy.p, 1, s);
+        hip_check(hipStreamSynchronize(s), "energy");
+    });
+}
+
+int csl_normalize(csl_lattice* h, double spin_length) {
+    return guarded([&] {
+        check_handle(h);
+        if (!h->has_state) fail(CSL_ERR_NOSTATE, "spins were never set");
+        hip_check(csl::launch_normalize(h->state.p, int(h->N), spin_length, h->stream), "normalize");
+        hip_check(hipStreamSynchronize(h->stream), "normalize");
+    });
+}
+
+int csl_magnetization(csl_lattice* h, double* mag_local, double* mag_staggered) {
+    return guarded([&] {
+        check_handle(h);
+        if (!mag_local || !mag_staggered) fail(CSL_ERR_NULL, "null output");
+        if (!h->has_state) fail(CSL_ERR_NOSTATE, "spins were never set");
+        // lattice_gpu.cu:729-760 downloads the state and sums on the host
+        // with sign (i % 2 == 0 ? 1 : -1); same arithmetic here.
+        std::vector<double> st(h->N * 3);
+        copy_back(st.data(), h->state.p, st.size(), h->stream);
+        hip_check(hipStreamSynchronize(h->stream), "download");
+        double ml[3] = {0, 0, 0}, ms[3] = {0, 0, 0};
+        for (size_t i = 0; i < h->N; ++i) {
+            const double sg = (i % 2 == 0) ? 1.0 : -1.0;
+            for (int d = 0; d < 3; ++d) {
+                ml[d] += st[i * 3 + d];
+                ms[d] += st[i * 3 + d] * sg;
+            }
+        }
+        for (int d = 0; d < 3; ++d) {
+            mag_local[d] = ml[d] / double(h->N);
+            mag_staggered[d] = ms[d] / double(h->N);
+        }
+    });
+}
+
+int csl_integrate_batched(csl_lattice* h, size_t B, const double* init, int init_per_replica,
+                          const double* t_pulse, const double* pulse_amp, double T0, double T1,
+                          double dt, size_t save_interval, const char* method, int tgrid, int block,
+                          size_t max_rec, double* times, double* mags, double* energy,
+                          double* final_states, size_t* n_rec_out) {
+    return guarded([&] {
+        check_handle(h);
+        if (B == 0) fail(CSL_ERR_ARG, "batch size must be >= 1");
+        if (!init || !t_pulse) fail(CSL_ERR_NULL, "null init / t_pulse");
+        if (!method_is_rk4(method)) fail(CSL_ERR_UNSUPPORTED, "batched mode implements \"rk4\" only");
+        if (save_interval == 0) fail(CSL_ERR_ARG, "save_interval must be >= 1");
+        if (B > 65535) fail(CSL_ERR_ARG, "at most 65535 trajectories per call");
+        auto grid = make_grid(T0, T1, dt, tgrid);
+        const size_t n_steps = grid.size() - 1;
+        const size_t n_rec = records_for(n_steps, save_interval);
+        if (n_rec_out) *n_rec_out = n_rec;
+        if ((times || mags || energy) && max_rec < n_rec)
+            fail(CSL_ERR_CAPACITY, "max_rec is smaller than the number of records");
+        const size_t nd = h->N * 3;
+        const size_t n_init = init_per_replica ? B * nd : nd;
+        check_finite(init, n_init, "initial state");
+        check_finite(t_pulse, 2 * B, "t_pulse");
+        hipStream_t s = h->stream;
+        h->bt_init.upload(init, n_init, s);
+        h->bt_tp.upload(t_pulse, 2 * B, s);
+        std::vector<double> am(2 * B, h->amp);
+        if (pulse_amp) {
+            check_finite(pulse_amp, 2 * B, "pulse_amp");
+            am.assign(pulse_amp, pulse_amp + 2 * B);
+        }
+        h->bt_amp.upload(am.data(), am.size(), s);
+        bool drive_on = false;
+        for (double a : am) drive_on |= (a != 0.0);
+        auto cands = persistent_candidates(h);
+        if (!cands.empty()) {
+            TuneVal tv = pick_persistent(h, B, block);
+            run_persistent(h, B, h->bt_init.p, init_per_replica ? nd : 0, h->bt_tp.p, h->bt_amp.p, grid, dt,
+                           save_interval, energy != nullptr, final_states != nullptr, tv, nullptr);
+            copy_back(final_states, h->bt_final.p, B * nd, s);
+        } else {
+            const csl::LatDev& L = h->L;
+            h->X.ensure(B * 3 * size_t(L.NP));
+            hip_check(csl::launch_perm_to_soa(L, h->bt_init.p, init_per_replica ? nd : 0, h->X.p, int(B), s),
+                      "perm");
+            run_stages(h, B, h->bt_tp.p, h->bt_amp.p, drive_on, grid, dt, save_interval, energy != nullptr,
+                       nullptr);
+            if (final_states) {
+                h->bt_final.ensure(B * nd);
+                hip_check(csl::launch_perm_to_aos(L, h->X.p, h->bt_final.p, int(B), s), "perm");
+                copy_back(final_states, h->bt_final.p, B * nd, s);
+            }
+        }
+        copy_back(mags, h->rec_mags.p, n_rec * B * 9, s);
+        copy_back(energy, h->rec_energy.p, n_rec * B, s);
+        hip_check(hipStreamSynchronize(s), "integrate_batched");
+        if (times) {
+            size_t r = 0;
+            for (size_t n = 0; n < n_steps; n += save_interval) times[r++] = grid[n];
+            times[r] = grid[n_steps];
+        }
+    });
+}
+
+int csl_pump_probe(csl_lattice* h, const double* ground, const double* field_in, double amp,
+                   double width, double freq, double tau_start, double tau_end, double tau_step,
+                   double T0, double T1, double T_step, size_t max_rec, size_t max_tau,
+                   size_t* n_rec_out, size_t* n_tau_out, double* times, double* M0, double* M1,
+                   double* M01, double* tau_values) {
+    return guarded([&] {
+        check_handle(h);
+        if (!ground || !field_in) fail(CSL_ERR_NULL, "null ground state / field_in");
+        if (!(tau_step != 0.0) || !std::isfinite(tau_step)) fail(CSL_ERR_ARG, "tau_step must be non-zero");
+        // tau grid (lattice_md.cpp:768; tau_start + i*tau_step as in the W2 loop :872)
+        const double q = std::abs((tau_end - tau_start) / tau_step);
+        if (!std::isfinite(q) || q > 1e6) fail(CSL_ERR_ARG, "tau grid too large");
+        const size_t n_tau = size_t(q) + 1;
+        if (n_tau_out) *n_tau_out = n_tau;
+        if (n_tau > max_tau) fail(CSL_ERR_CAPACITY, "max_tau is smaller than the number of delays");
+        const size_t A = h->n_atoms;
+        // Lattice::set_pulse: field_drive[p] = R_atom * field_in (lattice_md.cpp:215-216)
+        std::vector<double> fd(2 * A * 3, 0.0);
+        for (size_t a = 0; a < A; ++a)
+            for (int mu = 0; mu < 3; ++mu) {
+                double v = 0.0;
+                for (int nu = 0; nu < 3; ++nu) v += h->h_frames[a * 9 + mu * 3 + nu] * field_in[a * 3 + nu];
+                fd[a * 3 + mu] = v;
+                fd[(A + a) * 3 + mu] = v;
+            }
+        const int rc = csl_set_pulse(h, fd.data(), amp, width, freq, 0.0, 0.0);
+        if (rc != CSL_OK) fail(rc, g_err);
+        const size_t B = 1 + 2 * n_tau;
+        std::vector<double> tp(2 * B, 0.0), am(2 * B, 0.0), taus(n_tau);
+        for (size_t i = 0; i < n_tau; ++i) taus[i] = tau_start + double(i) * tau_step;
+        am[0] = amp;  // M0: pulse at 0 only
+        for (size_t i = 0; i < n_tau; ++i) {
+            tp[2 * (1 + i)] = taus[i];  // M1(tau): pulse at tau only
+            am[2 * (1 + i)] = amp;
+            tp[2 * (1 + n_tau + i) + 1] = taus[i];  // M01(tau): pulses at 0 and tau
+            am[2 * (1 + n_tau + i)] = amp;
+            am[2 * (1 + n_tau + i) + 1] = amp;
+        }
+        size_t n_steps = 0, n_rec = 0;
+        {
+            auto g = make_grid(T0, T1, T_step, CSL_TGRID_BOOST);
+            n_steps = g.size() - 1;
+            n_rec = records_for(n_steps, 1);
+        }
+        std::vector<double> tt(n_rec), mg(n_rec * B * 9);
+        size_t nr = 0;
+        const int rc2 = csl_integrate_batched(h, B, ground, 0, tp.data(), am.data(), T0, T1, T_step, 1, "rk4",
+                                              CSL_TGRID_BOOST, 0, n_rec, tt.data(), mg.data(), nullptr, nullptr,
+                                              &nr);
+        // reset the pulse as the reference drivers do (lattice_md.cpp:578-583)
+        std::vector<double> zero(2 * A * 3, 0.0);
+        (void)csl_set_pulse(h, zero.data(), 0.0, width, freq, 0.0, 0.0);
+        if (rc2 != CSL_OK) fail(rc2, g_err);
+        // CPU observer gate (lattice_md.cpp:532-554)
+        std::vector<size_t> keep;
+        double last = T0;
+        for (size_t r = 0; r < nr; ++r) {
+            const double t = tt[r];
+            if (t - last >= T_step - 1e-10 || t >= T1 - 1e-10) {
+                keep.push_back(r);
+                last = t;
+            }
+        }
+        if (n_rec_out) *n_rec_out = keep.size();
+        if (keep.size() > max_rec) fail(CSL_ERR_CAPACITY, "max_rec is smaller than the number of records");
+        for (size_t j = 0; j < keep.size(); ++j) {
+            const size_t r = keep[j];
+            if (times) times[j] = tt[r];
+            if (M0) std::memcpy(M0 + j * 9, &mg[(r * B + 0) * 9], 9 * sizeof(double));
+            for (size_t i = 0; i < n_tau; ++i) {
+                if (M1) std::memcpy(M1 + (i * max_rec + j) * 9, &mg[(r * B + 1 + i) * 9], 9 * sizeof(double));
+                if (M01)
+                    std::memcpy(M01 + (i * max_rec + j) * 9, &mg[(r * B + 1 + n_tau + i) * 9], 9 * sizeof(double));
+            }
+        }
+        if (tau_values) std::memcpy(tau_values, taus.data(), n_tau * sizeof(double));
+    });
+}
+
+int csl_autotune_batched(csl_lattice* h, size_t B, size_t n_probe_steps, int* best_block,
+                         double* best_ms) {
+    return guarded([&] {
+        check_handle(h);
+        if (B == 0 || n_probe_steps == 0) fail(CSL_ERR_ARG, "need B >= 1 and n_probe_steps >= 1");
+        auto cands = persistent_candidates(h);
+        if (cands.empty()) fail(CSL_ERR_UNSUPPORTED, "lattice does not fit the persistent kernel");
+        const size_t nd = h->N * 3;
+        hipStream_t s = h->stream;
+        // probe state: unit spins along a fixed direction
+        std::vector<double> init(nd);
+        for (size_t i = 0; i < h->N; ++i) {
+            init[3 * i] = 0.6;
+            init[3 * i + 1] = 0.0;
+            init[3 * i + 2] = 0.8;
+        }
+        h->bt_init.upload(init.data(), nd, s);
+        std::vector<double> tp(2 * B, 0.0), am(2 * B, h->amp);
+        h->bt_tp.upload(tp.data(), tp.size(), s);
+        h->bt_amp.upload(am.data(), am.size(), s);
+        std::vector<double> grid(n_probe_steps + 1);
+        for (size_t k = 0; k <= n_probe_steps; ++k) grid[k] = 0.01 * double(k);
+        double best = 1e300;
+        TuneVal bv = cands.front();
+        for (auto& c : cands) {
+            double ms = 0.0;
+            run_persistent(h, B, h->bt_init.p, 0, h->bt_tp.p, h->bt_amp.p, grid, 0.01, 1, true, false, c, &ms);
+            run_persistent(h, B, h->bt_init.p, 0, h->bt_tp.p, h->bt_amp.p, grid, 0.01, 1, true, false, c, &ms);
+            if (ms < best) {
+                best = ms;
+                bv = c;
+            }
+        }
+        const TuneKey key{h->L.NP, h->L.zmax, h->L.uniform, B >= 64 ? 2 : (B > 1 ? 1 : 0)};
+        g_tune[key] = bv;
+        if (best_block) *best_block = bv.block;
+        if (best_ms) *best_ms = best;
+    });
+}
+
+int csl_last_kernel_ms(csl_lattice* h, double* ms, size_t* launches) {
+    return guarded([&] {
+        if (!h) fail(CSL_ERR_NULL, "null lattice handle");
+        if (ms) *ms = h->last_ms;
+        if (launches) *launches = h->last_launches;
+    });
+}
+
+}  // extern "C"

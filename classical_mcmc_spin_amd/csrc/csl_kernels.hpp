@@ -1,0 +1,62 @@
+// csl_kernels.hpp — kernel argument blocks and host launchers of
+// csl_kernels.hip.
+#pragma once
+
+#include "csl_internal.hpp"
+
+namespace csl {
+
+// Persistent batched integration: one workgroup per trajectory.
+struct PersistArgs {
+    int B;
+    const double* init;      // AoS reference order; trajectory b starts at init + b*init_stride
+    size_t init_stride;      // 0: all trajectories share one initial state
+    const double* t_pulse;   // [B*2]
+    const double* amp;       // [B*2]
+    const double* tgrid;     // [n_steps+1] step start times (+ final time)
+    int n_steps;
+    int save_interval;
+    int n_rec;
+    double dt;
+    double* mags;            // [n_rec*B*9] or null
+    double* energy;          // [n_rec*B] or null
+    double* final_states;    // [B*N*3] AoS or null
+};
+
+// One RK4 stage over all sites of B trajectories (SoA [B][3][NP] buffers).
+struct StageArgs {
+    int B;
+    int stage;               // 0..3
+    double* x;               // state at step start (updated in stage 3)
+    double* acc;             // RK4 accumulator
+    const double* ycur;      // stage input
+    double* ynext;           // next stage input (unused in stage 3)
+    const double* fac;       // [B*2] drive factors of this stage, or null (no drive)
+    double dt;
+    int rec;                 // stage 0 of a recorded step: write block partials
+    double* partials;        // [B][nblk][10]
+};
+
+size_t persistent_lds_bytes(const LatDev& L, int block, bool dbuf);
+bool persistent_spt_supported(int spt);
+int persistent_block_limit(int spt);
+hipError_t launch_persistent(const LatDev& L, const PersistArgs& A, int block, int spt, bool dbuf,
+                             hipStream_t st);
+int stage_blocks(const LatDev& L);
+hipError_t launch_stage(const LatDev& L, const StageArgs& A, hipStream_t st);
+hipError_t launch_observe(const LatDev& L, const double* y, double* partials, int B,
+                          hipStream_t st);
+hipError_t launch_reduce_obs(const double* partials, int nblk, int B, int ri, double inv_n,
+                             double* mags, double* energy, hipStream_t st);
+hipError_t launch_eval(const LatDev& L, const double* y, double* H, double* dsdt, double f1,
+                       double f2, int drive_on, hipStream_t st);
+hipError_t launch_perm_to_soa(const LatDev& L, const double* aos, size_t aos_stride, double* soa,
+                              int B, hipStream_t st);
+hipError_t launch_perm_to_aos(const LatDev& L, const double* soa, double* aos, int B,
+                              hipStream_t st);
+hipError_t launch_normalize(double* s, int N, double len, hipStream_t st);
+hipError_t launch_drive_table(const double* tgrid, int n_steps, double dt, int B,
+                              const double* t_pulse, const double* amp, double width, double freq,
+                              double* fac, hipStream_t st);
+
+}  // namespace csl

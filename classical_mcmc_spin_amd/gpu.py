@@ -1,0 +1,288 @@
+"""Device lattice handle and the reference's simulator entry points on it.
+
+GPULattice wraps one csl_lattice handle (include/csl_gpu.h).  The driver
+functions below mirror the reference's Lattice simulator API for the
+Landau–Lifshitz path (src/core/lattice_md.cpp):
+
+  molecular_dynamics        Lattice::molecular_dynamics_cpu, method "rk4"
+                            (lattice_md.cpp:341-453)
+  single_pulse_drive        Lattice::single_pulse_drive (:493-586)
+  double_pulse_drive        Lattice::double_pulse_drive (:589-680)
+  pump_probe_spectroscopy   Lattice::pump_probe_spectroscopy (:683-926)
+
+with the same argument meaning, the same time grid (Boost integrate_const,
+lattice.h:4034-4039) and the same observer gating; every integration runs on
+the GPU through libcsl_hip.so.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from typing import Optional
+
+import numpy as np
+
+from . import _capi
+from ._capi import check, dptr, f64, lib, szptr
+from .model import Lattice
+
+
+class GPULattice:
+    """Owns a device copy of a Lattice's tables (csl_create_lattice_csr)."""
+
+    def __init__(self, lattice: Lattice, padded: bool = False):
+        if lattice.spin_dim != 3:
+            raise _capi.CslError(_capi.CSL_ERR_UNSUPPORTED, "spin_dim != 3 needs the mixed-lattice API")
+        self.lattice = lattice
+        L = lib()
+        h = C.c_void_p()
+        N, D, A = lattice.lattice_size, lattice.spin_dim, lattice.N_atoms
+        field = f64(lattice.field)
+        onsite = f64(lattice.onsite)
+        if padded:  # reference create_gpu_lattice_data layout
+            bil, par, cnt = lattice.padded_tables()
+            bil = f64(bil)
+            check(L.csl_create_lattice(N, D, A, lattice.num_bi, dptr(field), dptr(onsite), dptr(bil),
+                                       szptr(np.ascontiguousarray(par)), szptr(np.ascontiguousarray(cnt)),
+                                       C.byref(h)))
+        else:
+            J = f64(lattice.bi_J)
+            check(L.csl_create_lattice_csr(N, D, A, dptr(field), dptr(onsite),
+                                           szptr(np.ascontiguousarray(lattice.bi_offset)),
+                                           szptr(np.ascontiguousarray(lattice.bi_partner)), dptr(J),
+                                           C.byref(h)))
+        self.h = h
+        self.N, self.D, self.A = N, D, A
+        self.set_frames(lattice.frames, lattice.afm_signs)
+
+    # -- lifetime
+    def close(self):
+        if self.h:
+            check(lib().csl_destroy_lattice(self.h))
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def info(self):
+        vals = [C.c_size_t() for _ in range(5)]
+        u = C.c_int()
+        check(lib().csl_lattice_info(self.h, *[C.byref(v) for v in vals], C.byref(u)))
+        keys = ["N", "D", "n_atoms", "n_padded", "max_bonds"]
+        d = {k: v.value for k, v in zip(keys, vals)}
+        d["uniform"] = bool(u.value)
+        return d
+
+    # -- state and parameters
+    def set_frames(self, frames, afm_signs):
+        check(lib().csl_set_frames(self.h, dptr(f64(frames)), dptr(f64(afm_signs))))
+
+    def set_spins(self, spins):
+        s = f64(spins).reshape(-1)
+        check(lib().csl_set_spins(self.h, dptr(s), s.size))
+
+    def get_spins(self) -> np.ndarray:
+        out = np.empty(self.N * self.D)
+        check(lib().csl_get_spins(self.h, dptr(out), out.size))
+        return out.reshape(self.N, self.D)
+
+    def set_pulse(self, field_drive, amp, width, freq, t1, t2):
+        fd = f64(field_drive).reshape(-1)
+        if fd.size != 2 * self.A * self.D:
+            raise ValueError("field_drive must hold 2*N_atoms*spin_dim values")
+        check(lib().csl_set_pulse(self.h, dptr(fd), amp, width, freq, t1, t2))
+
+    def set_damping(self, alpha: float):
+        check(lib().csl_set_damping(self.h, float(alpha)))
+
+    # -- evaluation
+    def energy(self) -> float:
+        e = C.c_double()
+        check(lib().csl_energy(self.h, C.byref(e)))
+        return e.value
+
+    def local_field(self) -> np.ndarray:
+        out = np.empty(self.N * 3)
+        check(lib().csl_local_field(self.h, dptr(out), out.size))
+        return out.reshape(self.N, 3)
+
+    def rhs(self, t: float) -> np.ndarray:
+        out = np.empty(self.N * 3)
+        check(lib().csl_rhs(self.h, float(t), dptr(out), out.size))
+        return out.reshape(self.N, 3)
+
+    def magnetization(self):
+        ml, ms = np.empty(3), np.empty(3)
+        check(lib().csl_magnetization(self.h, dptr(ml), dptr(ms)))
+        return ml, ms
+
+    def normalize(self, spin_length: float = 1.0):
+        check(lib().csl_normalize(self.h, float(spin_length)))
+
+    def step(self, t: float, dt: float, method: str = "rk4"):
+        check(lib().csl_step(self.h, float(t), float(dt), method.encode()))
+
+    # -- integration
+    @staticmethod
+    def count_records(T0, T1, dt, save_interval=1, tgrid="gpu"):
+        ns, nr = C.c_size_t(), C.c_size_t()
+        check(lib().csl_count_records(T0, T1, dt, save_interval, _capi.TGRID[tgrid], C.byref(ns), C.byref(nr)))
+        return ns.value, nr.value
+
+    def integrate(self, T0, T1, dt, save_interval=1, method="rk4", tgrid="gpu", engine=0,
+                  want_states=False, want_energy=True):
+        """gpu::integrate_gpu (lattice_gpu_api.h:117-125) plus M/E records.
+        Returns dict(times, mags [n,3,3], energy [n], states [n,N,3] or None)."""
+        _, nr = self.count_records(T0, T1, dt, save_interval, tgrid)
+        times = np.empty(nr)
+        mags = np.empty(nr * 9)
+        energy = np.empty(nr) if want_energy else None
+        states = np.empty(nr * self.N * 3) if want_states else None
+        n = C.c_size_t()
+        check(lib().csl_integrate(self.h, T0, T1, dt, save_interval, method.encode(), _capi.TGRID[tgrid],
+                                  engine, nr, dptr(times), dptr(states), dptr(mags), dptr(energy),
+                                  C.byref(n)))
+        return {"times": times, "mags": mags.reshape(nr, 3, 3), "energy": energy,
+                "states": None if states is None else states.reshape(nr, self.N, 3)}
+
+    def integrate_batched(self, init, t_pulse, pulse_amp=None, T0=0.0, T1=1.0, dt=0.01,
+                          save_interval=1, method="rk4", tgrid="batched", block=0,
+                          want_energy=True, want_final=False):
+        """Generalised gpu::integrate_gpu_batched (lattice_gpu_api.h:200-214).
+        init: [N,3] shared or [B,N,3]; t_pulse, pulse_amp: [B,2].
+        Returns dict(times, mags [n,B,3,3], energy [n,B], final [B,N,3])."""
+        tp = f64(t_pulse).reshape(-1, 2)
+        B = tp.shape[0]
+        init = f64(init)
+        per = 1 if init.ndim == 3 else 0
+        if per and init.shape[0] != B:
+            raise ValueError("init has a different batch size than t_pulse")
+        amp = None if pulse_amp is None else f64(pulse_amp).reshape(B, 2)
+        _, nr = self.count_records(T0, T1, dt, save_interval, tgrid)
+        times = np.empty(nr)
+        mags = np.empty(nr * B * 9)
+        energy = np.empty(nr * B) if want_energy else None
+        final = np.empty(B * self.N * 3) if want_final else None
+        n = C.c_size_t()
+        check(lib().csl_integrate_batched(self.h, B, dptr(init.reshape(-1)), per, dptr(tp.reshape(-1)),
+                                          dptr(None if amp is None else amp.reshape(-1)), T0, T1, dt,
+                                          save_interval, method.encode(), _capi.TGRID[tgrid], block, nr,
+                                          dptr(times), dptr(mags), dptr(energy), dptr(final), C.byref(n)))
+        return {"times": times, "mags": mags.reshape(nr, B, 3, 3),
+                "energy": None if energy is None else energy.reshape(nr, B),
+                "final": None if final is None else final.reshape(B, self.N, 3)}
+
+    def autotune(self, B: int, n_probe_steps: int = 64):
+        blk, ms = C.c_int(), C.c_double()
+        check(lib().csl_autotune_batched(self.h, B, n_probe_steps, C.byref(blk), C.byref(ms)))
+        return blk.value, ms.value
+
+    def last_kernel_ms(self):
+        ms, n = C.c_double(), C.c_size_t()
+        check(lib().csl_last_kernel_ms(self.h, C.byref(ms), C.byref(n)))
+        return ms.value, n.value
+
+    def pump_probe(self, ground, field_in, amp, width, freq, tau_start, tau_end, tau_step,
+                   T0, T1, T_step):
+        """csl_pump_probe: M0, M1(tau), M01(tau) in one batched launch."""
+        g = f64(ground).reshape(-1)
+        fi = f64(field_in).reshape(self.A, 3)
+        n_tau_max = int(abs((tau_end - tau_start) / tau_step)) + 2
+        ns, _ = self.count_records(T0, T1, T_step, 1, "boost")
+        max_rec = ns + 1
+        times = np.empty(max_rec)
+        M0 = np.empty(max_rec * 9)
+        M1 = np.empty(n_tau_max * max_rec * 9)
+        M01 = np.empty(n_tau_max * max_rec * 9)
+        taus = np.empty(n_tau_max)
+        nr, nt = C.c_size_t(), C.c_size_t()
+        check(lib().csl_pump_probe(self.h, dptr(g), dptr(fi.reshape(-1)), amp, width, freq, tau_start,
+                                   tau_end, tau_step, T0, T1, T_step, max_rec, n_tau_max, C.byref(nr),
+                                   C.byref(nt), dptr(times), dptr(M0), dptr(M1), dptr(M01), dptr(taus)))
+        r, t = nr.value, nt.value
+        return {"times": times[:r], "M0": M0.reshape(max_rec, 3, 3)[:r],
+                "M1": M1.reshape(n_tau_max, max_rec, 3, 3)[:t, :r],
+                "M01": M01.reshape(n_tau_max, max_rec, 3, 3)[:t, :r], "tau": taus[:t]}
+
+
+# ---------------------------------------------------------------------------
+# Reference-style simulator entry points (CPU semantics, GPU arithmetic)
+
+def _drive_dirs(lattice: Lattice, field_in) -> np.ndarray:
+    """Lattice::set_pulse frame rotation field_drive = R_atom * B (lattice_md.cpp:215-216)."""
+    fi = np.asarray(field_in, dtype=float).reshape(lattice.N_atoms, lattice.spin_dim)
+    return np.einsum("aij,aj->ai", lattice.frames, fi)
+
+
+def _gate(times, T_start, T_end, step):
+    """Pulse-drive observer gate (lattice_md.cpp:532-554)."""
+    keep, last = [], T_start
+    for r, t in enumerate(times):
+        if t - last >= step - 1e-10 or t >= T_end - 1e-10:
+            keep.append(r)
+            last = t
+    return np.array(keep, dtype=np.int64)
+
+
+def molecular_dynamics(gl: GPULattice, spins, T_start, T_end, dt, save_interval=1, method="rk4",
+                       alpha_gilbert=0.0, engine=0):
+    """molecular_dynamics_cpu with method "rk4" (lattice_md.cpp:341-453): the
+    observer records M_antiferro, M_local, M_global and E/N every save_interval
+    integrate_const calls (HDF5 output omitted).  Returns dict(times, mags, energy)."""
+    gl.set_damping(alpha_gilbert)
+    gl.set_pulse(np.zeros(2 * gl.A * 3), 0.0, 1.0, 0.0, 0.0, 0.0)
+    gl.set_spins(spins)
+    return gl.integrate(T_start, T_end, dt, save_interval, method, tgrid="boost", engine=engine)
+
+
+def _pulse_run(gl: GPULattice, spins, fd, t1, t2, amp, width, freq, T_start, T_end, step):
+    gl.set_pulse(fd, amp, width, freq, t1, t2)
+    gl.set_spins(spins)
+    res = gl.integrate(T_start, T_end, step, 1, "rk4", tgrid="boost", want_energy=False)
+    gl.set_pulse(np.zeros_like(fd), 0.0, width, freq, 0.0, 0.0)  # reset (lattice_md.cpp:578-583)
+    keep = _gate(res["times"], T_start, T_end, step)
+    return res["times"][keep], res["mags"][keep]
+
+
+def single_pulse_drive(gl: GPULattice, spins, field_in, t_B, pulse_amp, pulse_width, pulse_freq,
+                       T_start, T_end, step_size, method="rk4"):
+    """Lattice::single_pulse_drive (lattice_md.cpp:493-586).  Returns
+    (times [n], mags [n, 3(M_antiferro, M_local, M_global), 3])."""
+    if method != "rk4":
+        raise _capi.CslError(_capi.CSL_ERR_UNSUPPORTED, "only method 'rk4' is implemented")
+    lat = gl.lattice
+    fd = np.concatenate([_drive_dirs(lat, field_in).reshape(-1), np.zeros(lat.N_atoms * 3)])
+    return _pulse_run(gl, spins, fd, t_B, 0.0, pulse_amp, pulse_width, pulse_freq, T_start, T_end, step_size)
+
+
+def double_pulse_drive(gl: GPULattice, spins, field_in_1, t_B_1, field_in_2, t_B_2, pulse_amp,
+                       pulse_width, pulse_freq, T_start, T_end, step_size, method="rk4"):
+    """Lattice::double_pulse_drive (lattice_md.cpp:589-680)."""
+    if method != "rk4":
+        raise _capi.CslError(_capi.CSL_ERR_UNSUPPORTED, "only method 'rk4' is implemented")
+    lat = gl.lattice
+    fd = np.concatenate([_drive_dirs(lat, field_in_1).reshape(-1), _drive_dirs(lat, field_in_2).reshape(-1)])
+    return _pulse_run(gl, spins, fd, t_B_1, t_B_2, pulse_amp, pulse_width, pulse_freq, T_start, T_end,
+                      step_size)
+
+
+def pump_probe_spectroscopy(gl: GPULattice, ground, field_in, pulse_amp, pulse_width, pulse_freq,
+                            tau_start, tau_end, tau_step, T_start, T_end, T_step):
+    """Lattice::pump_probe_spectroscopy (lattice_md.cpp:683-926) without HDF5:
+    M0, M1(tau) and M01(tau) for every delay, all integrated in one batched
+    launch.  Returns dict(times, M0 [n,3,3], M1 [n_tau,n,3,3], M01, tau)."""
+    return gl.pump_probe(ground, field_in, pulse_amp, pulse_width, pulse_freq, tau_start, tau_end,
+                         tau_step, T_start, T_end, T_step)
+
+
+def nonlinear_signal(res: dict) -> np.ndarray:
+    """M_NL(tau, t) = M01 - M0 - M1 (util/readers_new/reader_pyrochlore.py:760-830)."""
+    return res["M01"] - res["M0"][None] - res["M1"]

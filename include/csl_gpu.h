@@ -1,0 +1,210 @@
+/*
+ * csl_gpu.h — C ABI of the MI355X (gfx950) Landau–Lifshitz spin-dynamics
+ * library (libcsl_hip.so).
+ *
+ * This is the drop-in boundary for the reference's GPU host API
+ * (include/classical_spin/gpu/lattice_gpu_api.h, a C++ ABI with std::vector /
+ * std::string).  Every entry point below names the reference function it
+ * replaces (file:line).  All arguments are plain pointers, sizes and scalars;
+ * no C++ or torch types cross this boundary.  A C++ shim with the reference's
+ * exact `gpu::` signatures is provided on top of it in
+ * include/classical_spin_amd/lattice_gpu_api.h.
+ *
+ * Conventions
+ *   - Every function returns an int status: CSL_OK (0) on success, a
+ *     negative CSL_ERR_* code otherwise.  csl_last_error() returns the
+ *     message of the last failure on the calling thread.  (The reference
+ *     silently no-ops on a null handle, lattice_gpu.cu:660-710, and throws
+ *     std::runtime_error on CUDA errors, gpu_common_helpers.cuh:16-31; here
+ *     both become status codes.)
+ *   - Arrays are host memory, FP64, in the reference's layouts:
+ *       spins          AoS [N*D], site index ((i*d2+j)*d3+k)*n_atoms+atom
+ *       magnetisation  [t][b][3][D] with m = 0 M_antiferro, 1 M_local,
+ *                      2 M_global, each divided by N
+ *                      (BatchedMagResult, lattice_gpu_api.h:161-177)
+ *   - All state lives on the device in FP64.  The library picks the device
+ *     that is current when the handle is created and uses one stream per
+ *     handle.  Handles are not thread-safe (as in the reference).
+ */
+#ifndef CSL_GPU_H
+#define CSL_GPU_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes */
+#define CSL_OK               0
+#define CSL_ERR_NULL        -1   /* null handle / missing required pointer   */
+#define CSL_ERR_ARG         -2   /* inconsistent sizes or invalid value      */
+#define CSL_ERR_UNSUPPORTED -3   /* method / physics not implemented         */
+#define CSL_ERR_NOSTATE     -4   /* spins were never set on the handle       */
+#define CSL_ERR_HIP         -5   /* HIP runtime error                        */
+#define CSL_ERR_CAPACITY    -6   /* output buffer too small                  */
+#define CSL_ERR_NONFINITE   -7   /* NaN / Inf in an input state              */
+
+/* time-grid policies for the integrators (see DESIGN notes in csl_time.h):
+ *   CSL_TGRID_GPU     t += dt while t < T1                (integrator.cuh:470-487)
+ *   CSL_TGRID_BATCHED t += dt while t < T1 - 1e-12        (lattice_gpu.cu:557-565)
+ *   CSL_TGRID_BOOST   t_k = T0 + k*dt while t_k+dt <= T1  (Boost integrate_const,
+ *                                                          lattice.h:4034-4039) */
+#define CSL_TGRID_GPU     0
+#define CSL_TGRID_BATCHED 1
+#define CSL_TGRID_BOOST   2
+
+typedef struct csl_lattice csl_lattice;
+
+const char* csl_last_error(void);
+int csl_device_count(int* count);
+int csl_version(int* major, int* minor);
+
+/* ---- lattice handle -------------------------------------------------- */
+
+/* Replaces gpu::create_gpu_lattice_data (lattice_gpu_api.h:47-57): the
+ * reference's zero-padded layout — field [N*D], onsite [N*D*D], bilinear
+ * [N*max_bi*D*D], partners [N*max_bi] (size_t), counts [N]. */
+int csl_create_lattice(size_t N, size_t D, size_t n_atoms, size_t max_bi,
+                       const double* field, const double* onsite,
+                       const double* bilinear, const size_t* partners,
+                       const size_t* counts, csl_lattice** out);
+
+/* Same lattice from the CSR mirror bi_flat_offset [N+1] / bi_flat_partner /
+ * bi_flat_J (lattice.h:868-903). */
+int csl_create_lattice_csr(size_t N, size_t D, size_t n_atoms,
+                           const double* field, const double* onsite,
+                           const size_t* offset, const size_t* partner,
+                           const double* J, csl_lattice** out);
+
+/* Replaces gpu::destroy_gpu_lattice_data (lattice_gpu_api.h:62). */
+int csl_destroy_lattice(csl_lattice* h);
+
+/* Sizes of a handle: N, D, n_atoms, padded device sites, max bonds/site and
+ * whether the bond matrices were compressed to per-(sublattice, slot) form. */
+int csl_lattice_info(const csl_lattice* h, size_t* N, size_t* D, size_t* n_atoms,
+                     size_t* n_padded, size_t* max_bonds, int* uniform);
+
+/* Replaces gpu::set_gpu_pulse (lattice_gpu_api.h:67-75): field_drive is
+ * [2*n_atoms*D] (already frame-rotated, as Lattice::update_gpu_pulse sends
+ * it, lattice.h:5502-5522). */
+int csl_set_pulse(csl_lattice* h, const double* field_drive, double amp,
+                  double width, double freq, double t_pulse_1, double t_pulse_2);
+
+/* Gilbert damping alpha (Lattice::alpha_gilbert, lattice_md.cpp:88-98; the
+ * reference GPU path has no damping, lattice_gpu.cu:73-117). */
+int csl_set_damping(csl_lattice* h, double alpha);
+
+/* Sublattice frames [n_atoms*D*D] (row-major R, S_global = R S_local) and
+ * AFM signs [n_atoms] used by the on-device magnetisation reductions
+ * (integrate_gpu_batched arguments, lattice_gpu_api.h:200-214).  Defaults:
+ * identity frames, all signs +1. */
+int csl_set_frames(csl_lattice* h, const double* frames, const double* afm_signs);
+
+/* Replace gpu::set_gpu_spins / get_gpu_spins (lattice_gpu_api.h:83-97). */
+int csl_set_spins(csl_lattice* h, const double* spins, size_t n);
+int csl_get_spins(csl_lattice* h, double* spins, size_t n);
+
+/* ---- single trajectory ---------------------------------------------- */
+
+/* Replaces gpu::step_gpu (lattice_gpu_api.h:142-147).  method: "rk4". */
+int csl_step(csl_lattice* h, double t, double dt, const char* method);
+
+/* Replaces gpu::integrate_gpu (lattice_gpu_api.h:117-125).  Integrates the
+ * handle's spins from T0 to T1 with fixed dt on the given time grid policy,
+ * recording every save_interval steps plus the final state.  Any of
+ * times [max_rec], states [max_rec*N*D], mags [max_rec*3*D], energy
+ * [max_rec] (E/N) may be NULL.  *n_rec receives the number of records.
+ * engine: 0 = auto, 1 = grid-wide stage kernels (one launch per RK stage),
+ * 2 = persistent single-workgroup kernel (lattice resident in LDS). */
+int csl_integrate(csl_lattice* h, double T0, double T1, double dt,
+                  size_t save_interval, const char* method, int tgrid, int engine,
+                  size_t max_rec, double* times, double* states, double* mags,
+                  double* energy, size_t* n_rec);
+
+/* Total energy of the handle's spins (total_energy_flat, lattice.h:1485-1613;
+ * the reference GPU entry throws, lattice_gpu.cu:298-303). */
+int csl_energy(csl_lattice* h, double* E);
+
+/* Replaces gpu::normalize_spins_gpu (lattice_gpu_api.h:157). */
+int csl_normalize(csl_lattice* h, double spin_length);
+
+/* Replaces gpu::compute_magnetization_gpu (lattice_gpu_api.h:162-166):
+ * local sum and alternating-sign staggered sum, both /N. */
+int csl_magnetization(csl_lattice* h, double* mag_local, double* mag_staggered);
+
+/* Local field H_i without drive for every site, AoS [N*D]
+ * (get_local_field_flat, lattice.h:1653-1762) and dS/dt at time t
+ * (landau_lifshitz_flat, lattice_md.cpp:39-153) for the handle's spins. */
+int csl_local_field(csl_lattice* h, double* H, size_t n);
+int csl_rhs(csl_lattice* h, double t, double* dsdt, size_t n);
+
+/* ---- batched trajectories (persistent kernel) ------------------------ */
+
+/* Generalises gpu::integrate_gpu_batched (lattice_gpu_api.h:200-214).
+ *   B              number of trajectories
+ *   init           [N*D] shared by all (init_per_replica = 0) or [B*N*D]
+ *   t_pulse        [B*2] pulse centres per trajectory (reference: t_pulse_1
+ *                  shared, t_pulse_2 = tau2[b])
+ *   pulse_amp      [B*2] per-trajectory amplitude of pulse 1 / pulse 2
+ *                  (0 switches a pulse off; the reference pushes pulse 2 past
+ *                  T_end for M0, lattice.h:5688-5692).  NULL = the handle's
+ *                  amplitude for both pulses.
+ *   width/freq and the drive directions come from csl_set_pulse; frames and
+ *   AFM signs from csl_set_frames.
+ *   mags           [max_rec*B*3*D] BatchedMagResult layout; energy
+ *                  [max_rec*B] (E/N, may be NULL); final_states [B*N*D]
+ *                  (may be NULL); times [max_rec].
+ * method "rk4" only (the reference falls back to rk4, lattice_gpu.cu:781-784).
+ * block: workgroup size (0 = autotuned). */
+int csl_integrate_batched(csl_lattice* h, size_t B, const double* init,
+                          int init_per_replica, const double* t_pulse,
+                          const double* pulse_amp, double T0, double T1, double dt,
+                          size_t save_interval, const char* method, int tgrid,
+                          int block, size_t max_rec, double* times, double* mags,
+                          double* energy, double* final_states, size_t* n_rec);
+
+/* Number of records an integration will produce (same grid rules). */
+int csl_count_records(double T0, double T1, double dt, size_t save_interval,
+                      int tgrid, size_t* n_steps, size_t* n_rec);
+
+/* ---- 2DCS / pump-probe driver --------------------------------------- */
+
+/* The work of pump_probe_spectroscopy (lattice_md.cpp:683-926) and
+ * pump_probe_spectroscopy_gpu_batched (lattice.h:5617-5855) in one batched
+ * launch: M0 (pulse at 0), M1(tau) (pulse at tau) and M01(tau) (pulses at 0
+ * and tau) for every tau on the grid tau_start + i*tau_step.  field_in is
+ * the (global-frame) pulse direction per sublattice [n_atoms*D]; it is
+ * rotated by the frames as Lattice::set_pulse does.  Records follow the CPU
+ * pulse-drive observer (lattice_md.cpp:532-554): the t = T_start sample is
+ * dropped and samples sit on the Boost grid.  Outputs: times [max_rec],
+ * M0 [max_rec*3*D], M1 and M01 [max_tau*max_rec*3*D], tau_values
+ * [max_tau]. */
+int csl_pump_probe(csl_lattice* h, const double* ground, const double* field_in,
+                   double amp, double width, double freq,
+                   double tau_start, double tau_end, double tau_step,
+                   double T0, double T1, double T_step,
+                   size_t max_rec, size_t max_tau, size_t* n_rec, size_t* n_tau,
+                   double* times, double* M0, double* M1, double* M01,
+                   double* tau_values);
+
+/* ---- autotuner ------------------------------------------------------- */
+
+/* Times the persistent kernel for B trajectories over n_probe_steps at each
+ * candidate workgroup size and stores the fastest for this lattice shape;
+ * later csl_integrate_batched calls with block = 0 use it.  *best_block
+ * receives the choice, *best_ms its time. */
+int csl_autotune_batched(csl_lattice* h, size_t B, size_t n_probe_steps,
+                         int* best_block, double* best_ms);
+
+/* ---- timing support -------------------------------------------------- */
+
+/* Average duration (ms) of the dominant kernel of the last
+ * csl_integrate_batched / csl_integrate call on this handle, measured with
+ * HIP events on the handle's stream, and the number of launches it spans. */
+int csl_last_kernel_ms(csl_lattice* h, double* ms, size_t* launches);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* CSL_GPU_H */

@@ -1,0 +1,781 @@
+/*
+ * oracle.cpp — CPU FP64 restatement of the reference's Landau–Lifshitz path.
+ *
+ * TEST INFRASTRUCTURE ONLY: the checker for the HIP path and the "port" CPU
+ * baseline.  The product library never links, loads or calls this file.
+ *
+ * Built by oracle/Makefile with  g++ -O2 -fopenmp -ffp-contract=off  (no
+ * fast-math, no FMA contraction), so its arithmetic is the plain IEEE order
+ * written below.
+ *
+ * Pinning (each oracle function is checked by tests/ against something that
+ * is not a retyping of its own formula):
+ *   lattice expansion ....... neighbour counts/partners vs hand-derived
+ *                             honeycomb geometry; reverse bond = J^T; CSR
+ *                             order vs a brute-force enumeration
+ *                             (tests/test_oracle_model.py)
+ *   local field / RHS ....... H = dE/dS by central finite differences of
+ *                             orc_energy; dense O(N^2) brute force on 1x1x1 ..
+ *                             2x2x1 cells; S.dS/dt = 0; sum H.dS/dt = 0
+ *                             (tests/test_oracle_physics.py)
+ *   energy .................. closed-form FM / Neel energies of the Kitaev
+ *                             model; E conservation of undamped RK4 runs
+ *   SU(3) contraction ....... f_abc = -(i/4) Tr(l_a [l_b, l_c]) from the
+ *                             Gell-Mann matrices (numpy)
+ *   drive envelope .......... closed form exp(-(dt/2w)^2) cos(w dt) and a
+ *                             single-spin driven precession vs scipy
+ *   RK4 step ................ single spin in a static field: the RK4
+ *                             amplification of a rotation is known exactly;
+ *                             fitted global order 4
+ *   integrate_const grid .... step counts vs hand-evaluated cases
+ *   pulse segments .......... the reference's own test cases
+ *                             (test_pulse_chunking_alignment.cpp)
+ *   pump-probe driver ....... M1(tau) equals M0 shifted by tau when the ground
+ *                             state is stationary (W1 identity); serial and
+ *                             OpenMP tau loops agree
+ *   tangent linear .......... central finite differences of the trajectory
+ */
+#include "oracle.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <limits>
+#include <vector>
+
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+namespace {
+
+struct SiteBond {
+    size_t partner;
+    std::vector<double> J;   // D*D row-major copy (reference keeps one copy per bond)
+    int ucb;                 // unit-cell bond index
+    bool transposed;         // reverse bond carries J^T
+};
+
+}  // namespace
+
+struct OracleLattice {
+    int D = 3;
+    size_t n_atoms = 1, d1 = 1, d2 = 1, d3 = 1, N = 0;
+    std::vector<double> field;    // [N*D]
+    std::vector<double> onsite;   // [N*D*D]
+    std::vector<std::vector<SiteBond>> bonds;  // nested tables (lattice.h:785-803)
+    // flat mirror (lattice.h:868-903)
+    std::vector<size_t> bi_offset, bi_partner;
+    std::vector<double> bi_J;
+    std::vector<double> frames;    // [n_atoms*D*D]
+    std::vector<double> afm;       // [n_atoms]
+    // drive state (lattice.h:553-560)
+    std::vector<double> field_drive[2];  // [n_atoms*D]
+    double t_pulse[2] = {0.0, 0.0};
+    double amp = 0.0, width = 1.0, freq = 0.0;
+    double alpha = 0.0;
+};
+
+namespace {
+
+size_t flatten(const OracleLattice& L, size_t i, size_t j, size_t k, size_t atom) {
+    return ((i * L.d2 + j) * L.d3 + k) * L.n_atoms + atom;  // lattice.h:1057-1059
+}
+
+int wrap(int c, int n) {  // periodic_boundary (lattice.h:1064-1071): one wrap only
+    if (c < 0) return c + n;
+    if (c >= n) return c - n;
+    return c;
+}
+
+void build_flat(OracleLattice& L) {  // build_flat_bilinear_tables (lattice.h:868-903)
+    const size_t D2 = size_t(L.D) * L.D;
+    L.bi_offset.assign(L.N + 1, 0);
+    for (size_t i = 0; i < L.N; ++i) L.bi_offset[i + 1] = L.bi_offset[i] + L.bonds[i].size();
+    const size_t nb = L.bi_offset[L.N];
+    L.bi_partner.assign(nb, 0);
+    L.bi_J.assign(nb * D2, 0.0);
+    for (size_t i = 0; i < L.N; ++i)
+        for (size_t n = 0; n < L.bonds[i].size(); ++n) {
+            const size_t k = L.bi_offset[i] + n;
+            L.bi_partner[k] = L.bonds[i][n].partner;
+            std::copy(L.bonds[i][n].J.begin(), L.bonds[i][n].J.end(), L.bi_J.begin() + k * D2);
+        }
+}
+
+// Lattice::get_local_field_flat without twist/trilinear (lattice.h:1653-1711).
+void local_field(const OracleLattice& L, const double* x, size_t site, double* H) {
+    const int D = L.D;
+    for (int d = 0; d < D; ++d) H[d] = -L.field[site * D + d];
+    const double* S = x + site * D;
+    const double* A = &L.onsite[site * D * D];
+    for (int d = 0; d < D; ++d)
+        for (int e = 0; e < D; ++e) H[d] += 2.0 * A[d * D + e] * S[e];
+    const size_t D2 = size_t(D) * D;
+    for (size_t k = L.bi_offset[site]; k < L.bi_offset[site + 1]; ++k) {
+        const double* Sp = x + L.bi_partner[k] * D;
+        const double* J = &L.bi_J[k * D2];
+        for (int d = 0; d < D; ++d) {
+            double row = 0.0;
+            for (int e = 0; e < D; ++e) row += J[d * D + e] * Sp[e];
+            H[d] += row;
+        }
+    }
+}
+
+void drive_factors(const OracleLattice& L, double t, double& f1, double& f2) {
+    // lattice_md.cpp:45-56
+    const double dt1 = t - L.t_pulse[0];
+    const double dt2 = t - L.t_pulse[1];
+    f1 = L.amp * std::exp(-std::pow(dt1 / (2.0 * L.width), 2)) * std::cos(L.freq * dt1);
+    f2 = L.amp * std::exp(-std::pow(dt2 / (2.0 * L.width), 2)) * std::cos(L.freq * dt2);
+}
+
+// cross_prod_SU3_flat (simple_linear_alg.h:78-169), written out per triple.
+void su3_cross(const double* H, const double* S, double* out) {
+    static const int tri[9][3] = {{0, 1, 2}, {0, 3, 6}, {0, 4, 5}, {1, 3, 5}, {1, 4, 6},
+                                  {2, 3, 4}, {2, 5, 6}, {3, 4, 7}, {5, 6, 7}};
+    static const double val[9] = {1.0, 0.5, -0.5, 0.5, 0.5, 0.5, -0.5,
+                                  0.8660254037844386, 0.8660254037844386};
+    double d[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    for (int q = 0; q < 9; ++q) {
+        const int a = tri[q][0], b = tri[q][1], c = tri[q][2];
+        const double v = val[q];
+        d[a] += v * (H[b] * S[c] - H[c] * S[b]);
+        d[b] += v * (H[c] * S[a] - H[a] * S[c]);
+        d[c] += v * (H[a] * S[b] - H[b] * S[a]);
+    }
+    for (int i = 0; i < 8; ++i) out[i] = d[i];
+}
+
+// One site of landau_lifshitz_flat (lattice_md.cpp:70-127).
+void rhs_site(const OracleLattice& L, const double* x, double* dxdt, size_t i,
+              bool drive, double f1, double f2) {
+    const int D = L.D;
+    double H[16];
+    local_field(L, x, i, H);
+    if (drive) {  // apply_drive_field_flat (lattice_md.cpp:178-191)
+        const size_t atom = i % L.n_atoms;
+        const double* fd0 = &L.field_drive[0][atom * D];
+        const double* fd1 = &L.field_drive[1][atom * D];
+        for (int d = 0; d < D; ++d) H[d] -= fd0[d] * f1 + fd1[d] * f2;
+    }
+    const double* S = x + i * D;
+    double* o = dxdt + i * D;
+    if (D == 3) {
+        double dSx = H[1] * S[2] - H[2] * S[1];
+        double dSy = H[2] * S[0] - H[0] * S[2];
+        double dSz = H[0] * S[1] - H[1] * S[0];
+        if (L.alpha > 0.0) {  // lattice_md.cpp:88-98
+            const double a = S[1] * dSz - S[2] * dSy;
+            const double b = S[2] * dSx - S[0] * dSz;
+            const double c = S[0] * dSy - S[1] * dSx;
+            const double inv_S = 1.0 / std::sqrt(S[0] * S[0] + S[1] * S[1] + S[2] * S[2]);
+            dSx -= L.alpha * inv_S * a;
+            dSy -= L.alpha * inv_S * b;
+            dSz -= L.alpha * inv_S * c;
+        }
+        o[0] = dSx; o[1] = dSy; o[2] = dSz;
+    } else if (D == 8) {
+        su3_cross(H, S, o);
+    } else {
+        for (int d = 0; d < D; ++d) o[d] = 0.0;
+    }
+}
+
+void rhs(const OracleLattice& L, const double* x, double* dxdt, double t) {
+    const bool drive = (L.amp != 0.0);
+    double f1 = 0.0, f2 = 0.0;
+    if (drive) drive_factors(L, t, f1, f2);
+    const long n = long(L.N);
+#ifdef _OPENMP
+#pragma omp parallel for schedule(static) if (n >= 64)
+#endif
+    for (long i = 0; i < n; ++i) rhs_site(L, x, dxdt, size_t(i), drive, f1, f2);
+}
+
+// Boost.Odeint runge_kutta4 (explicit_generic_rk<4,4>): a = {1/2; 0,1/2; 0,0,1},
+// b = {1/6,1/3,1/3,1/6}, c = {0,1/2,1/2,1}; stage sums evaluated left to right.
+struct RK4Work {
+    std::vector<double> k1, k2, k3, k4, tmp;
+    explicit RK4Work(size_t n) : k1(n), k2(n), k3(n), k4(n), tmp(n) {}
+};
+
+template <class F>
+void rk4_step_generic(F&& f, double* x, size_t n, double t, double dt, RK4Work& w) {
+    const double b1 = 1.0 / 6.0, b2 = 1.0 / 3.0, b3 = 1.0 / 3.0, b4 = 1.0 / 6.0;
+    f(x, w.k1.data(), t);
+    for (size_t i = 0; i < n; ++i) w.tmp[i] = x[i] + (0.5 * dt) * w.k1[i];
+    f(w.tmp.data(), w.k2.data(), t + 0.5 * dt);
+    for (size_t i = 0; i < n; ++i) w.tmp[i] = x[i] + (0.5 * dt) * w.k2[i];
+    f(w.tmp.data(), w.k3.data(), t + 0.5 * dt);
+    for (size_t i = 0; i < n; ++i) w.tmp[i] = x[i] + (1.0 * dt) * w.k3[i];
+    f(w.tmp.data(), w.k4.data(), t + 1.0 * dt);
+    for (size_t i = 0; i < n; ++i)
+        x[i] = x[i] + (b1 * dt) * w.k1[i] + (b2 * dt) * w.k2[i] + (b3 * dt) * w.k3[i] +
+               (b4 * dt) * w.k4[i];
+}
+
+// boost::numeric::odeint::detail::less_eq_with_sign for dt > 0.
+bool less_eq_with_sign(double t1, double t2) {
+    return t1 - t2 <= std::numeric_limits<double>::epsilon();
+}
+
+// integrate_const (stepper_tag): observe(t_k) then step while t_k + dt <= end.
+template <class Step, class Obs>
+size_t integrate_const(Step&& step, double t0, double t1, double dt, Obs&& obs) {
+    double t = t0;
+    size_t k = 0;
+    while (less_eq_with_sign(t + dt, t1)) {
+        obs(t);
+        step(t, dt);
+        ++k;
+        t = t0 + double(k) * dt;
+    }
+    obs(t);
+    return k;
+}
+
+void mags(const OracleLattice& L, const double* x, double* out) {
+    // out = [M_antiferro | M_local | M_global], each D long, all divided by N.
+    const int D = L.D;
+    double Ml[16] = {0}, Ma[16] = {0}, Mg[16] = {0};
+    for (size_t i = 0; i < L.N; ++i)  // compute_magnetizations_from_flat (lattice.h:4962-4975)
+        for (int d = 0; d < D; ++d) Ml[d] += x[i * D + d];
+    for (size_t i = 0; i < L.N; ++i) {  // compute_magnetization_global_from_flat (:4461-4485)
+        const size_t atom = i % L.n_atoms;
+        const double* R = &L.frames[atom * D * D];
+        for (int mu = 0; mu < D; ++mu)
+            for (int nu = 0; nu < D; ++nu) Mg[mu] += R[mu * D + nu] * x[i * D + nu];
+    }
+    for (size_t i = 0; i < L.N; ++i) {  // compute_magnetization_staggered_from_flat (:4492-4512)
+        const size_t atom = i % L.n_atoms;
+        const double* R = &L.frames[atom * D * D];
+        const double s = L.afm[atom];
+        for (int mu = 0; mu < D; ++mu)
+            for (int nu = 0; nu < D; ++nu) Ma[mu] += s * R[mu * D + nu] * x[i * D + nu];
+    }
+    const double n = double(L.N);
+    for (int d = 0; d < D; ++d) {
+        out[d] = Ma[d] / n;
+        out[D + d] = Ml[d] / n;
+        out[2 * D + d] = Mg[d] / n;
+    }
+}
+
+struct Recorder {
+    size_t max_rec, n = 0;
+    double* times;
+    double* mag;
+    int MD3;
+    void push(double t, const double* m) {
+        if (n < max_rec) {
+            if (times) times[n] = t;
+            if (mag) std::memcpy(mag + n * MD3, m, sizeof(double) * MD3);
+        }
+        ++n;
+    }
+};
+
+// Segment list used by the pulse drivers (pulse_chunking.h:82-161).
+struct Seg {
+    double t0, t1, dt;
+};
+
+std::vector<Seg> pulse_segments(double T_start, double T_end, std::vector<double> centers,
+                                double window, double T_step, double free_dt_factor) {
+    std::vector<Seg> segs;
+    if (T_end <= T_start || T_step <= 0.0) return segs;
+    auto snap = [&](double t) { return T_start + std::round((t - T_start) / T_step) * T_step; };
+    const double T_end_s = T_start + std::floor((T_end - T_start) / T_step + 1e-9) * T_step;
+    if (T_end_s <= T_start) return segs;
+    std::vector<std::pair<double, double>> win;
+    for (double tc : centers) {
+        const double w0 = std::max(T_start, snap(tc - window));
+        const double w1 = std::min(T_end_s, snap(tc + window));
+        if (w0 < w1) win.emplace_back(w0, w1);
+    }
+    std::sort(win.begin(), win.end());
+    std::vector<std::pair<double, double>> merged;
+    for (auto& w : win) {
+        if (!merged.empty() && w.first <= merged.back().second)
+            merged.back().second = std::max(merged.back().second, w.second);
+        else
+            merged.push_back(w);
+    }
+    const double free_dt = std::max(T_step, free_dt_factor * T_step);
+    double cur = T_start;
+    for (auto& w : merged) {
+        if (cur < w.first) segs.push_back({cur, w.first, std::min(free_dt, w.first - cur)});
+        segs.push_back({w.first, w.second, T_step});
+        cur = w.second;
+    }
+    if (cur < T_end_s) segs.push_back({cur, T_end_s, std::min(free_dt, T_end_s - cur)});
+    return segs;
+}
+
+// Shared body of single_pulse_drive / double_pulse_drive (CPU branch).
+size_t pulse_drive(OracleLattice& L, const double* init, std::vector<double> centers,
+                   double T0, double T1, double step, int chunking, size_t max_rec,
+                   double* times, double* out_mags) {
+    const size_t n = L.N * L.D;
+    std::vector<double> x(init, init + n);
+    RK4Work w(n);
+    Recorder rec{max_rec, 0, times, out_mags, 3 * L.D};
+    double last = T0;
+    std::vector<double> m(3 * L.D);
+    auto obs = [&](double t) {  // lattice_md.cpp:532-554
+        if (t - last >= step - 1e-10 || t >= T1 - 1e-10) {
+            mags(L, x.data(), m.data());
+            rec.push(t, m.data());
+            last = t;
+        }
+    };
+    auto f = [&](const double* a, double* b, double t) { rhs(L, a, b, t); };
+    auto stp = [&](double t, double dt) { rk4_step_generic(f, x.data(), n, t, dt, w); };
+    if (chunking) {
+        // kPulseWindowSigmas = 9, kFreeDtFactor = 20 (pulse_chunking.h:173,178)
+        auto segs = pulse_segments(T0, T1, centers, 9.0 * L.width, step, 20.0);
+        for (auto& s : segs) integrate_const(stp, s.t0, s.t1, step, obs);
+    } else {
+        integrate_const(stp, T0, T1, step, obs);
+    }
+    // Reset pulse (lattice_md.cpp:578-583)
+    std::fill(L.field_drive[0].begin(), L.field_drive[0].end(), 0.0);
+    std::fill(L.field_drive[1].begin(), L.field_drive[1].end(), 0.0);
+    L.amp = 0.0;
+    return rec.n;
+}
+
+void set_pulse(OracleLattice& L, const double* f1, double t1, const double* f2, double t2,
+               double amp, double width, double freq) {
+    const int D = L.D;
+    for (size_t a = 0; a < L.n_atoms; ++a) {
+        const double* R = &L.frames[a * D * D];
+        for (int mu = 0; mu < D; ++mu) {
+            double s1 = 0.0, s2 = 0.0;
+            for (int nu = 0; nu < D; ++nu) {
+                s1 += R[mu * D + nu] * (f1 ? f1[a * D + nu] : 0.0);
+                s2 += R[mu * D + nu] * (f2 ? f2[a * D + nu] : 0.0);
+            }
+            L.field_drive[0][a * D + mu] = s1;
+            L.field_drive[1][a * D + mu] = s2;
+        }
+    }
+    L.t_pulse[0] = t1;
+    L.t_pulse[1] = t2;
+    L.amp = amp;
+    L.width = width;
+    L.freq = freq;
+}
+
+}  // namespace
+
+extern "C" {
+
+OracleLattice* orc_lattice_new(int spin_dim, int n_atoms, int d1, int d2, int d3, int n_bonds,
+                               const int* bond_src, const int* bond_partner, const int* bond_offset,
+                               const double* bond_J, const double* field_atom,
+                               const double* onsite_atom, const double* frames,
+                               const double* afm_signs) {
+    // a single periodic wrap (lattice.h:750-756) is only valid for offsets in [-L, L]
+    for (int b = 0; b < n_bonds; ++b)
+        for (int a = 0; a < 3; ++a) {
+            const int L = a == 0 ? d1 : (a == 1 ? d2 : d3);
+            if (bond_offset[3 * b + a] < -L || bond_offset[3 * b + a] > L) return nullptr;
+        }
+    auto* L = new OracleLattice();
+    L->D = spin_dim;
+    L->n_atoms = n_atoms;
+    L->d1 = d1; L->d2 = d2; L->d3 = d3;
+    L->N = size_t(n_atoms) * d1 * d2 * d3;
+    const int D = spin_dim;
+    const size_t D2 = size_t(D) * D;
+    L->field.assign(L->N * D, 0.0);
+    L->onsite.assign(L->N * D2, 0.0);
+    L->bonds.assign(L->N, {});
+    // Third pass of the Lattice constructor (lattice.h:744-807): sites in
+    // flatten order, unit-cell bonds of the atom in insertion order; forward
+    // J at the source, J^T at the partner.
+    for (int i = 0; i < d1; ++i)
+        for (int j = 0; j < d2; ++j)
+            for (int k = 0; k < d3; ++k)
+                for (int atom = 0; atom < n_atoms; ++atom) {
+                    const size_t s = flatten(*L, i, j, k, atom);
+                    for (int d = 0; d < D; ++d) L->field[s * D + d] = field_atom[atom * D + d];
+                    for (size_t q = 0; q < D2; ++q) L->onsite[s * D2 + q] = onsite_atom[atom * D2 + q];
+                    for (int b = 0; b < n_bonds; ++b) {
+                        if (bond_src[b] != atom) continue;
+                        const int pi = wrap(i + bond_offset[3 * b + 0], d1);
+                        const int pj = wrap(j + bond_offset[3 * b + 1], d2);
+                        const int pk = wrap(k + bond_offset[3 * b + 2], d3);
+                        const size_t p = flatten(*L, pi, pj, pk, bond_partner[b]);
+                        SiteBond fw{p, std::vector<double>(bond_J + b * D2, bond_J + (b + 1) * D2), b, false};
+                        SiteBond rv{s, std::vector<double>(D2), b, true};
+                        for (int r = 0; r < D; ++r)
+                            for (int c = 0; c < D; ++c) rv.J[r * D + c] = bond_J[b * D2 + c * D + r];
+                        L->bonds[s].push_back(std::move(fw));
+                        L->bonds[p].push_back(std::move(rv));
+                    }
+                }
+    build_flat(*L);
+    L->frames.assign(frames, frames + n_atoms * D2);
+    L->afm.assign(afm_signs, afm_signs + n_atoms);
+    L->field_drive[0].assign(n_atoms * D, 0.0);
+    L->field_drive[1].assign(n_atoms * D, 0.0);
+    return L;
+}
+
+OracleLattice* orc_lattice_from_csr(int spin_dim, int n_atoms, int d1, int d2, int d3,
+                                    const size_t* offset, const size_t* partner, const double* J,
+                                    const double* field_site, const double* onsite_site,
+                                    const double* frames, const double* afm_signs) {
+    auto* L = new OracleLattice();
+    L->D = spin_dim;
+    L->n_atoms = n_atoms;
+    L->d1 = d1; L->d2 = d2; L->d3 = d3;
+    L->N = size_t(n_atoms) * d1 * d2 * d3;
+    const size_t D2 = size_t(spin_dim) * spin_dim;
+    L->field.assign(field_site, field_site + L->N * spin_dim);
+    L->onsite.assign(onsite_site, onsite_site + L->N * D2);
+    L->bonds.assign(L->N, {});
+    for (size_t i = 0; i < L->N; ++i)
+        for (size_t k = offset[i]; k < offset[i + 1]; ++k)
+            L->bonds[i].push_back({partner[k], std::vector<double>(J + k * D2, J + (k + 1) * D2), -1, false});
+    build_flat(*L);
+    L->frames.assign(frames, frames + n_atoms * D2);
+    L->afm.assign(afm_signs, afm_signs + n_atoms);
+    L->field_drive[0].assign(n_atoms * spin_dim, 0.0);
+    L->field_drive[1].assign(n_atoms * spin_dim, 0.0);
+    return L;
+}
+
+void orc_lattice_free(OracleLattice* L) { delete L; }
+size_t orc_lattice_size(const OracleLattice* L) { return L->N; }
+size_t orc_total_bonds(const OracleLattice* L) { return L->bi_offset[L->N]; }
+
+void orc_flat_tables(const OracleLattice* L, size_t* offset, size_t* partner, double* J) {
+    std::copy(L->bi_offset.begin(), L->bi_offset.end(), offset);
+    std::copy(L->bi_partner.begin(), L->bi_partner.end(), partner);
+    std::copy(L->bi_J.begin(), L->bi_J.end(), J);
+}
+
+void orc_set_alpha(OracleLattice* L, double alpha) { L->alpha = alpha; }
+
+void orc_set_pulse(OracleLattice* L, const double* f1, double t1, const double* f2, double t2,
+                   double amp, double width, double freq) {
+    set_pulse(*L, f1, t1, f2, t2, amp, width, freq);
+}
+
+void orc_clear_pulse(OracleLattice* L) {
+    std::fill(L->field_drive[0].begin(), L->field_drive[0].end(), 0.0);
+    std::fill(L->field_drive[1].begin(), L->field_drive[1].end(), 0.0);
+    L->amp = 0.0;
+}
+
+void orc_local_field(const OracleLattice* L, const double* state, size_t site, double* H) {
+    local_field(*L, state, site, H);
+}
+
+void orc_rhs(const OracleLattice* L, const double* state, double* dsdt, double t) {
+    rhs(*L, state, dsdt, t);
+}
+
+double orc_energy(const OracleLattice* L, const double* x) {  // lattice.h:1485-1529
+    const int D = L->D;
+    double E = 0.0;
+    for (size_t i = 0; i < L->N; ++i) {
+        const double* Si = x + i * D;
+        for (int d = 0; d < D; ++d) E -= L->field[i * D + d] * Si[d];
+        const double* A = &L->onsite[i * D * D];
+        for (int d = 0; d < D; ++d)
+            for (int e = 0; e < D; ++e) E += Si[d] * A[d * D + e] * Si[e];
+        for (const auto& b : L->bonds[i]) {
+            if (b.partner > i) {
+                const double* Sj = x + b.partner * D;
+                for (int d = 0; d < D; ++d)
+                    for (int e = 0; e < D; ++e) E += Si[d] * b.J[d * D + e] * Sj[e];
+            }
+        }
+    }
+    return E;
+}
+
+void orc_mags(const OracleLattice* L, const double* state, double* out) { mags(*L, state, out); }
+
+void orc_rk4_step(const OracleLattice* L, double* state, double t, double dt) {
+    const size_t n = L->N * L->D;
+    RK4Work w(n);
+    auto f = [&](const double* a, double* b, double tt) { rhs(*L, a, b, tt); };
+    rk4_step_generic(f, state, n, t, dt, w);
+}
+
+size_t orc_boost_num_steps(double t0, double t1, double dt) {
+    size_t k = 0;
+    double t = t0;
+    while (less_eq_with_sign(t + dt, t1)) {
+        ++k;
+        t = t0 + double(k) * dt;
+    }
+    return k;
+}
+
+size_t orc_integrate_md(const OracleLattice* L, double* state, double T0, double T1, double dt,
+                        size_t save_interval, size_t max_rec, double* times, double* out_mags,
+                        double* energy) {
+    const size_t n = L->N * L->D;
+    RK4Work w(n);
+    size_t step_count = 0, nrec = 0;
+    std::vector<double> m(3 * L->D);
+    auto obs = [&](double t) {  // lattice_md.cpp:383-436 (renorm off, HDF5 omitted)
+        if (step_count % save_interval == 0) {
+            if (nrec < max_rec) {
+                mags(*L, state, m.data());
+                if (times) times[nrec] = t;
+                if (out_mags) std::memcpy(out_mags + nrec * 3 * L->D, m.data(), sizeof(double) * 3 * L->D);
+                if (energy) energy[nrec] = orc_energy(L, state) / double(L->N);
+            }
+            ++nrec;
+        }
+        ++step_count;
+    };
+    auto f = [&](const double* a, double* b, double tt) { rhs(*L, a, b, tt); };
+    auto stp = [&](double t, double h) { rk4_step_generic(f, state, n, t, h, w); };
+    integrate_const(stp, T0, T1, dt, obs);
+    return nrec;
+}
+
+size_t orc_single_pulse_drive(OracleLattice* L, const double* init, const double* field_in,
+                              double t_B, double amp, double width, double freq, double T0,
+                              double T1, double step, int chunking, size_t max_rec,
+                              double* times, double* out_mags) {
+    set_pulse(*L, field_in, t_B, nullptr, 0.0, amp, width, freq);  // lattice_md.cpp:512-513
+    return pulse_drive(*L, init, {t_B}, T0, T1, step, chunking, max_rec, times, out_mags);
+}
+
+size_t orc_double_pulse_drive(OracleLattice* L, const double* init, const double* f1, double tB1,
+                              const double* f2, double tB2, double amp, double width, double freq,
+                              double T0, double T1, double step, int chunking, size_t max_rec,
+                              double* times, double* out_mags) {
+    set_pulse(*L, f1, tB1, f2, tB2, amp, width, freq);  // lattice_md.cpp:611-612
+    return pulse_drive(*L, init, {tB1, tB2}, T0, T1, step, chunking, max_rec, times, out_mags);
+}
+
+double orc_max_dsdt_no_drive(const OracleLattice* L, const double* state) {
+    OracleLattice tmp = *L;
+    tmp.amp = 0.0;
+    std::vector<double> d(L->N * L->D);
+    rhs(tmp, state, d.data(), 0.0);
+    double mx = 0.0;
+    for (double v : d) mx = std::max(mx, std::abs(v));
+    return mx;
+}
+
+int orc_pump_probe(const OracleLattice* Lc, const double* ground, const double* field_in,
+                   double amp, double width, double freq, double tau_start, double tau_end,
+                   double tau_step, double T0, double T1, double T_step, int reuse_m0,
+                   double stat_tol, int n_outer, int chunking, size_t max_rec, size_t max_tau,
+                   size_t* n_rec, double* times, double* M0, double* M1, double* M01,
+                   double* tau_values, int* used_w1) {
+    OracleLattice L = *Lc;
+    const int MD3 = 3 * L.D;
+    // M0 = single pulse at t = 0 (lattice_md.cpp:730)
+    const size_t nr = orc_single_pulse_drive(&L, ground, field_in, 0.0, amp, width, freq, T0, T1,
+                                             T_step, chunking, max_rec, times, M0);
+    *n_rec = nr;
+    // M_ground_arr (lattice_md.cpp:737-742): staggered uses alternating signs
+    std::vector<double> mg(MD3);
+    {
+        const int D = L.D;
+        for (int d = 0; d < MD3; ++d) mg[d] = 0.0;
+        for (size_t i = 0; i < L.N; ++i) {
+            const double sgn = std::pow(-1.0, double(i));  // magnetization_local_antiferro
+            for (int d = 0; d < D; ++d) {
+                mg[d] += ground[i * D + d] * sgn;
+                mg[D + d] += ground[i * D + d];
+            }
+            const size_t atom = i % L.n_atoms;
+            const double* R = &L.frames[atom * D * D];
+            for (int mu = 0; mu < D; ++mu)
+                for (int nu = 0; nu < D; ++nu) mg[2 * D + mu] += R[mu * D + nu] * ground[i * D + nu];
+        }
+        for (int d = 0; d < MD3; ++d) mg[d] /= double(L.N);
+    }
+    bool can_reuse = false;
+    if (reuse_m0) can_reuse = orc_max_dsdt_no_drive(&L, ground) <= stat_tol;
+    if (used_w1) *used_w1 = can_reuse ? 1 : 0;
+    const int tau_steps = int(std::abs((tau_end - tau_start) / tau_step)) + 1;  // :768
+    if (size_t(tau_steps) > max_tau) return -tau_steps;
+
+    auto synth_M1 = [&](double tau, double* out) {  // synthesize_M1_from_M0 (:283-317)
+        if (nr == 0) return;
+        const double Ts = times[0];
+        for (size_t q = 0; q < nr && q < max_rec; ++q) {
+            const double ti = times[q];
+            const double* src;
+            if (ti < tau + Ts) {
+                src = mg.data();
+            } else {
+                long idx = std::lround((ti - tau - Ts) / T_step);
+                if (idx < 0) idx = 0;
+                if (idx >= long(nr)) idx = long(nr) - 1;
+                src = M0 + size_t(idx) * MD3;
+            }
+            std::memcpy(out + q * MD3, src, sizeof(double) * MD3);
+        }
+    };
+
+    if (n_outer <= 1) {
+        double cur = tau_start;  // serial path accumulates tau (:825-853)
+        for (int i = 0; i < tau_steps; ++i) {
+            tau_values[i] = cur;
+            double* m1 = M1 + size_t(i) * max_rec * MD3;
+            double* m01 = M01 + size_t(i) * max_rec * MD3;
+            std::vector<double> tt(max_rec);
+            if (can_reuse) synth_M1(cur, m1);
+            else orc_single_pulse_drive(&L, ground, field_in, cur, amp, width, freq, T0, T1, T_step,
+                                        chunking, max_rec, tt.data(), m1);
+            orc_double_pulse_drive(&L, ground, field_in, 0.0, field_in, cur, amp, width, freq, T0,
+                                   T1, T_step, chunking, max_rec, tt.data(), m01);
+            cur += tau_step;
+        }
+    } else {
+#ifdef _OPENMP
+        const int saved = omp_get_max_active_levels();
+        omp_set_max_active_levels(1);
+#pragma omp parallel num_threads(n_outer)
+#endif
+        {
+            OracleLattice local = L;  // per-thread deep clone (:863-868)
+            std::vector<double> tt(max_rec);
+#ifdef _OPENMP
+#pragma omp for schedule(dynamic, 1)
+#endif
+            for (int i = 0; i < tau_steps; ++i) {
+                const double tau = tau_start + i * tau_step;
+                tau_values[i] = tau;
+                double* m1 = M1 + size_t(i) * max_rec * MD3;
+                double* m01 = M01 + size_t(i) * max_rec * MD3;
+                if (can_reuse) synth_M1(tau, m1);
+                else orc_single_pulse_drive(&local, ground, field_in, tau, amp, width, freq, T0, T1,
+                                            T_step, chunking, max_rec, tt.data(), m1);
+                orc_double_pulse_drive(&local, ground, field_in, 0.0, field_in, tau, amp, width,
+                                       freq, T0, T1, T_step, chunking, max_rec, tt.data(), m01);
+            }
+        }
+#ifdef _OPENMP
+        omp_set_max_active_levels(saved);
+#endif
+    }
+    return tau_steps;
+}
+
+size_t orc_build_pulse_segments(double T0, double T1, const double* centers, size_t n_centers,
+                                double window, double T_step, double free_dt_factor,
+                                size_t max_seg, double* out) {
+    auto segs = pulse_segments(T0, T1, std::vector<double>(centers, centers + n_centers), window,
+                               T_step, free_dt_factor);
+    for (size_t i = 0; i < segs.size() && i < max_seg; ++i) {
+        out[3 * i + 0] = segs[i].t0;
+        out[3 * i + 1] = segs[i].t1;
+        out[3 * i + 2] = segs[i].dt;
+    }
+    return segs.size();
+}
+
+int orc_max_threads(void) {
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
+}
+
+/* ---------------- tangent-linear model (forward mode) ---------------- */
+
+size_t orc_tangent_md(OracleLattice* L, double* state, double* dstate, const double* dJ,
+                      const double* dB, double T0, double T1, double dt, size_t max_rec,
+                      double* out_mags, double* out_dmags) {
+    // Augmented system z = (x, dx): dz/dt = (f(x,p), f_x dx + f_p dp).  RK4 on
+    // z is the exact derivative of the discrete RK4 map (linear stage sums).
+    const int D = L->D;
+    const size_t n = L->N * D;
+    const size_t D2 = size_t(D) * D;
+    if (D != 3) return 0;
+    auto F = [&](const double* z, double* dz, double t) {
+        const double* x = z;
+        const double* dx = z + n;
+        rhs(*L, x, dz, t);
+        const bool drive = (L->amp != 0.0);
+        double f1 = 0.0, f2 = 0.0;
+        if (drive) drive_factors(*L, t, f1, f2);
+        for (size_t i = 0; i < L->N; ++i) {
+            double H[3], dH[3];
+            local_field(*L, x, i, H);
+            const size_t atom = i % L->n_atoms;
+            if (drive)
+                for (int d = 0; d < 3; ++d)
+                    H[d] -= L->field_drive[0][atom * 3 + d] * f1 + L->field_drive[1][atom * 3 + d] * f2;
+            for (int d = 0; d < 3; ++d) dH[d] = dB ? -dB[atom * 3 + d] : 0.0;
+            const double* A = &L->onsite[i * D2];
+            for (int d = 0; d < 3; ++d)
+                for (int e = 0; e < 3; ++e) dH[d] += 2.0 * A[d * 3 + e] * dx[i * 3 + e];
+            for (const auto& b : L->bonds[i]) {
+                const double* Sp = x + b.partner * 3;
+                const double* dSp = dx + b.partner * 3;
+                for (int d = 0; d < 3; ++d)
+                    for (int e = 0; e < 3; ++e) {
+                        dH[d] += b.J[d * 3 + e] * dSp[e];
+                        if (dJ) {
+                            const double* g = dJ + size_t(b.ucb) * 9;
+                            const double dj = b.transposed ? g[e * 3 + d] : g[d * 3 + e];
+                            dH[d] += dj * Sp[e];
+                        }
+                    }
+            }
+            const double* S = x + i * 3;
+            const double* dS = dx + i * 3;
+            double o[3];
+            // d(H x S) = dH x S + H x dS
+            o[0] = dH[1] * S[2] - dH[2] * S[1] + H[1] * dS[2] - H[2] * dS[1];
+            o[1] = dH[2] * S[0] - dH[0] * S[2] + H[2] * dS[0] - H[0] * dS[2];
+            o[2] = dH[0] * S[1] - dH[1] * S[0] + H[0] * dS[1] - H[1] * dS[0];
+            if (L->alpha > 0.0) {
+                // term = (alpha/|S|) (S (S.H) - H |S|^2)
+                const double S2 = S[0] * S[0] + S[1] * S[1] + S[2] * S[2];
+                const double nS = std::sqrt(S2);
+                const double SH = S[0] * H[0] + S[1] * H[1] + S[2] * H[2];
+                const double SdS = S[0] * dS[0] + S[1] * dS[1] + S[2] * dS[2];
+                const double dSH = dS[0] * H[0] + dS[1] * H[1] + dS[2] * H[2] +
+                                   S[0] * dH[0] + S[1] * dH[1] + S[2] * dH[2];
+                const double g = L->alpha / nS;
+                const double dg = -L->alpha * SdS / (nS * S2);
+                for (int d = 0; d < 3; ++d) {
+                    const double u = S[d] * SH - H[d] * S2;
+                    const double du = dS[d] * SH + S[d] * dSH - dH[d] * S2 - H[d] * 2.0 * SdS;
+                    o[d] += dg * u + g * du;
+                }
+            }
+            for (int d = 0; d < 3; ++d) dz[n + i * 3 + d] = o[d];
+        }
+    };
+    std::vector<double> z(2 * n);
+    std::copy(state, state + n, z.begin());
+    std::copy(dstate, dstate + n, z.begin() + n);
+    RK4Work w(2 * n);
+    size_t nrec = 0;
+    auto obs = [&](double) {
+        if (nrec < max_rec) {
+            if (out_mags) mags(*L, z.data(), out_mags + nrec * 9);
+            if (out_dmags) mags(*L, z.data() + n, out_dmags + nrec * 9);
+        }
+        ++nrec;
+    };
+    auto stp = [&](double t, double h) { rk4_step_generic(F, z.data(), 2 * n, t, h, w); };
+    integrate_const(stp, T0, T1, dt, obs);
+    std::copy(z.begin(), z.begin() + n, state);
+    std::copy(z.begin() + n, z.end(), dstate);
+    return nrec;
+}
+
+}  // extern "C"

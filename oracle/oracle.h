@@ -1,0 +1,125 @@
+/*
+ * oracle.h — CPU FP64 restatement of the reference's Landau–Lifshitz path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker that the HIP path is
+ * compared against (and the "port" CPU baseline timed beside it).  Nothing in
+ * the product library links it, loads it or calls it.
+ *
+ * Every function restates a reference function; the citation (path:line,
+ * relative to the reference repository root) is given next to each
+ * declaration.  Pinning of each function (which closed form / invariant /
+ * brute-force check covers it) is listed in the header of oracle.cpp.
+ *
+ * Conventions (all as in the reference):
+ *   - site index  ((i*d2 + j)*d3 + k)*n_atoms + atom        lattice.h:1057-1059
+ *   - state is AoS FP64 [N*D]
+ *   - H_i = -B_i + 2 A_i S_i + sum_n J_n S_p(n) - drive(t)   lattice.h:1653-1762,
+ *                                                            lattice_md.cpp:178-191
+ *   - dS/dt = H x S (SU(2)); d_a = f_abc H_b S_c (SU(3))     lattice_md.cpp:39-153
+ *   - magnetisation records are [M_antiferro, M_local, M_global] (3*D doubles)
+ */
+#ifndef CSL_ORACLE_H
+#define CSL_ORACLE_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct OracleLattice OracleLattice;
+
+/* Lattice(uc, d1, d2, d3) bond expansion + build_flat_bilinear_tables
+ * (lattice.h:601-855, :868-903).  Bonds are given in unit-cell insertion
+ * order (UnitCell::set_bilinear_interaction, unitcell.h:166-172). */
+OracleLattice* orc_lattice_new(int spin_dim, int n_atoms, int d1, int d2, int d3,
+                               int n_bonds, const int* bond_src, const int* bond_partner,
+                               const int* bond_offset, const double* bond_J,
+                               const double* field_atom, const double* onsite_atom,
+                               const double* frames, const double* afm_signs);
+/* Lattice from explicit reference tables (the CSR mirror plus per-site field
+ * and onsite matrices), for Hamiltonians that are not translation invariant.
+ * dims = (d1, d2, d3); N = n_atoms*d1*d2*d3. */
+OracleLattice* orc_lattice_from_csr(int spin_dim, int n_atoms, int d1, int d2, int d3,
+                                    const size_t* offset, const size_t* partner,
+                                    const double* J, const double* field_site,
+                                    const double* onsite_site, const double* frames,
+                                    const double* afm_signs);
+void   orc_lattice_free(OracleLattice* L);
+size_t orc_lattice_size(const OracleLattice* L);
+size_t orc_total_bonds(const OracleLattice* L);
+/* CSR mirror bi_flat_offset / bi_flat_partner / bi_flat_J (lattice.h:868-903). */
+void   orc_flat_tables(const OracleLattice* L, size_t* offset, size_t* partner, double* J);
+/* Gilbert damping alpha (lattice.h:563). */
+void   orc_set_alpha(OracleLattice* L, double alpha);
+/* Lattice::set_pulse: field_drive[p] = R_atom * field_in_p (lattice_md.cpp:210-224). */
+void   orc_set_pulse(OracleLattice* L, const double* field_in1, double t1,
+                     const double* field_in2, double t2,
+                     double amp, double width, double freq);
+/* Drive reset done at the end of *_pulse_drive (lattice_md.cpp:578-583). */
+void   orc_clear_pulse(OracleLattice* L);
+/* get_local_field_flat (lattice.h:1653-1762), no drive. */
+void   orc_local_field(const OracleLattice* L, const double* state, size_t site, double* H);
+/* landau_lifshitz_flat (lattice_md.cpp:39-153). */
+void   orc_rhs(const OracleLattice* L, const double* state, double* dsdt, double t);
+/* total_energy_flat (lattice.h:1485-1613). */
+double orc_energy(const OracleLattice* L, const double* state);
+/* [M_antiferro(staggered), M_local, M_global] as the pulse-drive observer
+ * assembles them (lattice_md.cpp:533-554, lattice.h:4461-4512). */
+void   orc_mags(const OracleLattice* L, const double* state, double* out);
+/* One Boost.Odeint runge_kutta4 do_step (explicit_generic_rk, 4 stages). */
+void   orc_rk4_step(const OracleLattice* L, double* state, double t, double dt);
+/* Number of integrate_const steps and the k-th grid time (Boost semantics:
+ * t_k = t0 + k*dt while t_k + dt - t1 <= eps). */
+size_t orc_boost_num_steps(double t0, double t1, double dt);
+/* molecular_dynamics_cpu with method "rk4" (lattice_md.cpp:341-453):
+ * observer every save_interval integrate_const calls, records t, mags, E/N. */
+size_t orc_integrate_md(const OracleLattice* L, double* state, double T0, double T1,
+                        double dt, size_t save_interval, size_t max_rec,
+                        double* times, double* mags, double* energy_per_site);
+/* single_pulse_drive / double_pulse_drive, CPU branch, method "rk4"
+ * (lattice_md.cpp:493-586, :589-680).  Returns the number of records. */
+size_t orc_single_pulse_drive(OracleLattice* L, const double* init, const double* field_in,
+                              double t_B, double amp, double width, double freq,
+                              double T0, double T1, double step, int chunking,
+                              size_t max_rec, double* times, double* mags);
+size_t orc_double_pulse_drive(OracleLattice* L, const double* init,
+                              const double* field_in1, double t_B1,
+                              const double* field_in2, double t_B2,
+                              double amp, double width, double freq,
+                              double T0, double T1, double step, int chunking,
+                              size_t max_rec, double* times, double* mags);
+/* max_dSdt_norm_no_drive (lattice_md.cpp:236-280). */
+double orc_max_dsdt_no_drive(const OracleLattice* L, const double* state);
+/* pump_probe_spectroscopy (lattice_md.cpp:683-926) minus HDF5: M0, M1(tau),
+ * M01(tau) for every tau; outer OpenMP over tau when n_outer > 1 (W2).
+ * Output blocks are [n_tau][max_rec][3*D].  Returns the number of tau points. */
+int    orc_pump_probe(const OracleLattice* L, const double* ground, const double* field_in,
+                      double amp, double width, double freq,
+                      double tau_start, double tau_end, double tau_step,
+                      double T0, double T1, double T_step,
+                      int reuse_m0, double stationarity_tol, int n_outer, int chunking,
+                      size_t max_rec, size_t max_tau, size_t* n_rec,
+                      double* times, double* M0, double* M1, double* M01,
+                      double* tau_values, int* used_w1);
+/* build_pulse_segments (pulse_chunking.h:82-161): out = [t0, t1, dt_init]*n. */
+size_t orc_build_pulse_segments(double T0, double T1, const double* centers, size_t n_centers,
+                                double window, double T_step, double free_dt_factor,
+                                size_t max_seg, double* out);
+/* OpenMP thread count actually available to the oracle. */
+int    orc_max_threads(void);
+
+/* Tangent-linear (forward-mode) RK4: propagates (state, dstate) together for
+ * a directional derivative of the flow w.r.t. the initial state and a
+ * parameter perturbation of the bond matrices dJ (per unit-cell bond,
+ * [n_bonds*D*D]) and of the per-atom field dB ([n_atoms*D]).  Used only to
+ * check adjoint gradients. */
+size_t orc_tangent_md(OracleLattice* L, double* state, double* dstate,
+                      const double* dJ, const double* dB,
+                      double T0, double T1, double dt, size_t max_rec,
+                      double* mags, double* dmags);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

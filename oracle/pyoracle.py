@@ -1,0 +1,236 @@
+"""ctypes wrapper of oracle/liboracle.so — TEST INFRASTRUCTURE ONLY.
+
+Used by tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg as the
+checker / CPU baseline.  The product package never imports this module.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "liboracle.so")
+
+_pd = C.POINTER(C.c_double)
+_pi = C.POINTER(C.c_int)
+_psz = C.POINTER(C.c_size_t)
+_sz, _d, _i, _vp = C.c_size_t, C.c_double, C.c_int, C.c_void_p
+
+_SIG = {
+    "orc_lattice_new": (_vp, [_i, _i, _i, _i, _i, _i, _pi, _pi, _pi, _pd, _pd, _pd, _pd, _pd]),
+    "orc_lattice_from_csr": (_vp, [_i, _i, _i, _i, _i, _psz, _psz, _pd, _pd, _pd, _pd, _pd]),
+    "orc_lattice_free": (None, [_vp]),
+    "orc_lattice_size": (_sz, [_vp]),
+    "orc_total_bonds": (_sz, [_vp]),
+    "orc_flat_tables": (None, [_vp, _psz, _psz, _pd]),
+    "orc_set_alpha": (None, [_vp, _d]),
+    "orc_set_pulse": (None, [_vp, _pd, _d, _pd, _d, _d, _d, _d]),
+    "orc_clear_pulse": (None, [_vp]),
+    "orc_local_field": (None, [_vp, _pd, _sz, _pd]),
+    "orc_rhs": (None, [_vp, _pd, _pd, _d]),
+    "orc_energy": (_d, [_vp, _pd]),
+    "orc_mags": (None, [_vp, _pd, _pd]),
+    "orc_rk4_step": (None, [_vp, _pd, _d, _d]),
+    "orc_boost_num_steps": (_sz, [_d, _d, _d]),
+    "orc_integrate_md": (_sz, [_vp, _pd, _d, _d, _d, _sz, _sz, _pd, _pd, _pd]),
+    "orc_single_pulse_drive": (_sz, [_vp, _pd, _pd, _d, _d, _d, _d, _d, _d, _d, _i, _sz, _pd, _pd]),
+    "orc_double_pulse_drive": (_sz, [_vp, _pd, _pd, _d, _pd, _d, _d, _d, _d, _d, _d, _d, _i, _sz, _pd, _pd]),
+    "orc_max_dsdt_no_drive": (_d, [_vp, _pd]),
+    "orc_pump_probe": (_i, [_vp, _pd, _pd, _d, _d, _d, _d, _d, _d, _d, _d, _d, _i, _d, _i, _i, _sz, _sz,
+                            _psz, _pd, _pd, _pd, _pd, _pd, _pi]),
+    "orc_build_pulse_segments": (_sz, [_d, _d, _pd, _sz, _d, _d, _d, _sz, _pd]),
+    "orc_max_threads": (_i, []),
+    "orc_tangent_md": (_sz, [_vp, _pd, _pd, _pd, _pd, _d, _d, _d, _sz, _pd, _pd]),
+}
+
+_lib = None
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            build()
+        L = C.CDLL(LIB)
+        for k, (r, a) in _SIG.items():
+            f = getattr(L, k)
+            f.restype = r
+            f.argtypes = a
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(_pd)
+
+
+def _f(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+class OracleLattice:
+    """Oracle lattice built from a model.UnitCell-like object (bonds in
+    insertion order) and dimensions."""
+
+    @classmethod
+    def from_csr(cls, D, n_atoms, dims, offset, partner, J, field_site, onsite_site, frames, afm):
+        self = cls.__new__(cls)
+        self.D, self.A = D, n_atoms
+        off = np.ascontiguousarray(offset, dtype=np.uintp)
+        par = np.ascontiguousarray(partner, dtype=np.uintp)
+        Jf, fs, os_, fr, af = (_f(J).reshape(-1), _f(field_site).reshape(-1), _f(onsite_site).reshape(-1),
+                               _f(frames).reshape(-1), _f(afm).reshape(-1))
+        self._keep = (off, par, Jf, fs, os_, fr, af)
+        self.h = lib().orc_lattice_from_csr(D, n_atoms, dims[0], dims[1], dims[2], off.ctypes.data_as(_psz),
+                                            par.ctypes.data_as(_psz), _p(Jf), _p(fs), _p(os_), _p(fr), _p(af))
+        self.N = lib().orc_lattice_size(self.h)
+        return self
+
+    def __init__(self, uc, d1, d2, d3):
+        self.D = uc.N
+        self.A = uc.N_atoms
+        nb = len(uc.bilinear)
+        src = np.array([b.source for b in uc.bilinear], dtype=np.int32)
+        par = np.array([b.partner for b in uc.bilinear], dtype=np.int32)
+        off = np.array([b.offset for b in uc.bilinear], dtype=np.int32).reshape(-1)
+        J = _f(np.array([b.J for b in uc.bilinear]).reshape(-1)) if nb else np.zeros(1)
+        self._keep = (src, par, off, J)
+        ip = lambda a: a.ctypes.data_as(_pi)
+        self.h = lib().orc_lattice_new(self.D, self.A, d1, d2, d3, nb, ip(src) if nb else None,
+                                       ip(par) if nb else None, ip(off) if nb else None, _p(J),
+                                       _p(_f(uc.field)), _p(_f(uc.onsite_interaction)),
+                                       _p(_f(uc.sublattice_frames)), _p(_f(uc.afm_sublattice_signs)))
+        if not self.h:
+            raise RuntimeError("oracle rejected the lattice (bond offset beyond the lattice size)")
+        self.N = lib().orc_lattice_size(self.h)
+
+    def __del__(self):
+        try:
+            lib().orc_lattice_free(self.h)
+        except Exception:
+            pass
+
+    def tables(self):
+        nb = lib().orc_total_bonds(self.h)
+        off = np.zeros(self.N + 1, dtype=np.uintp)
+        par = np.zeros(max(nb, 1), dtype=np.uintp)
+        J = np.zeros(max(nb, 1) * self.D * self.D)
+        lib().orc_flat_tables(self.h, off.ctypes.data_as(_psz), par.ctypes.data_as(_psz), _p(J))
+        return off, par[:nb], J[: nb * self.D * self.D].reshape(nb, self.D, self.D)
+
+    def set_alpha(self, a):
+        lib().orc_set_alpha(self.h, a)
+
+    def set_pulse(self, f1, t1, f2, t2, amp, width, freq):
+        lib().orc_set_pulse(self.h, _p(_f(f1)), t1, _p(None if f2 is None else _f(f2)), t2, amp, width, freq)
+
+    def clear_pulse(self):
+        lib().orc_clear_pulse(self.h)
+
+    def local_field(self, state, site):
+        H = np.zeros(self.D)
+        lib().orc_local_field(self.h, _p(_f(state)), site, _p(H))
+        return H
+
+    def rhs(self, state, t=0.0):
+        s = _f(state)
+        out = np.zeros_like(s)
+        lib().orc_rhs(self.h, _p(s), _p(out), t)
+        return out
+
+    def energy(self, state):
+        return lib().orc_energy(self.h, _p(_f(state)))
+
+    def mags(self, state):
+        out = np.zeros(3 * self.D)
+        lib().orc_mags(self.h, _p(_f(state)), _p(out))
+        return out.reshape(3, self.D)
+
+    def rk4_step(self, state, t, dt):
+        s = _f(state).copy()
+        lib().orc_rk4_step(self.h, _p(s), t, dt)
+        return s
+
+    def integrate_md(self, state, T0, T1, dt, save_interval=1):
+        n = lib().orc_boost_num_steps(T0, T1, dt)
+        mx = n // save_interval + 2
+        s = _f(state).copy()
+        t = np.zeros(mx)
+        m = np.zeros(mx * 3 * self.D)
+        e = np.zeros(mx)
+        nr = lib().orc_integrate_md(self.h, _p(s), T0, T1, dt, save_interval, mx, _p(t), _p(m), _p(e))
+        return {"times": t[:nr], "mags": m.reshape(mx, 3, self.D)[:nr], "energy": e[:nr], "final": s}
+
+    def single_pulse_drive(self, init, field_in, t_B, amp, width, freq, T0, T1, step, chunking=False):
+        n = lib().orc_boost_num_steps(T0, T1, step) + 4
+        t = np.zeros(n)
+        m = np.zeros(n * 3 * self.D)
+        nr = lib().orc_single_pulse_drive(self.h, _p(_f(init)), _p(_f(field_in)), t_B, amp, width, freq, T0, T1,
+                                          step, int(chunking), n, _p(t), _p(m))
+        return t[:nr], m.reshape(n, 3, self.D)[:nr]
+
+    def double_pulse_drive(self, init, f1, t1, f2, t2, amp, width, freq, T0, T1, step, chunking=False):
+        n = lib().orc_boost_num_steps(T0, T1, step) + 4
+        t = np.zeros(n)
+        m = np.zeros(n * 3 * self.D)
+        nr = lib().orc_double_pulse_drive(self.h, _p(_f(init)), _p(_f(f1)), t1, _p(_f(f2)), t2, amp, width, freq,
+                                          T0, T1, step, int(chunking), n, _p(t), _p(m))
+        return t[:nr], m.reshape(n, 3, self.D)[:nr]
+
+    def max_dsdt_no_drive(self, state):
+        return lib().orc_max_dsdt_no_drive(self.h, _p(_f(state)))
+
+    def pump_probe(self, ground, field_in, amp, width, freq, tau_start, tau_end, tau_step, T0, T1, T_step,
+                   reuse_m0=False, stat_tol=1e-6, n_outer=1, chunking=False):
+        mr = lib().orc_boost_num_steps(T0, T1, T_step) + 4
+        mt = int(abs((tau_end - tau_start) / tau_step)) + 2
+        D3 = 3 * self.D
+        times = np.zeros(mr)
+        M0 = np.zeros(mr * D3)
+        M1 = np.zeros(mt * mr * D3)
+        M01 = np.zeros(mt * mr * D3)
+        taus = np.zeros(mt)
+        nrec = C.c_size_t()
+        w1 = C.c_int()
+        nt = lib().orc_pump_probe(self.h, _p(_f(ground)), _p(_f(field_in)), amp, width, freq, tau_start, tau_end,
+                                  tau_step, T0, T1, T_step, int(reuse_m0), stat_tol, n_outer, int(chunking), mr,
+                                  mt, C.byref(nrec), _p(times), _p(M0), _p(M1), _p(M01), _p(taus),
+                                  C.byref(w1))
+        if nt < 0:
+            raise RuntimeError("tau grid larger than buffer")
+        r = nrec.value
+        return {"times": times[:r], "M0": M0.reshape(mr, 3, self.D)[:r],
+                "M1": M1.reshape(mt, mr, 3, self.D)[:nt, :r], "M01": M01.reshape(mt, mr, 3, self.D)[:nt, :r],
+                "tau": taus[:nt], "used_w1": bool(w1.value)}
+
+    def tangent_md(self, state, dstate, dJ, dB, T0, T1, dt):
+        n = lib().orc_boost_num_steps(T0, T1, dt) + 2
+        s = _f(state).copy()
+        ds = _f(dstate).copy()
+        m = np.zeros(n * 9)
+        dm = np.zeros(n * 9)
+        nr = lib().orc_tangent_md(self.h, _p(s), _p(ds), _p(None if dJ is None else _f(dJ)),
+                                  _p(None if dB is None else _f(dB)), T0, T1, dt, n, _p(m), _p(dm))
+        return m.reshape(n, 3, 3)[:nr], dm.reshape(n, 3, 3)[:nr]
+
+
+def boost_num_steps(T0, T1, dt):
+    return lib().orc_boost_num_steps(T0, T1, dt)
+
+
+def pulse_segments(T0, T1, centers, window, T_step, free_dt_factor):
+    c = _f(centers)
+    out = np.zeros(3 * 64)
+    n = lib().orc_build_pulse_segments(T0, T1, _p(c), c.size, window, T_step, free_dt_factor, 64, _p(out))
+    return out.reshape(64, 3)[:n]
+
+
+def max_threads():
+    return lib().orc_max_threads()
