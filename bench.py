@@ -1,0 +1,211 @@
+#!/usr/bin/env python3
+"""Benchmark of the batched FP64 2DCS workload on one MI355X.
+
+Workload (one "step" = one pass of the hot path over one batch):
+  Kitaev honeycomb 36x36x1 (2592 SU(2) spins), K=-1, Gamma=0.25, Gamma'=-0.02,
+  static h = 0.1 along [111] (unit direction), Gaussian pulses A=0.5, w=0.5
+  along global x, omega = 0; pulse 1 at t=0, delays tau = 0, 0.25, ..., 63.75
+  (256 points).  M0 + 256 M1(tau) + 256 M01(tau) = 513 trajectories of 8192
+  RK4 steps (dt = 0.01, t in [-10, 71.92]) from one seeded relaxed state tilted
+  by N(0, 0.01) noise and renormalised; M(t) recorded every step.
+
+Prints one JSON line (rank 0).  `value` = trajectories/s of the batched
+persistent kernel; the plain one-trajectory path, the oracle CPU baseline and
+the error against the oracle are reported beside it.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import platform
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "LL RK4 site-steps/s + 2DCS trajectories/s, batched vs plain, at fixed FP64 error"
+HBM_PEAK = 8.0e12          # MI355X HBM3E, B/s
+BYTES_PER_SITE_STEP = 192  # 4 RK stages x (read + write) x 3 doubles x 8 B
+FLOPS_PER_SITE_STEP = 306  # z = 3 SU(2) site, RK4
+
+L1 = L2 = 36
+DT = 0.01
+T0 = -10.0
+N_STEPS = 8192
+T1 = T0 + (N_STEPS + 0.5) * DT     # Boost grid: exactly N_STEPS steps
+TAU0, TAU_STEP, N_TAU = 0.0, 0.25, 256
+AMP, WIDTH, FREQ = 0.5, 0.5, 0.0
+
+
+def build_workload():
+    from classical_mcmc_spin_amd import model, seeding
+    uc = model.build_kitaev_honeycomb(K=-1.0, Gamma=0.25, Gammap=-0.02, field_strength=0.1,
+                                      field_direction=np.ones(3) / np.sqrt(3.0))
+    lat = model.Lattice(uc, L1, L2, 1)
+    relaxed = seeding.quench(lat, seeding.random_spins(lat.lattice_size, 1234), sweeps=200)
+    init = seeding.tilt(relaxed, 0.01, seed=4321)
+    field_in = np.zeros((lat.N_atoms, 3))
+    field_in[:, 0] = 1.0
+    return uc, lat, init, field_in
+
+
+def replica_tables(n_tau):
+    taus = TAU0 + TAU_STEP * np.arange(n_tau)
+    B = 1 + 2 * n_tau
+    tp = np.zeros((B, 2))
+    am = np.zeros((B, 2))
+    am[0, 0] = AMP                                   # M0
+    tp[1:1 + n_tau, 0] = taus                        # M1(tau)
+    am[1:1 + n_tau, 0] = AMP
+    tp[1 + n_tau:, 1] = taus                         # M01(tau)
+    am[1 + n_tau:, :] = AMP
+    return tp, am, taus
+
+
+def cpu_baseline(uc, init, field_in, n_tau_sample):
+    """Oracle (FP64 C++/OpenMP restatement, outer OpenMP over tau with serial
+    inner RHS as the reference's W2 path) on a bounded sample of the workload."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle
+    o = pyoracle.OracleLattice(uc, L1, L2, 1)
+    threads = pyoracle.max_threads()
+    t = time.perf_counter()
+    o.pump_probe(init.reshape(-1), field_in, AMP, WIDTH, FREQ, TAU0, TAU0 + TAU_STEP * (n_tau_sample - 1),
+                 TAU_STEP, T0, T1, DT, reuse_m0=False, n_outer=threads)
+    el = time.perf_counter() - t
+    n_traj = 1 + 2 * n_tau_sample
+    return {"value": n_traj / el, "unit": "trajectories/s", "cores": threads, "kind": "port",
+            "sample": f"{n_traj} trajectories (M0 + {n_tau_sample} M1 + {n_tau_sample} M01) x {N_STEPS} "
+                      f"RK4 steps of the same 36x36 lattice, {el:.1f} s on {threads} OpenMP threads "
+                      f"({platform.processor() or platform.machine()})",
+            "site_steps_per_s": n_traj * N_STEPS * L1 * L2 * 2 / el}
+
+
+def parity_check(uc, lat, gl, init, field_in, n_steps=500):
+    """Short-horizon error of the batched kernel against the oracle (3
+    trajectories: M0, M1(tau=0.25), M01(tau=0.25))."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle
+    o = pyoracle.OracleLattice(uc, L1, L2, 1)
+    Tend = T0 + (n_steps + 0.5) * DT
+    tp = np.array([[0.0, 0.0], [0.25, 0.0], [0.0, 0.25]])
+    am = np.array([[AMP, 0.0], [AMP, 0.0], [AMP, AMP]])
+    res = gl.integrate_batched(init, tp, am, T0, Tend, DT, tgrid="boost", want_final=True)
+    err_m = err_s = 0.0
+    f = field_in
+    for b in range(3):
+        o.set_pulse(f * (am[b, 0] != 0), tp[b, 0], f * (am[b, 1] != 0), tp[b, 1], AMP, WIDTH, FREQ)
+        ref = o.integrate_md(init.reshape(-1), T0, Tend, DT)
+        err_m = max(err_m, float(np.max(np.abs(res["mags"][:, b] - ref["mags"]))))
+        err_s = max(err_s, float(np.max(np.abs(res["final"][b] - ref["final"].reshape(-1, 3)))))
+    return err_m, err_s
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--cpu-tau", type=int, default=6, help="tau points in the CPU baseline sample")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--block", type=int, default=0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}): the batched 2DCS "
+              "trajectories shard by batch across GPUs with no exchange; multi-GPU runs are not "
+              "part of this build.", file=sys.stderr)
+        sys.exit(2)
+
+    from classical_mcmc_spin_amd import GPULattice
+
+    uc, lat, init, field_in = build_workload()
+    tp, am, taus = replica_tables(N_TAU)
+    B = tp.shape[0]
+    N = lat.lattice_size
+    gl = GPULattice(lat)
+    fd = np.einsum("aij,aj->ai", lat.frames, field_in).reshape(-1)
+    gl.set_pulse(np.concatenate([fd, fd]), AMP, WIDTH, FREQ, 0.0, 0.0)
+
+    tuned_block, tuned_ms = gl.autotune(B, 64)
+    print(f"[bench] autotuned block {tuned_block} ({tuned_ms:.2f} ms / 64 steps)", file=sys.stderr, flush=True)
+
+    def one_step():
+        return gl.integrate_batched(init, tp, am, T0, T1, DT, tgrid="boost", block=args.block,
+                                    want_energy=False)
+
+    for _ in range(args.warmup):
+        one_step()
+    kernel_ms = []
+    t = time.perf_counter()
+    for _ in range(args.steps):
+        res = one_step()
+        kernel_ms.append(gl.last_kernel_ms()[0])
+    el = time.perf_counter() - t
+    ms_step = 1e3 * el / args.steps
+    traj_s = B * args.steps / el
+    n_rec = res["mags"].shape[0]
+    assert n_rec == N_STEPS + 1 and np.all(np.isfinite(res["mags"]))
+
+    # roofline of the dominant kernel (k_persistent, one launch per step)
+    kms = float(np.mean(kernel_ms))
+    site_steps_launch = B * N * N_STEPS
+    achieved = site_steps_launch * BYTES_PER_SITE_STEP / (kms * 1e-3)
+
+    # plain one-trajectory-per-launch paths (M01 at tau = 0.25), one trajectory
+    gl.set_spins(init)
+    gl.set_pulse(np.concatenate([fd, fd]), AMP, WIDTH, FREQ, 0.0, 0.25)
+    t = time.perf_counter()
+    gl.integrate(T0, T1, DT, 1, tgrid="boost", engine=1, want_energy=False)
+    plain_stage_s = time.perf_counter() - t
+    gl.set_spins(init)
+    t = time.perf_counter()
+    gl.integrate(T0, T1, DT, 1, tgrid="boost", engine=2, want_energy=False)
+    plain_persist_s = time.perf_counter() - t
+    gl.set_pulse(np.concatenate([fd, fd]), AMP, WIDTH, FREQ, 0.0, 0.0)
+
+    err_m, err_s = parity_check(uc, lat, gl, init, field_in)
+
+    line = {
+        "metric": METRIC,
+        "value": traj_s,
+        "unit": "trajectories/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "2dcs_kitaev_honeycomb_36x36x1_513traj_8192steps_rk4_dt0.01",
+                   "lattice": [L1, L2, 1], "sites": N, "trajectories": B, "rk4_steps": N_STEPS,
+                   "record_every": 1, "block": tuned_block if args.block == 0 else args.block},
+        "site_steps_per_s": traj_s * N * N_STEPS,
+        "plain_stage_kernels_traj_per_s": 1.0 / plain_stage_s,
+        "plain_persistent_traj_per_s": 1.0 / plain_persist_s,
+        "batched_over_plain": traj_s * plain_stage_s,
+        "max_abs_err_vs_oracle_500_steps": {"M": err_m, "S": err_s},
+        "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK, "traffic": None,
+                     "kernel": "k_persistent", "kernel_ms": kms,
+                     "algorithmic_bytes_per_launch": site_steps_launch * BYTES_PER_SITE_STEP,
+                     "fp64_tflops": site_steps_launch * FLOPS_PER_SITE_STEP / (kms * 1e-3) / 1e12},
+    }
+    traffic_file = os.path.join(ROOT, "profiles", "traffic_k_persistent_2dcs_36x36.txt")
+    if os.path.exists(traffic_file):
+        for ln in open(traffic_file):
+            if ln.startswith("traffic_bytes_per_launch:"):
+                line["roofline"]["traffic"] = float(ln.split(":", 1)[1])
+    if not args.no_cpu:
+        line["cpu_baseline"] = cpu_baseline(uc, init, field_in, args.cpu_tau)
+    gl.close()
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -2,6 +2,8 @@
 
 The shared library lands next to this file so it travels with the source tree.
 Run ``python -m classical_mcmc_spin_amd.build`` or call :func:`build`.
+Objects compile in parallel; the persistent kernel is compiled once per
+sites-per-thread value (-DCSL_SPT=n).
 """
 from __future__ import annotations
 
@@ -9,6 +11,7 @@ import os
 import shutil
 import subprocess
 import sys
+from concurrent.futures import ThreadPoolExecutor
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
@@ -16,15 +19,17 @@ ROOT = os.path.dirname(HERE)
 LIB_NAME = "libcsl_hip.so"
 LIB_PATH = os.path.join(HERE, LIB_NAME)
 
-HIP_SOURCES = ["csl_kernels.hip", "csl_capi.hip"]
-CXX_SOURCES = ["gpu_api_shim.cpp"]
+# (source, object suffix, extra flags)
+HIP_UNITS = [("csl_kernels.hip", "", []), ("csl_capi.hip", "", [])] + [
+    ("csl_persistent.hip", f"_spt{n}", [f"-DCSL_SPT={n}"]) for n in (1, 2, 3, 4, 6)]
+CXX_UNITS = ["gpu_api_shim.cpp"]
 
 HIPCC_FLAGS = [
     "-O3",
     "--offload-arch=gfx950",
     "-std=c++17",
     "-fPIC",
-    # packed FP32 ops are kept out of every HIP object (see DESIGN.md)
+    # packed FP32 ops are kept out of every HIP object
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wno-unused-command-line-argument",
 ]
@@ -37,28 +42,41 @@ def hipcc() -> str:
     return path
 
 
-def build(verbose: bool = False) -> str:
+def build(verbose: bool = False, jobs: int = 0) -> str:
     """Compile every HIP/C++ source into classical_mcmc_spin_amd/libcsl_hip.so."""
     objdir = os.path.join(HERE, "_build")
     os.makedirs(objdir, exist_ok=True)
-    objs = []
     inc = ["-I", os.path.join(ROOT, "include"), "-I", CSRC]
-    for src in HIP_SOURCES:
-        obj = os.path.join(objdir, src + ".o")
-        cmd = [hipcc(), *HIPCC_FLAGS, *inc, "-c", os.path.join(CSRC, src), "-o", obj]
-        _run(cmd, verbose)
+    cmds, objs = [], []
+    for src, suffix, extra in HIP_UNITS:
+        obj = os.path.join(objdir, src + suffix + ".o")
+        cmds.append([hipcc(), *HIPCC_FLAGS, *extra, *inc, "-c", os.path.join(CSRC, src), "-o", obj])
         objs.append(obj)
-    for src in CXX_SOURCES:
-        if not os.path.exists(os.path.join(CSRC, src)):
-            continue
+    for src in CXX_UNITS:
         obj = os.path.join(objdir, src + ".o")
-        cmd = [hipcc(), "-O2", "-std=c++17", "-fPIC", *inc, "-c", os.path.join(CSRC, src), "-o", obj]
-        _run(cmd, verbose)
+        cmds.append([hipcc(), "-O2", "-std=c++17", "-fPIC", *inc, "-c", os.path.join(CSRC, src), "-o", obj])
         objs.append(obj)
+    jobs = jobs or min(len(cmds), max(1, min(16, os.cpu_count() or 1)))
+    with ThreadPoolExecutor(max_workers=jobs) as ex:
+        list(ex.map(lambda c: _run(c, verbose), cmds))
     tmp = LIB_PATH + ".tmp"
     _run([hipcc(), "--offload-arch=gfx950", "-shared", "-fPIC", *objs, "-o", tmp], verbose)
     os.replace(tmp, LIB_PATH)
+    build_cxx_test(verbose)
     return LIB_PATH
+
+
+def build_cxx_test(verbose: bool = False) -> str:
+    """C++ program that drives the gpu:: drop-in API (tests/cxx), linked
+    against the in-tree library with an rpath to it."""
+    src = os.path.join(ROOT, "tests", "cxx", "test_gpu_api_shim.cpp")
+    exe = os.path.join(ROOT, "tests", "cxx", "test_gpu_api_shim")
+    if not os.path.exists(src):
+        return ""
+    cmd = ["g++", "-O2", "-std=c++17", "-I", os.path.join(ROOT, "include"), src, "-o", exe,
+           "-L", HERE, "-lcsl_hip", f"-Wl,-rpath,{HERE}", "-Wl,-rpath,$ORIGIN/../../classical_mcmc_spin_amd"]
+    _run(cmd, verbose)
+    return exe
 
 
 def _run(cmd, verbose):
@@ -68,9 +86,11 @@ def _run(cmd, verbose):
     if res.returncode != 0:
         sys.stderr.write(res.stdout + res.stderr)
         raise RuntimeError(f"build step failed: {' '.join(cmd[:3])} ... ({res.returncode})")
-    if verbose and (res.stdout or res.stderr):
-        sys.stderr.write(res.stdout + res.stderr)
+    if verbose:
+        msg = "\n".join(l for l in (res.stdout + res.stderr).splitlines() if "packed-fp32-ops" not in l)
+        if msg.strip():
+            sys.stderr.write(msg + "\n")
 
 
 if __name__ == "__main__":
-    print(build(verbose=True))
+    print(build(verbose="-v" in sys.argv))

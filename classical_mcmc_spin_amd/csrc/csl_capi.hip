@@ -184,13 +184,57 @@ void build_tables(csl_lattice* h, size_t N, size_t D, size_t n_atoms, const doub
         const double v = q < N * D ? field[q] : q < N * D * (D + 1) ? onsite[q - N * D] : J[q - N * D * (D + 1)];
         if (!std::isfinite(v)) fail(CSL_ERR_NONFINITE, "non-finite Hamiltonian coefficient");
     }
-    // translation invariance of bond matrices (per sublattice and slot)
+    // Translation invariance of the bond matrices per (sublattice, slot).
+    // The reference's per-site bond order (lattice.h:785-803) depends on
+    // which neighbour visits first, which changes on cells where a bond
+    // wraps; the majority slot sequence of each sublattice is the canonical
+    // one and sites whose bonds are a permutation of it are re-ordered to it
+    // (slot_of[k] = position of CSR bond k in the device slot order).  Only
+    // the order of that site's FP64 bond sum changes.
+    std::vector<size_t> slot_of(off[N]);
+    for (size_t k = 0; k < off[N]; ++k) slot_of[k] = k;  // identity (CSR order) by default
     bool uni = true;
-    for (size_t i = 0; i < N && uni; ++i) {
-        const size_t r = i % n_atoms;  // same sublattice in cell 0
-        const size_t c = off[i + 1] - off[i];
-        if (c != off[r + 1] - off[r]) { uni = false; break; }
-        if (std::memcmp(J + off[i] * 9, J + off[r] * 9, c * 9 * sizeof(double)) != 0) uni = false;
+    std::vector<size_t> canon(n_atoms, 0);  // representative site of each sublattice
+    {
+        auto same_seq = [&](size_t a, size_t b) {
+            const size_t ca = off[a + 1] - off[a];
+            return ca == off[b + 1] - off[b] &&
+                   std::memcmp(J + off[a] * 9, J + off[b] * 9, ca * 9 * sizeof(double)) == 0;
+        };
+        for (size_t at = 0; at < n_atoms && uni; ++at) {
+            // majority sequence: Boyer-Moore vote over the sites of this sublattice
+            size_t cand = at, votes = 0;
+            for (size_t i = at; i < N; i += n_atoms) {
+                if (votes == 0) { cand = i; votes = 1; }
+                else if (same_seq(i, cand)) ++votes;
+                else --votes;
+            }
+            canon[at] = cand;
+            const size_t c = off[cand + 1] - off[cand];
+            for (size_t i = at; i < N && uni; i += n_atoms) {
+                if (same_seq(i, cand)) {
+                    for (size_t q = 0; q < c; ++q) slot_of[off[i] + q] = q;
+                    continue;
+                }
+                if (off[i + 1] - off[i] != c) { uni = false; break; }
+                std::vector<char> used(c, 0);
+                for (size_t q = 0; q < c && uni; ++q) {  // canonical slot q takes an equal unused bond
+                    bool found = false;
+                    for (size_t r = 0; r < c; ++r)
+                        if (!used[r] && std::memcmp(J + (off[i] + r) * 9, J + (off[cand] + q) * 9,
+                                                    9 * sizeof(double)) == 0) {
+                            used[r] = 1;
+                            slot_of[off[i] + r] = q;
+                            found = true;
+                            break;
+                        }
+                    if (!found) uni = false;
+                }
+            }
+        }
+        if (!uni)
+            for (size_t i = 0; i < N; ++i)
+                for (size_t k = off[i]; k < off[i + 1]; ++k) slot_of[k] = k - off[i];
     }
     bool site_uni = true;
     for (size_t i = 0; i < N && site_uni; ++i) {
@@ -207,14 +251,16 @@ void build_tables(csl_lattice* h, size_t N, size_t D, size_t n_atoms, const doub
         const int p = dev(i);
         const int c = int(off[i + 1] - off[i]);
         cnt[p] = c;
-        for (int k = 0; k < zmax; ++k) col[size_t(k) * NP + p] = k < c ? dev(partner[off[i] + k]) : p;
+        for (int k = 0; k < zmax; ++k) col[size_t(k) * NP + p] = p;  // unused slots point at the site itself
         for (int k = 0; k < c; ++k) {
+            const size_t q = slot_of[off[i] + k];
+            col[q * NP + p] = dev(partner[off[i] + k]);
             const double* src = J + (off[i] + k) * 9;
-            double* dst = uni ? &mats[((i % n_atoms) * zmax + k) * 9] : &mats[(size_t(k) * NP + p) * 9];
+            double* dst = uni ? &mats[((i % n_atoms) * zmax + q) * 9] : &mats[(q * NP + p) * 9];
             std::memcpy(dst, src, 9 * sizeof(double));
         }
     }
-    for (size_t a = 0; a < n_atoms; ++a) cnt_atom[a] = int(off[a + 1] - off[a]);
+    for (size_t a = 0; a < n_atoms; ++a) cnt_atom[a] = int(off[canon[a] + 1] - off[canon[a]]);
     std::vector<double> f, o;
     if (site_uni) {
         f.assign(field, field + n_atoms * 3);

@@ -37,6 +37,27 @@ struct StageArgs {
     double* partials;        // [B][nblk][10]
 };
 
+// Largest workgroup a persistent instantiation accepts.  Smaller bounds give
+// each wave a larger VGPR budget (512 / waves-per-SIMD): FP64 state and
+// accumulator cost 12 VGPRs per owned site, so threads that own more sites
+// run in smaller workgroups.
+template <int SPT>
+constexpr int persistent_max_block() {
+    return SPT <= 2 ? 1024 : (SPT <= 4 ? 768 : 512);
+}
+
+// Launcher of the persistent kernel for one sites-per-thread value; each
+// specialisation is defined in its own object (csl_persistent.hip built with
+// -DCSL_SPT=n).
+template <int SPT>
+hipError_t launch_persistent_spt(const LatDev& L, const PersistArgs& A, int block, bool dbuf, size_t lds,
+                                 hipStream_t st);
+template <> hipError_t launch_persistent_spt<1>(const LatDev&, const PersistArgs&, int, bool, size_t, hipStream_t);
+template <> hipError_t launch_persistent_spt<2>(const LatDev&, const PersistArgs&, int, bool, size_t, hipStream_t);
+template <> hipError_t launch_persistent_spt<3>(const LatDev&, const PersistArgs&, int, bool, size_t, hipStream_t);
+template <> hipError_t launch_persistent_spt<4>(const LatDev&, const PersistArgs&, int, bool, size_t, hipStream_t);
+template <> hipError_t launch_persistent_spt<6>(const LatDev&, const PersistArgs&, int, bool, size_t, hipStream_t);
+
 size_t persistent_lds_bytes(const LatDev& L, int block, bool dbuf);
 bool persistent_spt_supported(int spt);
 int persistent_block_limit(int spt);

@@ -1,0 +1,289 @@
+// csl_persistent.hip — the persistent batched RK4 kernel.
+//
+// Compiled once per sites-per-thread value (CSL_SPT, set by the build) so the
+// template instantiations build in parallel; each object defines
+// launch_persistent_spt<CSL_SPT>.
+//
+// One workgroup per trajectory; the whole lattice state of that trajectory
+// lives in LDS (SoA), each thread owns SPT sites and keeps their RK4 state x
+// and accumulator in VGPRs; all steps run inside one launch; M(t) and E(t)
+// are reduced in-kernel (wave64 shuffles + LDS) every save_interval steps.
+// Replaces the per-step launch sequence of integrate_gpu_batched_internal
+// (lattice_gpu.cu:455-577).
+#include "csl_device.hpp"
+
+#ifndef CSL_SPT
+#error "CSL_SPT (sites per thread) must be defined by the build"
+#endif
+
+namespace csl {
+
+// persistent batched kernel
+
+
+template <int SPT, int ZMAX, bool UNI, bool DBUF>
+__global__ void __launch_bounds__(persistent_max_block<SPT>()) k_persistent(LatDev L0, PersistArgs A) {
+    const LatDev& L = L0;
+    extern __shared__ double smem[];
+    const int NP = L.NP;
+    double* bufA = smem;
+    double* bufB = DBUF ? smem + 3 * NP : smem;
+    double* drv = smem + (DBUF ? 6 : 3) * NP;  // [2 parity][3 stage times][2 pulses]
+    double* red = drv + 12;                    // [nwaves][10]
+
+    const int b = blockIdx.x;
+    const int T = blockDim.x;
+    const int tid = threadIdx.x;
+    const int wave = uniform(tid >> 6);
+    const int lane = tid & (kWave - 1);
+    const int nwaves = T >> 6;
+
+    const double tp1 = A.t_pulse[2 * b], tp2 = A.t_pulse[2 * b + 1];
+    const double am1 = A.amp[2 * b], am2 = A.amp[2 * b + 1];
+    const bool drive_on = (am1 != 0.0) || (am2 != 0.0);
+    const double dt = A.dt;
+    const double alpha = L.alpha;
+
+    // initial state -> LDS (device order)
+    const double* init = A.init + size_t(b) * A.init_stride;
+    for (int p = tid; p < NP; p += T) {
+        const int atom = p / L.npad, cell = p - atom * L.npad;
+        double v0 = 0.0, v1 = 0.0, v2 = 0.0;
+        if (cell < L.n_cells) {
+            const size_t i = size_t(cell) * L.n_atoms + atom;
+            v0 = init[3 * i];
+            v1 = init[3 * i + 1];
+            v2 = init[3 * i + 2];
+        }
+        bufA[p] = v0;
+        bufA[NP + p] = v1;
+        bufA[2 * NP + p] = v2;
+    }
+
+    // per-thread ownership: sites p = tid + m*T
+    double x[SPT][3], acc[SPT][3];
+    int nb[SPT][ZMAX];
+    int cnt[SPT], atm[SPT];
+    bool act[SPT];
+#pragma unroll
+    for (int m = 0; m < SPT; ++m) {
+        const int p = tid + m * T;
+        const int pw = wave * kWave + m * T;  // wave-uniform
+        atm[m] = uniform(pw < NP ? pw / L.npad : 0);
+        act[m] = active_site(L, p);
+        const int c = UNI ? uniform(pw < NP ? as_const(L.cnt_atom)[atm[m]] : 0) : (p < NP ? L.cnt[p] : 0);
+        cnt[m] = c;
+#pragma unroll
+        for (int k = 0; k < ZMAX; ++k) nb[m][k] = (act[m] && k < c) ? L.col[size_t(k) * NP + p] : 0;
+        acc[m][0] = acc[m][1] = acc[m][2] = 0.0;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int m = 0; m < SPT; ++m) {
+        const int p = tid + m * T;
+        if (act[m]) {
+            x[m][0] = bufA[p];
+            x[m][1] = bufA[NP + p];
+            x[m][2] = bufA[2 * NP + p];
+        } else {
+            x[m][0] = x[m][1] = x[m][2] = 0.0;
+        }
+    }
+
+    const double b16 = (1.0 / 6.0) * dt, b13 = (1.0 / 3.0) * dt;
+    const double inv_n = 1.0 / double(L.N);
+
+    // Drive envelopes of the three stage times of step n (t, t+dt/2, t+dt)
+    // for both pulses, written by 6 lanes of wave 0 into parity slot n&1.
+    // Slot n&1 is written at the top of step n-1 (step 0: before the loop)
+    // and read during step n; the barriers of step n-1 separate the two.
+    auto write_drive = [&](int n) {
+        if (drive_on && wave == 0 && lane < 6) {
+            const int st = lane >> 1, pl = lane & 1;
+            const double t = A.tgrid[n];
+            const double ts = st == 0 ? t : (st == 1 ? t + 0.5 * dt : t + 1.0 * dt);
+            drv[(n & 1) * 6 + lane] = drive_factor(pl ? am2 : am1, ts, pl ? tp2 : tp1, L.width, L.freq);
+        }
+    };
+    // wave-reduce the observables of the owned sites and publish them in red[]
+    auto reduce_obs = [&](const LatDev& Lr, const double* e) {
+        double part[10];
+#pragma unroll
+        for (int j = 0; j < 10; ++j) part[j] = 0.0;
+#pragma unroll
+        for (int m = 0; m < SPT; ++m) {
+            if (act[m]) {
+                cptr<double> R = as_const(Lr.frames) + atm[m] * 9;
+                const double sg = as_const(Lr.afm)[atm[m]];
+#pragma unroll
+                for (int d = 0; d < 3; ++d) {
+                    const double g = R[d * 3 + 0] * x[m][0] + R[d * 3 + 1] * x[m][1] + R[d * 3 + 2] * x[m][2];
+                    part[d] += sg * g;
+                    part[3 + d] += x[m][d];
+                    part[6 + d] += g;
+                }
+                part[9] += e[m];
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < 10; ++j) {
+            const double v = wave_sum(part[j]);
+            if (lane == 0) red[wave * 10 + j] = v;
+        }
+    };
+    auto publish = [&](int ri) {
+        if (wave == 0 && lane < 10) {
+            double v = 0.0;
+            for (int w = 0; w < nwaves; ++w) v += red[w * 10 + lane];
+            if (lane < 9) {
+                if (A.mags) A.mags[(size_t(ri) * A.B + b) * 9 + lane] = v * inv_n;
+            } else if (A.energy) {
+                A.energy[size_t(ri) * A.B + b] = v * inv_n;
+            }
+        }
+    };
+
+    write_drive(0);
+    __syncthreads();
+
+    for (int n = 0; n < A.n_steps; ++n) {
+        const int par = n & 1;
+        const bool rec = (n % A.save_interval) == 0;
+        const int ri = n / A.save_interval;
+        if (n + 1 < A.n_steps) write_drive(n + 1);
+#pragma unroll 1
+        for (int s = 0; s < 4; ++s) {
+            const LatDev L = refresh(L0);
+            const double* cur = (DBUF && (s & 1)) ? bufB : bufA;
+            double* nxt = DBUF ? ((s & 1) ? bufA : bufB) : bufA;
+            const int st = s == 0 ? 0 : (s == 3 ? 2 : 1);
+            const double f1 = drive_on ? drv[par * 6 + st * 2 + 0] : 0.0;
+            const double f2 = drive_on ? drv[par * 6 + st * 2 + 1] : 0.0;
+            const double cdt = (s == 2) ? 1.0 * dt : 0.5 * dt;
+            const double bdt = (s == 0 || s == 3) ? b16 : b13;
+            const bool obs = (s == 0) && rec;
+            double e[SPT];
+            double yn[DBUF ? 1 : SPT][3];
+#pragma unroll
+            for (int m = 0; m < SPT; ++m) {
+                const int p = tid + m * T;
+                e[m] = 0.0;
+                if (act[m]) {
+                    const int atom = atm[m];
+                    const double S[3] = {cur[p], cur[NP + p], cur[2 * NP + p]};
+                    double H[3];
+                    local_field_soa<ZMAX, UNI>(L, cur, NP, p, atom, nb[m], cnt[m], S, H);
+                    if (obs) e[m] = site_energy(L, atom, p, S, H);
+                    if (drive_on) sub_drive(L, atom, f1, f2, H);
+                    double k[3];
+                    ll_su2(H, S, alpha, k);
+                    double y[3];
+#pragma unroll
+                    for (int d = 0; d < 3; ++d) {
+                        if (s == 0) acc[m][d] = x[m][d] + bdt * k[d];
+                        else if (s < 3) acc[m][d] = acc[m][d] + bdt * k[d];
+                        else x[m][d] = acc[m][d] + bdt * k[d];
+                        y[d] = (s < 3) ? x[m][d] + cdt * k[d] : x[m][d];
+                    }
+                    if (DBUF) {
+                        nxt[p] = y[0];
+                        nxt[NP + p] = y[1];
+                        nxt[2 * NP + p] = y[2];
+                    } else {
+#pragma unroll
+                        for (int d = 0; d < 3; ++d) yn[DBUF ? 0 : m][d] = y[d];
+                    }
+                }
+            }
+            // stage 0: x still holds the state at t_n (the record's state)
+            if (obs) reduce_obs(L, e);
+            if (!DBUF) {
+                __syncthreads();
+#pragma unroll
+                for (int m = 0; m < SPT; ++m) {
+                    const int p = tid + m * T;
+                    if (act[m]) {
+                        nxt[p] = yn[DBUF ? 0 : m][0];
+                        nxt[NP + p] = yn[DBUF ? 0 : m][1];
+                        nxt[2 * NP + p] = yn[DBUF ? 0 : m][2];
+                    }
+                }
+            }
+            __syncthreads();
+            if (obs) publish(ri);
+        }
+    }
+
+    // final record: state after the last step (lattice_gpu.cu:567-568)
+    {
+        const LatDev L = refresh(L0);
+        double e[SPT];
+#pragma unroll
+        for (int m = 0; m < SPT; ++m) {
+            const int p = tid + m * T;
+            e[m] = 0.0;
+            if (act[m]) {
+                double H[3];
+                local_field_soa<ZMAX, UNI>(L, bufA, NP, p, atm[m], nb[m], cnt[m], x[m], H);
+                e[m] = site_energy(L, atm[m], p, x[m], H);
+            }
+        }
+        reduce_obs(L, e);
+        __syncthreads();
+        publish(A.n_rec - 1);
+        if (A.final_states) {
+            double* out = A.final_states + size_t(b) * L.N * 3;
+#pragma unroll
+            for (int m = 0; m < SPT; ++m) {
+                const int p = tid + m * T;
+                if (act[m]) {
+                    const int cell = p - atm[m] * L.npad;
+                    const size_t i = size_t(cell) * L.n_atoms + atm[m];
+                    out[3 * i] = x[m][0];
+                    out[3 * i + 1] = x[m][1];
+                    out[3 * i + 2] = x[m][2];
+                }
+            }
+        }
+    }
+}
+
+
+namespace {
+
+template <int SPT, int ZMAX, bool UNI, bool DBUF>
+hipError_t launch_p(const LatDev& L, const PersistArgs& A, int block, size_t lds, hipStream_t st) {
+    auto kern = k_persistent<SPT, ZMAX, UNI, DBUF>;
+    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(kern, dim3(A.B), dim3(block), lds, st, L, A);
+    return hipGetLastError();
+}
+
+template <int SPT, int ZMAX, bool UNI>
+hipError_t launch_p_db(const LatDev& L, const PersistArgs& A, int block, bool dbuf, size_t lds,
+                       hipStream_t st) {
+    return dbuf ? launch_p<SPT, ZMAX, UNI, true>(L, A, block, lds, st)
+                : launch_p<SPT, ZMAX, UNI, false>(L, A, block, lds, st);
+}
+
+template <int SPT, int ZMAX>
+hipError_t launch_p_uni(const LatDev& L, const PersistArgs& A, int block, bool dbuf, size_t lds,
+                        hipStream_t st) {
+    return L.uniform ? launch_p_db<SPT, ZMAX, true>(L, A, block, dbuf, lds, st)
+                     : launch_p_db<SPT, ZMAX, false>(L, A, block, dbuf, lds, st);
+}
+
+}  // namespace
+
+template <>
+hipError_t launch_persistent_spt<CSL_SPT>(const LatDev& L, const PersistArgs& A, int block, bool dbuf,
+                                          size_t lds, hipStream_t st) {
+    if (L.zmax <= 3) return launch_p_uni<CSL_SPT, 3>(L, A, block, dbuf, lds, st);
+    if (L.zmax <= 6) return launch_p_uni<CSL_SPT, 6>(L, A, block, dbuf, lds, st);
+    if (L.zmax <= 12) return launch_p_uni<CSL_SPT, 12>(L, A, block, dbuf, lds, st);
+    return hipErrorInvalidValue;
+}
+
+}  // namespace csl
