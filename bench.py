@@ -113,6 +113,11 @@ def main():
     ap.add_argument("--cpu-tau", type=int, default=6, help="tau points in the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--block", type=int, default=0)
+    ap.add_argument("--launch", default="", help="fixed persistent launch shape 'block,spt,dbuf' "
+                                                   "instead of the autotuner's choice")
+    ap.add_argument("--profile", action="store_true",
+                    help="only the timed batched launches (for rocprofv3 runs): no autotune sweep, "
+                         "plain paths, parity check or CPU baseline")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}): the batched 2DCS "
@@ -130,8 +135,16 @@ def main():
     fd = np.einsum("aij,aj->ai", lat.frames, field_in).reshape(-1)
     gl.set_pulse(np.concatenate([fd, fd]), AMP, WIDTH, FREQ, 0.0, 0.0)
 
-    tuned_block, tuned_ms = gl.autotune(B, 64)
-    print(f"[bench] autotuned block {tuned_block} ({tuned_ms:.2f} ms / 64 steps)", file=sys.stderr, flush=True)
+    if args.launch:
+        blk, spt, db = (int(v) for v in args.launch.split(","))
+        gl.set_launch_config(B, blk, spt, db)
+        tuned_block, tuned_ms = blk, 0.0
+    elif args.profile:
+        tuned_block, tuned_ms = args.block, 0.0
+    else:
+        tuned_block, tuned_ms = gl.autotune(B, 64)
+        print(f"[bench] autotuned block {tuned_block} ({tuned_ms:.2f} ms / 64 steps)", file=sys.stderr,
+              flush=True)
 
     def one_step():
         return gl.integrate_batched(init, tp, am, T0, T1, DT, tgrid="boost", block=args.block,
@@ -154,6 +167,12 @@ def main():
     kms = float(np.mean(kernel_ms))
     site_steps_launch = B * N * N_STEPS
     achieved = site_steps_launch * BYTES_PER_SITE_STEP / (kms * 1e-3)
+    launch_cfg = gl.last_launch_config()
+    if args.profile:
+        print(json.dumps({"profile": True, "kernel_ms": kms, "ms_per_step": ms_step, "launch": launch_cfg,
+                          "traj_per_s": traj_s}), flush=True)
+        gl.close()
+        return
 
     # plain one-trajectory-per-launch paths (M01 at tau = 0.25), one trajectory
     gl.set_spins(init)
@@ -184,7 +203,7 @@ def main():
         "data": "synthetic",
         "config": {"workload": "2dcs_kitaev_honeycomb_36x36x1_513traj_8192steps_rk4_dt0.01",
                    "lattice": [L1, L2, 1], "sites": N, "trajectories": B, "rk4_steps": N_STEPS,
-                   "record_every": 1, "block": tuned_block if args.block == 0 else args.block},
+                   "record_every": 1, "launch": launch_cfg},
         "site_steps_per_s": traj_s * N * N_STEPS,
         "plain_stage_kernels_traj_per_s": 1.0 / plain_stage_s,
         "plain_persistent_traj_per_s": 1.0 / plain_persist_s,

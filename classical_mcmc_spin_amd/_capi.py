@@ -61,6 +61,9 @@ SIGNATURES = {
                             _pd, _pd, _pd, _pd, _pd]),
     "csl_autotune_batched": (_i, [_vp, _sz, _sz, _pi, _pd]),
     "csl_last_kernel_ms": (_i, [_vp, _pd, _psz]),
+    "csl_last_launch_config": (_i, [_vp, _pi, _pi, _pi]),
+    "csl_set_launch_config": (_i, [_vp, _sz, _i, _i, _i]),
+    "csl_set_replica_hamiltonians": (_i, [_vp, _sz, _pd, _pd]),
 }
 
 

@@ -118,7 +118,8 @@ struct TuneKey {
 };
 struct TuneVal {
     int block, spt;
-    bool dbuf;
+    int flags;  // csl::kPersistDoubleBuffer | csl::kPersistUnrollStages
+    bool dbuf() const { return flags & csl::kPersistDoubleBuffer; }
 };
 std::map<TuneKey, TuneVal> g_tune;
 
@@ -143,7 +144,13 @@ struct csl_lattice {
     // workspaces
     DBuf<double> X, ACC, Y1, Y2, partials, fac, tgrid, rec_mags, rec_energy, rec_states, aux;
     DBuf<double> bt_init, bt_tp, bt_amp, bt_final;
+    // CSR -> device slot mapping of the handle's tables (for per-replica tables)
+    std::vector<size_t> h_off, h_slot_of;
+    // per-trajectory Hamiltonians (csl_set_replica_hamiltonians)
+    DBuf<double> rep_mats, rep_field;
+    size_t rep_B = 0, rep_mats_stride = 0, rep_field_stride = 0;
     double last_ms = 0.0;
+    int last_block = 0, last_spt = 0, last_dbuf = 0;
     size_t last_launches = 0;
     ~csl_lattice() {
         if (stream) (void)hipStreamDestroy(stream);
@@ -261,6 +268,9 @@ void build_tables(csl_lattice* h, size_t N, size_t D, size_t n_atoms, const doub
         }
     }
     for (size_t a = 0; a < n_atoms; ++a) cnt_atom[a] = int(off[canon[a] + 1] - off[canon[a]]);
+    h->h_off = off;
+    h->h_slot_of = slot_of;
+    h->rep_B = 0;
     std::vector<double> f, o;
     if (site_uni) {
         f.assign(field, field + n_atoms * 3);
@@ -272,6 +282,12 @@ void build_tables(csl_lattice* h, size_t N, size_t D, size_t n_atoms, const doub
             std::memcpy(&f[size_t(dev(i)) * 3], field + i * 3, 3 * sizeof(double));
             std::memcpy(&o[size_t(dev(i)) * 9], onsite + i * 9, 9 * sizeof(double));
         }
+    }
+    // the kernels use 2A (lattice.h:1660-1665 multiplies by 2.0; exact)
+    bool has_onsite = false;
+    for (double& v : o) {
+        has_onsite |= (v != 0.0);
+        v *= 2.0;
     }
     hipStream_t s = h->stream;
     h->col.upload(col.data(), col.size(), s);
@@ -299,6 +315,7 @@ void build_tables(csl_lattice* h, size_t N, size_t D, size_t n_atoms, const doub
     L.zmax = std::max(zmax, 1);
     L.uniform = uni ? 1 : 0;
     L.site_uniform = site_uni ? 1 : 0;
+    L.has_onsite = has_onsite ? 1 : 0;
     L.col = h->col.p;
     L.cnt = h->cnt.p;
     L.cnt_atom = h->cnt_atom.p;
@@ -343,8 +360,9 @@ std::vector<TuneVal> persistent_candidates(const csl_lattice* h) {
         block = (block + 63) / 64 * 64;
         if (block > csl::persistent_block_limit(spt) || block < 64) continue;
         if (block * spt - NP >= 64 * spt) continue;  // a whole idle wave per site slot
-        for (bool db : {false, true})
-            if (csl::persistent_lds_bytes(h->L, block, db) <= h->lds_limit) c.push_back({block, spt, db});
+        for (int fl = 0; fl < 4; ++fl)
+            if (csl::persistent_lds_bytes(h->L, block, fl & csl::kPersistDoubleBuffer) <= h->lds_limit)
+                c.push_back({block, spt, fl});
     }
     return c;
 }
@@ -360,10 +378,10 @@ TuneVal pick_persistent(const csl_lattice* h, size_t B, int block_req) {
     const TuneKey key{h->L.NP, h->L.zmax, h->L.uniform, B >= 64 ? 2 : (B > 1 ? 1 : 0)};
     auto it = g_tune.find(key);
     if (it != g_tune.end()) return it->second;
-    // heuristic: single LDS buffer with the fewest sites per thread whose
-    // block is <= 1024
+    // heuristic: two LDS buffers (one barrier per stage) with the fewest
+    // sites per thread that fit, runtime stage index
     for (auto& c : cands)
-        if (!c.dbuf) return c;
+        if (c.flags == csl::kPersistDoubleBuffer) return c;
     return cands.front();
 }
 
@@ -400,11 +418,19 @@ void run_persistent(csl_lattice* h, size_t B, const double* d_init, size_t init_
     A.mags = h->rec_mags.p;
     A.energy = want_energy ? h->rec_energy.p : nullptr;
     A.final_states = want_final ? h->bt_final.p : nullptr;
+    csl::LatDev L = h->L;
+    if (h->rep_B) {
+        if (B != h->rep_B) fail(CSL_ERR_ARG, "batch size differs from the per-replica Hamiltonians set");
+        L.mats = h->rep_mats.p;
+        L.field = h->rep_field.p;
+        A.mats_stride = h->rep_mats_stride;
+        A.field_stride = h->rep_field_stride;
+    }
     hipEvent_t e0, e1;
     hip_check(hipEventCreate(&e0), "event");
     hip_check(hipEventCreate(&e1), "event");
     hip_check(hipEventRecord(e0, s), "event record");
-    hip_check(csl::launch_persistent(h->L, A, tv.block, tv.spt, tv.dbuf, s), "k_persistent launch");
+    hip_check(csl::launch_persistent(L, A, tv.block, tv.spt, tv.flags, s), "k_persistent launch");
     hip_check(hipEventRecord(e1, s), "event record");
     hip_check(hipEventSynchronize(e1), "k_persistent");
     float ms = 0.f;
@@ -412,6 +438,9 @@ void run_persistent(csl_lattice* h, size_t B, const double* d_init, size_t init_
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     h->last_ms = ms;
+    h->last_block = tv.block;
+    h->last_spt = tv.spt;
+    h->last_dbuf = tv.flags;
     h->last_launches = 1;
     if (ms_out) *ms_out = ms;
 }
@@ -886,6 +915,7 @@ int csl_integrate_batched(csl_lattice* h, size_t B, const double* init, int init
                            save_interval, energy != nullptr, final_states != nullptr, tv, nullptr);
             copy_back(final_states, h->bt_final.p, B * nd, s);
         } else {
+            if (h->rep_B) fail(CSL_ERR_UNSUPPORTED, "per-replica Hamiltonians need the persistent kernel");
             const csl::LatDev& L = h->L;
             h->X.ensure(B * 3 * size_t(L.NP));
             hip_check(csl::launch_perm_to_soa(L, h->bt_init.p, init_per_replica ? nd : 0, h->X.p, int(B), s),
@@ -1012,6 +1042,14 @@ int csl_autotune_batched(csl_lattice* h, size_t B, size_t n_probe_steps, int* be
         for (size_t k = 0; k <= n_probe_steps; ++k) grid[k] = 0.01 * double(k);
         double best = 1e300;
         TuneVal bv = cands.front();
+        // probe with the handle's shared tables (per-replica tables, if any,
+        // have their own batch size)
+        struct RepGuard {
+            csl_lattice* h;
+            size_t saved;
+            ~RepGuard() { h->rep_B = saved; }
+        } guard{h, h->rep_B};
+        h->rep_B = 0;
         for (auto& c : cands) {
             double ms = 0.0;
             run_persistent(h, B, h->bt_init.p, 0, h->bt_tp.p, h->bt_amp.p, grid, 0.01, 1, true, false, c, &ms);
@@ -1025,6 +1063,86 @@ int csl_autotune_batched(csl_lattice* h, size_t B, size_t n_probe_steps, int* be
         g_tune[key] = bv;
         if (best_block) *best_block = bv.block;
         if (best_ms) *best_ms = best;
+    });
+}
+
+int csl_set_replica_hamiltonians(csl_lattice* h, size_t B, const double* J, const double* field) {
+    return guarded([&] {
+        check_handle(h);
+        if (B == 0) {  // clear
+            h->rep_B = 0;
+            return;
+        }
+        if (!J || !field) fail(CSL_ERR_NULL, "null replica tables");
+        const csl::LatDev& L = h->L;
+        const size_t N = h->N, A = h->n_atoms, nnz = h->h_off[N];
+        check_finite(J, B * nnz * 9, "replica bond matrices");
+        check_finite(field, B * N * 3, "replica fields");
+        const auto& off = h->h_off;
+        const auto& slot = h->h_slot_of;
+        const size_t ms = L.uniform ? A * size_t(L.zmax) * 9 : size_t(L.zmax) * L.NP * 9;
+        const size_t fs = L.site_uniform ? A * 3 : size_t(L.NP) * 3;
+        std::vector<double> mats(B * ms, 0.0), fld(B * fs, 0.0);
+        std::vector<char> seen(L.uniform ? A * size_t(L.zmax) : 0);
+        for (size_t b = 0; b < B; ++b) {
+            const double* Jb = J + b * nnz * 9;
+            const double* Fb = field + b * N * 3;
+            double* mb = &mats[b * ms];
+            double* fb = &fld[b * fs];
+            std::fill(seen.begin(), seen.end(), 0);
+            for (size_t i = 0; i < N; ++i) {
+                const size_t atom = i % A, cell = i / A;
+                const size_t p = atom * L.npad + cell;
+                for (size_t k = off[i]; k < off[i + 1]; ++k) {
+                    const size_t q = slot[k];
+                    if (L.uniform) {
+                        double* dst = mb + (atom * L.zmax + q) * 9;
+                        if (!seen[atom * L.zmax + q]) {
+                            std::memcpy(dst, Jb + k * 9, 9 * sizeof(double));
+                            seen[atom * L.zmax + q] = 1;
+                        } else if (std::memcmp(dst, Jb + k * 9, 9 * sizeof(double)) != 0) {
+                            fail(CSL_ERR_ARG, "replica bond matrices are not translation invariant like the handle's");
+                        }
+                    } else {
+                        std::memcpy(mb + (q * L.NP + p) * 9, Jb + k * 9, 9 * sizeof(double));
+                    }
+                }
+                if (L.site_uniform) {
+                    if (cell == 0) std::memcpy(fb + atom * 3, Fb + i * 3, 3 * sizeof(double));
+                    else if (std::memcmp(fb + atom * 3, Fb + i * 3, 3 * sizeof(double)) != 0)
+                        fail(CSL_ERR_ARG, "replica fields are not translation invariant like the handle's");
+                } else {
+                    std::memcpy(fb + p * 3, Fb + i * 3, 3 * sizeof(double));
+                }
+            }
+        }
+        h->rep_mats.upload(mats.data(), mats.size(), h->stream);
+        h->rep_field.upload(fld.data(), fld.size(), h->stream);
+        hip_check(hipStreamSynchronize(h->stream), "replica upload");
+        h->rep_B = B;
+        h->rep_mats_stride = ms;
+        h->rep_field_stride = fs;
+    });
+}
+
+int csl_set_launch_config(csl_lattice* h, size_t B, int block, int sites_per_thread, int double_buffer) {
+    return guarded([&] {
+        check_handle(h);
+        for (const auto& c : persistent_candidates(h))
+            if (c.block == block && c.spt == sites_per_thread && c.flags == double_buffer) {
+                g_tune[TuneKey{h->L.NP, h->L.zmax, h->L.uniform, B >= 64 ? 2 : (B > 1 ? 1 : 0)}] = c;
+                return;
+            }
+        fail(CSL_ERR_ARG, "launch shape does not fit this lattice");
+    });
+}
+
+int csl_last_launch_config(csl_lattice* h, int* block, int* sites_per_thread, int* double_buffer) {
+    return guarded([&] {
+        if (!h) fail(CSL_ERR_NULL, "null lattice handle");
+        if (block) *block = h->last_block;
+        if (sites_per_thread) *sites_per_thread = h->last_spt;
+        if (double_buffer) *double_buffer = h->last_dbuf;
     });
 }
 

@@ -49,12 +49,13 @@ struct LatDev {
     int zmax;       // max bonds per site
     int uniform;    // bond matrices per (sublattice, slot)
     int site_uniform;  // field / onsite per sublattice
+    int has_onsite;    // any non-zero onsite matrix (the tables hold 2A)
     const int* __restrict__ col;        // [zmax*NP]
     const int* __restrict__ cnt;        // [NP] bonds of each site
     const int* __restrict__ cnt_atom;   // [n_atoms] (uniform handles)
     const double* __restrict__ mats;    // see header comment
     const double* __restrict__ field;   // [n_atoms*3] or [NP*3]
-    const double* __restrict__ onsite;  // [n_atoms*9] or [NP*9]
+    const double* __restrict__ onsite;  // 2A: [n_atoms*9] or [NP*9]
     const double* __restrict__ frames;  // [n_atoms*9]
     const double* __restrict__ afm;     // [n_atoms]
     const double* __restrict__ fdrive;  // [2*n_atoms*3]
@@ -79,28 +80,34 @@ __device__ __forceinline__ const double* bond_matrix(const LatDev& L, int atom, 
                      : L.mats + (size_t(k) * L.NP + p) * 9;
 }
 
-// H = -B + 2 A S  (lattice.h:1655-1666)
+// H = -B + 2 A S  (lattice.h:1655-1666).  The tables hold 2A (doubling is
+// exact, so (2A)*S equals the reference's 2.0*A*S); handles without onsite
+// terms skip the product.
 __device__ __forceinline__ void field_onsite(const LatDev& L, int atom, int p, const double S[3],
                                              double H[3]) {
     if (L.site_uniform) {
         cptr<double> B = as_const(L.field) + atom * 3;
-        cptr<double> A = as_const(L.onsite) + atom * 9;
+        H[0] = -B[0];
+        H[1] = -B[1];
+        H[2] = -B[2];
+        if (!L.has_onsite) return;
+        cptr<double> A2 = as_const(L.onsite) + atom * 9;
 #pragma unroll
         for (int d = 0; d < 3; ++d) {
-            double h = -B[d];
 #pragma unroll
-            for (int e = 0; e < 3; ++e) h += 2.0 * A[d * 3 + e] * S[e];
-            H[d] = h;
+            for (int e = 0; e < 3; ++e) H[d] += A2[d * 3 + e] * S[e];
         }
         return;
     }
     const double* B = L.field + size_t(p) * 3;
-    const double* A = L.onsite + size_t(p) * 9;
+    const double* A2 = L.onsite + size_t(p) * 9;
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
         double h = -B[d];
+        if (L.has_onsite) {
 #pragma unroll
-        for (int e = 0; e < 3; ++e) h += 2.0 * A[d * 3 + e] * S[e];
+            for (int e = 0; e < 3; ++e) h += A2[d * 3 + e] * S[e];
+        }
         H[d] = h;
     }
 }

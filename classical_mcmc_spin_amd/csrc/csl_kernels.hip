@@ -285,15 +285,16 @@ size_t persistent_lds_bytes(const LatDev& L, int block, bool dbuf) {
     return sizeof(double) * (size_t(dbuf ? 6 : 3) * L.NP + 12 + size_t(block / kWave) * 10);
 }
 
-hipError_t launch_persistent(const LatDev& L, const PersistArgs& A, int block, int spt, bool dbuf,
+hipError_t launch_persistent(const LatDev& L, const PersistArgs& A, int block, int spt, int flags,
                              hipStream_t st) {
+    const bool dbuf = flags & kPersistDoubleBuffer;
     const size_t lds = persistent_lds_bytes(L, block, dbuf);
     switch (spt) {
-        case 1: return launch_persistent_spt<1>(L, A, block, dbuf, lds, st);
-        case 2: return launch_persistent_spt<2>(L, A, block, dbuf, lds, st);
-        case 3: return launch_persistent_spt<3>(L, A, block, dbuf, lds, st);
-        case 4: return launch_persistent_spt<4>(L, A, block, dbuf, lds, st);
-        case 6: return launch_persistent_spt<6>(L, A, block, dbuf, lds, st);
+        case 1: return launch_persistent_spt<1>(L, A, block, flags, lds, st);
+        case 2: return launch_persistent_spt<2>(L, A, block, flags, lds, st);
+        case 3: return launch_persistent_spt<3>(L, A, block, flags, lds, st);
+        case 4: return launch_persistent_spt<4>(L, A, block, flags, lds, st);
+        case 6: return launch_persistent_spt<6>(L, A, block, flags, lds, st);
         default: return hipErrorInvalidValue;
     }
 }

@@ -21,6 +21,10 @@ struct PersistArgs {
     double* mags;            // [n_rec*B*9] or null
     double* energy;          // [n_rec*B] or null
     double* final_states;    // [B*N*3] AoS or null
+    // per-trajectory Hamiltonians: trajectory b reads L.mats + b*mats_stride
+    // and L.field + b*field_stride (0 = shared tables)
+    size_t mats_stride;
+    size_t field_stride;
 };
 
 // One RK4 stage over all sites of B trajectories (SoA [B][3][NP] buffers).
@@ -49,19 +53,23 @@ constexpr int persistent_max_block() {
 // Launcher of the persistent kernel for one sites-per-thread value; each
 // specialisation is defined in its own object (csl_persistent.hip built with
 // -DCSL_SPT=n).
+// Launch-shape flags of the persistent kernel.
+constexpr int kPersistDoubleBuffer = 1;  // two LDS state buffers, one barrier per stage
+constexpr int kPersistUnrollStages = 2;  // the four RK stages specialised at compile time
+
 template <int SPT>
-hipError_t launch_persistent_spt(const LatDev& L, const PersistArgs& A, int block, bool dbuf, size_t lds,
+hipError_t launch_persistent_spt(const LatDev& L, const PersistArgs& A, int block, int flags, size_t lds,
                                  hipStream_t st);
-template <> hipError_t launch_persistent_spt<1>(const LatDev&, const PersistArgs&, int, bool, size_t, hipStream_t);
-template <> hipError_t launch_persistent_spt<2>(const LatDev&, const PersistArgs&, int, bool, size_t, hipStream_t);
-template <> hipError_t launch_persistent_spt<3>(const LatDev&, const PersistArgs&, int, bool, size_t, hipStream_t);
-template <> hipError_t launch_persistent_spt<4>(const LatDev&, const PersistArgs&, int, bool, size_t, hipStream_t);
-template <> hipError_t launch_persistent_spt<6>(const LatDev&, const PersistArgs&, int, bool, size_t, hipStream_t);
+template <> hipError_t launch_persistent_spt<1>(const LatDev&, const PersistArgs&, int, int, size_t, hipStream_t);
+template <> hipError_t launch_persistent_spt<2>(const LatDev&, const PersistArgs&, int, int, size_t, hipStream_t);
+template <> hipError_t launch_persistent_spt<3>(const LatDev&, const PersistArgs&, int, int, size_t, hipStream_t);
+template <> hipError_t launch_persistent_spt<4>(const LatDev&, const PersistArgs&, int, int, size_t, hipStream_t);
+template <> hipError_t launch_persistent_spt<6>(const LatDev&, const PersistArgs&, int, int, size_t, hipStream_t);
 
 size_t persistent_lds_bytes(const LatDev& L, int block, bool dbuf);
 bool persistent_spt_supported(int spt);
 int persistent_block_limit(int spt);
-hipError_t launch_persistent(const LatDev& L, const PersistArgs& A, int block, int spt, bool dbuf,
+hipError_t launch_persistent(const LatDev& L, const PersistArgs& A, int block, int spt, int flags,
                              hipStream_t st);
 int stage_blocks(const LatDev& L);
 hipError_t launch_stage(const LatDev& L, const StageArgs& A, hipStream_t st);

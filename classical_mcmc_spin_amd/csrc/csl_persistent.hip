@@ -5,11 +5,18 @@
 // launch_persistent_spt<CSL_SPT>.
 //
 // One workgroup per trajectory; the whole lattice state of that trajectory
-// lives in LDS (SoA), each thread owns SPT sites and keeps their RK4 state x
-// and accumulator in VGPRs; all steps run inside one launch; M(t) and E(t)
-// are reduced in-kernel (wave64 shuffles + LDS) every save_interval steps.
+// lives in LDS, each thread owns SPT sites and keeps their RK4 state x and
+// accumulator in VGPRs; all steps run inside one launch; M(t) and E(t) are
+// reduced in-kernel (wave64 DPP sums + LDS) every save_interval steps.
 // Replaces the per-step launch sequence of integrate_gpu_batched_internal
 // (lattice_gpu.cu:455-577).
+//
+// LDS state layout (one buffer; two when DBUF): the x,y components of every
+// site packed as a 16-byte pair, then the z plane, so a neighbour gather is
+// one ds_read_b128 plus one ds_read_b64:
+//     [ (x,y)[0..NP) | z[0..NP) ]          3*NP doubles per buffer
+#include <type_traits>
+
 #include "csl_device.hpp"
 
 #ifndef CSL_SPT
@@ -18,16 +25,82 @@
 
 namespace csl {
 
-// persistent batched kernel
+namespace {
 
+// Spin state of one trajectory in LDS (see file header).
+struct LdsState {
+    double2* xy;
+    double* z;
+    __device__ __forceinline__ void load(int q, double S[3]) const {
+        const double2 v = xy[q];
+        S[0] = v.x;
+        S[1] = v.y;
+        S[2] = z[q];
+    }
+    __device__ __forceinline__ void store(int q, const double S[3]) const {
+        xy[q] = make_double2(S[0], S[1]);
+        z[q] = S[2];
+    }
+};
 
-template <int SPT, int ZMAX, bool UNI, bool DBUF>
-__global__ void __launch_bounds__(persistent_max_block<SPT>()) k_persistent(LatDev L0, PersistArgs A) {
+__device__ __forceinline__ LdsState lds_state(double* base, int NP) {
+    return LdsState{reinterpret_cast<double2*>(base), base + 2 * NP};
+}
+
+// Local field of site p (no drive) gathering neighbours from LDS.
+template <int ZMAX, bool UNI>
+__device__ __forceinline__ void local_field_lds(const LatDev& L, const LdsState& y, int p, int atom,
+                                                const int* nb, int cnt, const double S[3], double H[3]) {
+    field_onsite(L, atom, p, S, H);
+#pragma unroll
+    for (int k = 0; k < ZMAX; ++k) {
+        if (k < cnt) {
+            double Sp[3];
+            y.load(nb[k], Sp);
+            if (UNI)
+                add_bond(as_const(L.mats) + (atom * L.zmax + k) * 9, Sp, H);
+            else
+                add_bond(L.mats + (size_t(k) * L.NP + p) * 9, Sp, H);
+        }
+    }
+}
+
+// One DPP move of a double across lanes (both 32-bit halves with the same
+// control); lanes outside row_mask receive 0.
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ double dpp_f64(double v) {
+    const int lo = __double2loint(v), hi = __double2hiint(v);
+    const int rlo = __builtin_amdgcn_update_dpp(0, lo, CTRL, ROW_MASK, 0xf, false);
+    const int rhi = __builtin_amdgcn_update_dpp(0, hi, CTRL, ROW_MASK, 0xf, false);
+    return __hiloint2double(rhi, rlo);
+}
+
+// Sum over the 64 lanes of a wavefront with DPP row operations (no LDS
+// round trip); the total is valid in lane 63.  Must be called with all lanes
+// active.
+__device__ __forceinline__ double wave_sum_lane63(double v) {
+    v += dpp_f64<0xB1, 0xf>(v);   // quad_perm [1,0,3,2]
+    v += dpp_f64<0x4E, 0xf>(v);   // quad_perm [2,3,0,1]   -> quad sums
+    v += dpp_f64<0x124, 0xf>(v);  // row_ror:4
+    v += dpp_f64<0x128, 0xf>(v);  // row_ror:8             -> row (16-lane) sums
+    v += dpp_f64<0x142, 0xa>(v);  // row_bcast:15 into rows 1 and 3
+    v += dpp_f64<0x143, 0xc>(v);  // row_bcast:31 into rows 2 and 3 -> total in row 3
+    return v;
+}
+
+}  // namespace
+
+template <int SPT, int ZMAX, bool UNI, bool DBUF, bool UNROLL>
+__global__ void __launch_bounds__(persistent_max_block<SPT>()) k_persistent(LatDev Lk, PersistArgs A) {
+    // this trajectory's Hamiltonian tables
+    LatDev L0 = Lk;
+    L0.mats += size_t(blockIdx.x) * A.mats_stride;
+    L0.field += size_t(blockIdx.x) * A.field_stride;
     const LatDev& L = L0;
     extern __shared__ double smem[];
     const int NP = L.NP;
-    double* bufA = smem;
-    double* bufB = DBUF ? smem + 3 * NP : smem;
+    const LdsState bufA = lds_state(smem, NP);
+    const LdsState bufB = DBUF ? lds_state(smem + 3 * NP, NP) : bufA;
     double* drv = smem + (DBUF ? 6 : 3) * NP;  // [2 parity][3 stage times][2 pulses]
     double* red = drv + 12;                    // [nwaves][10]
 
@@ -48,16 +121,14 @@ __global__ void __launch_bounds__(persistent_max_block<SPT>()) k_persistent(LatD
     const double* init = A.init + size_t(b) * A.init_stride;
     for (int p = tid; p < NP; p += T) {
         const int atom = p / L.npad, cell = p - atom * L.npad;
-        double v0 = 0.0, v1 = 0.0, v2 = 0.0;
+        double v[3] = {0.0, 0.0, 0.0};
         if (cell < L.n_cells) {
             const size_t i = size_t(cell) * L.n_atoms + atom;
-            v0 = init[3 * i];
-            v1 = init[3 * i + 1];
-            v2 = init[3 * i + 2];
+            v[0] = init[3 * i];
+            v[1] = init[3 * i + 1];
+            v[2] = init[3 * i + 2];
         }
-        bufA[p] = v0;
-        bufA[NP + p] = v1;
-        bufA[2 * NP + p] = v2;
+        bufA.store(p, v);
     }
 
     // per-thread ownership: sites p = tid + m*T
@@ -82,9 +153,7 @@ __global__ void __launch_bounds__(persistent_max_block<SPT>()) k_persistent(LatD
     for (int m = 0; m < SPT; ++m) {
         const int p = tid + m * T;
         if (act[m]) {
-            x[m][0] = bufA[p];
-            x[m][1] = bufA[NP + p];
-            x[m][2] = bufA[2 * NP + p];
+            bufA.load(p, x[m]);
         } else {
             x[m][0] = x[m][1] = x[m][2] = 0.0;
         }
@@ -92,13 +161,17 @@ __global__ void __launch_bounds__(persistent_max_block<SPT>()) k_persistent(LatD
 
     const double b16 = (1.0 / 6.0) * dt, b13 = (1.0 / 3.0) * dt;
     const double inv_n = 1.0 / double(L.N);
+    const bool want_e = A.energy != nullptr;
+    // The last wave owns the fewest real sites (padding sits at the end), so
+    // it evaluates the drive envelopes.
+    const int drive_wave = nwaves - 1;
 
     // Drive envelopes of the three stage times of step n (t, t+dt/2, t+dt)
-    // for both pulses, written by 6 lanes of wave 0 into parity slot n&1.
+    // for both pulses, written by 6 lanes of one wave into parity slot n&1.
     // Slot n&1 is written at the top of step n-1 (step 0: before the loop)
     // and read during step n; the barriers of step n-1 separate the two.
     auto write_drive = [&](int n) {
-        if (drive_on && wave == 0 && lane < 6) {
+        if (drive_on && wave == drive_wave && lane < 6) {
             const int st = lane >> 1, pl = lane & 1;
             const double t = A.tgrid[n];
             const double ts = st == 0 ? t : (st == 1 ? t + 0.5 * dt : t + 1.0 * dt);
@@ -127,8 +200,9 @@ __global__ void __launch_bounds__(persistent_max_block<SPT>()) k_persistent(LatD
         }
 #pragma unroll
         for (int j = 0; j < 10; ++j) {
-            const double v = wave_sum(part[j]);
-            if (lane == 0) red[wave * 10 + j] = v;
+            if (j == 9 && !want_e) break;
+            const double v = wave_sum_lane63(part[j]);
+            if (lane == kWave - 1) red[wave * 10 + j] = v;
         }
     };
     auto publish = [&](int ri) {
@@ -146,16 +220,15 @@ __global__ void __launch_bounds__(persistent_max_block<SPT>()) k_persistent(LatD
     write_drive(0);
     __syncthreads();
 
-    for (int n = 0; n < A.n_steps; ++n) {
-        const int par = n & 1;
-        const bool rec = (n % A.save_interval) == 0;
-        const int ri = n / A.save_interval;
-        if (n + 1 < A.n_steps) write_drive(n + 1);
-#pragma unroll 1
-        for (int s = 0; s < 4; ++s) {
+    // One RK4 stage of step n.  Called with a runtime stage index (one copy of
+    // the code, selects on s) or with compile-time constants (UNROLL: the four
+    // stages are specialised and the selects disappear).
+    auto stage = [&](auto s_, int par, bool rec, int ri) {
+        const int s = static_cast<int>(s_);
+        {
             const LatDev L = refresh(L0);
-            const double* cur = (DBUF && (s & 1)) ? bufB : bufA;
-            double* nxt = DBUF ? ((s & 1) ? bufA : bufB) : bufA;
+            const LdsState cur = (DBUF && (s & 1)) ? bufB : bufA;
+            const LdsState nxt = DBUF ? ((s & 1) ? bufA : bufB) : bufA;
             const int st = s == 0 ? 0 : (s == 3 ? 2 : 1);
             const double f1 = drive_on ? drv[par * 6 + st * 2 + 0] : 0.0;
             const double f2 = drive_on ? drv[par * 6 + st * 2 + 1] : 0.0;
@@ -170,10 +243,11 @@ __global__ void __launch_bounds__(persistent_max_block<SPT>()) k_persistent(LatD
                 e[m] = 0.0;
                 if (act[m]) {
                     const int atom = atm[m];
-                    const double S[3] = {cur[p], cur[NP + p], cur[2 * NP + p]};
+                    double S[3];
+                    cur.load(p, S);
                     double H[3];
-                    local_field_soa<ZMAX, UNI>(L, cur, NP, p, atom, nb[m], cnt[m], S, H);
-                    if (obs) e[m] = site_energy(L, atom, p, S, H);
+                    local_field_lds<ZMAX, UNI>(L, cur, p, atom, nb[m], cnt[m], S, H);
+                    if (obs && want_e) e[m] = site_energy(L, atom, p, S, H);
                     if (drive_on) sub_drive(L, atom, f1, f2, H);
                     double k[3];
                     ll_su2(H, S, alpha, k);
@@ -186,9 +260,7 @@ __global__ void __launch_bounds__(persistent_max_block<SPT>()) k_persistent(LatD
                         y[d] = (s < 3) ? x[m][d] + cdt * k[d] : x[m][d];
                     }
                     if (DBUF) {
-                        nxt[p] = y[0];
-                        nxt[NP + p] = y[1];
-                        nxt[2 * NP + p] = y[2];
+                        nxt.store(p, y);
                     } else {
 #pragma unroll
                         for (int d = 0; d < 3; ++d) yn[DBUF ? 0 : m][d] = y[d];
@@ -202,15 +274,27 @@ __global__ void __launch_bounds__(persistent_max_block<SPT>()) k_persistent(LatD
 #pragma unroll
                 for (int m = 0; m < SPT; ++m) {
                     const int p = tid + m * T;
-                    if (act[m]) {
-                        nxt[p] = yn[DBUF ? 0 : m][0];
-                        nxt[NP + p] = yn[DBUF ? 0 : m][1];
-                        nxt[2 * NP + p] = yn[DBUF ? 0 : m][2];
-                    }
+                    if (act[m]) nxt.store(p, yn[DBUF ? 0 : m]);
                 }
             }
             __syncthreads();
             if (obs) publish(ri);
+        }
+    };
+
+    for (int n = 0; n < A.n_steps; ++n) {
+        const int par = n & 1;
+        const bool rec = (n % A.save_interval) == 0;
+        const int ri = n / A.save_interval;
+        if (n + 1 < A.n_steps) write_drive(n + 1);
+        if constexpr (UNROLL) {
+            stage(std::integral_constant<int, 0>{}, par, rec, ri);
+            stage(std::integral_constant<int, 1>{}, par, rec, ri);
+            stage(std::integral_constant<int, 2>{}, par, rec, ri);
+            stage(std::integral_constant<int, 3>{}, par, rec, ri);
+        } else {
+#pragma unroll 1
+            for (int s = 0; s < 4; ++s) stage(s, par, rec, ri);
         }
     }
 
@@ -224,7 +308,7 @@ __global__ void __launch_bounds__(persistent_max_block<SPT>()) k_persistent(LatD
             e[m] = 0.0;
             if (act[m]) {
                 double H[3];
-                local_field_soa<ZMAX, UNI>(L, bufA, NP, p, atm[m], nb[m], cnt[m], x[m], H);
+                local_field_lds<ZMAX, UNI>(L, bufA, p, atm[m], nb[m], cnt[m], x[m], H);
                 e[m] = site_energy(L, atm[m], p, x[m], H);
             }
         }
@@ -248,12 +332,11 @@ __global__ void __launch_bounds__(persistent_max_block<SPT>()) k_persistent(LatD
     }
 }
 
-
 namespace {
 
-template <int SPT, int ZMAX, bool UNI, bool DBUF>
+template <int SPT, int ZMAX, bool UNI, bool DBUF, bool UNROLL>
 hipError_t launch_p(const LatDev& L, const PersistArgs& A, int block, size_t lds, hipStream_t st) {
-    auto kern = k_persistent<SPT, ZMAX, UNI, DBUF>;
+    auto kern = k_persistent<SPT, ZMAX, UNI, DBUF, UNROLL>;
     hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
                                        hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));
     if (e != hipSuccess) return e;
@@ -262,27 +345,31 @@ hipError_t launch_p(const LatDev& L, const PersistArgs& A, int block, size_t lds
 }
 
 template <int SPT, int ZMAX, bool UNI>
-hipError_t launch_p_db(const LatDev& L, const PersistArgs& A, int block, bool dbuf, size_t lds,
+hipError_t launch_p_db(const LatDev& L, const PersistArgs& A, int block, int flags, size_t lds,
                        hipStream_t st) {
-    return dbuf ? launch_p<SPT, ZMAX, UNI, true>(L, A, block, lds, st)
-                : launch_p<SPT, ZMAX, UNI, false>(L, A, block, lds, st);
+    const bool dbuf = flags & kPersistDoubleBuffer, unroll = flags & kPersistUnrollStages;
+    if (unroll)
+        return dbuf ? launch_p<SPT, ZMAX, UNI, true, true>(L, A, block, lds, st)
+                    : launch_p<SPT, ZMAX, UNI, false, true>(L, A, block, lds, st);
+    return dbuf ? launch_p<SPT, ZMAX, UNI, true, false>(L, A, block, lds, st)
+                : launch_p<SPT, ZMAX, UNI, false, false>(L, A, block, lds, st);
 }
 
 template <int SPT, int ZMAX>
-hipError_t launch_p_uni(const LatDev& L, const PersistArgs& A, int block, bool dbuf, size_t lds,
+hipError_t launch_p_uni(const LatDev& L, const PersistArgs& A, int block, int flags, size_t lds,
                         hipStream_t st) {
-    return L.uniform ? launch_p_db<SPT, ZMAX, true>(L, A, block, dbuf, lds, st)
-                     : launch_p_db<SPT, ZMAX, false>(L, A, block, dbuf, lds, st);
+    return L.uniform ? launch_p_db<SPT, ZMAX, true>(L, A, block, flags, lds, st)
+                     : launch_p_db<SPT, ZMAX, false>(L, A, block, flags, lds, st);
 }
 
 }  // namespace
 
 template <>
-hipError_t launch_persistent_spt<CSL_SPT>(const LatDev& L, const PersistArgs& A, int block, bool dbuf,
+hipError_t launch_persistent_spt<CSL_SPT>(const LatDev& L, const PersistArgs& A, int block, int flags,
                                           size_t lds, hipStream_t st) {
-    if (L.zmax <= 3) return launch_p_uni<CSL_SPT, 3>(L, A, block, dbuf, lds, st);
-    if (L.zmax <= 6) return launch_p_uni<CSL_SPT, 6>(L, A, block, dbuf, lds, st);
-    if (L.zmax <= 12) return launch_p_uni<CSL_SPT, 12>(L, A, block, dbuf, lds, st);
+    if (L.zmax <= 3) return launch_p_uni<CSL_SPT, 3>(L, A, block, flags, lds, st);
+    if (L.zmax <= 6) return launch_p_uni<CSL_SPT, 6>(L, A, block, flags, lds, st);
+    if (L.zmax <= 12) return launch_p_uni<CSL_SPT, 12>(L, A, block, flags, lds, st);
     return hipErrorInvalidValue;
 }
 

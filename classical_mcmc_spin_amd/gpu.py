@@ -185,6 +185,31 @@ class GPULattice:
         check(lib().csl_autotune_batched(self.h, B, n_probe_steps, C.byref(blk), C.byref(ms)))
         return blk.value, ms.value
 
+    def set_replica_hamiltonians(self, lattices=None, J=None, field=None):
+        """Per-trajectory Hamiltonians for integrate_batched: either a list of
+        Lattice objects with this lattice's structure, or arrays J [B,nnz,3,3]
+        and field [B,N,3].  None clears them."""
+        if lattices is None and J is None:
+            check(lib().csl_set_replica_hamiltonians(self.h, 0, None, None))
+            return
+        if lattices is not None:
+            J = np.stack([l.bi_J for l in lattices])
+            field = np.stack([l.field for l in lattices])
+        J = f64(J)
+        field = f64(field)
+        check(lib().csl_set_replica_hamiltonians(self.h, J.shape[0], dptr(J.reshape(-1)),
+                                                 dptr(field.reshape(-1))))
+
+    def set_launch_config(self, B, block, sites_per_thread, flags):
+        """flags: bit 0 two LDS state buffers, bit 1 compile-time RK stages."""
+        check(lib().csl_set_launch_config(self.h, B, block, sites_per_thread, int(flags)))
+
+    def last_launch_config(self):
+        b, s, d = C.c_int(), C.c_int(), C.c_int()
+        check(lib().csl_last_launch_config(self.h, C.byref(b), C.byref(s), C.byref(d)))
+        return {"block": b.value, "sites_per_thread": s.value, "double_buffer": bool(d.value & 1),
+                "unrolled_stages": bool(d.value & 2)}
+
     def last_kernel_ms(self):
         ms, n = C.c_double(), C.c_size_t()
         check(lib().csl_last_kernel_ms(self.h, C.byref(ms), C.byref(n)))

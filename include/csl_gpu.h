@@ -164,6 +164,15 @@ int csl_integrate_batched(csl_lattice* h, size_t B, const double* init,
                           int block, size_t max_rec, double* times, double* mags,
                           double* energy, double* final_states, size_t* n_rec);
 
+/* Per-trajectory Hamiltonians for csl_integrate_batched: bond matrices
+ * J [B*nnz*D*D] in the handle's CSR bond order (same sparsity) and fields
+ * [B*N*D], e.g. one coupling set per parameter start.  They must have the
+ * handle's structure (translation invariant when the handle is).  B = 0
+ * clears them.  No reference counterpart: the reference runs one
+ * Hamiltonian per Lattice (lattice_gpu.cu:336-397 shares the tables across
+ * replicas). */
+int csl_set_replica_hamiltonians(csl_lattice* h, size_t B, const double* J, const double* field);
+
 /* Number of records an integration will produce (same grid rules). */
 int csl_count_records(double T0, double T1, double dt, size_t save_interval,
                       int tgrid, size_t* n_steps, size_t* n_rec);
@@ -203,6 +212,16 @@ int csl_autotune_batched(csl_lattice* h, size_t B, size_t n_probe_steps,
  * csl_integrate_batched / csl_integrate call on this handle, measured with
  * HIP events on the handle's stream, and the number of launches it spans. */
 int csl_last_kernel_ms(csl_lattice* h, double* ms, size_t* launches);
+
+/* Launch shape of the last persistent-kernel run on this handle: workgroup
+ * size, sites per thread and flags (bit 0: two LDS state buffers, one
+ * barrier per RK stage; bit 1: the four RK stages specialised at compile
+ * time). */
+int csl_last_launch_config(csl_lattice* h, int* block, int* sites_per_thread, int* flags);
+
+/* Fix the persistent launch shape used for this lattice shape and batch size
+ * class (as the autotuner would); rejects shapes that do not fit. */
+int csl_set_launch_config(csl_lattice* h, size_t B, int block, int sites_per_thread, int flags);
 
 #ifdef __cplusplus
 }

@@ -125,6 +125,37 @@ def test_batched_per_replica_pulses_and_states(po):
     o.clear_pulse()
 
 
+def test_batched_per_replica_hamiltonians(po):
+    """One Kitaev parameter set per trajectory (parameter starts), damped and
+    driven, vs one oracle run per parameter set."""
+    params = [(-1.0, 0.25, -0.02, 0.05), (-1.1, 0.22, -0.018, 0.045), (-0.9, 0.27, -0.021, 0.055)]
+    lats, ucs = [], []
+    for K, G, Gp, h in params:
+        uc, lat = kitaev(6, 6, h=h, K=K, Gamma=G, Gammap=Gp)
+        ucs.append(uc)
+        lats.append(lat)
+    s = random_state(lats[0], 17)
+    f = unit_x_field(lats[0])
+    fd = np.einsum("aij,aj->ai", lats[0].frames, f).reshape(-1)
+    B = len(params)
+    T0, T1, dt = -1.0, 2.005, 0.01
+    with GPULattice(lats[0]) as gl:
+        gl.set_damping(0.02)
+        gl.set_pulse(np.concatenate([fd, fd]), 0.5, 0.5, 0.0, 0.0, 0.0)
+        gl.set_replica_hamiltonians(lats)
+        res = gl.integrate_batched(s, np.zeros((B, 2)), np.tile([0.5, 0.0], (B, 1)), T0, T1, dt,
+                                   tgrid="boost", want_final=True)
+        gl.set_replica_hamiltonians(None)
+    for b in range(B):
+        o = po.OracleLattice(ucs[b], 6, 6, 1)
+        o.set_alpha(0.02)
+        o.set_pulse(f, 0.0, f * 0.0, 0.0, 0.5, 0.5, 0.0)
+        ref = o.integrate_md(s.reshape(-1), T0, T1, dt)
+        assert np.max(np.abs(res["mags"][:, b] - ref["mags"])) <= TOL_TRAJ
+        assert np.max(np.abs(res["energy"][:, b] - ref["energy"])) <= TOL_TRAJ
+        assert np.max(np.abs(res["final"][b] - ref["final"].reshape(-1, 3))) <= TOL_TRAJ
+
+
 def test_pump_probe_2dcs_small(po):
     """2DCS driver (M0, M1(tau), M01(tau)) vs the oracle pump_probe_spectroscopy."""
     uc, lat, o = _pair(po, 6, 6, h=0.1)
