@@ -42,27 +42,14 @@ AMP, WIDTH, FREQ = 0.5, 0.5, 0.0
 
 
 def build_workload():
-    from classical_mcmc_spin_amd import model, seeding
-    uc = model.build_kitaev_honeycomb(K=-1.0, Gamma=0.25, Gammap=-0.02, field_strength=0.1,
-                                      field_direction=np.ones(3) / np.sqrt(3.0))
-    lat = model.Lattice(uc, L1, L2, 1)
-    relaxed = seeding.quench(lat, seeding.random_spins(lat.lattice_size, 1234), sweeps=200)
-    init = seeding.tilt(relaxed, 0.01, seed=4321)
-    field_in = np.zeros((lat.N_atoms, 3))
-    field_in[:, 0] = 1.0
-    return uc, lat, init, field_in
+    from classical_mcmc_spin_amd import workloads
+    return workloads.kitaev_2dcs(L1, L2)
 
 
 def replica_tables(n_tau):
+    from classical_mcmc_spin_amd import workloads
     taus = TAU0 + TAU_STEP * np.arange(n_tau)
-    B = 1 + 2 * n_tau
-    tp = np.zeros((B, 2))
-    am = np.zeros((B, 2))
-    am[0, 0] = AMP                                   # M0
-    tp[1:1 + n_tau, 0] = taus                        # M1(tau)
-    am[1:1 + n_tau, 0] = AMP
-    tp[1 + n_tau:, 1] = taus                         # M01(tau)
-    am[1 + n_tau:, :] = AMP
+    tp, am = workloads.pump_probe_replicas(taus, AMP)
     return tp, am, taus
 
 
@@ -105,6 +92,69 @@ def parity_check(uc, lat, gl, init, field_in, n_steps=500):
     return err_m, err_s
 
 
+PYRO_L, PYRO_B, PYRO_STEPS, PYRO_DT = 16, 64, 4096, 0.02
+
+
+def bench_pyrochlore(args):
+    """64 samples x 4096 RK4 steps (dt = 0.02) of the 16^3-cell conventional
+    pyrochlore lattice (65 536 sites, z = 6), all-in-all-out + tilt starts,
+    M(t) and E(t) every step.  The lattice is too large for one workgroup's
+    LDS, so the grid-wide stage kernels run (4 launches per step)."""
+    from classical_mcmc_spin_amd import GPULattice, workloads
+    lat = workloads.pyrochlore_lattice(PYRO_L)
+    N = lat.lattice_size
+    init, _ = workloads.pyrochlore_tilted_aiao(lat, PYRO_B, seed=1234)
+    T1p = (PYRO_STEPS + 0.5) * PYRO_DT
+    zero = np.zeros((PYRO_B, 2))
+    gl = GPULattice(lat)
+
+    def one_step():
+        return gl.integrate_batched(init, zero, zero, 0.0, T1p, PYRO_DT, tgrid="boost")
+
+    for _ in range(args.warmup):
+        one_step()
+    kernel_ms = []
+    t = time.perf_counter()
+    for _ in range(args.steps):
+        res = one_step()
+        kernel_ms.append(gl.last_kernel_ms()[0])
+    el = time.perf_counter() - t
+    assert res["mags"].shape[0] == PYRO_STEPS + 1 and np.all(np.isfinite(res["mags"]))
+    gl.close()
+    kms = float(np.mean(kernel_ms))
+    site_steps = PYRO_B * N * PYRO_STEPS
+    samples_s = PYRO_B * args.steps / el
+    bytes_launch = PYRO_B * N * BYTES_PER_SITE_STEP / 4       # one RK stage of every site
+    line = {
+        "metric": METRIC, "value": samples_s, "unit": "trajectories/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": 1e3 * el / args.steps,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "pyrochlore_16x16x16_conventional_64samples_4096steps_rk4_dt0.02",
+                   "sites": N, "trajectories": PYRO_B, "rk4_steps": PYRO_STEPS, "record_every": 1},
+        "site_steps_per_s": samples_s * N * PYRO_STEPS,
+        "roofline": {"bound": "hbm", "achieved": bytes_launch / (kms * 1e-3) / 1e9,
+                     "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                     "frac": bytes_launch / (kms * 1e-3) / HBM_PEAK, "traffic": None,
+                     "kernel": "k_stage", "kernel_ms": kms, "algorithmic_bytes_per_launch": bytes_launch},
+    }
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import pyoracle
+        o = pyoracle.OracleLattice(lat.unit_cell, PYRO_L, PYRO_L, PYRO_L)
+        n = 60
+        t = time.perf_counter()
+        o.integrate_md(init[0].reshape(-1), 0.0, (n + 0.5) * PYRO_DT, PYRO_DT)
+        el = time.perf_counter() - t
+        line["cpu_baseline"] = {
+            "value": n / el / PYRO_STEPS, "unit": "trajectories/s", "cores": pyoracle.max_threads(),
+            "kind": "port",
+            "sample": f"1 sample x {n} RK4 steps of the same lattice ({el:.1f} s, OpenMP over sites "
+                      f"inside each RHS as the reference's MD path)",
+            "site_steps_per_s": n * N / el}
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -115,6 +165,9 @@ def main():
     ap.add_argument("--block", type=int, default=0)
     ap.add_argument("--launch", default="", help="fixed persistent launch shape 'block,spt,dbuf' "
                                                    "instead of the autotuner's choice")
+    ap.add_argument("--workload", default="2dcs", choices=["2dcs", "pyrochlore"],
+                    help="2dcs: the batched 2DCS workload (default); pyrochlore: 64 samples of the "
+                         "65 536-site pyrochlore lattice (grid-wide stage kernels)")
     ap.add_argument("--profile", action="store_true",
                     help="only the timed batched launches (for rocprofv3 runs): no autotune sweep, "
                          "plain paths, parity check or CPU baseline")
@@ -125,6 +178,8 @@ def main():
               "part of this build.", file=sys.stderr)
         sys.exit(2)
 
+    if args.workload == "pyrochlore":
+        return bench_pyrochlore(args)
     from classical_mcmc_spin_amd import GPULattice
 
     uc, lat, init, field_in = build_workload()

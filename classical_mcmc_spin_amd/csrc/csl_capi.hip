@@ -378,10 +378,11 @@ TuneVal pick_persistent(const csl_lattice* h, size_t B, int block_req) {
     const TuneKey key{h->L.NP, h->L.zmax, h->L.uniform, B >= 64 ? 2 : (B > 1 ? 1 : 0)};
     auto it = g_tune.find(key);
     if (it != g_tune.end()) return it->second;
-    // heuristic: two LDS buffers (one barrier per stage) with the fewest
-    // sites per thread that fit, runtime stage index
-    for (auto& c : cands)
-        if (c.flags == csl::kPersistDoubleBuffer) return c;
+    // heuristic: two LDS buffers (one barrier per stage) and compile-time
+    // stages with the fewest sites per thread that fit
+    for (int want : {csl::kPersistDoubleBuffer | csl::kPersistUnrollStages, csl::kPersistDoubleBuffer})
+        for (auto& c : cands)
+            if (c.flags == want) return c;
     return cands.front();
 }
 
@@ -517,6 +518,9 @@ void run_stages(csl_lattice* h, size_t B, const double* d_tp, const double* d_am
     (void)hipEventDestroy(e1);
     h->last_ms = launches ? ms / double(launches) : 0.0;
     h->last_launches = launches;
+    h->last_block = -1;  // grid-wide stage kernels
+    h->last_spt = 0;
+    h->last_dbuf = 0;
 }
 
 void copy_back(double* host, const double* dev, size_t n, hipStream_t s) {
@@ -908,8 +912,9 @@ int csl_integrate_batched(csl_lattice* h, size_t B, const double* init, int init
         h->bt_amp.upload(am.data(), am.size(), s);
         bool drive_on = false;
         for (double a : am) drive_on |= (a != 0.0);
+        if (block < -1) fail(CSL_ERR_ARG, "block must be >= -1");
         auto cands = persistent_candidates(h);
-        if (!cands.empty()) {
+        if (!cands.empty() && block >= 0) {
             TuneVal tv = pick_persistent(h, B, block);
             run_persistent(h, B, h->bt_init.p, init_per_replica ? nd : 0, h->bt_tp.p, h->bt_amp.p, grid, dt,
                            save_interval, energy != nullptr, final_states != nullptr, tv, nullptr);

@@ -16,16 +16,34 @@ namespace csl {
 
 // grid-wide fused stage kernel (plain path, any lattice size, B in grid.y)
 
+// Workgroups are handed to the 8 XCDs round-robin in launch order; when the
+// grid size is a multiple of 8, XCD x is given the contiguous logical range
+// [x T/8, (x+1) T/8) instead, so the workgroups of one trajectory share one
+// L2 and its neighbour gathers hit there.
+__device__ __forceinline__ void xcd_block(int& bx, int& by) {
+    const int nx = gridDim.x;
+    const int T = nx * gridDim.y;
+    bx = blockIdx.x;
+    by = blockIdx.y;
+    if ((T & 7) == 0) {
+        const int h = bx + by * nx;
+        const int logical = (h & 7) * (T >> 3) + (h >> 3);
+        bx = logical % nx;
+        by = logical / nx;
+    }
+}
+
 template <int ZMAX, bool UNI>
 __global__ void __launch_bounds__(256) k_stage(LatDev L, StageArgs A) {
     __shared__ double red[4 * 10];
     const int NP = L.NP;
-    const int b = blockIdx.y;
+    int bx, b;
+    xcd_block(bx, b);
     const int tid = threadIdx.x;
-    const int p = blockIdx.x * blockDim.x + tid;
+    const int p = bx * blockDim.x + tid;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int lane = tid & (kWave - 1);
-    const int pw = blockIdx.x * blockDim.x + wave * kWave;
+    const int pw = bx * blockDim.x + wave * kWave;
     const int atom = uniform(pw < NP ? pw / L.npad : 0);
     const size_t off = size_t(b) * 3 * NP;
     const double* cur = A.ycur + off;
@@ -54,18 +72,29 @@ __global__ void __launch_bounds__(256) k_stage(LatDev L, StageArgs A) {
         if (drive_on) sub_drive(L, atom, f1, f2, H);
         double k[3];
         ll_su2(H, S, L.alpha, k);
+        // stage 0 reads its input from x itself, so S is x there; stage 3
+        // needs only the accumulator
+        if (s == 0) {
 #pragma unroll
-        for (int d = 0; d < 3; ++d) {
-            const size_t q = size_t(d) * NP + p;
-            const double xd = x[q];
-            double xn = xd;
-            if (s == 0) acc[q] = xd + bdt * k[d];
-            else if (s < 3) acc[q] = acc[q] + bdt * k[d];
-            else {
-                xn = acc[q] + bdt * k[d];
-                x[q] = xn;
+            for (int d = 0; d < 3; ++d) {
+                const size_t q = size_t(d) * NP + p;
+                acc[q] = S[d] + bdt * k[d];
+                nxt[q] = S[d] + cdt * k[d];
             }
-            if (s < 3) nxt[q] = xd + cdt * k[d];
+        } else if (s < 3) {
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+                const size_t q = size_t(d) * NP + p;
+                const double xd = x[q];
+                acc[q] = acc[q] + bdt * k[d];
+                nxt[q] = xd + cdt * k[d];
+            }
+        } else {
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+                const size_t q = size_t(d) * NP + p;
+                x[q] = acc[q] + bdt * k[d];
+            }
         }
     }
     if (A.rec) {
@@ -78,7 +107,7 @@ __global__ void __launch_bounds__(256) k_stage(LatDev L, StageArgs A) {
         if (tid < 10) {
             double v = 0.0;
             for (int w = 0; w < int(blockDim.x >> 6); ++w) v += red[w * 10 + tid];
-            A.partials[(size_t(b) * gridDim.x + blockIdx.x) * 10 + tid] = v;
+            A.partials[(size_t(b) * gridDim.x + bx) * 10 + tid] = v;
         }
     }
 }
@@ -124,19 +153,36 @@ __global__ void __launch_bounds__(256) k_observe(LatDev L, const double* __restr
 }
 
 // one block per trajectory: sum block partials into record ri
-__global__ void __launch_bounds__(64) k_reduce_obs(const double* __restrict__ partials, int nblk,
-                                                   int B, int ri, double inv_n,
-                                                   double* __restrict__ mags,
-                                                   double* __restrict__ energy) {
+__global__ void __launch_bounds__(256) k_reduce_obs(const double* __restrict__ partials, int nblk,
+                                                    int B, int ri, double inv_n,
+                                                    double* __restrict__ mags,
+                                                    double* __restrict__ energy) {
+    // one workgroup per trajectory: strided partial sums per thread, wave64
+    // butterfly sums, then the 4 wave totals in a fixed order
+    __shared__ double red[10][4];
     const int b = blockIdx.x;
-    const int j = threadIdx.x;
-    if (j >= 10) return;
-    double v = 0.0;
-    for (int k = 0; k < nblk; ++k) v += partials[(size_t(b) * nblk + k) * 10 + j];
-    if (j < 9) {
-        if (mags) mags[(size_t(ri) * B + b) * 9 + j] = v * inv_n;
-    } else if (energy) {
-        energy[size_t(ri) * B + b] = v * inv_n;
+    const int t = threadIdx.x;
+    double v[10];
+#pragma unroll
+    for (int j = 0; j < 10; ++j) v[j] = 0.0;
+    for (int k = t; k < nblk; k += 256) {
+        const double* q = partials + (size_t(b) * nblk + k) * 10;
+#pragma unroll
+        for (int j = 0; j < 10; ++j) v[j] += q[j];
+    }
+#pragma unroll
+    for (int j = 0; j < 10; ++j) v[j] = wave_sum(v[j]);
+    if ((t & (kWave - 1)) == 0)
+#pragma unroll
+        for (int j = 0; j < 10; ++j) red[j][t >> 6] = v[j];
+    __syncthreads();
+    if (t < 10) {
+        const double tot = (red[t][0] + red[t][1]) + (red[t][2] + red[t][3]);
+        if (t < 9) {
+            if (mags) mags[(size_t(ri) * B + b) * 9 + t] = tot * inv_n;
+        } else if (energy) {
+            energy[size_t(ri) * B + b] = tot * inv_n;
+        }
     }
 }
 
@@ -335,7 +381,7 @@ hipError_t launch_observe(const LatDev& L, const double* y, double* partials, in
 
 hipError_t launch_reduce_obs(const double* partials, int nblk, int B, int ri, double inv_n,
                              double* mags, double* energy, hipStream_t st) {
-    hipLaunchKernelGGL(k_reduce_obs, dim3(B), dim3(64), 0, st, partials, nblk, B, ri, inv_n, mags,
+    hipLaunchKernelGGL(k_reduce_obs, dim3(B), dim3(256), 0, st, partials, nblk, B, ri, inv_n, mags,
                        energy);
     return hipGetLastError();
 }

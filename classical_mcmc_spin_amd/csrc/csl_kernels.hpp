@@ -33,7 +33,7 @@ struct StageArgs {
     int stage;               // 0..3
     double* x;               // state at step start (updated in stage 3)
     double* acc;             // RK4 accumulator
-    const double* ycur;      // stage input
+    const double* ycur;      // stage input (must be x in stage 0)
     double* ynext;           // next stage input (unused in stage 3)
     const double* fac;       // [B*2] drive factors of this stage, or null (no drive)
     double dt;

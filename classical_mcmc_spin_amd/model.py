@@ -143,6 +143,96 @@ def build_kitaev_honeycomb(K=-1.0, Gamma=0.25, Gammap=-0.02, J=0.0, field_streng
     return uc
 
 
+_PYRO_POS = np.array([[0.125, 0.125, 0.125], [0.125, -0.125, -0.125],
+                      [-0.125, 0.125, -0.125], [-0.125, -0.125, 0.125]])
+_FCC = np.array([[0.0, 0.0, 0.0], [0.0, 0.5, 0.5], [0.5, 0.0, 0.5], [0.5, 0.5, 0.0]])
+# bond angle theta of each sublattice pair: the "Jz" (0), "Jx" (2pi/3) and
+# "Jy" (4pi/3) bond types of build_pyrochlore_non_kramer
+# (unitcell_builders.cpp:420-433)
+_PYRO_THETA = {(0, 1): 0.0, (0, 2): 2 * math.pi / 3, (0, 3): 4 * math.pi / 3,
+               (1, 2): 4 * math.pi / 3, (1, 3): 2 * math.pi / 3, (2, 3): 0.0}
+
+
+def pyrochlore_frames() -> np.ndarray:
+    """Local frames of the four pyrochlore sublattices, columns (x_a, y_a, z_a)
+    with z_a along the local <111> axis (unitcell.h:322-351)."""
+    s2, s3, s6 = math.sqrt(2.0), math.sqrt(3.0), math.sqrt(6.0)
+    z = np.array([[1, 1, 1], [1, -1, -1], [-1, 1, -1], [-1, -1, 1]]) / s3
+    y = np.array([[0, -1, 1], [0, 1, -1], [0, -1, -1], [0, 1, 1]]) / s2
+    x = np.array([[-2, 1, 1], [-2, -1, -1], [2, 1, -1], [2, -1, 1]]) / s6
+    return np.stack([np.stack([x[a], y[a], z[a]], axis=1) for a in range(4)])
+
+
+class Pyrochlore(UnitCell):
+    """Primitive 4-site FCC pyrochlore cell with its local frames
+    (unitcell.h:312-356)."""
+
+    def __init__(self, spin_dim: int = 3):
+        super().__init__(spin_dim, 4, _PYRO_POS, [[0, 0.5, 0.5], [0.5, 0, 0.5], [0.5, 0.5, 0]])
+        if spin_dim == 3:
+            for a, R in enumerate(pyrochlore_frames()):
+                self.set_sublattice_frame(R, a)
+
+
+def pyrochlore_local_exchange(theta: float, Jzz: float, Jpm: float, Jpmpm: float,
+                              Jzpm: float = 0.0) -> np.ndarray:
+    """Nearest-neighbour exchange in the two sites' local frames,
+        Jzz s^z s^z - Jpm (s+ s- + h.c.) + Jpmpm (g s+ s+ + h.c.)
+          + Jzpm (s^z (z s+ + z* s-) + i<->j),   g = exp(i theta), z = -g*,
+    written as a 3x3 matrix on (s^x, s^y, s^z).  With Jzpm = 0 it is the
+    J(theta) of build_pyrochlore_non_kramer (unitcell_builders.cpp:406-413);
+    the Jzpm column/row is this build's extension for the Jzz/J+-/J+-+-/Jz+-
+    model."""
+    c, s = math.cos(theta), math.sin(theta)
+    return np.array([[-2 * Jpm + 2 * Jpmpm * c, -2 * Jpmpm * s, -2 * Jzpm * c],
+                     [-2 * Jpmpm * s, -2 * Jpm - 2 * Jpmpm * c, -2 * Jzpm * s],
+                     [-2 * Jzpm * c, -2 * Jzpm * s, Jzz]])
+
+
+def build_pyrochlore_conventional(Jzz=1.0, Jpm=0.05, Jpmpm=0.02, Jzpm=-0.01,
+                                  field_strength=0.0, field_direction=(0.0, 0.0, 1.0)) -> UnitCell:
+    """16-site conventional cubic pyrochlore cell (lattice constant 1) with
+    global-frame nearest-neighbour bonds.
+
+    Sites are the 4 FCC translations x the 4 tetrahedron sublattices of
+    unitcell.h:312-320 (atom = 4*fcc + sublattice); each site has 6 nearest
+    neighbours at distance sqrt(2)/4.  Spins are stored in the global frame,
+    so a bond a->b carries J_ab = R_a J_loc(theta_ab) R_b^T with R the
+    sublattice frames and J_loc from `pyrochlore_local_exchange`; frames of
+    the cell are the identity (observables are global-frame sums).  The
+    reference only builds the primitive 4-site cell; an L^3 supercell of this
+    cell is the cubic L x L x L lattice of 16 L^3 sites.  The field is
+    h*direction on every site (direction used as given)."""
+    R = pyrochlore_frames()
+    pos = np.array([_FCC[f] + _PYRO_POS[a] for f in range(4) for a in range(4)])
+    uc = UnitCell(3, 16, pos, np.eye(3))
+    dnn = math.sqrt(2.0) / 4.0
+    offsets = [(i, j, k) for i in (-1, 0, 1) for j in (-1, 0, 1) for k in (-1, 0, 1)]
+    for s in range(16):
+        for t in range(s + 1, 16):
+            a, b = s % 4, t % 4
+            for o in offsets:
+                d = pos[t] + np.asarray(o, float) - pos[s]
+                if abs(np.linalg.norm(d) - dnn) > 1e-9:
+                    continue
+                key = (min(a, b), max(a, b))
+                Jl = pyrochlore_local_exchange(_PYRO_THETA[key], Jzz, Jpm, Jpmpm, Jzpm)
+                uc.set_bilinear_interaction(R[a] @ Jl @ R[b].T, s, t, o)
+    d = np.asarray(field_direction, dtype=float)
+    for s in range(16):
+        uc.set_field(field_strength * d, s)
+    return uc
+
+
+def all_in_all_out(lat: "Lattice") -> np.ndarray:
+    """All-in-all-out state of a pyrochlore lattice built from
+    `build_pyrochlore_conventional`: every spin along its sublattice's local
+    z axis (pointing out of the up tetrahedra)."""
+    z = pyrochlore_frames()[:, :, 2]
+    sub = (np.arange(lat.lattice_size) % lat.N_atoms) % 4
+    return z[sub] * lat.spin_length
+
+
 class Lattice:
     """Periodic L1 x L2 x L3 expansion of a unit cell with the reference's
     flat bond tables (lattice.h:601-903).

@@ -156,7 +156,9 @@ int csl_rhs(csl_lattice* h, double t, double* dsdt, size_t n);
  *                  [max_rec*B] (E/N, may be NULL); final_states [B*N*D]
  *                  (may be NULL); times [max_rec].
  * method "rk4" only (the reference falls back to rk4, lattice_gpu.cu:781-784).
- * block: workgroup size (0 = autotuned). */
+ * block: workgroup size of the one-workgroup-per-trajectory kernel
+ *        (0 = autotuned); -1 selects the grid-wide stage kernels (the path
+ *        lattices too large for one workgroup's LDS always take). */
 int csl_integrate_batched(csl_lattice* h, size_t B, const double* init,
                           int init_per_replica, const double* t_pulse,
                           const double* pulse_amp, double T0, double T1, double dt,
@@ -213,10 +215,11 @@ int csl_autotune_batched(csl_lattice* h, size_t B, size_t n_probe_steps,
  * HIP events on the handle's stream, and the number of launches it spans. */
 int csl_last_kernel_ms(csl_lattice* h, double* ms, size_t* launches);
 
-/* Launch shape of the last persistent-kernel run on this handle: workgroup
- * size, sites per thread and flags (bit 0: two LDS state buffers, one
- * barrier per RK stage; bit 1: the four RK stages specialised at compile
- * time). */
+/* Launch shape of the last integration on this handle: workgroup size,
+ * sites per thread and flags of the one-workgroup-per-trajectory kernel
+ * (flags bit 0: two LDS state buffers, one barrier per RK stage; bit 1: the
+ * four RK stages specialised at compile time), or block = -1 (sites per
+ * thread and flags 0) after the grid-wide stage kernels. */
 int csl_last_launch_config(csl_lattice* h, int* block, int* sites_per_thread, int* flags);
 
 /* Fix the persistent launch shape used for this lattice shape and batch size

@@ -779,3 +779,406 @@ size_t orc_tangent_md(OracleLattice* L, double* state, double* dstate, const dou
 }
 
 }  // extern "C"
+
+/* ------------------------------------------------------------------------
+ * Mixed SU(2) + SU(3) lattice (MixedLattice, mixed_lattice.h / _md.cpp).
+ * State [N2*3 | N3*8] (mixed_lattice.h:3878-3900).  Supported terms: fields,
+ * onsite 2A, SU(2)-SU(2) / SU(3)-SU(3) bilinears, SU(2)-SU(3) bilinears (J at
+ * the SU(2) site, J^T at the SU(3) site), two-pulse drives per species,
+ * SU(2) Gilbert damping and SU(3) Bloch damping.
+ * ---------------------------------------------------------------------- */
+struct MixedOracle {
+    OracleLattice* s2 = nullptr;
+    OracleLattice* s3 = nullptr;
+    size_t N2 = 0, N3 = 0;
+    std::vector<size_t> m2_off, m2_par, m3_off, m3_par;
+    std::vector<double> m2_J, m3_J;  // [nnz][3][8], [nnz][8][3]
+    double alpha = 0.0;
+    double gamma[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    std::vector<double> eq;          // [N3*8]
+};
+
+namespace {
+
+// build_mixed_interactions, bilinear part (mixed_lattice.h:773-800)
+void mixed_bonds(MixedOracle& M, int d1, int d2, int d3, int A2, int A3, int nb, const int* src,
+                 const int* par, const int* off, const double* J) {
+    std::vector<std::vector<std::pair<size_t, const double*>>> b2(M.N2), b3(M.N3);
+    for (int i = 0; i < d1; ++i)
+        for (int j = 0; j < d2; ++j)
+            for (int k = 0; k < d3; ++k)
+                for (int atom = 0; atom < A2; ++atom) {
+                    const size_t s = ((size_t(i) * d2 + j) * d3 + k) * A2 + atom;
+                    for (int b = 0; b < nb; ++b) {
+                        if (src[b] != atom) continue;
+                        const int pi = wrap(i + off[3 * b], d1), pj = wrap(j + off[3 * b + 1], d2),
+                                  pk = wrap(k + off[3 * b + 2], d3);
+                        const size_t p = ((size_t(pi) * d2 + pj) * d3 + pk) * A3 + par[b];
+                        b2[s].push_back({p, J + b * 24});
+                        b3[p].push_back({s, J + b * 24});
+                    }
+                }
+    M.m2_off.assign(M.N2 + 1, 0);
+    M.m3_off.assign(M.N3 + 1, 0);
+    for (size_t s = 0; s < M.N2; ++s) {
+        M.m2_off[s + 1] = M.m2_off[s] + b2[s].size();
+        for (auto& e : b2[s]) {
+            M.m2_par.push_back(e.first);
+            M.m2_J.insert(M.m2_J.end(), e.second, e.second + 24);
+        }
+    }
+    for (size_t s = 0; s < M.N3; ++s) {
+        M.m3_off[s + 1] = M.m3_off[s] + b3[s].size();
+        for (auto& e : b3[s]) {
+            M.m3_par.push_back(e.first);
+            for (int r = 0; r < 8; ++r)  // J^T (8x3)
+                for (int c = 0; c < 3; ++c) M.m3_J.push_back(e.second[c * 8 + r]);
+        }
+    }
+}
+
+// get_local_field_SU2_flat_into without trilinears / assisted bonds
+// (mixed_lattice_md.cpp:716-879); no drive.
+void mixed_field_su2(const MixedOracle& M, const double* x, size_t site, double* H) {
+    const OracleLattice& L = *M.s2;
+    const double* S = x + site * 3;
+    const double* x3 = x + M.N2 * 3;
+    for (int a = 0; a < 3; ++a) H[a] = -L.field[site * 3 + a];
+    const double* A = &L.onsite[site * 9];
+    for (int a = 0; a < 3; ++a) {
+        double acc = 0.0;
+        for (int b = 0; b < 3; ++b) acc += A[a * 3 + b] * S[b];
+        H[a] += 2.0 * acc;
+    }
+    for (size_t k = L.bi_offset[site]; k < L.bi_offset[site + 1]; ++k) {
+        const double* Sp = x + L.bi_partner[k] * 3;
+        const double* J = &L.bi_J[k * 9];
+        for (int a = 0; a < 3; ++a) {
+            double acc = 0.0;
+            for (int b = 0; b < 3; ++b) acc += J[a * 3 + b] * Sp[b];
+            H[a] += acc;
+        }
+    }
+    for (size_t k = M.m2_off[site]; k < M.m2_off[site + 1]; ++k) {
+        const double* lp = x3 + M.m2_par[k] * 8;
+        const double* J = &M.m2_J[k * 24];
+        for (int a = 0; a < 3; ++a) {
+            double acc = 0.0;
+            for (int c = 0; c < 8; ++c) acc += J[a * 8 + c] * lp[c];
+            H[a] += acc;
+        }
+    }
+}
+
+// get_local_field_SU3_flat_into (mixed_lattice_md.cpp:967-1105); no drive.
+void mixed_field_su3(const MixedOracle& M, const double* x, size_t site, double* H) {
+    const OracleLattice& L = *M.s3;
+    const double* x3 = x + M.N2 * 3;
+    const double* S = x3 + site * 8;
+    for (int a = 0; a < 8; ++a) H[a] = -L.field[site * 8 + a];
+    const double* A = &L.onsite[site * 64];
+    for (int a = 0; a < 8; ++a) {
+        double acc = 0.0;
+        for (int b = 0; b < 8; ++b) acc += A[a * 8 + b] * S[b];
+        H[a] += 2.0 * acc;
+    }
+    for (size_t k = L.bi_offset[site]; k < L.bi_offset[site + 1]; ++k) {
+        const double* Sp = x3 + L.bi_partner[k] * 8;
+        const double* J = &L.bi_J[k * 64];
+        for (int a = 0; a < 8; ++a) {
+            double acc = 0.0;
+            for (int b = 0; b < 8; ++b) acc += J[a * 8 + b] * Sp[b];
+            H[a] += acc;
+        }
+    }
+    for (size_t k = M.m3_off[site]; k < M.m3_off[site + 1]; ++k) {
+        const double* Sp = x + M.m3_par[k] * 3;
+        const double* J = &M.m3_J[k * 24];
+        for (int a = 0; a < 8; ++a) {
+            double acc = 0.0;
+            for (int b = 0; b < 3; ++b) acc += J[a * 3 + b] * Sp[b];
+            H[a] += acc;
+        }
+    }
+}
+
+// MixedLattice::landau_lifshitz, both species standard (mixed_lattice_md.cpp:433-605)
+void mixed_rhs(const MixedOracle& M, const double* x, double* dxdt, double t) {
+    const OracleLattice& L2 = *M.s2;
+    const OracleLattice& L3 = *M.s3;
+    double f21 = 0.0, f22 = 0.0, f31 = 0.0, f32 = 0.0;
+    if (L2.amp != 0.0) drive_factors(L2, t, f21, f22);   // drive_envelopes_SU2 (:265-286)
+    if (L3.amp != 0.0) drive_factors(L3, t, f31, f32);   // drive_envelopes_SU3 (:288-313)
+    const long n2 = long(M.N2), n3 = long(M.N3);
+#ifdef _OPENMP
+#pragma omp parallel if (n2 + n3 >= 64)
+#endif
+    {
+#ifdef _OPENMP
+#pragma omp for schedule(static) nowait
+#endif
+        for (long si = 0; si < n2; ++si) {
+            const size_t s = size_t(si);
+            double H[3];
+            mixed_field_su2(M, x, s, H);
+            if (f21 != 0.0 || f22 != 0.0) {
+                const size_t atom = s % L2.n_atoms;
+                const double* fd0 = &L2.field_drive[0][atom * 3];
+                const double* fd1 = &L2.field_drive[1][atom * 3];
+                for (int a = 0; a < 3; ++a) H[a] -= fd0[a] * f21 + fd1[a] * f22;
+            }
+            const double* S = x + s * 3;
+            double* o = dxdt + s * 3;
+            o[0] = H[1] * S[2] - H[2] * S[1];
+            o[1] = H[2] * S[0] - H[0] * S[2];
+            o[2] = H[0] * S[1] - H[1] * S[0];
+            if (M.alpha != 0.0) {  // (:570-578)
+                const double S2 = S[0] * S[0] + S[1] * S[1] + S[2] * S[2];
+                const double SdotH = S[0] * H[0] + S[1] * H[1] + S[2] * H[2];
+                const double inv_S = (S2 > 0.0) ? M.alpha / std::sqrt(S2) : 0.0;
+                for (int a = 0; a < 3; ++a) o[a] += inv_S * (S[a] * SdotH - H[a] * S2);
+            }
+        }
+#ifdef _OPENMP
+#pragma omp for schedule(static)
+#endif
+        for (long si = 0; si < n3; ++si) {
+            const size_t s = size_t(si);
+            double H[8];
+            mixed_field_su3(M, x, s, H);
+            if (f31 != 0.0 || f32 != 0.0) {
+                const size_t atom = s % L3.n_atoms;
+                const double* fd0 = &L3.field_drive[0][atom * 8];
+                const double* fd1 = &L3.field_drive[1][atom * 8];
+                for (int a = 0; a < 8; ++a) H[a] -= fd0[a] * f31 + fd1[a] * f32;
+            }
+            const double* lam = x + M.N2 * 3 + s * 8;
+            double* o = dxdt + M.N2 * 3 + s * 8;
+            su3_cross(H, lam, o);
+            for (int a = 0; a < 8; ++a)  // Bloch damping (:596-605)
+                if (M.gamma[a] != 0.0) o[a] -= M.gamma[a] * (lam[a] - M.eq[s * 8 + a]);
+        }
+    }
+}
+
+// total_energy_flat without trilinears (mixed_lattice.h:1783-1900)
+double mixed_energy(const MixedOracle& M, const double* x) {
+    const OracleLattice& L2 = *M.s2;
+    const OracleLattice& L3 = *M.s3;
+    const double* x3 = x + M.N2 * 3;
+    double e = 0.0;
+    for (size_t i = 0; i < M.N2; ++i) {
+        const double* s = x + i * 3;
+        for (int d = 0; d < 3; ++d) e -= s[d] * L2.field[i * 3 + d];
+        for (int a = 0; a < 3; ++a)
+            for (int b = 0; b < 3; ++b) e += s[a] * L2.onsite[i * 9 + a * 3 + b] * s[b];
+        for (size_t k = L2.bi_offset[i]; k < L2.bi_offset[i + 1]; ++k) {
+            const double* p = x + L2.bi_partner[k] * 3;
+            for (int a = 0; a < 3; ++a)
+                for (int b = 0; b < 3; ++b) e += 0.5 * s[a] * L2.bi_J[k * 9 + a * 3 + b] * p[b];
+        }
+        for (size_t k = M.m2_off[i]; k < M.m2_off[i + 1]; ++k) {
+            const double* p = x3 + M.m2_par[k] * 8;
+            for (int a = 0; a < 3; ++a)
+                for (int b = 0; b < 8; ++b) e += 0.5 * s[a] * M.m2_J[k * 24 + a * 8 + b] * p[b];
+        }
+    }
+    for (size_t i = 0; i < M.N3; ++i) {
+        const double* s = x3 + i * 8;
+        for (int d = 0; d < 8; ++d) e -= s[d] * L3.field[i * 8 + d];
+        for (int a = 0; a < 8; ++a)
+            for (int b = 0; b < 8; ++b) e += s[a] * L3.onsite[i * 64 + a * 8 + b] * s[b];
+        for (size_t k = L3.bi_offset[i]; k < L3.bi_offset[i + 1]; ++k) {
+            const double* p = x3 + L3.bi_partner[k] * 8;
+            for (int a = 0; a < 8; ++a)
+                for (int b = 0; b < 8; ++b) e += 0.5 * s[a] * L3.bi_J[k * 64 + a * 8 + b] * p[b];
+        }
+        for (size_t k = M.m3_off[i]; k < M.m3_off[i + 1]; ++k) {
+            const double* p = x + M.m3_par[k] * 3;
+            for (int a = 0; a < 8; ++a)
+                for (int b = 0; b < 3; ++b) e += 0.5 * s[a] * M.m3_J[k * 24 + a * 3 + b] * p[b];
+        }
+    }
+    return e;
+}
+
+// MD observer record (mixed_lattice_md.cpp:1249-1281):
+// [M2_anti, M2_local, M2_global (3 each), M3_anti, M3_local, M3_global (8 each), E/(N2+N3)]
+void mixed_record(const MixedOracle& M, const double* x, double* out) {
+    const OracleLattice& L2 = *M.s2;
+    const OracleLattice& L3 = *M.s3;
+    double ml2[3] = {0, 0, 0}, ma2[3] = {0, 0, 0}, mg2[3] = {0, 0, 0};
+    for (size_t i = 0; i < M.N2; ++i)  // compute_sublattice_magnetizations_from_flat
+        for (int d = 0; d < 3; ++d) ml2[d] += x[i * 3 + d];
+    for (size_t i = 0; i < M.N2; ++i) {  // compute_magnetization_global_SU2_from_flat
+        const double* R = &L2.frames[(i % L2.n_atoms) * 9];
+        for (int mu = 0; mu < 3; ++mu)
+            for (int nu = 0; nu < 3; ++nu) mg2[mu] += R[mu * 3 + nu] * x[i * 3 + nu];
+    }
+    for (size_t i = 0; i < M.N2; ++i) {  // compute_magnetization_staggered_SU2_from_flat
+        const size_t atom = i % L2.n_atoms;
+        const double* R = &L2.frames[atom * 9];
+        const double sg = L2.afm[atom];
+        for (int mu = 0; mu < 3; ++mu)
+            for (int nu = 0; nu < 3; ++nu) ma2[mu] += sg * R[mu * 3 + nu] * x[i * 3 + nu];
+    }
+    const double* x3 = x + M.N2 * 3;
+    double ml3[8] = {0}, ma3[8] = {0}, mg3[8] = {0};
+    for (size_t i = 0; i < M.N3; ++i) {
+        const double sg = (i % 2 == 0) ? 1.0 : -1.0;
+        for (int d = 0; d < 8; ++d) {
+            ml3[d] += x3[i * 8 + d];
+            ma3[d] += x3[i * 8 + d] * sg;
+        }
+    }
+    for (size_t i = 0; i < M.N3; ++i) {  // compute_magnetization_global_SU3_from_flat
+        const double* R = &L3.frames[(i % L3.n_atoms) * 64];
+        for (int mu = 0; mu < 8; ++mu)
+            for (int nu = 0; nu < 8; ++nu) mg3[mu] += R[mu * 8 + nu] * x3[i * 8 + nu];
+    }
+    const double n2 = double(M.N2), n3 = double(M.N3);
+    for (int d = 0; d < 3; ++d) {
+        out[d] = ma2[d] / n2;
+        out[3 + d] = ml2[d] / n2;
+        out[6 + d] = mg2[d] / n2;
+    }
+    for (int d = 0; d < 8; ++d) {
+        out[9 + d] = ma3[d] / n3;
+        out[17 + d] = ml3[d] / n3;
+        out[25 + d] = mg3[d] / n3;
+    }
+    out[33] = mixed_energy(M, x) / (n2 + n3);
+}
+
+}  // namespace
+
+extern "C" {
+
+MixedOracle* orc_mixed_new(int d1, int d2, int d3, int A2, const double* field2, const double* onsite2,
+                           const double* frames2, const double* afm2, int nb2, const int* src2,
+                           const int* par2, const int* off2, const double* J2, int A3,
+                           const double* field3, const double* onsite3, const double* frames3, int nb3,
+                           const int* src3, const int* par3, const int* off3, const double* J3, int nbm,
+                           const int* srcm, const int* parm, const int* offm, const double* Jm) {
+    for (int b = 0; b < nbm; ++b)
+        for (int a = 0; a < 3; ++a) {
+            const int L = a == 0 ? d1 : (a == 1 ? d2 : d3);
+            if (offm[3 * b + a] < -L || offm[3 * b + a] > L) return nullptr;
+        }
+    std::vector<double> afm3(A3, 1.0);
+    OracleLattice* l2 = orc_lattice_new(3, A2, d1, d2, d3, nb2, src2, par2, off2, J2, field2, onsite2,
+                                        frames2, afm2);
+    OracleLattice* l3 = orc_lattice_new(8, A3, d1, d2, d3, nb3, src3, par3, off3, J3, field3, onsite3,
+                                        frames3, afm3.data());
+    if (!l2 || !l3) {
+        orc_lattice_free(l2);
+        orc_lattice_free(l3);
+        return nullptr;
+    }
+    auto* M = new MixedOracle();
+    M->s2 = l2;
+    M->s3 = l3;
+    M->N2 = l2->N;
+    M->N3 = l3->N;
+    M->eq.assign(M->N3 * 8, 0.0);
+    mixed_bonds(*M, d1, d2, d3, A2, A3, nbm, srcm, parm, offm, Jm);
+    return M;
+}
+
+void orc_mixed_free(MixedOracle* M) {
+    if (!M) return;
+    orc_lattice_free(M->s2);
+    orc_lattice_free(M->s3);
+    delete M;
+}
+
+void orc_mixed_sizes(const MixedOracle* M, size_t* N2, size_t* N3, size_t* nnz22, size_t* nnz33,
+                     size_t* nnz23) {
+    *N2 = M->N2;
+    *N3 = M->N3;
+    *nnz22 = M->s2->bi_offset[M->N2];
+    *nnz33 = M->s3->bi_offset[M->N3];
+    *nnz23 = M->m2_off[M->N2];
+}
+
+void orc_mixed_tables(const MixedOracle* M, size_t* off22, size_t* par22, double* J22, size_t* off33,
+                      size_t* par33, double* J33, size_t* off23, size_t* par23, double* J23, size_t* off32,
+                      size_t* par32, double* J32) {
+    std::copy(M->s2->bi_offset.begin(), M->s2->bi_offset.end(), off22);
+    std::copy(M->s2->bi_partner.begin(), M->s2->bi_partner.end(), par22);
+    std::copy(M->s2->bi_J.begin(), M->s2->bi_J.end(), J22);
+    std::copy(M->s3->bi_offset.begin(), M->s3->bi_offset.end(), off33);
+    std::copy(M->s3->bi_partner.begin(), M->s3->bi_partner.end(), par33);
+    std::copy(M->s3->bi_J.begin(), M->s3->bi_J.end(), J33);
+    std::copy(M->m2_off.begin(), M->m2_off.end(), off23);
+    std::copy(M->m2_par.begin(), M->m2_par.end(), par23);
+    std::copy(M->m2_J.begin(), M->m2_J.end(), J23);
+    std::copy(M->m3_off.begin(), M->m3_off.end(), off32);
+    std::copy(M->m3_par.begin(), M->m3_par.end(), par32);
+    std::copy(M->m3_J.begin(), M->m3_J.end(), J32);
+}
+
+void orc_mixed_set_damping(MixedOracle* M, double alpha, const double* gamma8, const double* eq) {
+    M->alpha = alpha;
+    for (int a = 0; a < 8; ++a) M->gamma[a] = gamma8 ? gamma8[a] : 0.0;
+    if (eq) std::copy(eq, eq + M->N3 * 8, M->eq.begin());
+    else std::fill(M->eq.begin(), M->eq.end(), 0.0);
+}
+
+// set_pulse_SU2 / set_pulse_SU3 (mixed_lattice_md.cpp:114-263): drive
+// directions are F^T * B_global per atom.
+void orc_mixed_set_pulse(MixedOracle* M, int species, const double* field_in1, double t1,
+                         const double* field_in2, double t2, double amp, double width, double freq) {
+    OracleLattice& L = species == 3 ? *M->s3 : *M->s2;
+    const int D = L.D;
+    for (size_t a = 0; a < L.n_atoms; ++a) {
+        const double* R = &L.frames[a * D * D];
+        for (int mu = 0; mu < D; ++mu) {
+            double v1 = 0.0, v2 = 0.0;
+            for (int nu = 0; nu < D; ++nu) {
+                v1 += R[nu * D + mu] * field_in1[a * D + nu];
+                v2 += R[nu * D + mu] * field_in2[a * D + nu];
+            }
+            L.field_drive[0][a * D + mu] = v1;
+            L.field_drive[1][a * D + mu] = v2;
+        }
+    }
+    L.t_pulse[0] = t1;
+    L.t_pulse[1] = t2;
+    L.amp = amp;
+    L.width = width;
+    L.freq = freq;
+}
+
+void orc_mixed_local_field(const MixedOracle* M, const double* x, int species, size_t site, double* H) {
+    if (species == 3) mixed_field_su3(*M, x, site, H);
+    else mixed_field_su2(*M, x, site, H);
+}
+
+void orc_mixed_rhs(const MixedOracle* M, const double* x, double* dxdt, double t) { mixed_rhs(*M, x, dxdt, t); }
+
+double orc_mixed_energy(const MixedOracle* M, const double* x) { return mixed_energy(*M, x); }
+
+void orc_mixed_record(const MixedOracle* M, const double* x, double* out34) { mixed_record(*M, x, out34); }
+
+// Boost runge_kutta4 + integrate_const over the flat mixed state, one
+// record (34 doubles) per observed time.
+size_t orc_mixed_integrate_md(const MixedOracle* M, double* state, double T0, double T1, double dt,
+                              size_t max_rec, double* times, double* recs) {
+    const size_t n = M->N2 * 3 + M->N3 * 8;
+    RK4Work w(n);
+    size_t nrec = 0;
+    auto obs = [&](double t) {
+        if (nrec < max_rec) {
+            if (times) times[nrec] = t;
+            if (recs) mixed_record(*M, state, recs + nrec * 34);
+        }
+        ++nrec;
+    };
+    auto F = [&](const double* x, double* dx, double t) { mixed_rhs(*M, x, dx, t); };
+    auto stp = [&](double t, double h) { rk4_step_generic(F, state, n, t, h, w); };
+    integrate_const(stp, T0, T1, dt, obs);
+    return nrec;
+}
+
+}  // extern "C"

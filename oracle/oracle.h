@@ -119,6 +119,45 @@ size_t orc_tangent_md(OracleLattice* L, double* state, double* dstate,
                       double T0, double T1, double dt, size_t max_rec,
                       double* mags, double* dmags);
 
+/* ---- mixed SU(2) + SU(3) lattice (MixedLattice) -------------------------
+ * Expansion: build_sublattice per species (mixed_lattice.h:591-700) and the
+ * SU(2)-SU(3) bilinears (mixed_lattice.h:773-800: J [3x8] at the SU(2)
+ * source, J^T at the SU(3) partner).  State [N2*3 | N3*8]. */
+typedef struct MixedOracle MixedOracle;
+MixedOracle* orc_mixed_new(int d1, int d2, int d3,
+                           int A2, const double* field2, const double* onsite2,
+                           const double* frames2, const double* afm2,
+                           int nb2, const int* src2, const int* par2, const int* off2, const double* J2,
+                           int A3, const double* field3, const double* onsite3, const double* frames3,
+                           int nb3, const int* src3, const int* par3, const int* off3, const double* J3,
+                           int nbm, const int* srcm, const int* parm, const int* offm, const double* Jm);
+void   orc_mixed_free(MixedOracle* M);
+void   orc_mixed_sizes(const MixedOracle* M, size_t* N2, size_t* N3, size_t* nnz22,
+                       size_t* nnz33, size_t* nnz23);
+void   orc_mixed_tables(const MixedOracle* M, size_t* off22, size_t* par22, double* J22,
+                        size_t* off33, size_t* par33, double* J33, size_t* off23,
+                        size_t* par23, double* J23, size_t* off32, size_t* par32, double* J32);
+/* SU(2) Gilbert alpha (mixed_lattice_md.cpp:570-578), SU(3) Bloch rates
+ * gamma[8] towards eq [N3*8] (:596-605; NULL = zeros). */
+void   orc_mixed_set_damping(MixedOracle* M, double alpha, const double* gamma8, const double* eq);
+/* set_pulse_SU2 / set_pulse_SU3 (mixed_lattice_md.cpp:114-263; F^T B). */
+void   orc_mixed_set_pulse(MixedOracle* M, int species, const double* field_in1, double t1,
+                           const double* field_in2, double t2, double amp, double width,
+                           double freq);
+/* get_local_field_SU2/SU3_flat_into without drive (mixed_lattice_md.cpp:716-1105). */
+void   orc_mixed_local_field(const MixedOracle* M, const double* x, int species, size_t site,
+                             double* H);
+/* MixedLattice::landau_lifshitz (mixed_lattice_md.cpp:433-605). */
+void   orc_mixed_rhs(const MixedOracle* M, const double* x, double* dxdt, double t);
+/* total_energy_flat (mixed_lattice.h:1783-1900). */
+double orc_mixed_energy(const MixedOracle* M, const double* x);
+/* MD observer record (mixed_lattice_md.cpp:1249-1281): [M2_anti, M2_local,
+ * M2_global, M3_anti, M3_local, M3_global, E/(N2+N3)] = 34 doubles. */
+void   orc_mixed_record(const MixedOracle* M, const double* x, double* out34);
+/* Boost RK4 + integrate_const over [T0, T1]; records every step. */
+size_t orc_mixed_integrate_md(const MixedOracle* M, double* state, double T0, double T1,
+                              double dt, size_t max_rec, double* times, double* recs);
+
 #ifdef __cplusplus
 }
 #endif

@@ -44,6 +44,18 @@ _SIG = {
     "orc_build_pulse_segments": (_sz, [_d, _d, _pd, _sz, _d, _d, _d, _sz, _pd]),
     "orc_max_threads": (_i, []),
     "orc_tangent_md": (_sz, [_vp, _pd, _pd, _pd, _pd, _d, _d, _d, _sz, _pd, _pd]),
+    "orc_mixed_new": (_vp, [_i, _i, _i, _i, _pd, _pd, _pd, _pd, _i, _pi, _pi, _pi, _pd,
+                            _i, _pd, _pd, _pd, _i, _pi, _pi, _pi, _pd, _i, _pi, _pi, _pi, _pd]),
+    "orc_mixed_free": (None, [_vp]),
+    "orc_mixed_sizes": (None, [_vp, _psz, _psz, _psz, _psz, _psz]),
+    "orc_mixed_tables": (None, [_vp, _psz, _psz, _pd, _psz, _psz, _pd, _psz, _psz, _pd, _psz, _psz, _pd]),
+    "orc_mixed_set_damping": (None, [_vp, _d, _pd, _pd]),
+    "orc_mixed_set_pulse": (None, [_vp, _i, _pd, _d, _pd, _d, _d, _d, _d]),
+    "orc_mixed_local_field": (None, [_vp, _pd, _i, _sz, _pd]),
+    "orc_mixed_rhs": (None, [_vp, _pd, _pd, _d]),
+    "orc_mixed_energy": (_d, [_vp, _pd]),
+    "orc_mixed_record": (None, [_vp, _pd, _pd]),
+    "orc_mixed_integrate_md": (_sz, [_vp, _pd, _d, _d, _d, _sz, _pd, _pd]),
 }
 
 _lib = None
@@ -219,6 +231,108 @@ class OracleLattice:
         nr = lib().orc_tangent_md(self.h, _p(s), _p(ds), _p(None if dJ is None else _f(dJ)),
                                   _p(None if dB is None else _f(dB)), T0, T1, dt, n, _p(m), _p(dm))
         return m.reshape(n, 3, 3)[:nr], dm.reshape(n, 3, 3)[:nr]
+
+
+def _bonds(cell):
+    nb = len(cell.bilinear)
+    src = np.array([b.source for b in cell.bilinear], dtype=np.int32)
+    par = np.array([b.partner for b in cell.bilinear], dtype=np.int32)
+    off = np.array([b.offset for b in cell.bilinear], dtype=np.int32).reshape(-1)
+    J = _f(np.array([b.J for b in cell.bilinear]).reshape(-1)) if nb else np.zeros(1)
+    return nb, src, par, off, J
+
+
+class MixedOracle:
+    """Oracle mixed SU(2)+SU(3) lattice built from a MixedUnitCell-like
+    object (SU2_cell, SU3_cell, bilinear_SU2_SU3) and dimensions."""
+
+    REC = 34
+
+    def __init__(self, muc, d1, d2, d3):
+        c2, c3 = muc.SU2_cell, muc.SU3_cell
+        if c2.N != 3 or c3.N != 8:
+            raise ValueError("mixed oracle handles spin_dim 3 + 8")
+        b2, b3 = _bonds(c2), _bonds(c3)
+        mb = muc.bilinear_SU2_SU3
+        nbm = len(mb)
+        srcm = np.array([b.source for b in mb], dtype=np.int32)
+        parm = np.array([b.partner for b in mb], dtype=np.int32)
+        offm = np.array([b.offset for b in mb], dtype=np.int32).reshape(-1)
+        Jm = _f(np.array([b.J for b in mb]).reshape(-1)) if nbm else np.zeros(1)
+        arrs = [_f(c2.field), _f(c2.onsite_interaction), _f(c2.sublattice_frames), _f(c2.afm_sublattice_signs),
+                _f(c3.field), _f(c3.onsite_interaction), _f(c3.sublattice_frames)]
+        self._keep = (b2, b3, srcm, parm, offm, Jm, arrs)
+        ip = lambda a, n: a.ctypes.data_as(_pi) if n else None
+        self.h = lib().orc_mixed_new(d1, d2, d3, c2.N_atoms, _p(arrs[0]), _p(arrs[1]), _p(arrs[2]), _p(arrs[3]),
+                                     b2[0], ip(b2[1], b2[0]), ip(b2[2], b2[0]), ip(b2[3], b2[0]), _p(b2[4]),
+                                     c3.N_atoms, _p(arrs[4]), _p(arrs[5]), _p(arrs[6]),
+                                     b3[0], ip(b3[1], b3[0]), ip(b3[2], b3[0]), ip(b3[3], b3[0]), _p(b3[4]),
+                                     nbm, ip(srcm, nbm), ip(parm, nbm), ip(offm, nbm), _p(Jm))
+        if not self.h:
+            raise RuntimeError("oracle rejected the mixed lattice (bond offset beyond the lattice size)")
+        n2, n3, a, b, c = (C.c_size_t() for _ in range(5))
+        lib().orc_mixed_sizes(self.h, C.byref(n2), C.byref(n3), C.byref(a), C.byref(b), C.byref(c))
+        self.N2, self.N3 = n2.value, n3.value
+        self.nnz = (a.value, b.value, c.value)
+
+    def __del__(self):
+        try:
+            lib().orc_mixed_free(self.h)
+        except Exception:
+            pass
+
+    @property
+    def n_state(self):
+        return self.N2 * 3 + self.N3 * 8
+
+    def tables(self):
+        n22, n33, n23 = self.nnz
+        o22, o33 = np.zeros(self.N2 + 1, np.uintp), np.zeros(self.N3 + 1, np.uintp)
+        o23, o32 = np.zeros(self.N2 + 1, np.uintp), np.zeros(self.N3 + 1, np.uintp)
+        p22, p33 = np.zeros(max(n22, 1), np.uintp), np.zeros(max(n33, 1), np.uintp)
+        p23, p32 = np.zeros(max(n23, 1), np.uintp), np.zeros(max(n23, 1), np.uintp)
+        J22, J33 = np.zeros(max(n22, 1) * 9), np.zeros(max(n33, 1) * 64)
+        J23, J32 = np.zeros(max(n23, 1) * 24), np.zeros(max(n23, 1) * 24)
+        z = lambda a: a.ctypes.data_as(_psz)
+        lib().orc_mixed_tables(self.h, z(o22), z(p22), _p(J22), z(o33), z(p33), _p(J33), z(o23), z(p23), _p(J23),
+                               z(o32), z(p32), _p(J32))
+        return {"22": (o22, p22[:n22], J22[:n22 * 9].reshape(-1, 3, 3)),
+                "33": (o33, p33[:n33], J33[:n33 * 64].reshape(-1, 8, 8)),
+                "23": (o23, p23[:n23], J23[:n23 * 24].reshape(-1, 3, 8)),
+                "32": (o32, p32[:n23], J32[:n23 * 24].reshape(-1, 8, 3))}
+
+    def set_damping(self, alpha=0.0, gamma=None, eq=None):
+        lib().orc_mixed_set_damping(self.h, alpha, _p(None if gamma is None else _f(gamma)),
+                                    _p(None if eq is None else _f(eq)))
+
+    def set_pulse(self, species, f1, t1, f2, t2, amp, width, freq):
+        lib().orc_mixed_set_pulse(self.h, species, _p(_f(f1)), t1, _p(_f(f2)), t2, amp, width, freq)
+
+    def local_field(self, x, species, site):
+        H = np.zeros(8)
+        lib().orc_mixed_local_field(self.h, _p(_f(x)), species, site, _p(H))
+        return H[:3] if species == 2 else H
+
+    def rhs(self, x, t=0.0):
+        out = np.zeros(self.n_state)
+        lib().orc_mixed_rhs(self.h, _p(_f(x)), _p(out), t)
+        return out
+
+    def energy(self, x):
+        return lib().orc_mixed_energy(self.h, _p(_f(x)))
+
+    def record(self, x):
+        out = np.zeros(self.REC)
+        lib().orc_mixed_record(self.h, _p(_f(x)), _p(out))
+        return out
+
+    def integrate_md(self, state, T0, T1, dt):
+        n = lib().orc_boost_num_steps(T0, T1, dt) + 2
+        s = _f(state).copy()
+        t = np.zeros(n)
+        r = np.zeros(n * self.REC)
+        nr = lib().orc_mixed_integrate_md(self.h, _p(s), T0, T1, dt, n, _p(t), _p(r))
+        return {"times": t[:nr], "records": r.reshape(n, self.REC)[:nr], "final": s}
 
 
 def boost_num_steps(T0, T1, dt):
