@@ -155,6 +155,91 @@ def bench_pyrochlore(args):
     print(json.dumps(line), flush=True)
 
 
+MIX_L, MIX_NTAU, MIX_STEPS, MIX_DT = 8, 128, 8192, 0.005
+
+
+def bench_mixed(args):
+    """Mixed SU(2)+SU(3) 2DCS batch: TmFeO3-like 8x8x8 cells (2048 Fe SU(2)
+    + 2048 Tm SU(3) sites), pulses A = 0.5, w = 0.5 along x (Fe) and
+    lambda_2 (Tm) at 0 and tau = 0, 0.05, ..., 6.35 (128 delays):
+    M0 + 128 M1 + 128 M01 = 257 trajectories x 8192 RK4 steps (dt = 0.005),
+    records every step.  Grid-wide stage kernels (the 180 KB state exceeds
+    one workgroup's LDS)."""
+    from classical_mcmc_spin_amd import gpu_mixed as GM
+    from classical_mcmc_spin_amd import mixed_model as mm
+    from classical_mcmc_spin_amd import seeding
+    muc = mm.build_tmfeo3_mixed()
+    ml = mm.MixedLattice(muc, MIX_L, MIX_L, MIX_L)
+    g = ml.flat_state(seeding.random_spins(ml.N2, 1234), mm.random_su3_states(ml.N3, 4321))
+    taus = 0.05 * np.arange(MIX_NTAU)
+    B = 1 + 2 * MIX_NTAU
+    tp = np.zeros((B, 2))
+    sc = np.zeros((B, 2))
+    sc[0, 0] = 1.0
+    tp[1:1 + MIX_NTAU, 0] = taus
+    sc[1:1 + MIX_NTAU, 0] = 1.0
+    tp[1 + MIX_NTAU:, 1] = taus
+    sc[1 + MIX_NTAU:, :] = 1.0
+    f2 = np.zeros((4, 3))
+    f2[:, 0] = 1.0
+    f3 = np.zeros((4, 8))
+    f3[:, 1] = 1.0
+    T0 = -1.0
+    T1m = T0 + (MIX_STEPS + 0.5) * MIX_DT
+    gm = GM.GPUMixedLattice(ml)
+    GM._set_pulses(gm, f2, f2, f3, f3, 0.0, 0.0, AMP, WIDTH, 0.0, AMP, WIDTH, 0.0)
+
+    def one_step():
+        return gm.integrate_batched(g, tp, sc, T0, T1m, MIX_DT)
+
+    for _ in range(args.warmup):
+        one_step()
+    kernel_ms = []
+    t = time.perf_counter()
+    for _ in range(args.steps):
+        res = one_step()
+        kernel_ms.append(gm.last_kernel_ms()[0])
+    el = time.perf_counter() - t
+    assert res["records"].shape[0] == MIX_STEPS + 1 and np.all(np.isfinite(res["records"]))
+    gm.close()
+    kms = float(np.mean(kernel_ms))
+    traj_s = B * args.steps / el
+    n_sites = ml.N2 + ml.N3
+    # one stage: SU(2) sites read+write 3 doubles, SU(3) sites 8 doubles
+    bytes_launch = B * (ml.N2 * 3 + ml.N3 * 8) * 2 * 8
+    line = {
+        "metric": METRIC, "value": traj_s, "unit": "trajectories/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": 1e3 * el / args.steps,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "2dcs_tmfeo3_mixed_8x8x8_257traj_8192steps_rk4_dt0.005",
+                   "sites_su2": ml.N2, "sites_su3": ml.N3, "trajectories": B, "rk4_steps": MIX_STEPS,
+                   "record_every": 1},
+        "site_steps_per_s": traj_s * n_sites * MIX_STEPS,
+        "roofline": {"bound": "hbm", "achieved": bytes_launch / (kms * 1e-3) / 1e9,
+                     "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                     "frac": bytes_launch / (kms * 1e-3) / HBM_PEAK, "traffic": None,
+                     "kernel": "k_mixed_stage", "kernel_ms": kms, "algorithmic_bytes_per_launch": bytes_launch},
+    }
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import pyoracle
+        o = pyoracle.MixedOracle(muc, MIX_L, MIX_L, MIX_L)
+        o.set_pulse(2, f2, 0.0, f2 * 0.0, 0.0, AMP, WIDTH, 0.0)
+        o.set_pulse(3, f3, 0.0, f3 * 0.0, 0.0, AMP, WIDTH, 0.0)
+        n = 400
+        t = time.perf_counter()
+        o.integrate_md(g, T0, T0 + (n + 0.5) * MIX_DT, MIX_DT)
+        el = time.perf_counter() - t
+        line["cpu_baseline"] = {
+            "value": n / el / MIX_STEPS, "unit": "trajectories/s", "cores": pyoracle.max_threads(),
+            "kind": "port",
+            "sample": f"1 trajectory x {n} RK4 steps of the same lattice ({el:.1f} s, OpenMP over sites "
+                      f"inside each RHS as the reference's mixed MD path)",
+            "site_steps_per_s": n * n_sites / el}
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -165,9 +250,10 @@ def main():
     ap.add_argument("--block", type=int, default=0)
     ap.add_argument("--launch", default="", help="fixed persistent launch shape 'block,spt,dbuf' "
                                                    "instead of the autotuner's choice")
-    ap.add_argument("--workload", default="2dcs", choices=["2dcs", "pyrochlore"],
+    ap.add_argument("--workload", default="2dcs", choices=["2dcs", "pyrochlore", "mixed"],
                     help="2dcs: the batched 2DCS workload (default); pyrochlore: 64 samples of the "
-                         "65 536-site pyrochlore lattice (grid-wide stage kernels)")
+                         "65 536-site pyrochlore lattice; mixed: 257-trajectory SU(2)+SU(3) 2DCS batch "
+                         "(both on the grid-wide stage kernels)")
     ap.add_argument("--profile", action="store_true",
                     help="only the timed batched launches (for rocprofv3 runs): no autotune sweep, "
                          "plain paths, parity check or CPU baseline")
@@ -180,6 +266,8 @@ def main():
 
     if args.workload == "pyrochlore":
         return bench_pyrochlore(args)
+    if args.workload == "mixed":
+        return bench_mixed(args)
     from classical_mcmc_spin_amd import GPULattice
 
     uc, lat, init, field_in = build_workload()

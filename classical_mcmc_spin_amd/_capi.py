@@ -64,6 +64,25 @@ SIGNATURES = {
     "csl_last_launch_config": (_i, [_vp, _pi, _pi, _pi]),
     "csl_set_launch_config": (_i, [_vp, _sz, _i, _i, _i]),
     "csl_set_replica_hamiltonians": (_i, [_vp, _sz, _pd, _pd]),
+    # mixed SU(2) + SU(3) path (include/csl_mixed.h)
+    "csl_mixed_create": (_i, [_sz, _sz, _sz, _pd, _pd, _psz, _psz, _pd, _pd, _pd, _psz, _psz, _pd,
+                              _psz, _psz, _pd, _psz, _psz, _pd, C.POINTER(_vp)]),
+    "csl_mixed_destroy": (_i, [_vp]),
+    "csl_mixed_info": (_i, [_vp, _psz, _psz, _pi, _pi, _pi, _pi]),
+    "csl_mixed_set_frames": (_i, [_vp, _pd, _pd, _pd]),
+    "csl_mixed_set_pulse": (_i, [_vp, _i, _pd, _d, _d, _d, _d, _d]),
+    "csl_mixed_set_damping": (_i, [_vp, _d, _pd, _pd]),
+    "csl_mixed_set_spins": (_i, [_vp, _pd]),
+    "csl_mixed_get_spins": (_i, [_vp, _pd]),
+    "csl_mixed_record": (_i, [_vp, _pd]),
+    "csl_mixed_energy": (_i, [_vp, _pd]),
+    "csl_mixed_rhs": (_i, [_vp, _d, _pd]),
+    "csl_mixed_normalize": (_i, [_vp, _d, _d]),
+    "csl_mixed_step": (_i, [_vp, _d, _d, _cs]),
+    "csl_mixed_integrate": (_i, [_vp, _d, _d, _d, _sz, _cs, _i, _sz, _pd, _pd, _pd, _psz]),
+    "csl_mixed_integrate_batched": (_i, [_vp, _sz, _pd, _i, _pd, _pd, _d, _d, _d, _sz, _cs, _i, _sz, _pd,
+                                         _pd, _pd, _psz]),
+    "csl_mixed_last_kernel_ms": (_i, [_vp, _pd, _psz]),
 }
 
 

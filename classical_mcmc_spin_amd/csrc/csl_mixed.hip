@@ -1,0 +1,1174 @@
+// csl_mixed.hip — mixed SU(2) + SU(3) Landau–Lifshitz RK4 path: device
+// layout, gfx950 stage kernels and the C ABI of include/csl_mixed.h.
+//
+// Device layout: both species are renumbered sublattice-major like the
+// SU(2) path (p = atom * npad + cell, npad = n_cells rounded up to 64), so a
+// wavefront holds 64 sites of one sublattice of one species and the
+// per-(sublattice, slot) bond matrices of translation-invariant tables are
+// wave-uniform (scalar loads through the constant address space).  One
+// trajectory's state is [3][NP2] SU(2) planes followed by [8][NP3] SU(3)
+// planes, FP64.
+//
+// The state of the reference workload (2048 SU(2) + 2048 SU(3) sites,
+// 180 KB) exceeds one workgroup's LDS, so the path runs grid-wide stage
+// kernels: one launch per RK4 stage covers the SU(2) blocks and the SU(3)
+// blocks of every trajectory (B in the grid's second dimension), with the
+// drive envelopes evaluated once per workgroup and the M(t)/E(t) records
+// reduced per workgroup and then per trajectory.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "../../include/csl_mixed.h"
+#include "csl_host.hpp"
+#include "csl_internal.hpp"
+
+namespace csl {
+
+// One bond kind on the device (source species gathers partner species).
+struct Ell {
+    const int* __restrict__ col;        // [zmax][NP_src] partner device index
+    const int* __restrict__ cnt;        // [NP_src]
+    const int* __restrict__ cnt_atom;   // [A_src]
+    const double* __restrict__ mats;    // uniform: [A][zmax][m]; else [zmax][NP_src][m]
+    int zmax, uniform;
+};
+
+struct MixDev {
+    int N2, N3, A2, A3, n_cells, npad, NP2, NP3;
+    int site_uniform2, site_uniform3, has_onsite2, has_onsite3;
+    const double* __restrict__ field2;   // [A2][3] or [NP2][3]
+    const double* __restrict__ onsite2;  // 2A: [A2][9] or [NP2][9]
+    const double* __restrict__ field3;   // [A3][8] or [NP3][8]
+    const double* __restrict__ onsite3;  // 2A: [A3][64] or [NP3][64]
+    Ell e22, e23, e33, e32;
+    const double* __restrict__ frames2;  // [A2][9]
+    const double* __restrict__ afm2;     // [A2]
+    const double* __restrict__ frames3;  // [A3][64]
+    const double* __restrict__ fdrive2;  // [2][A2][3]
+    const double* __restrict__ fdrive3;  // [2][A3][8]
+    const double* __restrict__ eq3;      // [8][NP3] or null
+    double amp2, width2, freq2, amp3, width3, freq3;
+    double alpha;
+    double gamma[8];
+    int has_bloch;
+};
+
+struct MixStageArgs {
+    int stage;                 // 0..3
+    int nblk2;                 // workgroups of the SU(2) part
+    double* x;                 // [B][SZ] state at step start
+    double* acc;               // RK4 accumulator
+    const double* ycur;        // stage input (x itself in stage 0)
+    double* ynext;             // next stage input
+    double t, dt;              // stage time, step
+    const double* tp;          // [B][2] SU(2) pulse centres or null (no drive)
+    const double* tp3;         // [B][2] SU(3) pulse centres
+    const double* scale;       // [B][2] pulse scales or null (1)
+    int rec;                   // stage 0 of a recorded step: write partials
+    double* partials;          // [B][nblk][34]
+};
+
+constexpr int kRec = CSL_MIXED_RECORD;
+
+__device__ __forceinline__ size_t mix_stride(const MixDev& M) {
+    return size_t(3) * M.NP2 + size_t(8) * M.NP3;
+}
+
+// y_p = sum_k J_k x_partner(k), J m = DA x DB, accumulated per bond row by row
+template <int DA, int DB>
+__device__ __forceinline__ void gather_bonds(const Ell& E, const double* __restrict__ src, int NPsrc,
+                                             int NPpart, int atom, int p, double* H) {
+    const int c = E.uniform ? E.cnt_atom[atom] : E.cnt[p];
+    for (int k = 0; k < c; ++k) {
+        const int q = E.col[size_t(k) * NPsrc + p];
+        double s[DB];
+#pragma unroll
+        for (int e = 0; e < DB; ++e) s[e] = src[size_t(e) * NPpart + q];
+        if (E.uniform) {
+            const cptr<double> J = as_const(E.mats) + (size_t(atom) * E.zmax + k) * (DA * DB);
+#pragma unroll
+            for (int a = 0; a < DA; ++a) {
+                double r = 0.0;
+#pragma unroll
+                for (int e = 0; e < DB; ++e) r += J[a * DB + e] * s[e];
+                H[a] += r;
+            }
+        } else {
+            const double* J = E.mats + (size_t(k) * NPsrc + p) * (DA * DB);
+#pragma unroll
+            for (int a = 0; a < DA; ++a) {
+                double r = 0.0;
+#pragma unroll
+                for (int e = 0; e < DB; ++e) r += J[a * DB + e] * s[e];
+                H[a] += r;
+            }
+        }
+    }
+}
+
+// -B + 2A S of one site (tables hold 2A)
+template <int D>
+__device__ __forceinline__ void field_onsite(const double* __restrict__ field, const double* __restrict__ onsite,
+                                             int site_uniform, int has_onsite, int atom, int p,
+                                             const double* S, double* H) {
+    if (site_uniform) {
+        const cptr<double> f = as_const(field) + size_t(atom) * D;
+#pragma unroll
+        for (int a = 0; a < D; ++a) H[a] = -f[a];
+        if (has_onsite) {
+            const cptr<double> A = as_const(onsite) + size_t(atom) * D * D;
+#pragma unroll
+            for (int a = 0; a < D; ++a) {
+                double r = 0.0;
+#pragma unroll
+                for (int e = 0; e < D; ++e) r += A[a * D + e] * S[e];
+                H[a] += r;
+            }
+        }
+    } else {
+        const double* f = field + size_t(p) * D;
+#pragma unroll
+        for (int a = 0; a < D; ++a) H[a] = -f[a];
+        if (has_onsite) {
+            const double* A = onsite + size_t(p) * D * D;
+#pragma unroll
+            for (int a = 0; a < D; ++a) {
+                double r = 0.0;
+#pragma unroll
+                for (int e = 0; e < D; ++e) r += A[a * D + e] * S[e];
+                H[a] += r;
+            }
+        }
+    }
+}
+
+// SU(2) local field without drive at device site p of one trajectory y
+__device__ __forceinline__ void field_su2(const MixDev& M, const double* __restrict__ y, int atom, int p,
+                                          const double* S, double* H) {
+    field_onsite<3>(M.field2, M.onsite2, M.site_uniform2, M.has_onsite2, atom, p, S, H);
+    gather_bonds<3, 3>(M.e22, y, M.NP2, M.NP2, atom, p, H);
+    gather_bonds<3, 8>(M.e23, y + size_t(3) * M.NP2, M.NP2, M.NP3, atom, p, H);
+}
+
+__device__ __forceinline__ void field_su3(const MixDev& M, const double* __restrict__ y, int atom, int p,
+                                          const double* S, double* H) {
+    field_onsite<8>(M.field3, M.onsite3, M.site_uniform3, M.has_onsite3, atom, p, S, H);
+    gather_bonds<8, 8>(M.e33, y + size_t(3) * M.NP2, M.NP3, M.NP3, atom, p, H);
+    gather_bonds<8, 3>(M.e32, y, M.NP3, M.NP2, atom, p, H);
+}
+
+// d_a = f_abc H_b S_c over the 9 non-zero (a<b<c) structure constants
+// (cross_prod_SU3_flat, simple_linear_alg.h:78-169)
+__device__ __forceinline__ void su3_cross(const double* H, const double* S, double* d) {
+    constexpr double h = 0.5;
+    constexpr double r = 0.8660254037844386;  // sqrt(3)/2
+#pragma unroll
+    for (int i = 0; i < 8; ++i) d[i] = 0.0;
+    auto tri = [&](int a, int b, int c, double v) {
+        d[a] += v * (H[b] * S[c] - H[c] * S[b]);
+        d[b] += v * (H[c] * S[a] - H[a] * S[c]);
+        d[c] += v * (H[a] * S[b] - H[b] * S[a]);
+    };
+    tri(0, 1, 2, 1.0);
+    tri(0, 3, 6, h);
+    tri(0, 4, 5, -h);
+    tri(1, 3, 5, h);
+    tri(1, 4, 6, h);
+    tri(2, 3, 4, h);
+    tri(2, 5, 6, -h);
+    tri(3, 4, 7, r);
+    tri(5, 6, 7, r);
+}
+
+__device__ __forceinline__ double envelope(double amp, double t, double tc, double width, double freq) {
+    const double d = t - tc;
+    const double a = d / (2.0 * width);
+    return amp * exp(-(a * a)) * cos(freq * d);
+}
+
+// Workgroup -> (block, trajectory) with the XCD-contiguous mapping of k_stage.
+__device__ __forceinline__ void mix_block(int& bx, int& by) {
+    const int nx = gridDim.x;
+    const int T = nx * gridDim.y;
+    bx = blockIdx.x;
+    by = blockIdx.y;
+    if ((T & 7) == 0) {
+        const int h = bx + by * nx;
+        const int logical = (h & 7) * (T >> 3) + (h >> 3);
+        bx = logical % nx;
+        by = logical / nx;
+    }
+}
+
+// One RK4 stage of every SU(2) and SU(3) site of every trajectory.
+__global__ void __launch_bounds__(256) k_mixed_stage(MixDev M, MixStageArgs A) {
+    __shared__ double fac[4];
+    __shared__ double red[4][kRec];
+    int bx, b;
+    mix_block(bx, b);
+    const int tid = threadIdx.x;
+    const int lane = tid & (kWave - 1);
+    const int wave = uniform(tid >> 6);
+    const bool is3 = bx >= A.nblk2;                       // uniform per workgroup
+    const size_t off = size_t(b) * mix_stride(M);
+    const double* cur = A.ycur + off;
+    double* nxt = A.ynext + off;
+    double* x = A.x + off;
+    double* acc = A.acc + off;
+    if (A.tp) {  // drive envelopes of this trajectory at the stage time
+        if (tid < 4) {
+            const int sp = tid >> 1, k = tid & 1;
+            const double amp = sp ? M.amp3 : M.amp2;
+            double f = 0.0;
+            if (amp != 0.0) {
+                f = envelope(amp, A.t, (sp ? A.tp3 : A.tp)[size_t(b) * 2 + k], sp ? M.width3 : M.width2,
+                             sp ? M.freq3 : M.freq2);
+                if (A.scale) f *= A.scale[size_t(b) * 2 + k];
+            }
+            fac[tid] = f;
+        }
+        __syncthreads();
+    }
+    const int s = A.stage;
+    const double dt = A.dt;
+    const double cdt = (s == 2) ? 1.0 * dt : 0.5 * dt;
+    const double bdt = (s == 0 || s == 3) ? (1.0 / 6.0) * dt : (1.0 / 3.0) * dt;
+    double part[kRec];
+#pragma unroll
+    for (int j = 0; j < kRec; ++j) part[j] = 0.0;
+    if (!is3) {
+        const int p = bx * 256 + tid;
+        const int pw = uniform(bx * 256 + wave * kWave);
+        const int atom = uniform(pw < M.NP2 ? pw / M.npad : 0);
+        if (p < M.NP2 && (p - atom * M.npad) < M.n_cells) {
+            const int NP = M.NP2;
+            const double S[3] = {cur[p], cur[NP + p], cur[2 * NP + p]};
+            double H[3];
+            field_su2(M, cur, atom, p, S, H);
+            if (A.rec) {
+                const cptr<double> R = as_const(M.frames2) + size_t(atom) * 9;
+                const double sg = M.afm2[atom];
+                const double B0 = M.site_uniform2 ? M.field2[atom * 3 + 0] : M.field2[size_t(p) * 3 + 0];
+                const double B1 = M.site_uniform2 ? M.field2[atom * 3 + 1] : M.field2[size_t(p) * 3 + 1];
+                const double B2 = M.site_uniform2 ? M.field2[atom * 3 + 2] : M.field2[size_t(p) * 3 + 2];
+#pragma unroll
+                for (int mu = 0; mu < 3; ++mu) {
+                    const double g = R[mu * 3] * S[0] + R[mu * 3 + 1] * S[1] + R[mu * 3 + 2] * S[2];
+                    part[mu] = sg * g;
+                    part[3 + mu] = S[mu];
+                    part[6 + mu] = g;
+                }
+                part[33] = 0.5 * (S[0] * (H[0] - B0) + S[1] * (H[1] - B1) + S[2] * (H[2] - B2));
+            }
+            if (A.tp) {
+                const double f1 = fac[0], f2 = fac[1];
+                if (f1 != 0.0 || f2 != 0.0) {
+                    const cptr<double> d0 = as_const(M.fdrive2) + size_t(atom) * 3;
+                    const cptr<double> d1 = as_const(M.fdrive2) + size_t(M.A2 + atom) * 3;
+#pragma unroll
+                    for (int a = 0; a < 3; ++a) H[a] -= d0[a] * f1 + d1[a] * f2;
+                }
+            }
+            double k[3] = {H[1] * S[2] - H[2] * S[1], H[2] * S[0] - H[0] * S[2], H[0] * S[1] - H[1] * S[0]};
+            if (M.alpha != 0.0) {  // mixed_lattice_md.cpp:570-578
+                const double S2 = S[0] * S[0] + S[1] * S[1] + S[2] * S[2];
+                const double SH = S[0] * H[0] + S[1] * H[1] + S[2] * H[2];
+                const double g = (S2 > 0.0) ? M.alpha / sqrt(S2) : 0.0;
+#pragma unroll
+                for (int a = 0; a < 3; ++a) k[a] += g * (S[a] * SH - H[a] * S2);
+            }
+            if (s == 0) {
+#pragma unroll
+                for (int d = 0; d < 3; ++d) {
+                    const size_t q = size_t(d) * NP + p;
+                    acc[q] = S[d] + bdt * k[d];
+                    nxt[q] = S[d] + cdt * k[d];
+                }
+            } else if (s < 3) {
+#pragma unroll
+                for (int d = 0; d < 3; ++d) {
+                    const size_t q = size_t(d) * NP + p;
+                    const double xd = x[q];
+                    acc[q] = acc[q] + bdt * k[d];
+                    nxt[q] = xd + cdt * k[d];
+                }
+            } else {
+#pragma unroll
+                for (int d = 0; d < 3; ++d) {
+                    const size_t q = size_t(d) * NP + p;
+                    x[q] = acc[q] + bdt * k[d];
+                }
+            }
+        }
+    } else {
+        const int p = (bx - A.nblk2) * 256 + tid;
+        const int pw = uniform((bx - A.nblk2) * 256 + wave * kWave);
+        const int atom = uniform(pw < M.NP3 ? pw / M.npad : 0);
+        const int cell = p - atom * M.npad;
+        if (p < M.NP3 && cell < M.n_cells) {
+            const int NP = M.NP3;
+            const size_t o3 = size_t(3) * M.NP2;
+            double S[8];
+#pragma unroll
+            for (int a = 0; a < 8; ++a) S[a] = cur[o3 + size_t(a) * NP + p];
+            double H[8];
+            field_su3(M, cur, atom, p, S, H);
+            if (A.rec) {
+                const cptr<double> R = as_const(M.frames3) + size_t(atom) * 64;
+                const double sg = ((cell * M.A3 + atom) & 1) ? -1.0 : 1.0;
+                double e = 0.0;
+#pragma unroll
+                for (int mu = 0; mu < 8; ++mu) {
+                    double g = 0.0;
+#pragma unroll
+                    for (int nu = 0; nu < 8; ++nu) g += R[mu * 8 + nu] * S[nu];
+                    part[9 + mu] = sg * S[mu];
+                    part[17 + mu] = S[mu];
+                    part[25 + mu] = g;
+                    const double Bm = M.site_uniform3 ? M.field3[atom * 8 + mu] : M.field3[size_t(p) * 8 + mu];
+                    e += S[mu] * (H[mu] - Bm);
+                }
+                part[33] = 0.5 * e;
+            }
+            if (A.tp) {
+                const double f1 = fac[2], f2 = fac[3];
+                if (f1 != 0.0 || f2 != 0.0) {
+                    const cptr<double> d0 = as_const(M.fdrive3) + size_t(atom) * 8;
+                    const cptr<double> d1 = as_const(M.fdrive3) + size_t(M.A3 + atom) * 8;
+#pragma unroll
+                    for (int a = 0; a < 8; ++a) H[a] -= d0[a] * f1 + d1[a] * f2;
+                }
+            }
+            double k[8];
+            su3_cross(H, S, k);
+            if (M.has_bloch) {  // mixed_lattice_md.cpp:596-605
+#pragma unroll
+                for (int a = 0; a < 8; ++a)
+                    if (M.gamma[a] != 0.0) k[a] -= M.gamma[a] * (S[a] - M.eq3[size_t(a) * NP + p]);
+            }
+            if (s == 0) {
+#pragma unroll
+                for (int d = 0; d < 8; ++d) {
+                    const size_t q = o3 + size_t(d) * NP + p;
+                    acc[q] = S[d] + bdt * k[d];
+                    nxt[q] = S[d] + cdt * k[d];
+                }
+            } else if (s < 3) {
+#pragma unroll
+                for (int d = 0; d < 8; ++d) {
+                    const size_t q = o3 + size_t(d) * NP + p;
+                    const double xd = x[q];
+                    acc[q] = acc[q] + bdt * k[d];
+                    nxt[q] = xd + cdt * k[d];
+                }
+            } else {
+#pragma unroll
+                for (int d = 0; d < 8; ++d) {
+                    const size_t q = o3 + size_t(d) * NP + p;
+                    x[q] = acc[q] + bdt * k[d];
+                }
+            }
+        }
+    }
+    if (A.rec) {
+        // SU(2) workgroups own record entries 0..8 and 33, SU(3) ones 9..33
+        const int j0 = is3 ? 9 : 0, j1 = is3 ? 33 : 9;
+#pragma unroll
+        for (int j = 0; j < kRec; ++j) {
+            if ((j >= j0 && j < j1) || j == 33) {
+                const double v = wave_sum(part[j]);
+                if (lane == 0) red[wave][j] = v;
+            }
+        }
+        __syncthreads();
+        if (tid < kRec) {
+            const bool mine = (tid >= j0 && tid < j1) || tid == 33;
+            const double v = mine ? (red[0][tid] + red[1][tid]) + (red[2][tid] + red[3][tid]) : 0.0;
+            A.partials[(size_t(b) * gridDim.x + bx) * kRec + tid] = v;
+        }
+    }
+}
+
+// One workgroup per trajectory: partials -> record ri
+__global__ void __launch_bounds__(256) k_mixed_reduce(const double* __restrict__ partials, int nblk, int B,
+                                                      int ri, double inv2, double inv3, double invE,
+                                                      double* __restrict__ recs) {
+    __shared__ double red[kRec][4];
+    const int b = blockIdx.x;
+    const int t = threadIdx.x;
+    double v[kRec];
+#pragma unroll
+    for (int j = 0; j < kRec; ++j) v[j] = 0.0;
+    for (int k = t; k < nblk; k += 256) {
+        const double* q = partials + (size_t(b) * nblk + k) * kRec;
+#pragma unroll
+        for (int j = 0; j < kRec; ++j) v[j] += q[j];
+    }
+#pragma unroll
+    for (int j = 0; j < kRec; ++j) v[j] = wave_sum(v[j]);
+    if ((t & (kWave - 1)) == 0)
+#pragma unroll
+        for (int j = 0; j < kRec; ++j) red[j][t >> 6] = v[j];
+    __syncthreads();
+    if (t < kRec) {
+        const double tot = (red[t][0] + red[t][1]) + (red[t][2] + red[t][3]);
+        const double sc = t < 9 ? inv2 : (t < 33 ? inv3 : invE);
+        recs[(size_t(ri) * B + b) * kRec + t] = tot * sc;
+    }
+}
+
+// Local fields (no drive) and dS/dt at time t with drive factors f[4] of one state
+__global__ void __launch_bounds__(256) k_mixed_eval(MixDev M, const double* __restrict__ y, double f21,
+                                                    double f22, double f31, double f32,
+                                                    double* __restrict__ dsdt) {
+    const int nblk2 = (M.NP2 + 255) / 256;
+    const int bx = blockIdx.x;
+    const int tid = threadIdx.x;
+    const int wave = uniform(tid >> 6);
+    if (bx < nblk2) {
+        const int p = bx * 256 + tid;
+        const int pw = uniform(bx * 256 + wave * kWave);
+        const int atom = uniform(pw < M.NP2 ? pw / M.npad : 0);
+        if (p >= M.NP2 || p - atom * M.npad >= M.n_cells) return;
+        const int NP = M.NP2;
+        const double S[3] = {y[p], y[NP + p], y[2 * NP + p]};
+        double H[3];
+        field_su2(M, y, atom, p, S, H);
+        if (f21 != 0.0 || f22 != 0.0)
+            for (int a = 0; a < 3; ++a)
+                H[a] -= M.fdrive2[atom * 3 + a] * f21 + M.fdrive2[(M.A2 + atom) * 3 + a] * f22;
+        double k[3] = {H[1] * S[2] - H[2] * S[1], H[2] * S[0] - H[0] * S[2], H[0] * S[1] - H[1] * S[0]};
+        if (M.alpha != 0.0) {
+            const double S2 = S[0] * S[0] + S[1] * S[1] + S[2] * S[2];
+            const double SH = S[0] * H[0] + S[1] * H[1] + S[2] * H[2];
+            const double g = (S2 > 0.0) ? M.alpha / sqrt(S2) : 0.0;
+            for (int a = 0; a < 3; ++a) k[a] += g * (S[a] * SH - H[a] * S2);
+        }
+        for (int d = 0; d < 3; ++d) dsdt[size_t(d) * NP + p] = k[d];
+    } else {
+        const int p = (bx - nblk2) * 256 + tid;
+        const int pw = uniform((bx - nblk2) * 256 + wave * kWave);
+        const int atom = uniform(pw < M.NP3 ? pw / M.npad : 0);
+        if (p >= M.NP3 || p - atom * M.npad >= M.n_cells) return;
+        const int NP = M.NP3;
+        const size_t o3 = size_t(3) * M.NP2;
+        double S[8], H[8], k[8];
+        for (int a = 0; a < 8; ++a) S[a] = y[o3 + size_t(a) * NP + p];
+        field_su3(M, y, atom, p, S, H);
+        if (f31 != 0.0 || f32 != 0.0)
+            for (int a = 0; a < 8; ++a)
+                H[a] -= M.fdrive3[atom * 8 + a] * f31 + M.fdrive3[(M.A3 + atom) * 8 + a] * f32;
+        su3_cross(H, S, k);
+        if (M.has_bloch)
+            for (int a = 0; a < 8; ++a)
+                if (M.gamma[a] != 0.0) k[a] -= M.gamma[a] * (S[a] - M.eq3[size_t(a) * NP + p]);
+        for (int d = 0; d < 8; ++d) dsdt[o3 + size_t(d) * NP + p] = k[d];
+    }
+}
+
+// flat reference state(s) [N2*3 | N3*8] <-> device SoA, B trajectories
+__global__ void k_mixed_perm(MixDev M, const double* __restrict__ src, size_t src_stride, double* __restrict__ dst,
+                             size_t dst_stride, int to_soa) {
+    const int b = blockIdx.y;
+    const long i = long(blockIdx.x) * blockDim.x + threadIdx.x;   // flat component index
+    const long n2 = long(M.N2) * 3, n = n2 + long(M.N3) * 8;
+    if (i >= n) return;
+    size_t q;
+    if (i < n2) {
+        const int site = int(i / 3), d = int(i % 3);
+        const int p = (site % M.A2) * M.npad + site / M.A2;
+        q = size_t(d) * M.NP2 + p;
+    } else {
+        const long j = i - n2;
+        const int site = int(j / 8), d = int(j % 8);
+        const int p = (site % M.A3) * M.npad + site / M.A3;
+        q = size_t(3) * M.NP2 + size_t(d) * M.NP3 + p;
+    }
+    if (to_soa) dst[size_t(b) * dst_stride + q] = src[size_t(b) * src_stride + size_t(i)];
+    else dst[size_t(b) * dst_stride + size_t(i)] = src[size_t(b) * src_stride + q];
+}
+
+__global__ void k_mixed_normalize(MixDev M, double* __restrict__ y, double len2, double len3) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < M.NP2) {
+        const double a = y[i], b = y[M.NP2 + i], c = y[2 * M.NP2 + i];
+        const double n = sqrt(a * a + b * b + c * c);
+        if (n > 0.0) {
+            const double s = len2 / n;
+            y[i] = a * s;
+            y[M.NP2 + i] = b * s;
+            y[2 * M.NP2 + i] = c * s;
+        }
+    }
+    if (i < M.NP3) {
+        double* z = y + size_t(3) * M.NP2;
+        double n2 = 0.0;
+        for (int a = 0; a < 8; ++a) n2 += z[size_t(a) * M.NP3 + i] * z[size_t(a) * M.NP3 + i];
+        if (n2 > 0.0) {
+            const double s = len3 / sqrt(n2);
+            for (int a = 0; a < 8; ++a) z[size_t(a) * M.NP3 + i] *= s;
+        }
+    }
+}
+
+}  // namespace csl
+
+using namespace cslh;
+
+namespace {
+
+// Host form of one ELL bond kind.
+struct EllHost {
+    int zmax = 0;
+    bool uniform = true;
+    std::vector<int> col, cnt, cnt_atom;
+    std::vector<double> mats;
+};
+
+// CSR (reference per-site order) -> ELL in the sublattice-major layout.  For
+// translation-invariant tables the bonds of each site are put in its
+// sublattice's majority slot order (sites on wrapped cells list the same
+// bonds in another order) and one matrix per (sublattice, slot) is kept.
+EllHost build_ell(size_t N, size_t A, int npad, const size_t* off, const size_t* par, const double* J, int m,
+                  size_t N_partner, size_t A_partner) {
+    EllHost E;
+    const int NP = int(A) * npad;
+    for (size_t i = 0; i < N; ++i) {
+        if (off[i + 1] < off[i]) fail(CSL_ERR_ARG, "bond offsets must be non-decreasing");
+        E.zmax = std::max<int>(E.zmax, int(off[i + 1] - off[i]));
+    }
+    if (E.zmax > 64) fail(CSL_ERR_UNSUPPORTED, "more than 64 bonds of one kind per site");
+    for (size_t k = 0; k < off[N]; ++k) {
+        if (par[k] >= N_partner) fail(CSL_ERR_ARG, "bond partner index out of range");
+        for (int q = 0; q < m; ++q)
+            if (!std::isfinite(J[k * m + q])) fail(CSL_ERR_NONFINITE, "non-finite bond matrix");
+    }
+    auto pdev = [&](size_t s) { return int(s % A_partner) * npad + int(s / A_partner); };
+    auto sdev = [&](size_t s) { return int(s % A) * npad + int(s / A); };
+    std::vector<size_t> slot_of(off[N]);
+    for (size_t i = 0; i < N; ++i)
+        for (size_t k = off[i]; k < off[i + 1]; ++k) slot_of[k] = k - off[i];
+    std::vector<size_t> canon(A, 0);
+    auto same = [&](size_t a, size_t b) {
+        const size_t ca = off[a + 1] - off[a];
+        return ca == off[b + 1] - off[b] && std::memcmp(J + off[a] * m, J + off[b] * m, ca * m * sizeof(double)) == 0;
+    };
+    bool uni = true;
+    for (size_t at = 0; at < A && uni; ++at) {
+        size_t cand = at, votes = 0;
+        for (size_t i = at; i < N; i += A) {
+            if (votes == 0) { cand = i; votes = 1; }
+            else if (same(i, cand)) ++votes;
+            else --votes;
+        }
+        canon[at] = cand;
+        const size_t c = off[cand + 1] - off[cand];
+        for (size_t i = at; i < N && uni; i += A) {
+            if (same(i, cand)) continue;
+            if (off[i + 1] - off[i] != c) { uni = false; break; }
+            std::vector<char> used(c, 0);
+            for (size_t q = 0; q < c && uni; ++q) {
+                bool found = false;
+                for (size_t r = 0; r < c; ++r)
+                    if (!used[r] && std::memcmp(J + (off[i] + r) * m, J + (off[cand] + q) * m, m * sizeof(double)) == 0) {
+                        used[r] = 1;
+                        slot_of[off[i] + r] = q;
+                        found = true;
+                        break;
+                    }
+                if (!found) uni = false;
+            }
+        }
+    }
+    if (!uni)
+        for (size_t i = 0; i < N; ++i)
+            for (size_t k = off[i]; k < off[i + 1]; ++k) slot_of[k] = k - off[i];
+    E.uniform = uni;
+    const int z = std::max(E.zmax, 1);
+    E.col.assign(size_t(z) * NP, 0);
+    E.cnt.assign(NP, 0);
+    E.cnt_atom.assign(A, 0);
+    E.mats.assign(uni ? A * size_t(z) * m : size_t(z) * NP * m, 0.0);
+    for (size_t i = 0; i < N; ++i) {
+        const int p = sdev(i);
+        const int c = int(off[i + 1] - off[i]);
+        E.cnt[p] = c;
+        for (int k = 0; k < c; ++k) {
+            const size_t q = slot_of[off[i] + k];
+            E.col[q * NP + p] = pdev(par[off[i] + k]);
+            double* dst = uni ? &E.mats[((i % A) * z + q) * m] : &E.mats[(q * NP + p) * m];
+            std::memcpy(dst, J + (off[i] + k) * m, m * sizeof(double));
+        }
+    }
+    for (size_t a = 0; a < A; ++a) E.cnt_atom[a] = int(off[canon[a] + 1] - off[canon[a]]);
+    return E;
+}
+
+struct EllDev {
+    DBuf<int> col, cnt, cnt_atom;
+    DBuf<double> mats;
+    void upload(const EllHost& E, csl::Ell& d, hipStream_t s) {
+        col.upload(E.col.data(), E.col.size(), s);
+        cnt.upload(E.cnt.data(), E.cnt.size(), s);
+        cnt_atom.upload(E.cnt_atom.data(), E.cnt_atom.size(), s);
+        mats.upload(E.mats.data(), E.mats.size(), s);
+        d.col = col.p;
+        d.cnt = cnt.p;
+        d.cnt_atom = cnt_atom.p;
+        d.mats = mats.p;
+        d.zmax = std::max(E.zmax, 1);
+        d.uniform = E.uniform ? 1 : 0;
+    }
+};
+
+}  // namespace
+
+struct csl_mixed {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    size_t N2 = 0, N3 = 0, A2 = 0, A3 = 0, n_cells = 0;
+    csl::MixDev M{};
+    EllDev e22, e23, e33, e32;
+    DBuf<double> field2, onsite2, field3, onsite3, frames2, afm2, frames3, fdrive2, fdrive3, eq3;
+    DBuf<double> state;                      // SoA [SZ]
+    bool has_state = false;
+    DBuf<double> X, ACC, Y1, Y2, partials, recs, tp, tp3, scale, init, flat, aux;
+    double h_tp2[2] = {0.0, 0.0}, h_tp3[2] = {0.0, 0.0};  // single-trajectory pulse centres
+    double last_ms = 0.0;
+    size_t last_launches = 0;
+    ~csl_mixed() {
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+    size_t sz() const { return size_t(3) * M.NP2 + size_t(8) * M.NP3; }
+    size_t nflat() const { return N2 * 3 + N3 * 8; }
+};
+
+namespace {
+
+csl_mixed* check_mixed(csl_mixed* h) {
+    if (!h) fail(CSL_ERR_NULL, "null mixed-lattice handle");
+    hip_check(hipSetDevice(h->device), "hipSetDevice");
+    return h;
+}
+
+// per-site field / onsite -> per-sublattice when translation invariant; 2A
+template <int D>
+void site_tables(size_t N, size_t A, int npad, const double* field, const double* onsite, DBuf<double>& f_dev,
+                 DBuf<double>& o_dev, int& site_uniform, int& has_onsite, hipStream_t s) {
+    check_finite(field, N * D, "field");
+    check_finite(onsite, N * D * D, "onsite");
+    bool uni = true;
+    for (size_t i = 0; i < N && uni; ++i) {
+        const size_t r = i % A;
+        if (std::memcmp(field + i * D, field + r * D, D * sizeof(double)) ||
+            std::memcmp(onsite + i * D * D, onsite + r * D * D, D * D * sizeof(double)))
+            uni = false;
+    }
+    std::vector<double> f, o;
+    if (uni) {
+        f.assign(field, field + A * D);
+        o.assign(onsite, onsite + A * D * D);
+    } else {
+        const size_t NP = A * npad;
+        f.assign(NP * D, 0.0);
+        o.assign(NP * D * D, 0.0);
+        for (size_t i = 0; i < N; ++i) {
+            const size_t p = (i % A) * npad + i / A;
+            std::memcpy(&f[p * D], field + i * D, D * sizeof(double));
+            std::memcpy(&o[p * D * D], onsite + i * D * D, D * D * sizeof(double));
+        }
+    }
+    bool any = false;
+    for (double& v : o) {
+        any |= (v != 0.0);
+        v *= 2.0;
+    }
+    site_uniform = uni ? 1 : 0;
+    has_onsite = any ? 1 : 0;
+    f_dev.upload(f.data(), f.size(), s);
+    o_dev.upload(o.data(), o.size(), s);
+}
+
+int blocks_of(int n) { return (n + 255) / 256; }
+
+void to_soa(csl_mixed* h, const double* d_flat, size_t src_stride, double* d_soa, size_t B) {
+    const long n = long(h->nflat());
+    dim3 g(unsigned((n + 255) / 256), unsigned(B));
+    hipLaunchKernelGGL(csl::k_mixed_perm, g, dim3(256), 0, h->stream, h->M, d_flat, src_stride, d_soa, h->sz(), 1);
+    hip_check(hipGetLastError(), "k_mixed_perm");
+}
+
+void to_flat(csl_mixed* h, const double* d_soa, double* d_flat, size_t B) {
+    const long n = long(h->nflat());
+    dim3 g(unsigned((n + 255) / 256), unsigned(B));
+    hipLaunchKernelGGL(csl::k_mixed_perm, g, dim3(256), 0, h->stream, h->M, d_soa, h->sz(), d_flat, h->nflat(), 0);
+    hip_check(hipGetLastError(), "k_mixed_perm");
+}
+
+// Integrate B trajectories whose SoA states are in h->X; records into h->recs
+// ([n_rec][B][34]); optional state snapshots (flat, B = 1) into d_states.
+void run_mixed(csl_mixed* h, size_t B, const double* d_tp, const double* d_tp3, const double* d_scale,
+               const std::vector<double>& grid, double dt, size_t si, double* d_states) {
+    const size_t n_steps = grid.size() - 1;
+    const size_t n_rec = records_for(n_steps, si);
+    hipStream_t s = h->stream;
+    const size_t plane = B * h->sz();
+    h->ACC.ensure(plane);
+    h->Y1.ensure(plane);
+    h->Y2.ensure(plane);
+    const int nblk2 = blocks_of(h->M.NP2), nblk3 = blocks_of(h->M.NP3);
+    const int nblk = nblk2 + nblk3;
+    h->partials.ensure(B * size_t(nblk) * CSL_MIXED_RECORD);
+    h->recs.ensure(n_rec * B * CSL_MIXED_RECORD);
+    const bool drive = d_tp && (h->M.amp2 != 0.0 || h->M.amp3 != 0.0);
+    const double inv2 = 1.0 / double(h->N2), inv3 = 1.0 / double(h->N3), invE = 1.0 / double(h->N2 + h->N3);
+    hipEvent_t e0, e1;
+    hip_check(hipEventCreate(&e0), "event");
+    hip_check(hipEventCreate(&e1), "event");
+    hip_check(hipEventRecord(e0, s), "event record");
+    size_t launches = 0;
+    const dim3 grid_dim{unsigned(nblk), unsigned(B), 1u};
+    auto stage = [&](int st, double t, int rec) {
+        csl::MixStageArgs A{};
+        A.stage = st;
+        A.nblk2 = nblk2;
+        A.x = h->X.p;
+        A.acc = h->ACC.p;
+        A.ycur = st == 0 ? h->X.p : (st == 2 ? h->Y2.p : h->Y1.p);
+        A.ynext = st == 0 ? h->Y1.p : (st == 1 ? h->Y2.p : (st == 2 ? h->Y1.p : h->Y2.p));
+        A.t = t;
+        A.dt = dt;
+        A.tp = drive ? d_tp : nullptr;
+        A.tp3 = drive ? d_tp3 : nullptr;
+        A.scale = drive ? d_scale : nullptr;
+        A.rec = rec;
+        A.partials = h->partials.p;
+        hipLaunchKernelGGL(csl::k_mixed_stage, grid_dim, dim3(256), 0, s, h->M, A);
+        hip_check(hipGetLastError(), "k_mixed_stage");
+        ++launches;
+    };
+    auto reduce = [&](size_t ri) {
+        hipLaunchKernelGGL(csl::k_mixed_reduce, dim3(unsigned(B)), dim3(256), 0, s, h->partials.p, nblk, int(B),
+                           int(ri), inv2, inv3, invE, h->recs.p);
+        hip_check(hipGetLastError(), "k_mixed_reduce");
+    };
+    for (size_t n = 0; n < n_steps; ++n) {
+        const bool rec = (n % si) == 0;
+        const size_t ri = n / si;
+        if (rec && d_states) to_flat(h, h->X.p, d_states + ri * h->nflat(), 1);
+        const double t = grid[n];
+        // Boost runge_kutta4 stage times t, t + dt/2, t + dt/2, t + dt
+        stage(0, t, rec ? 1 : 0);
+        stage(1, t + 0.5 * dt, 0);
+        stage(2, t + 0.5 * dt, 0);
+        stage(3, t + 1.0 * dt, 0);
+        if (rec) reduce(ri);
+    }
+    hip_check(hipEventRecord(e1, s), "event record");
+    // final record: a stage-0 evaluation whose outputs go to scratch
+    {
+        h->aux.ensure(plane);
+        csl::MixStageArgs A{};
+        A.stage = 0;
+        A.nblk2 = nblk2;
+        A.x = h->X.p;
+        A.acc = h->aux.p;
+        A.ycur = h->X.p;
+        A.ynext = h->aux.p;
+        A.t = grid[n_steps];
+        A.dt = dt;
+        A.rec = 1;
+        A.partials = h->partials.p;
+        hipLaunchKernelGGL(csl::k_mixed_stage, grid_dim, dim3(256), 0, s, h->M, A);
+        hip_check(hipGetLastError(), "k_mixed_stage");
+        reduce(n_rec - 1);
+    }
+    if (d_states) to_flat(h, h->X.p, d_states + (n_rec - 1) * h->nflat(), 1);
+    hip_check(hipStreamSynchronize(s), "mixed integration");
+    float ms = 0.f;
+    hip_check(hipEventElapsedTime(&ms, e0, e1), "elapsed");
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    h->last_ms = launches ? ms / double(launches) : 0.0;
+    h->last_launches = launches;
+}
+
+void need_state(const csl_mixed* h) {
+    if (!h->has_state) fail(CSL_ERR_NOSTATE, "no spin state set");
+}
+
+}  // namespace
+
+extern "C" {
+
+int csl_mixed_create(size_t A2, size_t A3, size_t n_cells, const double* field2, const double* onsite2,
+                     const size_t* off22, const size_t* par22, const double* J22, const double* field3,
+                     const double* onsite3, const size_t* off33, const size_t* par33, const double* J33,
+                     const size_t* off23, const size_t* par23, const double* J23, const size_t* off32,
+                     const size_t* par32, const double* J32, csl_mixed** out) {
+    return guarded([&] {
+        if (!out || !field2 || !onsite2 || !field3 || !onsite3 || !off22 || !off33 || !off23 || !off32)
+            fail(CSL_ERR_NULL, "null table pointer");
+        *out = nullptr;
+        if (A2 == 0 || A3 == 0 || n_cells == 0) fail(CSL_ERR_ARG, "empty lattice");
+        const size_t N2 = A2 * n_cells, N3 = A3 * n_cells;
+        if (N2 > size_t(1) << 26 || N3 > size_t(1) << 26) fail(CSL_ERR_ARG, "lattice too large");
+        if ((off22[N2] && (!par22 || !J22)) || (off33[N3] && (!par33 || !J33)) || (off23[N2] && (!par23 || !J23)) ||
+            (off32[N3] && (!par32 || !J32)))
+            fail(CSL_ERR_NULL, "null bond table");
+        if (off23[N2] != off32[N3]) fail(CSL_ERR_ARG, "SU(2)-SU(3) and SU(3)-SU(2) tables differ in size");
+        auto* h = new csl_mixed();
+        try {
+            hip_check(hipGetDevice(&h->device), "hipGetDevice");
+            hip_check(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking), "hipStreamCreate");
+            hipStream_t s = h->stream;
+            h->N2 = N2;
+            h->N3 = N3;
+            h->A2 = A2;
+            h->A3 = A3;
+            h->n_cells = n_cells;
+            const int npad = int((n_cells + 63) / 64 * 64);
+            csl::MixDev& M = h->M;
+            M.N2 = int(N2);
+            M.N3 = int(N3);
+            M.A2 = int(A2);
+            M.A3 = int(A3);
+            M.n_cells = int(n_cells);
+            M.npad = npad;
+            M.NP2 = int(A2) * npad;
+            M.NP3 = int(A3) * npad;
+            const double dummy = 0.0;
+            EllHost e22 = build_ell(N2, A2, npad, off22, par22, J22 ? J22 : &dummy, 9, N2, A2);
+            EllHost e23 = build_ell(N2, A2, npad, off23, par23, J23 ? J23 : &dummy, 24, N3, A3);
+            EllHost e33 = build_ell(N3, A3, npad, off33, par33, J33 ? J33 : &dummy, 64, N3, A3);
+            EllHost e32 = build_ell(N3, A3, npad, off32, par32, J32 ? J32 : &dummy, 24, N2, A2);
+            h->e22.upload(e22, M.e22, s);
+            h->e23.upload(e23, M.e23, s);
+            h->e33.upload(e33, M.e33, s);
+            h->e32.upload(e32, M.e32, s);
+            site_tables<3>(N2, A2, npad, field2, onsite2, h->field2, h->onsite2, M.site_uniform2, M.has_onsite2, s);
+            site_tables<8>(N3, A3, npad, field3, onsite3, h->field3, h->onsite3, M.site_uniform3, M.has_onsite3, s);
+            M.field2 = h->field2.p;
+            M.onsite2 = h->onsite2.p;
+            M.field3 = h->field3.p;
+            M.onsite3 = h->onsite3.p;
+            std::vector<double> f2(A2 * 9, 0.0), a2(A2, 1.0), f3(A3 * 64, 0.0);
+            for (size_t a = 0; a < A2; ++a)
+                for (int d = 0; d < 3; ++d) f2[a * 9 + d * 4] = 1.0;
+            for (size_t a = 0; a < A3; ++a)
+                for (int d = 0; d < 8; ++d) f3[a * 64 + d * 9] = 1.0;
+            h->frames2.upload(f2.data(), f2.size(), s);
+            h->afm2.upload(a2.data(), a2.size(), s);
+            h->frames3.upload(f3.data(), f3.size(), s);
+            std::vector<double> z2(2 * A2 * 3, 0.0), z3(2 * A3 * 8, 0.0);
+            h->fdrive2.upload(z2.data(), z2.size(), s);
+            h->fdrive3.upload(z3.data(), z3.size(), s);
+            M.frames2 = h->frames2.p;
+            M.afm2 = h->afm2.p;
+            M.frames3 = h->frames3.p;
+            M.fdrive2 = h->fdrive2.p;
+            M.fdrive3 = h->fdrive3.p;
+            M.width2 = M.width3 = 1.0;
+            hip_check(hipStreamSynchronize(s), "table upload");
+        } catch (...) {
+            delete h;
+            throw;
+        }
+        *out = h;
+    });
+}
+
+int csl_mixed_destroy(csl_mixed* h) {
+    return guarded([&] {
+        if (!h) return;
+        (void)hipSetDevice(h->device);
+        (void)hipStreamSynchronize(h->stream);
+        delete h;
+    });
+}
+
+int csl_mixed_info(csl_mixed* h, size_t* N2, size_t* N3, int* u22, int* u23, int* u33, int* u32) {
+    return guarded([&] {
+        check_mixed(h);
+        if (N2) *N2 = h->N2;
+        if (N3) *N3 = h->N3;
+        if (u22) *u22 = h->M.e22.uniform;
+        if (u23) *u23 = h->M.e23.uniform;
+        if (u33) *u33 = h->M.e33.uniform;
+        if (u32) *u32 = h->M.e32.uniform;
+    });
+}
+
+int csl_mixed_set_frames(csl_mixed* h, const double* frames2, const double* afm2, const double* frames3) {
+    return guarded([&] {
+        check_mixed(h);
+        if (frames2) {
+            check_finite(frames2, h->A2 * 9, "frames2");
+            h->frames2.upload(frames2, h->A2 * 9, h->stream);
+        }
+        if (afm2) {
+            check_finite(afm2, h->A2, "afm2");
+            h->afm2.upload(afm2, h->A2, h->stream);
+        }
+        if (frames3) {
+            check_finite(frames3, h->A3 * 64, "frames3");
+            h->frames3.upload(frames3, h->A3 * 64, h->stream);
+        }
+        hip_check(hipStreamSynchronize(h->stream), "frames");
+    });
+}
+
+int csl_mixed_set_pulse(csl_mixed* h, int species, const double* drive, double amp, double width, double freq,
+                        double t1, double t2) {
+    return guarded([&] {
+        check_mixed(h);
+        if (species != 2 && species != 3) fail(CSL_ERR_ARG, "species must be 2 (SU(2)) or 3 (SU(3))");
+        if (!drive) fail(CSL_ERR_NULL, "null drive");
+        if (!(width > 0.0) || !std::isfinite(amp) || !std::isfinite(freq) || !std::isfinite(t1) || !std::isfinite(t2))
+            fail(CSL_ERR_ARG, "pulse needs finite parameters and width > 0");
+        const size_t n = species == 2 ? 2 * h->A2 * 3 : 2 * h->A3 * 8;
+        check_finite(drive, n, "drive");
+        (species == 2 ? h->fdrive2 : h->fdrive3).upload(drive, n, h->stream);
+        if (species == 2) {
+            h->M.amp2 = amp;
+            h->M.width2 = width;
+            h->M.freq2 = freq;
+        } else {
+            h->M.amp3 = amp;
+            h->M.width3 = width;
+            h->M.freq3 = freq;
+        }
+        double* tc = species == 2 ? h->h_tp2 : h->h_tp3;
+        tc[0] = t1;
+        tc[1] = t2;
+        hip_check(hipStreamSynchronize(h->stream), "pulse");
+    });
+}
+
+int csl_mixed_set_damping(csl_mixed* h, double alpha, const double* gamma8, const double* eq) {
+    return guarded([&] {
+        check_mixed(h);
+        if (!std::isfinite(alpha)) fail(CSL_ERR_ARG, "non-finite alpha");
+        h->M.alpha = alpha;
+        bool any = false;
+        for (int a = 0; a < 8; ++a) {
+            h->M.gamma[a] = gamma8 ? gamma8[a] : 0.0;
+            if (!std::isfinite(h->M.gamma[a])) fail(CSL_ERR_ARG, "non-finite gamma");
+            any |= h->M.gamma[a] != 0.0;
+        }
+        h->M.has_bloch = any ? 1 : 0;
+        std::vector<double> e(size_t(8) * h->M.NP3, 0.0);
+        if (eq) {
+            check_finite(eq, h->N3 * 8, "eq");
+            for (size_t i = 0; i < h->N3; ++i) {
+                const size_t p = (i % h->A3) * h->M.npad + i / h->A3;
+                for (int a = 0; a < 8; ++a) e[size_t(a) * h->M.NP3 + p] = eq[i * 8 + a];
+            }
+        }
+        h->eq3.upload(e.data(), e.size(), h->stream);
+        h->M.eq3 = h->eq3.p;
+        hip_check(hipStreamSynchronize(h->stream), "damping");
+    });
+}
+
+int csl_mixed_set_spins(csl_mixed* h, const double* flat) {
+    return guarded([&] {
+        check_mixed(h);
+        if (!flat) fail(CSL_ERR_NULL, "null spins");
+        check_finite(flat, h->nflat(), "spins");
+        h->flat.upload(flat, h->nflat(), h->stream);
+        h->state.ensure(h->sz());
+        hip_check(hipMemsetAsync(h->state.p, 0, h->sz() * sizeof(double), h->stream), "memset");
+        to_soa(h, h->flat.p, h->nflat(), h->state.p, 1);
+        hip_check(hipStreamSynchronize(h->stream), "set spins");
+        h->has_state = true;
+    });
+}
+
+int csl_mixed_get_spins(csl_mixed* h, double* flat) {
+    return guarded([&] {
+        check_mixed(h);
+        need_state(h);
+        if (!flat) fail(CSL_ERR_NULL, "null output");
+        h->flat.ensure(h->nflat());
+        to_flat(h, h->state.p, h->flat.p, 1);
+        copy_back(flat, h->flat.p, h->nflat(), h->stream);
+        hip_check(hipStreamSynchronize(h->stream), "get spins");
+    });
+}
+
+int csl_mixed_record(csl_mixed* h, double* out34) {
+    return guarded([&] {
+        check_mixed(h);
+        need_state(h);
+        if (!out34) fail(CSL_ERR_NULL, "null output");
+        // a zero-step integration of a copy of the state yields the record
+        h->X.ensure(h->sz());
+        hip_check(hipMemcpyAsync(h->X.p, h->state.p, h->sz() * sizeof(double), hipMemcpyDeviceToDevice, h->stream),
+                  "copy");
+        std::vector<double> grid{0.0};
+        run_mixed(h, 1, nullptr, nullptr, nullptr, grid, 1.0, 1, nullptr);
+        copy_back(out34, h->recs.p, CSL_MIXED_RECORD, h->stream);
+        hip_check(hipStreamSynchronize(h->stream), "record");
+    });
+}
+
+int csl_mixed_energy(csl_mixed* h, double* energy) {
+    return guarded([&] {
+        if (!energy) fail(CSL_ERR_NULL, "null output");
+        double r[CSL_MIXED_RECORD];
+        const int rc = csl_mixed_record(h, r);
+        if (rc != CSL_OK) fail(rc, g_err);
+        *energy = r[33] * double(h->N2 + h->N3);
+    });
+}
+
+int csl_mixed_rhs(csl_mixed* h, double t, double* dsdt) {
+    return guarded([&] {
+        check_mixed(h);
+        need_state(h);
+        if (!dsdt) fail(CSL_ERR_NULL, "null output");
+        auto env = [&](double amp, double w, double fr, double c) {
+            const double d = t - c;
+            const double a = d / (2.0 * w);
+            return amp * std::exp(-(a * a)) * std::cos(fr * d);
+        };
+        const csl::MixDev& M = h->M;
+        const double f21 = M.amp2 != 0.0 ? env(M.amp2, M.width2, M.freq2, h->h_tp2[0]) : 0.0;
+        const double f22 = M.amp2 != 0.0 ? env(M.amp2, M.width2, M.freq2, h->h_tp2[1]) : 0.0;
+        const double f31 = M.amp3 != 0.0 ? env(M.amp3, M.width3, M.freq3, h->h_tp3[0]) : 0.0;
+        const double f32 = M.amp3 != 0.0 ? env(M.amp3, M.width3, M.freq3, h->h_tp3[1]) : 0.0;
+        h->aux.ensure(h->sz());
+        hip_check(hipMemsetAsync(h->aux.p, 0, h->sz() * sizeof(double), h->stream), "memset");
+        const int nb = blocks_of(M.NP2) + blocks_of(M.NP3);
+        hipLaunchKernelGGL(csl::k_mixed_eval, dim3(nb), dim3(256), 0, h->stream, M, h->state.p, f21, f22, f31, f32,
+                           h->aux.p);
+        hip_check(hipGetLastError(), "k_mixed_eval");
+        h->flat.ensure(h->nflat());
+        to_flat(h, h->aux.p, h->flat.p, 1);
+        copy_back(dsdt, h->flat.p, h->nflat(), h->stream);
+        hip_check(hipStreamSynchronize(h->stream), "rhs");
+    });
+}
+
+int csl_mixed_normalize(csl_mixed* h, double len2, double len3) {
+    return guarded([&] {
+        check_mixed(h);
+        need_state(h);
+        const int n = std::max(h->M.NP2, h->M.NP3);
+        hipLaunchKernelGGL(csl::k_mixed_normalize, dim3(blocks_of(n)), dim3(256), 0, h->stream, h->M, h->state.p,
+                           len2, len3);
+        hip_check(hipGetLastError(), "k_mixed_normalize");
+        hip_check(hipStreamSynchronize(h->stream), "normalize");
+    });
+}
+
+int csl_mixed_integrate(csl_mixed* h, double T0, double T1, double dt, size_t save_interval, const char* method,
+                        int tgrid, size_t max_rec, double* times, double* states, double* recs, size_t* n_rec_out) {
+    return guarded([&] {
+        check_mixed(h);
+        need_state(h);
+        if (!method_is_rk4(method)) fail(CSL_ERR_UNSUPPORTED, "the mixed path implements \"rk4\"");
+        if (save_interval == 0) fail(CSL_ERR_ARG, "save_interval must be >= 1");
+        auto grid = make_grid(T0, T1, dt, tgrid);
+        const size_t n_steps = grid.size() - 1;
+        const size_t n_rec = records_for(n_steps, save_interval);
+        if (n_rec_out) *n_rec_out = n_rec;
+        if ((times || states || recs) && max_rec < n_rec) fail(CSL_ERR_CAPACITY, "max_rec is smaller than the number of records");
+        hipStream_t s = h->stream;
+        h->X.ensure(h->sz());
+        hip_check(hipMemcpyAsync(h->X.p, h->state.p, h->sz() * sizeof(double), hipMemcpyDeviceToDevice, s), "copy");
+        const double one[2] = {1.0, 1.0};
+        h->scale.upload(one, 2, s);
+        h->tp.upload(h->h_tp2, 2, s);
+        h->tp3.upload(h->h_tp3, 2, s);
+        DBuf<double> st;
+        if (states) st.ensure(n_rec * h->nflat());
+        run_mixed(h, 1, h->tp.p, h->tp3.p, h->scale.p, grid, dt, save_interval, states ? st.p : nullptr);
+        hip_check(hipMemcpyAsync(h->state.p, h->X.p, h->sz() * sizeof(double), hipMemcpyDeviceToDevice, s), "copy");
+        copy_back(recs, h->recs.p, n_rec * CSL_MIXED_RECORD, s);
+        if (states) copy_back(states, st.p, n_rec * h->nflat(), s);
+        hip_check(hipStreamSynchronize(s), "integrate");
+        if (times)
+            for (size_t r = 0; r < n_rec; ++r) times[r] = grid[std::min(r * save_interval, n_steps)];
+    });
+}
+
+int csl_mixed_step(csl_mixed* h, double t, double dt, const char* method) {
+    return guarded([&] {
+        check_mixed(h);
+        need_state(h);
+        if (!method_is_rk4(method)) fail(CSL_ERR_UNSUPPORTED, "the mixed path implements \"rk4\"");
+        if (!(dt > 0.0) || !std::isfinite(t)) fail(CSL_ERR_ARG, "step needs finite t and dt > 0");
+        hipStream_t s = h->stream;
+        h->X.ensure(h->sz());
+        hip_check(hipMemcpyAsync(h->X.p, h->state.p, h->sz() * sizeof(double), hipMemcpyDeviceToDevice, s), "copy");
+        const double one[2] = {1.0, 1.0};
+        h->scale.upload(one, 2, s);
+        h->tp.upload(h->h_tp2, 2, s);
+        h->tp3.upload(h->h_tp3, 2, s);
+        std::vector<double> grid{t, t + dt};
+        run_mixed(h, 1, h->tp.p, h->tp3.p, h->scale.p, grid, dt, 1, nullptr);
+        hip_check(hipMemcpyAsync(h->state.p, h->X.p, h->sz() * sizeof(double), hipMemcpyDeviceToDevice, s), "copy");
+        hip_check(hipStreamSynchronize(s), "step");
+    });
+}
+
+int csl_mixed_integrate_batched(csl_mixed* h, size_t B, const double* init, int init_per_replica,
+                                const double* t_pulse, const double* pulse_scale, double T0, double T1, double dt,
+                                size_t save_interval, const char* method, int tgrid, size_t max_rec, double* times,
+                                double* recs, double* final_states, size_t* n_rec_out) {
+    return guarded([&] {
+        check_mixed(h);
+        if (B == 0) fail(CSL_ERR_ARG, "batch size must be >= 1");
+        if (B > 65535) fail(CSL_ERR_ARG, "at most 65535 trajectories per call");
+        if (!init || !t_pulse) fail(CSL_ERR_NULL, "null init / t_pulse");
+        if (!method_is_rk4(method)) fail(CSL_ERR_UNSUPPORTED, "the mixed path implements \"rk4\"");
+        if (save_interval == 0) fail(CSL_ERR_ARG, "save_interval must be >= 1");
+        auto grid = make_grid(T0, T1, dt, tgrid);
+        const size_t n_steps = grid.size() - 1;
+        const size_t n_rec = records_for(n_steps, save_interval);
+        if (n_rec_out) *n_rec_out = n_rec;
+        if ((times || recs) && max_rec < n_rec) fail(CSL_ERR_CAPACITY, "max_rec is smaller than the number of records");
+        const size_t nf = h->nflat();
+        const size_t n_init = init_per_replica ? B * nf : nf;
+        check_finite(init, n_init, "initial state");
+        check_finite(t_pulse, 2 * B, "t_pulse");
+        hipStream_t s = h->stream;
+        h->init.upload(init, n_init, s);
+        h->tp.upload(t_pulse, 2 * B, s);
+        std::vector<double> sc(2 * B, 1.0);
+        if (pulse_scale) {
+            check_finite(pulse_scale, 2 * B, "pulse_scale");
+            sc.assign(pulse_scale, pulse_scale + 2 * B);
+        }
+        h->scale.upload(sc.data(), sc.size(), s);
+        h->X.ensure(B * h->sz());
+        hip_check(hipMemsetAsync(h->X.p, 0, B * h->sz() * sizeof(double), s), "memset");
+        to_soa(h, h->init.p, init_per_replica ? nf : 0, h->X.p, B);
+        run_mixed(h, B, h->tp.p, h->tp.p, h->scale.p, grid, dt, save_interval, nullptr);
+        copy_back(recs, h->recs.p, n_rec * B * CSL_MIXED_RECORD, s);
+        if (final_states) {
+            h->flat.ensure(B * nf);
+            to_flat(h, h->X.p, h->flat.p, B);
+            copy_back(final_states, h->flat.p, B * nf, s);
+        }
+        hip_check(hipStreamSynchronize(s), "integrate_batched");
+        if (times)
+            for (size_t r = 0; r < n_rec; ++r) times[r] = grid[std::min(r * save_interval, n_steps)];
+    });
+}
+
+int csl_mixed_last_kernel_ms(csl_mixed* h, double* ms, size_t* launches) {
+    return guarded([&] {
+        if (!h) fail(CSL_ERR_NULL, "null mixed-lattice handle");
+        if (ms) *ms = h->last_ms;
+        if (launches) *launches = h->last_launches;
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,262 @@
+"""Device handle of a mixed SU(2) + SU(3) lattice and the reference's
+MixedLattice simulator entry points on it.
+
+GPUMixedLattice wraps one csl_mixed handle (include/csl_mixed.h).  The
+drivers mirror src/core/mixed_lattice_md.cpp and mixed_lattice.h:
+
+  molecular_dynamics        MixedLattice::molecular_dynamics_cpu, "rk4"
+                            (mixed_lattice_md.cpp:1208-1333)
+  single_pulse_drive        MixedLattice::single_pulse_drive (mixed_lattice.h:4754-4881)
+  double_pulse_drive        MixedLattice::double_pulse_drive (mixed_lattice.h:4883-)
+  pump_probe_spectroscopy   MixedLattice::pump_probe_spectroscopy (mixed_lattice_md.cpp:1400-1750),
+                            every M0 / M1(tau) / M01(tau) trajectory in one batched call
+
+Records are the observer's [M_SU2 (antiferro, local, global), M_SU3
+(antiferro, local, global)] and E/(N2+N3); times follow Boost integrate_const
+with dt = step_size and the pulse drivers' observer gate (the t = T_start
+sample is dropped, mixed_lattice.h:4797-4799).  The reference's pulse-window
+chunking integrates grid-snapped segments with the same step, so for RK4 it
+gives the same samples as one unchunked run.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _capi
+from ._capi import check, dptr, f64, lib, szptr
+from .gpu import _gate
+from .mixed_model import MixedLattice
+
+REC = 34
+
+
+def split_records(r):
+    """[..., 34] -> dict(M2 [..., 3, 3], M3 [..., 3, 8], energy [...])."""
+    r = np.asarray(r)
+    return {"M2": r[..., :9].reshape(r.shape[:-1] + (3, 3)),
+            "M3": r[..., 9:33].reshape(r.shape[:-1] + (3, 8)),
+            "energy": r[..., 33]}
+
+
+class GPUMixedLattice:
+    """Owns a device copy of a MixedLattice's tables (csl_mixed_create)."""
+
+    def __init__(self, ml: MixedLattice):
+        self.lattice = ml
+        s2, s3 = ml.su2, ml.su3
+        if s2.spin_dim != 3 or s3.spin_dim != 8:
+            raise _capi.CslError(_capi.CSL_ERR_UNSUPPORTED, "mixed path handles spin_dim 3 + 8")
+        self.A2, self.A3 = s2.N_atoms, s3.N_atoms
+        self.N2, self.N3 = s2.lattice_size, s3.lattice_size
+        n_cells = self.N2 // self.A2
+        keep = [f64(s2.field), f64(s2.onsite), szptr(s2.bi_offset), szptr(s2.bi_partner), f64(s2.bi_J),
+                f64(s3.field), f64(s3.onsite), szptr(s3.bi_offset), szptr(s3.bi_partner), f64(s3.bi_J),
+                szptr(ml.m2_offset), szptr(ml.m2_partner), f64(ml.m2_J),
+                szptr(ml.m3_offset), szptr(ml.m3_partner), f64(ml.m3_J)]
+        h = C.c_void_p()
+        args = []
+        for a in keep:
+            args.append(a.ctypes.data_as(_capi._psz) if a.dtype == np.uintp else dptr(a))
+        check(lib().csl_mixed_create(self.A2, self.A3, n_cells, *args, C.byref(h)))
+        self.h = h
+        check(lib().csl_mixed_set_frames(self.h, dptr(f64(s2.frames)), dptr(f64(s2.afm_signs)),
+                                         dptr(f64(s3.frames))))
+
+    # -- lifetime
+    def close(self):
+        if getattr(self, "h", None):
+            lib().csl_mixed_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # -- state and parameters
+    @property
+    def n_state(self):
+        return self.N2 * 3 + self.N3 * 8
+
+    def info(self):
+        n2, n3 = C.c_size_t(), C.c_size_t()
+        u = [C.c_int() for _ in range(4)]
+        check(lib().csl_mixed_info(self.h, C.byref(n2), C.byref(n3), *(C.byref(x) for x in u)))
+        return {"N2": n2.value, "N3": n3.value, "uniform": tuple(bool(x.value) for x in u)}
+
+    def set_spins(self, flat):
+        check(lib().csl_mixed_set_spins(self.h, dptr(f64(flat).reshape(-1))))
+
+    def get_spins(self) -> np.ndarray:
+        out = np.empty(self.n_state)
+        check(lib().csl_mixed_get_spins(self.h, dptr(out)))
+        return out
+
+    def set_pulse(self, species: int, drive, amp, width, freq, t1, t2):
+        """set_gpu_pulse_SU2 / SU3: drive [2, n_atoms, D] already F^T B."""
+        check(lib().csl_mixed_set_pulse(self.h, species, dptr(f64(drive).reshape(-1)), amp, width, freq, t1, t2))
+
+    def set_damping(self, alpha=0.0, gamma=None, eq=None):
+        check(lib().csl_mixed_set_damping(self.h, alpha, dptr(None if gamma is None else f64(gamma)),
+                                          dptr(None if eq is None else f64(eq).reshape(-1))))
+
+    def record(self):
+        out = np.empty(REC)
+        check(lib().csl_mixed_record(self.h, dptr(out)))
+        return out
+
+    def energy(self) -> float:
+        e = C.c_double()
+        check(lib().csl_mixed_energy(self.h, C.byref(e)))
+        return e.value
+
+    def rhs(self, t: float) -> np.ndarray:
+        out = np.empty(self.n_state)
+        check(lib().csl_mixed_rhs(self.h, t, dptr(out)))
+        return out
+
+    def normalize(self, len2=1.0, len3=1.0):
+        check(lib().csl_mixed_normalize(self.h, len2, len3))
+
+    def step(self, t, dt, method="rk4"):
+        check(lib().csl_mixed_step(self.h, t, dt, method.encode()))
+
+    def drive_dirs(self, species: int, field_in) -> np.ndarray:
+        """F^T B per atom (set_pulse_SU2/SU3, mixed_lattice_md.cpp:130-135, 208-213)."""
+        lat = self.lattice.su2 if species == 2 else self.lattice.su3
+        fi = np.asarray(field_in, dtype=float).reshape(lat.N_atoms, lat.spin_dim)
+        return np.einsum("aji,aj->ai", lat.frames, fi)
+
+    # -- integration
+    def integrate(self, T0, T1, dt, save_interval=1, method="rk4", tgrid="boost", want_states=False):
+        _, nr = _count(T0, T1, dt, save_interval, tgrid)
+        times = np.empty(nr)
+        recs = np.empty(nr * REC)
+        states = np.empty(nr * self.n_state) if want_states else None
+        n = C.c_size_t()
+        check(lib().csl_mixed_integrate(self.h, T0, T1, dt, save_interval, method.encode(), _capi.TGRID[tgrid], nr,
+                                        dptr(times), dptr(states), dptr(recs), C.byref(n)))
+        out = {"times": times, "records": recs.reshape(nr, REC)}
+        if want_states:
+            out["states"] = states.reshape(nr, self.n_state)
+        return out
+
+    def integrate_batched(self, init, t_pulse, pulse_scale=None, T0=0.0, T1=1.0, dt=0.01, save_interval=1,
+                          method="rk4", tgrid="boost", want_final=False):
+        tp = f64(t_pulse).reshape(-1, 2)
+        B = tp.shape[0]
+        init = f64(init)
+        per = 1 if init.ndim == 2 else 0
+        if per and init.shape[0] != B:
+            raise ValueError("init has a different batch size than t_pulse")
+        sc = None if pulse_scale is None else f64(pulse_scale).reshape(B, 2)
+        _, nr = _count(T0, T1, dt, save_interval, tgrid)
+        times = np.empty(nr)
+        recs = np.empty(nr * B * REC)
+        final = np.empty(B * self.n_state) if want_final else None
+        n = C.c_size_t()
+        check(lib().csl_mixed_integrate_batched(self.h, B, dptr(init.reshape(-1)), per, dptr(tp.reshape(-1)),
+                                                dptr(None if sc is None else sc.reshape(-1)), T0, T1, dt,
+                                                save_interval, method.encode(), _capi.TGRID[tgrid], nr, dptr(times),
+                                                dptr(recs), dptr(final), C.byref(n)))
+        return {"times": times, "records": recs.reshape(nr, B, REC),
+                "final": None if final is None else final.reshape(B, self.n_state)}
+
+    def last_kernel_ms(self):
+        ms, n = C.c_double(), C.c_size_t()
+        check(lib().csl_mixed_last_kernel_ms(self.h, C.byref(ms), C.byref(n)))
+        return ms.value, n.value
+
+
+def _count(T0, T1, dt, save_interval, tgrid):
+    st, nr = C.c_size_t(), C.c_size_t()
+    check(lib().csl_count_records(T0, T1, dt, save_interval, _capi.TGRID[tgrid], C.byref(st), C.byref(nr)))
+    return st.value, nr.value
+
+
+def _set_pulses(gm: GPUMixedLattice, f2a, f2b, f3a, f3b, t1, t2, amp2, w2, fr2, amp3, w3, fr3):
+    d2 = np.stack([gm.drive_dirs(2, f2a), gm.drive_dirs(2, f2b)])
+    d3 = np.stack([gm.drive_dirs(3, f3a), gm.drive_dirs(3, f3b)])
+    gm.set_pulse(2, d2, amp2, w2, fr2, t1, t2)
+    gm.set_pulse(3, d3, amp3, w3, fr3, t1, t2)
+
+
+def _reset(gm: GPUMixedLattice):
+    gm.set_pulse(2, np.zeros((2, gm.A2, 3)), 0.0, 1.0, 0.0, 0.0, 0.0)
+    gm.set_pulse(3, np.zeros((2, gm.A3, 8)), 0.0, 1.0, 0.0, 0.0, 0.0)
+
+
+def molecular_dynamics(gm: GPUMixedLattice, flat, T_start, T_end, dt, save_interval=1, method="rk4"):
+    """MixedLattice::molecular_dynamics_cpu with "rk4": a record every
+    save_interval integrate_const calls (HDF5 output omitted)."""
+    if method != "rk4":
+        raise _capi.CslError(_capi.CSL_ERR_UNSUPPORTED, "only method 'rk4' is implemented")
+    _reset(gm)
+    gm.set_spins(flat)
+    r = gm.integrate(T_start, T_end, dt, save_interval, method)
+    out = split_records(r["records"])
+    out["times"] = r["times"]
+    return out
+
+
+def single_pulse_drive(gm, flat, field_in_SU2, field_in_SU3, t_B, amp2, w2, fr2, amp3, w3, fr3,
+                       T_start, T_end, step_size):
+    """MixedLattice::single_pulse_drive: returns dict(times, M2, M3, energy)."""
+    z2, z3 = np.zeros((gm.A2, 3)), np.zeros((gm.A3, 8))
+    _set_pulses(gm, field_in_SU2, z2, field_in_SU3, z3, t_B, 0.0, amp2, w2, fr2, amp3, w3, fr3)
+    return _gated_run(gm, flat, T_start, T_end, step_size)
+
+
+def double_pulse_drive(gm, flat, field_in_SU2_1, field_in_SU3_1, t_B_1, field_in_SU2_2, field_in_SU3_2, t_B_2,
+                       amp2, w2, fr2, amp3, w3, fr3, T_start, T_end, step_size):
+    """MixedLattice::double_pulse_drive: returns dict(times, M2, M3, energy)."""
+    _set_pulses(gm, field_in_SU2_1, field_in_SU2_2, field_in_SU3_1, field_in_SU3_2, t_B_1, t_B_2, amp2, w2, fr2,
+                amp3, w3, fr3)
+    return _gated_run(gm, flat, T_start, T_end, step_size)
+
+
+def _gated_run(gm, flat, T_start, T_end, step):
+    gm.set_spins(flat)
+    r = gm.integrate(T_start, T_end, step, 1, "rk4")
+    _reset(gm)
+    keep = _gate(r["times"], T_start, T_end, step)
+    out = split_records(r["records"][keep])
+    out["times"] = r["times"][keep]
+    return out
+
+
+def pump_probe_spectroscopy(gm, ground, field_in_SU2, field_in_SU3, amp2, w2, fr2, amp3, w3, fr3,
+                            tau_start, tau_end, tau_step, T_start, T_end, T_step):
+    """MixedLattice::pump_probe_spectroscopy without HDF5 output: M0 (pump
+    at 0), M1(tau) (pump at tau) and M01(tau) (pumps at 0 and tau) for
+    n_tau = floor(|tau_end - tau_start| / tau_step) + 1 delays, integrated as
+    1 + 2 n_tau trajectories in one batched call.  Returns dict(times, tau,
+    M0, M1, M01) with each M a split_records dict."""
+    n_tau = int(abs((tau_end - tau_start) / tau_step)) + 1
+    taus = tau_start + tau_step * np.arange(n_tau)
+    _set_pulses(gm, field_in_SU2, field_in_SU2, field_in_SU3, field_in_SU3, 0.0, 0.0, amp2, w2, fr2,
+                amp3, w3, fr3)
+    B = 1 + 2 * n_tau
+    tp = np.zeros((B, 2))
+    sc = np.zeros((B, 2))
+    sc[0, 0] = 1.0
+    tp[1:1 + n_tau, 0] = taus
+    sc[1:1 + n_tau, 0] = 1.0
+    tp[1 + n_tau:, 1] = taus
+    sc[1 + n_tau:, :] = 1.0
+    r = gm.integrate_batched(np.asarray(ground, float).reshape(-1), tp, sc, T_start, T_end, T_step)
+    _reset(gm)
+    keep = _gate(r["times"], T_start, T_end, T_step)
+    rec = r["records"][keep]
+    return {"times": r["times"][keep], "tau": taus, "M0": split_records(rec[:, 0]),
+            "M1": split_records(np.swapaxes(rec[:, 1:1 + n_tau], 0, 1)),
+            "M01": split_records(np.swapaxes(rec[:, 1 + n_tau:], 0, 1))}
