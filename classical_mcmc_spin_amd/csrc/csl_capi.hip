@@ -1035,6 +1035,140 @@ int csl_set_replica_hamiltonians(csl_lattice* h, size_t B, const double* J, cons
     });
 }
 
+int csl_adjoint_gradient(csl_lattice* h, size_t B, const double* init, int init_per_replica,
+                         const double* t_pulse, const double* pulse_amp, double T0, double T1, double dt,
+                         int tgrid, const double* target, size_t n_target, const double* weights,
+                         size_t n_params, const double* dJ, const double* dB, size_t ckpt_interval,
+                         double* loss, double* grad, double* records, double* grad_init) {
+    return guarded([&] {
+        check_handle(h);
+        if (B == 0 || B > 65535) fail(CSL_ERR_ARG, "batch size must be 1 .. 65535");
+        if (!init || !t_pulse || !target || !weights || !loss || !grad) fail(CSL_ERR_NULL, "null argument");
+        if (n_params > 8) fail(CSL_ERR_ARG, "at most 8 parameters");
+        if (n_params && (!dJ || !dB)) fail(CSL_ERR_NULL, "null parameter tangents");
+        csl::LatDev L = h->L;
+        if (!L.uniform) fail(CSL_ERR_UNSUPPORTED, "the adjoint needs translation-invariant bond tables");
+        if (L.zmax > 6) fail(CSL_ERR_UNSUPPORTED, "the adjoint handles up to 6 bonds per site");
+        if (csl::adjoint_sites_per_thread(L) == 0)
+            fail(CSL_ERR_UNSUPPORTED, "lattice too large for the one-workgroup adjoint kernel");
+        if (h->rep_B && h->rep_B != B) fail(CSL_ERR_ARG, "replica Hamiltonians were set for another batch size");
+        auto grid = make_grid(T0, T1, dt, tgrid);
+        const size_t n_steps = grid.size() - 1;
+        if (n_steps == 0) fail(CSL_ERR_ARG, "empty time grid");
+        if (n_steps > size_t(std::numeric_limits<int>::max() / 4)) fail(CSL_ERR_ARG, "too many steps");
+        const size_t n_rec = n_steps + 1;
+        if (n_target != n_rec) fail(CSL_ERR_CAPACITY, "target must hold one record per step plus the final one");
+        const size_t nd = h->N * 3;
+        check_finite(init, init_per_replica ? B * nd : nd, "initial state");
+        check_finite(t_pulse, 2 * B, "t_pulse");
+        check_finite(target, n_rec * 9, "target");
+        check_finite(weights, 9, "weights");
+        const size_t A_ = h->n_atoms, N = h->N, nnz = h->h_off[N];
+        if (n_params) {
+            check_finite(dJ, n_params * nnz * 9, "dJ");
+            check_finite(dB, n_params * N * 3, "dB");
+        }
+        // parameter tangents -> per (sublattice, slot) / per sublattice
+        std::vector<double> dm(std::max<size_t>(n_params, 1) * A_ * L.zmax * 9, 0.0),
+            dfl(std::max<size_t>(n_params, 1) * A_ * 3, 0.0);
+        for (size_t q = 0; q < n_params; ++q) {
+            std::vector<char> seen(A_ * L.zmax, 0);
+            for (size_t i = 0; i < N; ++i) {
+                const size_t atom = i % A_;
+                for (size_t k = h->h_off[i]; k < h->h_off[i + 1]; ++k) {
+                    double* dst = &dm[((q * A_ + atom) * L.zmax + h->h_slot_of[k]) * 9];
+                    const double* src = dJ + (q * nnz + k) * 9;
+                    if (!seen[atom * L.zmax + h->h_slot_of[k]]) {
+                        std::memcpy(dst, src, 9 * sizeof(double));
+                        seen[atom * L.zmax + h->h_slot_of[k]] = 1;
+                    } else if (std::memcmp(dst, src, 9 * sizeof(double)) != 0) {
+                        fail(CSL_ERR_ARG, "dJ is not translation invariant like the handle's tables");
+                    }
+                }
+                const double* fb = dB + (q * N + i) * 3;
+                double* fd = &dfl[(q * A_ + atom) * 3];
+                if (i < A_) std::memcpy(fd, fb, 3 * sizeof(double));
+                else if (std::memcmp(fd, fb, 3 * sizeof(double)) != 0)
+                    fail(CSL_ERR_ARG, "dB is not translation invariant");
+            }
+        }
+        hipStream_t s = h->stream;
+        DBuf<double> d_dm, d_df, d_tgt, d_ckpt, d_seg, d_rec, d_loss, d_grad, d_gi;
+        d_dm.upload(dm.data(), dm.size(), s);
+        d_df.upload(dfl.data(), dfl.size(), s);
+        d_tgt.upload(target, n_rec * 9, s);
+        h->bt_init.upload(init, init_per_replica ? B * nd : nd, s);
+        h->bt_tp.upload(t_pulse, 2 * B, s);
+        std::vector<double> am(2 * B, h->amp);
+        if (pulse_amp) {
+            check_finite(pulse_amp, 2 * B, "pulse_amp");
+            am.assign(pulse_amp, pulse_amp + 2 * B);
+        }
+        h->bt_amp.upload(am.data(), am.size(), s);
+        bool drive_on = false;
+        for (double a : am) drive_on |= (a != 0.0);
+        h->tgrid.upload(grid.data(), grid.size(), s);
+        if (drive_on) {
+            h->fac.ensure(n_steps * 3 * B * 2);
+            hip_check(csl::launch_drive_table(h->tgrid.p, int(n_steps), dt, int(B), h->bt_tp.p, h->bt_amp.p, L.width,
+                                              L.freq, h->fac.p, s),
+                      "drive table");
+        }
+        const size_t C = ckpt_interval ? ckpt_interval : size_t(std::ceil(std::sqrt(double(n_steps))));
+        const size_t n_ckpt = (n_steps + C - 1) / C;
+        const size_t SZ = size_t(3) * L.NP;
+        d_ckpt.ensure(B * n_ckpt * SZ);
+        d_seg.ensure(B * C * SZ);
+        d_rec.ensure(n_rec * B * 9);
+        d_loss.ensure(B);
+        d_grad.ensure(B * std::max<size_t>(n_params, 1));
+        if (grad_init) d_gi.ensure(B * nd);
+        csl::AdjArgs Aa{};
+        Aa.B = int(B);
+        Aa.n_steps = int(n_steps);
+        Aa.ckpt_interval = int(C);
+        Aa.n_ckpt = int(n_ckpt);
+        Aa.P = int(n_params);
+        Aa.dt = dt;
+        Aa.init = h->bt_init.p;
+        Aa.init_stride = init_per_replica ? nd : 0;
+        Aa.fac = drive_on ? h->fac.p : nullptr;
+        Aa.target = d_tgt.p;
+        for (int j = 0; j < 9; ++j) Aa.w[j] = weights[j];
+        Aa.dmats = d_dm.p;
+        Aa.dfield = d_df.p;
+        Aa.ckpt = d_ckpt.p;
+        Aa.seg = d_seg.p;
+        Aa.rec = d_rec.p;
+        Aa.loss = d_loss.p;
+        Aa.grad = d_grad.p;
+        Aa.grad_init = grad_init ? d_gi.p : nullptr;
+        if (h->rep_B) {
+            L.mats = h->rep_mats.p;
+            L.field = h->rep_field.p;
+            Aa.mats_stride = h->rep_mats_stride;
+            Aa.field_stride = h->rep_field_stride;
+        }
+        hipEvent_t e0, e1;
+        hip_check(hipEventCreate(&e0), "event");
+        hip_check(hipEventCreate(&e1), "event");
+        hip_check(hipEventRecord(e0, s), "event record");
+        hip_check(csl::launch_adjoint(L, Aa, s), "k_adjoint launch");
+        hip_check(hipEventRecord(e1, s), "event record");
+        copy_back(loss, d_loss.p, B, s);
+        copy_back(grad, d_grad.p, B * n_params, s);
+        copy_back(records, d_rec.p, n_rec * B * 9, s);
+        copy_back(grad_init, d_gi.p, B * nd, s);
+        hip_check(hipStreamSynchronize(s), "adjoint");
+        float ms = 0.f;
+        hip_check(hipEventElapsedTime(&ms, e0, e1), "elapsed");
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        h->last_ms = ms;
+        h->last_launches = 1;
+    });
+}
+
 int csl_set_launch_config(csl_lattice* h, size_t B, int block, int sites_per_thread, int double_buffer) {
     return guarded([&] {
         check_handle(h);

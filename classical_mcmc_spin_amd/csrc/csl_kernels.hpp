@@ -27,6 +27,35 @@ struct PersistArgs {
     size_t field_stride;
 };
 
+// Discrete-adjoint gradient run (csl_adjoint.hip): one workgroup per trajectory.
+struct AdjArgs {
+    int B;
+    int n_steps;
+    int ckpt_interval;         // steps per checkpoint segment
+    int n_ckpt;                // ceil(n_steps / ckpt_interval)
+    int P;                     // parameters (<= 8)
+    double dt;
+    const double* init;        // AoS reference order; trajectory b at init + b*init_stride
+    size_t init_stride;
+    const double* fac;         // [n_steps][3][B][2] drive factors or null
+    const double* target;      // [n_steps+1][9] target records
+    double w[9];               // loss weights of the 9 record components
+    const double* dmats;       // [P][n_atoms][zmax][9] dJ/dtheta per (sublattice, slot)
+    const double* dfield;      // [P][n_atoms][3]      dB/dtheta per sublattice
+    double* ckpt;              // [B][n_ckpt][3*NP] checkpoint states (SoA)
+    double* seg;               // [B][ckpt_interval][3*NP] states of one segment
+    double* rec;               // [n_steps+1][B][9] forward records
+    double* loss;              // [B]
+    double* grad;              // [B][P]
+    double* grad_init;         // [B][N*3] dL/dx_0 (AoS) or null
+    size_t mats_stride;        // per-trajectory Hamiltonians (0 = shared)
+    size_t field_stride;
+};
+
+size_t adjoint_lds_bytes(const LatDev& L, int block);
+int adjoint_sites_per_thread(const LatDev& L);
+hipError_t launch_adjoint(const LatDev& L, const AdjArgs& A, hipStream_t st);
+
 // One RK4 stage over all sites of B trajectories (SoA [B][3][NP] buffers).
 struct StageArgs {
     int B;

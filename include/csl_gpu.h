@@ -226,6 +226,36 @@ int csl_last_launch_config(csl_lattice* h, int* block, int* sites_per_thread, in
  * class (as the autotuner would); rejects shapes that do not fit. */
 int csl_set_launch_config(csl_lattice* h, size_t B, int block, int sites_per_thread, int flags);
 
+/* Gradient of a trajectory loss by the discrete adjoint of RK4.  The
+ * reference has no gradient path; this entry is for fitting Hamiltonian
+ * parameters to a target M(t) (the reference's MD path is the forward model:
+ * lattice_md.cpp:39-153 RHS, Boost runge_kutta4 steps, lattice_md.cpp:383-436
+ * observer).
+ * For each trajectory b (initial state init [N*3] shared or [B*N*3] with
+ * init_per_replica; pulses t_pulse / pulse_amp [B*2] as in
+ * csl_integrate_batched; the handle's per-replica Hamiltonians when set with
+ * csl_set_replica_hamiltonians for this B), integrate T0 -> T1 (grid policy
+ * tgrid), record m = [M_anti, M_local, M_global] at every step and after the
+ * last, and return
+ *   loss[b]   = sum_n sum_j weights[j] (m_bj(t_n) - target[n*9 + j])^2
+ *   grad[b*P + p] = d loss[b] / d theta_p
+ * for P = n_params <= 8 parameters entering the Hamiltonian linearly:
+ * dJ [P*nnz*D*D] in the handle's CSR bond order and dB [P*N*D] per site
+ * (both translation invariant when the handle's tables are).  n_target must be
+ * n_steps + 1.  The states are checkpointed every ckpt_interval steps
+ * (0 = ceil(sqrt(n_steps))) and segments are recomputed on the way back.
+ * Optional outputs: records [n_target*B*9] (the forward m), grad_init
+ * [B*N*3] (d loss / d initial state).  Requires translation-invariant bond
+ * tables whose reverse bonds carry J^T (the Lattice expansion,
+ * lattice.h:790-803), at most 6 bonds per site and 3*NP <= 6144 doubles
+ * (one workgroup per trajectory). */
+int csl_adjoint_gradient(csl_lattice* h, size_t B, const double* init, int init_per_replica,
+                         const double* t_pulse, const double* pulse_amp, double T0, double T1,
+                         double dt, int tgrid, const double* target, size_t n_target,
+                         const double* weights, size_t n_params, const double* dJ,
+                         const double* dB, size_t ckpt_interval, double* loss, double* grad,
+                         double* records, double* grad_init);
+
 #ifdef __cplusplus
 }
 #endif
