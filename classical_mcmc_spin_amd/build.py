@@ -20,7 +20,8 @@ LIB_NAME = "libcsl_hip.so"
 LIB_PATH = os.path.join(HERE, LIB_NAME)
 
 # (source, object suffix, extra flags)
-HIP_UNITS = [("csl_kernels.hip", "", []), ("csl_capi.hip", "", []), ("csl_mixed.hip", "", []), ("csl_adjoint.hip", "", [])] + [
+HIP_UNITS = [("csl_kernels.hip", "", []), ("csl_capi.hip", "", []), ("csl_mixed.hip", "", []), ("csl_adjoint.hip", "", []),
+             ("csl_dssf.hip", "", [])] + [
     ("csl_persistent.hip", f"_spt{n}", [f"-DCSL_SPT={n}"]) for n in (1, 2, 3, 4, 6)]
 CXX_UNITS = ["gpu_api_shim.cpp", "mixed_gpu_api_shim.cpp"]
 
@@ -60,7 +61,7 @@ def build(verbose: bool = False, jobs: int = 0) -> str:
     with ThreadPoolExecutor(max_workers=jobs) as ex:
         list(ex.map(lambda c: _run(c, verbose), cmds))
     tmp = LIB_PATH + ".tmp"
-    _run([hipcc(), "--offload-arch=gfx950", "-shared", "-fPIC", *objs, "-o", tmp], verbose)
+    _run([hipcc(), "--offload-arch=gfx950", "-shared", "-fPIC", *objs, "-L/opt/rocm/lib", "-lhipfft", "-Wl,-rpath,/opt/rocm/lib", "-o", tmp], verbose)
     os.replace(tmp, LIB_PATH)
     build_cxx_test(verbose)
     return LIB_PATH

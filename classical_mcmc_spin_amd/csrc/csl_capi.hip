@@ -10,12 +10,15 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <functional>
 #include <limits>
 #include <map>
 #include <stdexcept>
 #include <string>
 #include <tuple>
 #include <vector>
+
+#include <hipfft/hipfft.h>
 
 #include "../../include/csl_gpu.h"
 #include "csl_host.hpp"
@@ -359,7 +362,8 @@ void run_persistent(csl_lattice* h, size_t B, const double* d_init, size_t init_
 // grid.y; X must already hold the SoA initial state(s).
 void run_stages(csl_lattice* h, size_t B, const double* d_tp, const double* d_amp, bool drive_on,
                 const std::vector<double>& grid, double dt, size_t si, bool want_energy,
-                double* d_states_out /* [n_rec][B][N*3] or null */) {
+                double* d_states_out /* [n_rec][B][N*3] or null */,
+                const std::function<void(size_t)>& at_step = {} /* called with X = x_n */) {
     const size_t n_steps = grid.size() - 1;
     const size_t n_rec = records_for(n_steps, si);
     hipStream_t s = h->stream;
@@ -388,6 +392,7 @@ void run_stages(csl_lattice* h, size_t B, const double* d_tp, const double* d_am
     for (size_t n = 0; n < n_steps; ++n) {
         const bool rec = (n % si) == 0;
         const size_t ri = n / si;
+        if (at_step) at_step(n);
         if (rec && d_states_out)
             hip_check(csl::launch_perm_to_aos(L, h->X.p, d_states_out + ri * B * h->N * 3, int(B), s),
                       "state record");
@@ -1166,6 +1171,93 @@ int csl_adjoint_gradient(csl_lattice* h, size_t B, const double* init, int init_
         (void)hipEventDestroy(e1);
         h->last_ms = ms;
         h->last_launches = 1;
+    });
+}
+
+int csl_structure_factor(csl_lattice* h, size_t B, const double* init, int init_per_replica, double T0,
+                         double T1, double dt, int tgrid, size_t sample_interval, size_t L1, size_t L2,
+                         size_t L3, const double* positions, size_t max_t, double* sqw, size_t* n_t_out,
+                         double* mags) {
+    return guarded([&] {
+        check_handle(h);
+        if (B == 0 || B > 65535) fail(CSL_ERR_ARG, "batch size must be 1 .. 65535");
+        if (!init || !positions || !sqw) fail(CSL_ERR_NULL, "null argument");
+        if (sample_interval == 0) fail(CSL_ERR_ARG, "sample_interval must be >= 1");
+        const csl::LatDev& L = h->L;
+        if (L1 * L2 * L3 != size_t(L.n_cells)) fail(CSL_ERR_ARG, "L1*L2*L3 must equal the number of unit cells");
+        check_finite(positions, h->n_atoms * 3, "positions");
+        auto grid = make_grid(T0, T1, dt, tgrid);
+        const size_t n_steps = grid.size() - 1;
+        const size_t n_t = n_steps / sample_interval;
+        if (n_t_out) *n_t_out = n_t;
+        if (n_t < 2) fail(CSL_ERR_ARG, "fewer than two time samples");
+        const size_t Q = L1 * L2 * L3;
+        if (max_t < n_t) fail(CSL_ERR_CAPACITY, "max_t is smaller than the number of time samples");
+        const size_t nd = h->N * 3;
+        check_finite(init, init_per_replica ? B * nd : nd, "initial state");
+        hipStream_t s = h->stream;
+        h->bt_init.upload(init, init_per_replica ? B * nd : nd, s);
+        h->X.ensure(B * 3 * size_t(L.NP));
+        hip_check(hipMemsetAsync(h->X.p, 0, B * 3 * size_t(L.NP) * sizeof(double), s), "memset");
+        hip_check(csl::launch_perm_to_soa(L, h->bt_init.p, init_per_replica ? nd : 0, h->X.p, int(B), s), "perm");
+        DBuf<double> pos, cin, cf, sqt, out;
+        pos.upload(positions, h->n_atoms * 3, s);
+        const size_t nbatch3 = B * h->n_atoms * 3;
+        cin.ensure(2 * nbatch3 * Q);
+        cf.ensure(2 * nbatch3 * Q);
+        const size_t R = B * 3 * Q;
+        sqt.ensure(2 * n_t * R);
+        out.ensure(Q * n_t);
+        hipfftHandle p3 = 0, p1 = 0;
+        auto fft_check = [](hipfftResult r, const char* what) {
+            if (r != HIPFFT_SUCCESS) fail(CSL_ERR_HIP, std::string(what) + " failed (hipFFT status " + std::to_string(int(r)) + ")");
+        };
+        struct PlanGuard {
+            hipfftHandle& a;
+            hipfftHandle& b;
+            ~PlanGuard() {
+                if (a) hipfftDestroy(a);
+                if (b) hipfftDestroy(b);
+            }
+        } guard{p3, p1};
+        int dims3[3] = {int(L1), int(L2), int(L3)};
+        fft_check(hipfftCreate(&p3), "hipfftCreate");
+        size_t ws = 0;
+        fft_check(hipfftMakePlanMany(p3, 3, dims3, nullptr, 1, int(Q), nullptr, 1, int(Q), HIPFFT_Z2Z, int(nbatch3), &ws),
+                  "3-D plan");
+        fft_check(hipfftSetStream(p3, s), "hipfftSetStream");
+        int dims1[1] = {int(n_t)};
+        fft_check(hipfftCreate(&p1), "hipfftCreate");
+        fft_check(hipfftMakePlanMany(p1, 1, dims1, dims1, int(R), 1, dims1, int(R), 1, HIPFFT_Z2Z, int(R), &ws),
+                  "time plan");
+        fft_check(hipfftSetStream(p1, s), "hipfftSetStream");
+        auto* cin2 = reinterpret_cast<double2*>(cin.p);
+        auto* cf2 = reinterpret_cast<double2*>(cf.p);
+        auto* sqt2 = reinterpret_cast<double2*>(sqt.p);
+        auto sample = [&](size_t n) {
+            if (n % sample_interval || n / sample_interval >= n_t) return;
+            const size_t ti = n / sample_interval;
+            hip_check(csl::launch_sq_fill(L, h->X.p, int(B), cin2, s), "k_sq_fill");
+            fft_check(hipfftExecZ2Z(p3, reinterpret_cast<hipfftDoubleComplex*>(cin2),
+                                    reinterpret_cast<hipfftDoubleComplex*>(cf2), HIPFFT_FORWARD),
+                      "3-D FFT");
+            hip_check(csl::launch_sq_combine(cf2, int(B), int(h->n_atoms), int(L1), int(L2), int(L3), pos.p,
+                                             sqt2 + ti * R, s),
+                      "k_sq_combine");
+        };
+        std::vector<double> zero(2 * B, 0.0);
+        h->bt_tp.upload(zero.data(), zero.size(), s);
+        h->bt_amp.upload(zero.data(), zero.size(), s);
+        if (h->rep_B) fail(CSL_ERR_UNSUPPORTED, "per-replica Hamiltonians need the persistent kernel");
+        run_stages(h, B, h->bt_tp.p, h->bt_amp.p, false, grid, dt, 1, false, nullptr, sample);
+        fft_check(hipfftExecZ2Z(p1, reinterpret_cast<hipfftDoubleComplex*>(sqt2),
+                                reinterpret_cast<hipfftDoubleComplex*>(sqt2), HIPFFT_FORWARD),
+                  "time FFT");
+        hip_check(csl::launch_sq_power(sqt2, int(n_t), int(B), Q, 1.0 / (double(B) * double(h->N) * double(n_t)), out.p, s),
+                  "k_sq_power");
+        copy_back(sqw, out.p, Q * n_t, s);
+        copy_back(mags, h->rec_mags.p, (n_steps + 1) * B * 9, s);
+        hip_check(hipStreamSynchronize(s), "structure factor");
     });
 }
 

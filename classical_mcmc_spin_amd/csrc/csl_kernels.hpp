@@ -52,6 +52,12 @@ struct AdjArgs {
     size_t field_stride;
 };
 
+// structure factor (csl_dssf.hip)
+hipError_t launch_sq_fill(const LatDev& L, const double* x, int B, double2* out, hipStream_t st);
+hipError_t launch_sq_combine(const double2* F, int B, int n_atoms, int L1, int L2, int L3, const double* pos,
+                             double2* sqt, hipStream_t st);
+hipError_t launch_sq_power(const double2* sw, int T, int B, size_t Q, double scale, double* out, hipStream_t st);
+
 size_t adjoint_lds_bytes(const LatDev& L, int block);
 int adjoint_sites_per_thread(const LatDev& L);
 hipError_t launch_adjoint(const LatDev& L, const AdjArgs& A, hipStream_t st);
@@ -76,7 +82,7 @@ struct StageArgs {
 // run in smaller workgroups.
 template <int SPT>
 constexpr int persistent_max_block() {
-    return SPT <= 2 ? 1024 : (SPT <= 4 ? 768 : 512);
+    return SPT <= 3 ? 1024 : (SPT <= 4 ? 768 : 512);
 }
 
 // Launcher of the persistent kernel for one sites-per-thread value; each

@@ -180,6 +180,28 @@ class GPULattice:
                 "energy": None if energy is None else energy.reshape(nr, B),
                 "final": None if final is None else final.reshape(B, self.N, 3)}
 
+    def structure_factor(self, init, T0, T1, dt, sample_interval, dims, positions, tgrid="boost",
+                         want_mags=False):
+        """csl_structure_factor: S(q, w) [L1*L2*L3, n_t] of the undriven
+        trajectories started from init ([N,3] or [B,N,3]); positions are the
+        atoms' fractional coordinates [n_atoms, 3]."""
+        init = f64(init)
+        B = init.shape[0] if init.ndim == 3 else 1
+        per = 1 if init.ndim == 3 else 0
+        L1, L2, L3 = (int(v) for v in dims)
+        n_steps, _ = self.count_records(T0, T1, dt, 1, tgrid)
+        nt_max = n_steps // sample_interval
+        sqw = np.empty(L1 * L2 * L3 * max(nt_max, 1))
+        mags = np.empty((n_steps + 1) * B * 9) if want_mags else None
+        nt = C.c_size_t()
+        check(lib().csl_structure_factor(self.h, B, dptr(init.reshape(-1)), per, T0, T1, dt, _capi.TGRID[tgrid],
+                                         sample_interval, L1, L2, L3, dptr(f64(positions).reshape(-1)), nt_max,
+                                         dptr(sqw), C.byref(nt), dptr(mags)))
+        out = {"sqw": sqw[:L1 * L2 * L3 * nt.value].reshape(L1 * L2 * L3, nt.value)}
+        if want_mags:
+            out["mags"] = mags.reshape(n_steps + 1, B, 3, 3)
+        return out
+
     def autotune(self, B: int, n_probe_steps: int = 64):
         blk, ms = C.c_int(), C.c_double()
         check(lib().csl_autotune_batched(self.h, B, n_probe_steps, C.byref(blk), C.byref(ms)))

@@ -51,14 +51,13 @@ class GPUMixedLattice:
         self.A2, self.A3 = s2.N_atoms, s3.N_atoms
         self.N2, self.N3 = s2.lattice_size, s3.lattice_size
         n_cells = self.N2 // self.A2
-        keep = [f64(s2.field), f64(s2.onsite), szptr(s2.bi_offset), szptr(s2.bi_partner), f64(s2.bi_J),
-                f64(s3.field), f64(s3.onsite), szptr(s3.bi_offset), szptr(s3.bi_partner), f64(s3.bi_J),
-                szptr(ml.m2_offset), szptr(ml.m2_partner), f64(ml.m2_J),
-                szptr(ml.m3_offset), szptr(ml.m3_partner), f64(ml.m3_J)]
+        u = lambda a: np.ascontiguousarray(a, dtype=np.uintp)
+        keep = [f64(s2.field), f64(s2.onsite), u(s2.bi_offset), u(s2.bi_partner), f64(s2.bi_J),
+                f64(s3.field), f64(s3.onsite), u(s3.bi_offset), u(s3.bi_partner), f64(s3.bi_J),
+                u(ml.m2_offset), u(ml.m2_partner), f64(ml.m2_J),
+                u(ml.m3_offset), u(ml.m3_partner), f64(ml.m3_J)]
         h = C.c_void_p()
-        args = []
-        for a in keep:
-            args.append(a.ctypes.data_as(_capi._psz) if a.dtype == np.uintp else dptr(a))
+        args = [szptr(a) if a.dtype == np.uintp else dptr(a) for a in keep]
         check(lib().csl_mixed_create(self.A2, self.A3, n_cells, *args, C.byref(h)))
         self.h = h
         check(lib().csl_mixed_set_frames(self.h, dptr(f64(s2.frames)), dptr(f64(s2.afm_signs)),

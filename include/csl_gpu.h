@@ -45,7 +45,7 @@ extern "C" {
 #define CSL_ERR_CAPACITY    -6   /* output buffer too small                  */
 #define CSL_ERR_NONFINITE   -7   /* NaN / Inf in an input state              */
 
-/* time-grid policies for the integrators (see DESIGN notes in csl_time.h):
+/* time-grid policies for the integrators (the host computes the step times):
  *   CSL_TGRID_GPU     t += dt while t < T1                (integrator.cuh:470-487)
  *   CSL_TGRID_BATCHED t += dt while t < T1 - 1e-12        (lattice_gpu.cu:557-565)
  *   CSL_TGRID_BOOST   t_k = T0 + k*dt while t_k+dt <= T1  (Boost integrate_const,
@@ -225,6 +225,23 @@ int csl_last_launch_config(csl_lattice* h, int* block, int* sites_per_thread, in
 /* Fix the persistent launch shape used for this lattice shape and batch size
  * class (as the autotuner would); rejects shapes that do not fit. */
 int csl_set_launch_config(csl_lattice* h, size_t B, int block, int sites_per_thread, int flags);
+
+/* Dynamical structure factor of B undriven trajectories (the reference
+ * computes it in Python from saved spin snapshots,
+ * util/readers_new/reader_pyrochlore.py:150-270):
+ *   S_a(q, t) = sum_i exp(-i q.r_i) S_ia(t),  q = 2 pi (h/L1, k/L2, l/L3)
+ *   sqw[q * n_t + w] = 1/(B N n_t) sum_b sum_a |sum_t exp(-2 pi i w t/n_t) S_a(q, t)|^2
+ * with q = (h*L2 + k)*L3 + l, r_i = cell (i, j, k) + positions[atom]
+ * (fractional coordinates of the lattice vectors), t over the n_t =
+ * floor(n_steps / sample_interval) samples taken every sample_interval RK4
+ * steps from T0 (grid policy tgrid).  L1*L2*L3 must equal the number of
+ * cells.  Runs the grid-wide stage kernels; the FFTs are hipFFT.  mags
+ * [(n_steps+1)*B*9] (optional) receives M(t) at every step. */
+int csl_structure_factor(csl_lattice* h, size_t B, const double* init, int init_per_replica,
+                         double T0, double T1, double dt, int tgrid, size_t sample_interval,
+                         size_t L1, size_t L2, size_t L3, const double* positions, size_t max_t,
+                         double* sqw, size_t* n_t, double* mags);
+
 
 /* Gradient of a trajectory loss by the discrete adjoint of RK4.  The
  * reference has no gradient path; this entry is for fitting Hamiltonian

@@ -225,3 +225,19 @@ def test_random_su3_states():
     assert v.shape == (50, 8)
     assert np.allclose(np.linalg.norm(v, axis=1), 1.0)
     assert np.array_equal(v, mm.random_su3_states(50, 4))
+
+
+def test_mixed_handle_marshalling_without_device():
+    """The host wrapper marshals every table into csl_mixed_create; without a
+    GPU the library reports a HIP error (never a Python-side failure)."""
+    from classical_mcmc_spin_amd import _capi
+    from classical_mcmc_spin_amd.gpu_mixed import GPUMixedLattice
+    try:
+        has_gpu = _capi.device_count() > 0
+    except _capi.CslError:
+        has_gpu = False
+    if has_gpu:
+        pytest.skip("a GPU is present")
+    ml = mm.MixedLattice(rich_cell(), 2, 2, 2)
+    with pytest.raises(_capi.CslError):
+        GPUMixedLattice(ml)
