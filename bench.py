@@ -155,6 +155,143 @@ def bench_pyrochlore(args):
     print(json.dumps(line), flush=True)
 
 
+MD_L, MD_STEPS, MD_DT = 24, 10000, 0.01
+
+
+def bench_md(args):
+    """Plain one-trajectory MD: Kitaev 24x24x1 (1152 sites), K=-1, G=0.25,
+    G'=-0.02, h = 0, seeded random start, 10^4 RK4 steps of dt = 0.01, M(t)
+    and E(t) every step.  value = trajectories/s of the fastest device engine;
+    the three engines (grid-wide stage kernels launched eagerly, the same
+    launch sequence replayed from a hipGraph, one persistent workgroup) are
+    reported beside it."""
+    from classical_mcmc_spin_amd import GPULattice, model, seeding
+    uc = model.build_kitaev_honeycomb(K=-1.0, Gamma=0.25, Gammap=-0.02)
+    lat = model.Lattice(uc, MD_L, MD_L, 1)
+    s0 = seeding.random_spins(lat.lattice_size, 1234)
+    T1m = (MD_STEPS + 0.5) * MD_DT
+    gl = GPULattice(lat)
+    eng = {}
+    for name, e in (("stage_kernels", 1), ("stage_kernels_hipgraph", 3), ("persistent_workgroup", 2)):
+        for _ in range(max(args.warmup, 1)):
+            gl.set_spins(s0)
+            gl.integrate(0.0, T1m, MD_DT, 1, tgrid="boost", engine=e)
+        t = time.perf_counter()
+        for _ in range(args.steps):
+            gl.set_spins(s0)
+            r = gl.integrate(0.0, T1m, MD_DT, 1, tgrid="boost", engine=e)
+        el = (time.perf_counter() - t) / args.steps
+        assert r["mags"].shape[0] == MD_STEPS + 1 and np.all(np.isfinite(r["mags"]))
+        eng[name] = {"traj_per_s": 1.0 / el, "ms_per_traj": 1e3 * el,
+                     "site_steps_per_s": MD_STEPS * lat.lattice_size / el, "kernel_ms": gl.last_kernel_ms()[0]}
+    gl.close()
+    best = max(eng, key=lambda k: eng[k]["traj_per_s"])
+    v = eng[best]["traj_per_s"]
+    line = {
+        "metric": METRIC, "value": v, "unit": "trajectories/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": 1e3 / v, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "md_kitaev_honeycomb_24x24x1_1traj_10000steps_rk4_dt0.01", "sites": lat.lattice_size,
+                   "rk4_steps": MD_STEPS, "record_every": 1, "engine": best},
+        "site_steps_per_s": eng[best]["site_steps_per_s"], "engines": eng,
+        "roofline": {"bound": "hbm", "achieved": v * MD_STEPS * lat.lattice_size * BYTES_PER_SITE_STEP / 1e9,
+                     "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                     "frac": v * MD_STEPS * lat.lattice_size * BYTES_PER_SITE_STEP / HBM_PEAK, "traffic": None,
+                     "kernel": best},
+    }
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import pyoracle
+        o = pyoracle.OracleLattice(uc, MD_L, MD_L, 1)
+        n = 2000
+        t = time.perf_counter()
+        o.integrate_md(s0.reshape(-1), 0.0, (n + 0.5) * MD_DT, MD_DT)
+        el = time.perf_counter() - t
+        line["cpu_baseline"] = {
+            "value": n / el / MD_STEPS, "unit": "trajectories/s", "cores": pyoracle.max_threads(), "kind": "port",
+            "sample": f"{n} of the {MD_STEPS} RK4 steps ({el:.1f} s, OpenMP over sites inside each RHS as the "
+                      f"reference's MD path)",
+            "site_steps_per_s": n * lat.lattice_size / el}
+    print(json.dumps(line), flush=True)
+
+
+ADJ_L, ADJ_STEPS, ADJ_B = 24, 2000, 32
+
+
+def bench_adjoint(args):
+    """Adjoint gradients: Kitaev 24x24 (1152 sites), one pulse (A = 0.5,
+    w = 0.5 at t = 1), Gilbert alpha = 0.02, loss on M_global over 2000 RK4
+    steps (dt = 0.01) against a target run at (K, G, G', J, h) =
+    (-1, 0.25, -0.02, 0, 0.05); 32 parameter starts within +-10 %.
+    value = gradients (starts) per second."""
+    from classical_mcmc_spin_amd import adjoint as AD
+    from classical_mcmc_spin_amd import GPULattice, model, seeding
+    lat = model.Lattice(model.build_kitaev_honeycomb(), ADJ_L, ADJ_L, 1)
+    init = seeding.tilt(seeding.quench(lat, seeding.random_spins(lat.lattice_size, 21), sweeps=30), 0.05, seed=22)
+    dt = 0.01
+    T1a = (ADJ_STEPS + 0.5) * dt
+    star = np.array([-1.0, 0.25, -0.02, 0.0, 0.05])
+    tl = AD.kitaev_lattice(star, ADJ_L, ADJ_L)
+    with GPULattice(tl) as g0:
+        g0.set_damping(0.02)
+        fdir = np.zeros((2, 3))
+        fdir[:, 0] = 1.0
+        d = np.einsum("aij,aj->ai", tl.frames, fdir).reshape(-1)
+        g0.set_pulse(np.concatenate([d, d]), 0.5, 0.5, 0.0, 1.0, 0.0)
+        tr = g0.integrate_batched(init, np.array([[1.0, 0.0]]), np.array([[0.5, 0.0]]), 0.0, T1a, dt,
+                                  tgrid="boost", want_energy=False)
+    target = tr["mags"][:, 0].reshape(-1, 9)
+    rng = np.random.default_rng(8)
+    thetas = star * (1.0 + 0.1 * rng.uniform(-1, 1, size=(ADJ_B, 5)))
+    fit = AD.KitaevFit(ADJ_L, ADJ_L, init, target, 0.0, T1a, dt, pulse=(0.5, 0.5, 0.0, 1.0), alpha=0.02)
+    for _ in range(args.warmup):
+        fit(thetas)
+    t = time.perf_counter()
+    for _ in range(args.steps):
+        loss, grad = fit(thetas)
+    el = (time.perf_counter() - t) / args.steps
+    kms = fit.gl.last_kernel_ms()[0]
+    fit.close()
+    assert np.all(np.isfinite(grad))
+    v = ADJ_B / el
+    # forward-equivalent site-steps: forward + recomputation + transposed steps
+    line = {
+        "metric": METRIC, "value": v, "unit": "gradients/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": 1e3 * el, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "adjoint_kitaev_honeycomb_24x24x1_32starts_2000steps_rk4_dt0.01",
+                   "sites": lat.lattice_size, "starts": ADJ_B, "rk4_steps": ADJ_STEPS, "parameters": 5},
+        "site_steps_per_s": v * ADJ_STEPS * lat.lattice_size, "kernel_ms": kms,
+        "roofline": {"bound": "hbm",
+                     "achieved": v * ADJ_STEPS * lat.lattice_size * BYTES_PER_SITE_STEP / 1e9,
+                     "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                     "frac": v * ADJ_STEPS * lat.lattice_size * BYTES_PER_SITE_STEP / HBM_PEAK, "traffic": None,
+                     "kernel": "k_adjoint"},
+    }
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import pyoracle
+        uc = model.build_kitaev_honeycomb(K=thetas[0, 0], Gamma=thetas[0, 1], Gammap=thetas[0, 2], J=thetas[0, 3],
+                                          field_strength=thetas[0, 4], field_direction=(1.0, 1.0, 1.0))
+        o = pyoracle.OracleLattice(uc, ADJ_L, ADJ_L, 1)
+        o.set_alpha(0.02)
+        o.set_pulse(fdir, 1.0, fdir * 0.0, 0.0, 0.5, 0.5, 0.0)
+        eK = model.build_kitaev_honeycomb(K=1.0, Gamma=0.0, Gammap=0.0)
+        dJ = np.array([b.J for b in eK.bilinear]).reshape(-1)
+        n = 400
+        t = time.perf_counter()
+        o.tangent_md(init.reshape(-1), np.zeros(init.size), dJ, np.zeros(6), 0.0, (n + 0.5) * dt, dt)
+        el = time.perf_counter() - t
+        # a 5-parameter gradient by tangent-linear sweeps costs 5 such runs
+        line["cpu_baseline"] = {
+            "value": n / (5 * el * ADJ_STEPS), "unit": "gradients/s", "cores": pyoracle.max_threads(),
+            "kind": "port",
+            "sample": f"one tangent-linear sweep over {n} of the {ADJ_STEPS} steps ({el:.1f} s); a 5-parameter "
+                      f"gradient needs 5 sweeps",
+            "site_steps_per_s": n * lat.lattice_size / el}
+    print(json.dumps(line), flush=True)
+
+
 MIX_L, MIX_NTAU, MIX_STEPS, MIX_DT = 8, 128, 8192, 0.005
 
 
@@ -250,7 +387,7 @@ def main():
     ap.add_argument("--block", type=int, default=0)
     ap.add_argument("--launch", default="", help="fixed persistent launch shape 'block,spt,dbuf' "
                                                    "instead of the autotuner's choice")
-    ap.add_argument("--workload", default="2dcs", choices=["2dcs", "pyrochlore", "mixed"],
+    ap.add_argument("--workload", default="2dcs", choices=["2dcs", "pyrochlore", "mixed", "md", "adjoint"],
                     help="2dcs: the batched 2DCS workload (default); pyrochlore: 64 samples of the "
                          "65 536-site pyrochlore lattice; mixed: 257-trajectory SU(2)+SU(3) 2DCS batch "
                          "(both on the grid-wide stage kernels)")
@@ -268,6 +405,10 @@ def main():
         return bench_pyrochlore(args)
     if args.workload == "mixed":
         return bench_mixed(args)
+    if args.workload == "md":
+        return bench_md(args)
+    if args.workload == "adjoint":
+        return bench_adjoint(args)
     from classical_mcmc_spin_amd import GPULattice
 
     uc, lat, init, field_in = build_workload()

@@ -71,7 +71,11 @@ struct csl_lattice {
     double last_ms = 0.0;
     int last_block = 0, last_spt = 0, last_dbuf = 0;
     size_t last_launches = 0;
+    // stage-kernel launch sequence captured as a hipGraph (engine 3)
+    hipGraphExec_t graph_exec = nullptr;
+    std::vector<unsigned char> graph_key;
     ~csl_lattice() {
+        if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
         if (stream) (void)hipStreamDestroy(stream);
     }
 };
@@ -363,7 +367,8 @@ void run_persistent(csl_lattice* h, size_t B, const double* d_init, size_t init_
 void run_stages(csl_lattice* h, size_t B, const double* d_tp, const double* d_amp, bool drive_on,
                 const std::vector<double>& grid, double dt, size_t si, bool want_energy,
                 double* d_states_out /* [n_rec][B][N*3] or null */,
-                const std::function<void(size_t)>& at_step = {} /* called with X = x_n */) {
+                const std::function<void(size_t)>& at_step = {} /* called with X = x_n */,
+                bool use_graph = false) {
     const size_t n_steps = grid.size() - 1;
     const size_t n_rec = records_for(n_steps, si);
     hipStream_t s = h->stream;
@@ -377,18 +382,24 @@ void run_stages(csl_lattice* h, size_t B, const double* d_tp, const double* d_am
     h->rec_mags.ensure(n_rec * B * 9);
     if (want_energy) h->rec_energy.ensure(n_rec * B);
     h->tgrid.upload(grid.data(), grid.size(), s);
-    if (drive_on) {
-        h->fac.ensure(std::max<size_t>(n_steps * 3 * B * 2, 1));
-        hip_check(csl::launch_drive_table(h->tgrid.p, int(n_steps), dt, int(B), d_tp, d_amp,
-                                          L.width, L.freq, h->fac.p, s),
-                  "drive table");
-    }
+    if (drive_on) h->fac.ensure(std::max<size_t>(n_steps * 3 * B * 2, 1));
     const double inv_n = 1.0 / double(h->N);
     hipEvent_t e0, e1;
     hip_check(hipEventCreate(&e0), "event");
     hip_check(hipEventCreate(&e1), "event");
-    hip_check(hipEventRecord(e0, s), "event record");
     size_t launches = 0;
+    // The whole launch sequence of the integration (drive table, 4 stage
+    // kernels per step, record reductions, final record).  With use_graph it
+    // is captured once into a hipGraph and replayed while the key (shapes,
+    // device buffers, lattice parameters) is unchanged; inputs are refreshed
+    // in the same device buffers before every replay.
+    auto issue = [&](bool timed) {
+        if (drive_on)
+            hip_check(csl::launch_drive_table(h->tgrid.p, int(n_steps), dt, int(B), d_tp, d_amp, L.width, L.freq,
+                                              h->fac.p, s),
+                      "drive table");
+        if (timed) hip_check(hipEventRecord(e0, s), "event record");
+        launches = 0;
     for (size_t n = 0; n < n_steps; ++n) {
         const bool rec = (n % si) == 0;
         const size_t ri = n / si;
@@ -417,14 +428,56 @@ void run_stages(csl_lattice* h, size_t B, const double* d_tp, const double* d_am
                                              want_energy ? h->rec_energy.p : nullptr, s),
                       "reduce");
     }
-    hip_check(hipEventRecord(e1, s), "event record");
-    hip_check(csl::launch_observe(L, h->X.p, h->partials.p, int(B), s), "observe");
-    hip_check(csl::launch_reduce_obs(h->partials.p, nblk, int(B), int(n_rec - 1), inv_n, h->rec_mags.p,
-                                     want_energy ? h->rec_energy.p : nullptr, s),
-              "reduce");
-    if (d_states_out)
-        hip_check(csl::launch_perm_to_aos(L, h->X.p, d_states_out + (n_rec - 1) * B * h->N * 3, int(B), s),
-                  "state record");
+        if (timed) hip_check(hipEventRecord(e1, s), "event record");
+        hip_check(csl::launch_observe(L, h->X.p, h->partials.p, int(B), s), "observe");
+        hip_check(csl::launch_reduce_obs(h->partials.p, nblk, int(B), int(n_rec - 1), inv_n, h->rec_mags.p,
+                                         want_energy ? h->rec_energy.p : nullptr, s),
+                  "reduce");
+        if (d_states_out)
+            hip_check(csl::launch_perm_to_aos(L, h->X.p, d_states_out + (n_rec - 1) * B * h->N * 3, int(B), s),
+                      "state record");
+    };
+    if (!use_graph) {
+        issue(true);
+    } else {
+        if (at_step) fail(CSL_ERR_UNSUPPORTED, "per-step host hooks cannot be replayed from a graph");
+        // key: everything the captured launches bake in (the lattice block is
+        // copied from the handle, whose bytes are stable between calls)
+        std::vector<unsigned char> key;
+        auto put = [&](const void* p, size_t n) {
+            const auto* c = static_cast<const unsigned char*>(p);
+            key.insert(key.end(), c, c + n);
+        };
+        put(&h->L, sizeof(csl::LatDev));
+        const size_t shape[4] = {n_steps, si, B, size_t((drive_on ? 1 : 0) | (want_energy ? 2 : 0))};
+        put(shape, sizeof(shape));
+        put(&dt, sizeof(dt));
+        const void* ptrs[] = {h->X.p, h->ACC.p, h->Y1.p, h->Y2.p, h->partials.p, h->rec_mags.p,
+                              want_energy ? h->rec_energy.p : nullptr, drive_on ? h->fac.p : nullptr,
+                              h->tgrid.p, d_tp, d_amp, d_states_out};
+        put(ptrs, sizeof(ptrs));
+        if (!h->graph_exec || key != h->graph_key) {
+            if (h->graph_exec) (void)hipGraphExecDestroy(h->graph_exec);
+            h->graph_exec = nullptr;
+            hipGraph_t g = nullptr;
+            hip_check(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), "begin capture");
+            try {
+                issue(false);
+            } catch (...) {
+                (void)hipStreamEndCapture(s, &g);
+                if (g) (void)hipGraphDestroy(g);
+                throw;
+            }
+            hip_check(hipStreamEndCapture(s, &g), "end capture");
+            const hipError_t ie = hipGraphInstantiate(&h->graph_exec, g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            hip_check(ie, "graph instantiate");
+            h->graph_key = key;
+        }
+        hip_check(hipEventRecord(e0, s), "event record");
+        hip_check(hipGraphLaunch(h->graph_exec, s), "graph launch");
+        hip_check(hipEventRecord(e1, s), "event record");
+    }
     hip_check(hipStreamSynchronize(s), "stage integration");
     float ms = 0.f;
     hip_check(hipEventElapsedTime(&ms, e0, e1), "elapsed");
@@ -637,7 +690,7 @@ int csl_integrate(csl_lattice* h, double T0, double T1, double dt, size_t save_i
         bool persistent = false;
         if (engine == 2) persistent = true;
         else if (engine == 0) persistent = !states && !persistent_candidates(h).empty();
-        else if (engine != 1) fail(CSL_ERR_ARG, "engine must be 0, 1 or 2");
+        else if (engine != 1 && engine != 3) fail(CSL_ERR_ARG, "engine must be 0, 1, 2 or 3");
         if (persistent && states) fail(CSL_ERR_ARG, "the persistent engine does not record full states");
         if (persistent) {
             TuneVal tv = pick_persistent(h, 1, 0);
@@ -651,7 +704,7 @@ int csl_integrate(csl_lattice* h, double T0, double T1, double dt, size_t save_i
             hip_check(csl::launch_perm_to_soa(L, h->state.p, 0, h->X.p, 1, s), "perm");
             if (states) h->rec_states.ensure(n_rec * h->N * 3);
             run_stages(h, 1, h->bt_tp.p, h->bt_amp.p, drive_on, grid, dt, save_interval, energy != nullptr,
-                       states ? h->rec_states.p : nullptr);
+                       states ? h->rec_states.p : nullptr, {}, engine == 3);
             hip_check(csl::launch_perm_to_aos(L, h->X.p, h->state.p, 1, s), "perm");
         }
         copy_back(mags, h->rec_mags.p, n_rec * 9, s);

@@ -69,7 +69,7 @@ __global__ void __launch_bounds__(256) k_stage(LatDev L, StageArgs A) {
         double H[3];
         local_field_soa<ZMAX, UNI>(L, cur, NP, p, atom, nb, c, S, H);
         if (A.rec) site_obs(L, atom, p, S, H, part);
-        if (drive_on) sub_drive(L, atom, f1, f2, H);
+        if (drive_on && (f1 != 0.0 || f2 != 0.0)) sub_drive(L, atom, f1, f2, H);
         double k[3];
         ll_su2(H, S, L.alpha, k);
         // stage 0 reads its input from x itself, so S is x there; stage 3

@@ -248,7 +248,7 @@ __global__ void __launch_bounds__(persistent_max_block<SPT>()) k_persistent(LatD
                     double H[3];
                     local_field_lds<ZMAX, UNI>(L, cur, p, atom, nb[m], cnt[m], S, H);
                     if (obs && want_e) e[m] = site_energy(L, atom, p, S, H);
-                    if (drive_on) sub_drive(L, atom, f1, f2, H);
+                    if (drive_on && (f1 != 0.0 || f2 != 0.0)) sub_drive(L, atom, f1, f2, H);
                     double k[3];
                     ll_su2(H, S, alpha, k);
                     double y[3];

@@ -333,3 +333,34 @@ def pump_probe_spectroscopy(gl: GPULattice, ground, field_in, pulse_amp, pulse_w
 def nonlinear_signal(res: dict) -> np.ndarray:
     """M_NL(tau, t) = M01 - M0 - M1 (util/readers_new/reader_pyrochlore.py:760-830)."""
     return res["M01"] - res["M0"][None] - res["M1"]
+
+
+def nonlinear_spectrum(res: dict, mag_index: int = 2, omega_range=None):
+    """2-D spectrum |FFT2(M_NL)| of a pump-probe result on the GPU
+    (csl_nonlinear_spectrum; reader_pyrochlore.py:760-830): M_NL over the
+    last half of the times, zero-padded to powers of two, fftshifted.
+    mag_index selects M_antiferro (0), M_local (1) or M_global (2).  Returns
+    dict(M_NL_FF [P_tau, P_t, D], freq_tau, freq_T), frequencies from
+    numpy.fft.fftfreq; with omega_range the window |f| <= omega_range."""
+    M0 = f64(res["M0"][:, mag_index])
+    M1 = f64(res["M1"][:, :, mag_index])
+    M01 = f64(res["M01"][:, :, mag_index])
+    n_tau, n_t, D = M1.shape
+    dom = n_t // 2
+    Pt_ = 1 << max(0, (n_tau - 1).bit_length())
+    Pd_ = 1 << max(0, (dom - 1).bit_length())
+    out = np.empty(Pt_ * Pd_ * D)
+    pt, pd = C.c_size_t(), C.c_size_t()
+    check(lib().csl_nonlinear_spectrum(n_tau, n_t, D, dptr(M0), dptr(M1), dptr(M01), out.size, dptr(out),
+                                       C.byref(pt), C.byref(pd)))
+    ff = out.reshape(pt.value, pd.value, D)
+    times, taus = np.asarray(res["times"]), np.asarray(res["tau"])
+    dT = times[1] - times[0] if len(times) > 1 else 1.0
+    dtau = taus[1] - taus[0] if len(taus) > 1 else 1.0
+    fq_tau = np.fft.fftshift(np.fft.fftfreq(pt.value, d=dtau))
+    fq_T = np.fft.fftshift(np.fft.fftfreq(pd.value, d=dT))
+    if omega_range is not None:
+        it = (fq_tau >= -omega_range) & (fq_tau <= omega_range)
+        iT = (fq_T >= -omega_range) & (fq_T <= omega_range)
+        ff, fq_tau, fq_T = ff[np.ix_(it, iT)], fq_tau[it], fq_T[iT]
+    return {"M_NL_FF": ff, "freq_tau": fq_tau, "freq_T": fq_T}

@@ -116,7 +116,10 @@ int csl_step(csl_lattice* h, double t, double dt, const char* method);
  * times [max_rec], states [max_rec*N*D], mags [max_rec*3*D], energy
  * [max_rec] (E/N) may be NULL.  *n_rec receives the number of records.
  * engine: 0 = auto, 1 = grid-wide stage kernels (one launch per RK stage),
- * 2 = persistent single-workgroup kernel (lattice resident in LDS). */
+ * 2 = persistent single-workgroup kernel (lattice resident in LDS),
+ * 3 = the stage-kernel launch sequence captured once into a hipGraph and
+ * replayed by later calls with the same shapes, lattice parameters and
+ * device buffers (the inputs are refreshed in place before each replay). */
 int csl_integrate(csl_lattice* h, double T0, double T1, double dt,
                   size_t save_interval, const char* method, int tgrid, int engine,
                   size_t max_rec, double* times, double* states, double* mags,
@@ -242,6 +245,17 @@ int csl_structure_factor(csl_lattice* h, size_t B, const double* init, int init_
                          size_t L1, size_t L2, size_t L3, const double* positions, size_t max_t,
                          double* sqw, size_t* n_t, double* mags);
 
+
+/* 2-D nonlinear spectrum of a pump-probe scan
+ * (util/readers_new/reader_pyrochlore.py:760-830): with domain = n_t/2,
+ *   M_NL[tau][t] = M01[tau][n_t-domain+t] - M0[n_t-domain+t] - M1[tau][n_t-domain+t]
+ * per component d < D, zero-padded to P_tau x P_t (the next powers of two
+ * of n_tau and domain), 2-D FFT, both axes fftshifted;
+ * out[(i*P_t + j)*D + d] = |FFT|.  Inputs (host): M0 [n_t*D],
+ * M1 / M01 [n_tau*n_t*D]; max_out >= P_tau*P_t*D.  No handle needed. */
+int csl_nonlinear_spectrum(size_t n_tau, size_t n_t, size_t D, const double* M0, const double* M1,
+                           const double* M01, size_t max_out, double* out, size_t* P_tau,
+                           size_t* P_t);
 
 /* Gradient of a trajectory loss by the discrete adjoint of RK4.  The
  * reference has no gradient path; this entry is for fitting Hamiltonian

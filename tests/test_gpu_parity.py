@@ -199,3 +199,56 @@ def test_general_tables_non_uniform(po):
         res = G.molecular_dynamics(gl, s, 0.0, 2.005, 0.01)
     assert np.max(np.abs(res["mags"] - ref["mags"])) <= TOL_TRAJ
     assert np.max(np.abs(res["energy"] - ref["energy"])) <= TOL_TRAJ
+
+
+@pytest.mark.parametrize("n_tau,n_t", [(5, 41), (8, 64), (1, 10)])
+def test_nonlinear_spectrum_vs_numpy(n_tau, n_t):
+    """GPU M_NL 2-D spectrum against numpy.fft on the same arrays, with the
+    reference's windowing and padding (reader_pyrochlore.py:760-830)."""
+    rng = np.random.default_rng(n_tau * 100 + n_t)
+    res = {"M0": rng.normal(size=(n_t, 3, 3)), "M1": rng.normal(size=(n_tau, n_t, 3, 3)),
+           "M01": rng.normal(size=(n_tau, n_t, 3, 3)), "times": 0.1 * np.arange(n_t),
+           "tau": 0.25 * np.arange(n_tau)}
+    out = G.nonlinear_spectrum(res, mag_index=2)
+    dom = n_t // 2
+    mnl = res["M01"][:, -dom:, 2] - res["M0"][None, -dom:, 2] - res["M1"][:, -dom:, 2]
+    P1 = 2 ** int(np.ceil(np.log2(n_tau)))
+    P2 = 2 ** int(np.ceil(np.log2(dom)))
+    ref = np.stack([np.abs(np.fft.fftshift(np.fft.fft2(mnl[:, :, d], s=(P1, P2)))) for d in range(3)], -1)
+    assert out["M_NL_FF"].shape == ref.shape
+    assert np.max(np.abs(out["M_NL_FF"] - ref)) <= 1e-12 * max(1.0, np.max(ref))
+    assert np.allclose(out["freq_T"], np.fft.fftshift(np.fft.fftfreq(P2, d=0.1)))
+
+
+def test_graph_replay_matches_eager_launches(po):
+    """engine 3 (stage launches captured once into a hipGraph, then replayed)
+    against engine 1 (the same kernels launched eagerly): three initial
+    states, three rounds of replays with eager integrations and host
+    synchronisation between them, every replay bit-identical; the eager
+    result itself against the oracle."""
+    uc, lat, o = _pair(po, 6, 6, h=0.1)
+    f = unit_x_field(lat)
+    fd = np.einsum("aij,aj->ai", lat.frames, f).reshape(-1)
+    seeds = (1, 2, 3)
+    with GPULattice(lat) as gl:
+        gl.set_damping(0.01)
+        gl.set_pulse(np.concatenate([fd, fd]), 0.5, 0.5, 0.0, 0.3, 0.8)
+        eager = []
+        for sd in seeds:
+            gl.set_spins(random_state(lat, sd))
+            r = gl.integrate(0.0, 1.505, 0.01, 1, tgrid="boost", engine=1)
+            eager.append((r, gl.get_spins()))
+        for rnd in range(3):
+            for i, sd in enumerate(seeds):
+                gl.set_spins(random_state(lat, sd))
+                r = gl.integrate(0.0, 1.505, 0.01, 1, tgrid="boost", engine=3)
+                assert np.array_equal(r["times"], eager[i][0]["times"])
+                assert np.array_equal(r["mags"], eager[i][0]["mags"]), (rnd, i)
+                assert np.array_equal(r["energy"], eager[i][0]["energy"]), (rnd, i)
+                assert np.array_equal(gl.get_spins(), eager[i][1]), (rnd, i)
+                gl.set_spins(random_state(lat, 50 + i))
+                gl.integrate(0.0, 0.505, 0.01, 1, tgrid="boost", engine=1)
+    o.set_alpha(0.01)
+    o.set_pulse(f, 0.3, f, 0.8, 0.5, 0.5, 0.0)
+    ref = o.integrate_md(random_state(lat, 1).reshape(-1), 0.0, 1.505, 0.01)
+    assert np.max(np.abs(eager[0][0]["mags"] - ref["mags"])) <= TOL_TRAJ
