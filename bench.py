@@ -215,14 +215,15 @@ def bench_md(args):
     print(json.dumps(line), flush=True)
 
 
-ADJ_L, ADJ_STEPS, ADJ_B = 24, 2000, 32
+ADJ_L, ADJ_STEPS, ADJ_B = 24, 2000, 256
 
 
 def bench_adjoint(args):
     """Adjoint gradients: Kitaev 24x24 (1152 sites), one pulse (A = 0.5,
     w = 0.5 at t = 1), Gilbert alpha = 0.02, loss on M_global over 2000 RK4
     steps (dt = 0.01) against a target run at (K, G, G', J, h) =
-    (-1, 0.25, -0.02, 0, 0.05); 32 parameter starts within +-10 %.
+    (-1, 0.25, -0.02, 0, 0.05); 256 parameter starts within +-10 % (one
+    workgroup per start, one per CU: the LDS state of 1152 sites takes 83 KB).
     value = gradients (starts) per second."""
     from classical_mcmc_spin_amd import adjoint as AD
     from classical_mcmc_spin_amd import GPULattice, model, seeding
@@ -259,7 +260,7 @@ def bench_adjoint(args):
         "metric": METRIC, "value": v, "unit": "gradients/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": 1e3 * el, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f64", "data": "synthetic",
-        "config": {"workload": "adjoint_kitaev_honeycomb_24x24x1_32starts_2000steps_rk4_dt0.01",
+        "config": {"workload": f"adjoint_kitaev_honeycomb_{ADJ_L}x{ADJ_L}x1_{ADJ_B}starts_{ADJ_STEPS}steps_rk4_dt0.01",
                    "sites": lat.lattice_size, "starts": ADJ_B, "rk4_steps": ADJ_STEPS, "parameters": 5},
         "site_steps_per_s": v * ADJ_STEPS * lat.lattice_size, "kernel_ms": kms,
         "roofline": {"bound": "hbm",

@@ -21,7 +21,7 @@ LIB_PATH = os.path.join(HERE, LIB_NAME)
 
 # (source, object suffix, extra flags)
 HIP_UNITS = [("csl_kernels.hip", "", []), ("csl_capi.hip", "", []), ("csl_mixed.hip", "", []), ("csl_adjoint.hip", "", []),
-             ("csl_dssf.hip", "", []), ("csl_spectrum.hip", "", [])] + [
+             ("csl_dssf.hip", "", []), ("csl_spectrum.hip", "", []), ("csl_quench.hip", "", [])] + [
     ("csl_persistent.hip", f"_spt{n}", [f"-DCSL_SPT={n}"]) for n in (1, 2, 3, 4, 6)]
 CXX_UNITS = ["gpu_api_shim.cpp", "mixed_gpu_api_shim.cpp"]
 
@@ -64,7 +64,37 @@ def build(verbose: bool = False, jobs: int = 0) -> str:
     _run([hipcc(), "--offload-arch=gfx950", "-shared", "-fPIC", *objs, "-L/opt/rocm/lib", "-lhipfft", "-Wl,-rpath,/opt/rocm/lib", "-o", tmp], verbose)
     os.replace(tmp, LIB_PATH)
     build_cxx_test(verbose)
+    build_h5(verbose)
     return LIB_PATH
+
+
+# HDF5 C library locations probed for the writer library (prefix, include, lib)
+HDF5_PREFIXES = [("/opt/conda", "/opt/conda/include", "/opt/conda/lib"),
+                 ("/usr", "/usr/include/hdf5/serial", "/usr/lib/x86_64-linux-gnu/hdf5/serial"),
+                 ("/usr", "/usr/include", "/usr/lib/x86_64-linux-gnu")]
+
+
+def find_hdf5():
+    for _, inc, libdir in HDF5_PREFIXES:
+        if os.path.exists(os.path.join(inc, "hdf5.h")) and os.path.exists(os.path.join(libdir, "libhdf5.so")):
+            return inc, libdir
+    return None
+
+
+def build_h5(verbose: bool = False) -> str:
+    """libcsl_h5.so: the HDF5 writers (csrc/csl_h5.cpp) against the HDF5 C
+    library; skipped with a message when no HDF5 installation is found."""
+    found = find_hdf5()
+    if found is None:
+        sys.stderr.write("HDF5 C library not found: libcsl_h5.so (HDF5 output) not built\n")
+        return ""
+    inc, libdir = found
+    out = os.path.join(HERE, "libcsl_h5.so")
+    tmp = out + ".tmp"
+    _run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-I", os.path.join(ROOT, "include"), "-I", inc,
+          os.path.join(CSRC, "csl_h5.cpp"), "-o", tmp, f"-L{libdir}", "-lhdf5", f"-Wl,-rpath,{libdir}"], verbose)
+    os.replace(tmp, out)
+    return out
 
 
 def build_cxx_test(verbose: bool = False) -> str:

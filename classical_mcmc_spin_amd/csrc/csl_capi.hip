@@ -64,7 +64,7 @@ struct csl_lattice {
     DBuf<double> X, ACC, Y1, Y2, partials, fac, tgrid, rec_mags, rec_energy, rec_states, aux;
     DBuf<double> bt_init, bt_tp, bt_amp, bt_final;
     // CSR -> device slot mapping of the handle's tables (for per-replica tables)
-    std::vector<size_t> h_off, h_slot_of;
+    std::vector<size_t> h_off, h_slot_of, h_par;
     // per-trajectory Hamiltonians (csl_set_replica_hamiltonians)
     DBuf<double> rep_mats, rep_field;
     size_t rep_B = 0, rep_mats_stride = 0, rep_field_stride = 0;
@@ -192,6 +192,7 @@ void build_tables(csl_lattice* h, size_t N, size_t D, size_t n_atoms, const doub
     }
     for (size_t a = 0; a < n_atoms; ++a) cnt_atom[a] = int(off[canon[a] + 1] - off[canon[a]]);
     h->h_off = off;
+    h->h_par.assign(partner.begin(), partner.begin() + off[N]);
     h->h_slot_of = slot_of;
     h->rep_B = 0;
     std::vector<double> f, o;
@@ -1224,6 +1225,78 @@ int csl_adjoint_gradient(csl_lattice* h, size_t B, const double* init, int init_
         (void)hipEventDestroy(e1);
         h->last_ms = ms;
         h->last_launches = 1;
+    });
+}
+
+int csl_quench(csl_lattice* h, size_t B, double* states, size_t max_sweeps, double rel_tol, double spin_length,
+               size_t* sweeps_done) {
+    return guarded([&] {
+        check_handle(h);
+        if (B == 0 || B > 65535) fail(CSL_ERR_ARG, "batch size must be 1 .. 65535");
+        if (!states) fail(CSL_ERR_NULL, "null states");
+        if (!(rel_tol >= 0.0) || !(spin_length > 0.0)) fail(CSL_ERR_ARG, "need rel_tol >= 0 and spin_length > 0");
+        const size_t N = h->N, nd = N * 3;
+        check_finite(states, B * nd, "states");
+        const csl::LatDev& L = h->L;
+        // greedy colouring in site order: smallest colour unused by a neighbour
+        std::vector<int> color(N, -1);
+        int n_colors = 0;
+        for (size_t i = 0; i < N; ++i) {
+            std::vector<char> used(size_t(n_colors) + 1, 0);
+            for (size_t k = h->h_off[i]; k < h->h_off[i + 1]; ++k) {
+                const int c = color[h->h_par[k]];
+                if (c >= 0) used[size_t(c)] = 1;
+            }
+            int c = 0;
+            while (used[size_t(c)]) ++c;
+            color[i] = c;
+            n_colors = std::max(n_colors, c + 1);
+        }
+        std::vector<int> dcol(size_t(L.NP), -1);
+        for (size_t i = 0; i < N; ++i) dcol[(i % h->n_atoms) * L.npad + i / h->n_atoms] = color[i];
+        hipStream_t s = h->stream;
+        DBuf<int> d_col, d_frozen, d_sweeps;
+        DBuf<double> d_e, d_ep;
+        d_col.upload(dcol.data(), dcol.size(), s);
+        std::vector<int> zeros(B, 0);
+        d_frozen.upload(zeros.data(), B, s);
+        d_sweeps.upload(zeros.data(), B, s);
+        d_e.ensure(B);
+        d_ep.ensure(B);
+        h->bt_init.upload(states, B * nd, s);
+        h->X.ensure(B * 3 * size_t(L.NP));
+        hip_check(hipMemsetAsync(h->X.p, 0, B * 3 * size_t(L.NP) * sizeof(double), s), "memset");
+        hip_check(csl::launch_perm_to_soa(L, h->bt_init.p, nd, h->X.p, int(B), s), "perm");
+        const int nblk = csl::stage_blocks(L);
+        h->partials.ensure(B * size_t(nblk) * 10);
+        auto energy = [&](double* out) {
+            hip_check(csl::launch_observe(L, h->X.p, h->partials.p, int(B), s), "observe");
+            hip_check(csl::launch_reduce_obs(h->partials.p, nblk, int(B), 0, 1.0 / double(N), nullptr, out, s),
+                      "reduce");
+        };
+        energy(d_ep.p);
+        std::vector<int> fz(B, 0);
+        for (size_t sw = 0; sw < max_sweeps; ++sw) {
+            for (int c = 0; c < n_colors; ++c)
+                hip_check(csl::launch_quench_class(L, h->X.p, d_col.p, c, d_frozen.p, spin_length, int(B), s),
+                          "quench sweep");
+            energy(d_e.p);
+            hip_check(csl::launch_quench_check(d_e.p, d_ep.p, d_frozen.p, d_sweeps.p, int(B), int(sw), rel_tol, s),
+                      "quench check");
+            if ((sw + 1) % 32 == 0 || sw + 1 == max_sweeps) {
+                hip_check(hipMemcpyAsync(fz.data(), d_frozen.p, B * sizeof(int), hipMemcpyDeviceToHost, s), "D2H");
+                hip_check(hipStreamSynchronize(s), "quench");
+                if (std::all_of(fz.begin(), fz.end(), [](int v) { return v != 0; })) break;
+            }
+        }
+        h->bt_final.ensure(B * nd);
+        hip_check(csl::launch_perm_to_aos(L, h->X.p, h->bt_final.p, int(B), s), "perm");
+        copy_back(states, h->bt_final.p, B * nd, s);
+        std::vector<int> sd(B, 0);
+        hip_check(hipMemcpyAsync(sd.data(), d_sweeps.p, B * sizeof(int), hipMemcpyDeviceToHost, s), "D2H");
+        hip_check(hipStreamSynchronize(s), "quench");
+        if (sweeps_done)
+            for (size_t b = 0; b < B; ++b) sweeps_done[b] = size_t(sd[b]);
     });
 }
 

@@ -52,6 +52,12 @@ struct AdjArgs {
     size_t field_stride;
 };
 
+// zero-temperature relaxation (csl_quench.hip)
+hipError_t launch_quench_class(const LatDev& L, double* x, const int* color, int c, const int* frozen, double len,
+                               int B, hipStream_t st);
+hipError_t launch_quench_check(const double* e, double* e_prev, int* frozen, int* sweeps, int B, int sweep,
+                               double rel_tol, hipStream_t st);
+
 // structure factor (csl_dssf.hip)
 hipError_t launch_sq_fill(const LatDev& L, const double* x, int B, double2* out, hipStream_t st);
 hipError_t launch_sq_combine(const double2* F, int B, int n_atoms, int L1, int L2, int L3, const double* pos,

@@ -17,6 +17,7 @@ the GPU through libcsl_hip.so.
 from __future__ import annotations
 
 import ctypes as C
+import os
 from typing import Optional
 
 import numpy as np
@@ -180,6 +181,17 @@ class GPULattice:
                 "energy": None if energy is None else energy.reshape(nr, B),
                 "final": None if final is None else final.reshape(B, self.N, 3)}
 
+    def quench(self, states, max_sweeps=10000, rel_tol=1e-12, spin_length=1.0):
+        """csl_quench: relaxed copies of states ([N,3] or [B,N,3]) and the
+        sweep count of each (Lattice::greedy_quench, colour-class order)."""
+        st = f64(states).copy()
+        single = st.ndim == 2
+        st = st.reshape(-1, self.N, 3)
+        B = st.shape[0]
+        done = np.zeros(B, dtype=np.uintp)
+        check(lib().csl_quench(self.h, B, dptr(st.reshape(-1)), max_sweeps, rel_tol, spin_length, szptr(done)))
+        return (st[0] if single else st), done.astype(np.int64)
+
     def structure_factor(self, init, T0, T1, dt, sample_interval, dims, positions, tgrid="boost",
                          want_mags=False):
         """csl_structure_factor: S(q, w) [L1*L2*L3, n_t] of the undriven
@@ -280,14 +292,27 @@ def _gate(times, T_start, T_end, step):
 
 
 def molecular_dynamics(gl: GPULattice, spins, T_start, T_end, dt, save_interval=1, method="rk4",
-                       alpha_gilbert=0.0, engine=0):
+                       alpha_gilbert=0.0, engine=0, out_dir: Optional[str] = None):
     """molecular_dynamics_cpu with method "rk4" (lattice_md.cpp:341-453): the
     observer records M_antiferro, M_local, M_global and E/N every save_interval
-    integrate_const calls (HDF5 output omitted).  Returns dict(times, mags, energy)."""
+    integrate_const calls.  With out_dir the records and the spin snapshots
+    go to out_dir/trajectory.h5 in the HDF5MDWriter layout
+    (lattice_md.cpp:363-373, hdf5_io.h:83-300; libcsl_h5.so).
+    Returns dict(times, mags, energy[, states])."""
     gl.set_damping(alpha_gilbert)
     gl.set_pulse(np.zeros(2 * gl.A * 3), 0.0, 1.0, 0.0, 0.0, 0.0)
     gl.set_spins(spins)
-    return gl.integrate(T_start, T_end, dt, save_interval, method, tgrid="boost", engine=engine)
+    res = gl.integrate(T_start, T_end, dt, save_interval, method, tgrid="boost", engine=engine,
+                       want_states=out_dir is not None)
+    if out_dir is not None:
+        from . import h5io
+        lat = gl.lattice
+        os.makedirs(out_dir, exist_ok=True)
+        with h5io.MDWriter(os.path.join(out_dir, "trajectory.h5"), lat.lattice_size, lat.spin_dim, lat.N_atoms,
+                           (lat.dim1, lat.dim2, lat.dim3), method, dt, T_start, T_end, save_interval,
+                           lat.spin_length, positions=lat.positions) as w:
+            w.write_steps(res["times"], res["mags"], res["states"])
+    return res
 
 
 def _pulse_run(gl: GPULattice, spins, fd, t1, t2, amp, width, freq, T_start, T_end, step):
@@ -322,12 +347,38 @@ def double_pulse_drive(gl: GPULattice, spins, field_in_1, t_B_1, field_in_2, t_B
 
 
 def pump_probe_spectroscopy(gl: GPULattice, ground, field_in, pulse_amp, pulse_width, pulse_freq,
-                            tau_start, tau_end, tau_step, T_start, T_end, T_step):
-    """Lattice::pump_probe_spectroscopy (lattice_md.cpp:683-926) without HDF5:
-    M0, M1(tau) and M01(tau) for every delay, all integrated in one batched
-    launch.  Returns dict(times, M0 [n,3,3], M1 [n_tau,n,3,3], M01, tau)."""
-    return gl.pump_probe(ground, field_in, pulse_amp, pulse_width, pulse_freq, tau_start, tau_end,
-                         tau_step, T_start, T_end, T_step)
+                            tau_start, tau_end, tau_step, T_start, T_end, T_step,
+                            dir_name: Optional[str] = None, method: str = "rk4",
+                            Temp_start: float = 0.0, Temp_end: float = 0.0, n_anneal: int = 0,
+                            T_zero_quench: bool = False, quench_sweeps: int = 0):
+    """Lattice::pump_probe_spectroscopy (lattice_md.cpp:683-926): M0, M1(tau)
+    and M01(tau) for every delay, all integrated in one batched launch.
+    With dir_name the result goes to dir_name/pump_probe_spectroscopy.h5 in
+    the HDF5PumpProbeWriter layout (hdf5_io.h:729-1000; libcsl_h5.so), with
+    the ground state's E/N and |M_local| and the annealing parameters (which
+    only describe how the ground state was made) as metadata.
+    Returns dict(times, M0 [n,3,3], M1 [n_tau,n,3,3], M01, tau)."""
+    if method != "rk4":
+        raise _capi.CslError(_capi.CSL_ERR_UNSUPPORTED, "only method 'rk4' is implemented")
+    res = gl.pump_probe(ground, field_in, pulse_amp, pulse_width, pulse_freq, tau_start, tau_end,
+                        tau_step, T_start, T_end, T_step)
+    if dir_name is not None:
+        from . import h5io
+        lat = gl.lattice
+        os.makedirs(dir_name, exist_ok=True)
+        gl.set_spins(ground)
+        E_ground = gl.energy() / lat.lattice_size          # energy_density (lattice_md.cpp:716)
+        M_ground = np.linalg.norm(gl.magnetization()[0])   # |magnetization_local| (:717)
+        h5io.write_pump_probe(
+            os.path.join(dir_name, "pump_probe_spectroscopy.h5"), res, lattice_size=lat.lattice_size,
+            n_atoms=lat.N_atoms, dims=(lat.dim1, lat.dim2, lat.dim3), spin_dim=lat.spin_dim,
+            spin_length=lat.spin_length, pulse_amp=pulse_amp, pulse_width=pulse_width, pulse_freq=pulse_freq,
+            T_start=T_start, T_end=T_end, T_step=T_step, method=method, tau_start=tau_start, tau_end=tau_end,
+            tau_step=tau_step, ground_state_energy=E_ground, ground_state_magnetization=M_ground,
+            Temp_start=Temp_start, Temp_end=Temp_end, n_anneal=n_anneal, T_zero_quench=T_zero_quench,
+            quench_sweeps=quench_sweeps, pulse_field_direction=np.asarray(field_in, dtype=float),
+            positions=lat.positions)
+    return res
 
 
 def nonlinear_signal(res: dict) -> np.ndarray:

@@ -229,6 +229,17 @@ int csl_last_launch_config(csl_lattice* h, int* block, int* sites_per_thread, in
  * class (as the autotuner would); rejects shapes that do not fit. */
 int csl_set_launch_config(csl_lattice* h, size_t B, int block, int sites_per_thread, int flags);
 
+/* Zero-temperature relaxation of B states in place (states [B*N*3], AoS):
+ * Lattice::greedy_quench (lattice.h:3352-3368) — sweeps that set every spin
+ * antiparallel to its local field, S = -H/|H| * spin_length (skipped when
+ * |H| < 1e-15; deterministic_sweep, lattice.h:3334-3349), until the energy
+ * changes by at most rel_tol * (|E_prev| + 1e-18) or max_sweeps.  Sites are
+ * visited colour class by colour class of a greedy colouring of the bond
+ * graph (the reference draws them at random).  sweeps_done [B] (optional)
+ * receives each state's sweep count. */
+int csl_quench(csl_lattice* h, size_t B, double* states, size_t max_sweeps, double rel_tol,
+               double spin_length, size_t* sweeps_done);
+
 /* Dynamical structure factor of B undriven trajectories (the reference
  * computes it in Python from saved spin snapshots,
  * util/readers_new/reader_pyrochlore.py:150-270):

@@ -201,6 +201,29 @@ def test_general_tables_non_uniform(po):
     assert np.max(np.abs(res["energy"] - ref["energy"])) <= TOL_TRAJ
 
 
+def test_quench_matches_host_colour_sweeps():
+    """Device greedy quench vs the host colour-class sweeps (same colouring,
+    same update) over 40 sweeps; a converged quench lowers the energy and
+    leaves every spin along its local field."""
+    uc, lat = kitaev(6, 6, h=0.1)
+    s0 = np.stack([random_state(lat, 30 + b) for b in range(3)])
+    with GPULattice(lat) as gl:
+        q, done = gl.quench(s0, max_sweeps=40, rel_tol=0.0)
+        assert np.all(done <= 40)
+        for b in range(3):
+            ref = seeding.quench(lat, s0[b], sweeps=40, tol=-1.0)
+            assert np.max(np.abs(q[b] - ref)) <= 1e-10
+        q2, done2 = gl.quench(s0, max_sweeps=5000, rel_tol=1e-12)
+        assert np.all(done2 < 5000)
+        for b in range(3):
+            gl.set_spins(s0[b])
+            e0 = gl.energy()
+            gl.set_spins(q2[b])
+            assert gl.energy() < e0
+            H = gl.local_field()
+            assert np.max(np.linalg.norm(np.cross(H, q2[b]), axis=1)) <= 1e-4
+
+
 @pytest.mark.parametrize("n_tau,n_t", [(5, 41), (8, 64), (1, 10)])
 def test_nonlinear_spectrum_vs_numpy(n_tau, n_t):
     """GPU M_NL 2-D spectrum against numpy.fft on the same arrays, with the
