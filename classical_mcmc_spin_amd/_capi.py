@@ -59,6 +59,8 @@ SIGNATURES = {
     "csl_count_records": (_i, [_d, _d, _d, _sz, _i, _psz, _psz]),
     "csl_pump_probe": (_i, [_vp, _pd, _pd, _d, _d, _d, _d, _d, _d, _d, _d, _d, _sz, _sz, _psz, _psz,
                             _pd, _pd, _pd, _pd, _pd]),
+    "csl_pump_probe_w1": (_i, [_vp, _pd, _pd, _d, _d, _d, _d, _d, _d, _d, _d, _d, _i, _d, _sz, _sz, _psz,
+                               _psz, _pd, _pd, _pd, _pd, _pd, _pi, _pd]),
     "csl_autotune_batched": (_i, [_vp, _sz, _sz, _pi, _pd]),
     "csl_last_kernel_ms": (_i, [_vp, _pd, _psz]),
     "csl_last_launch_config": (_i, [_vp, _pi, _pi, _pi]),

@@ -913,10 +913,22 @@ int csl_pump_probe(csl_lattice* h, const double* ground, const double* field_in,
                    double T0, double T1, double T_step, size_t max_rec, size_t max_tau,
                    size_t* n_rec_out, size_t* n_tau_out, double* times, double* M0, double* M1,
                    double* M01, double* tau_values) {
+    return csl_pump_probe_w1(h, ground, field_in, amp, width, freq, tau_start, tau_end, tau_step, T0, T1, T_step,
+                             0, 0.0, max_rec, max_tau, n_rec_out, n_tau_out, times, M0, M1, M01, tau_values,
+                             nullptr, nullptr);
+}
+
+int csl_pump_probe_w1(csl_lattice* h, const double* ground, const double* field_in, double amp,
+                      double width, double freq, double tau_start, double tau_end, double tau_step,
+                      double T0, double T1, double T_step, int reuse_m0_for_m1, double stationarity_tol,
+                      size_t max_rec, size_t max_tau, size_t* n_rec_out, size_t* n_tau_out, double* times,
+                      double* M0, double* M1, double* M01, double* tau_values, int* used_w1,
+                      double* max_dsdt) {
     return guarded([&] {
         check_handle(h);
         if (!ground || !field_in) fail(CSL_ERR_NULL, "null ground state / field_in");
         if (!(tau_step != 0.0) || !std::isfinite(tau_step)) fail(CSL_ERR_ARG, "tau_step must be non-zero");
+        if (reuse_m0_for_m1 && !(stationarity_tol >= 0.0)) fail(CSL_ERR_ARG, "stationarity_tol must be >= 0");
         // tau grid (lattice_md.cpp:768; tau_start + i*tau_step as in the W2 loop :872)
         const double q = std::abs((tau_end - tau_start) / tau_step);
         if (!std::isfinite(q) || q > 1e6) fail(CSL_ERR_ARG, "tau grid too large");
@@ -924,6 +936,23 @@ int csl_pump_probe(csl_lattice* h, const double* ground, const double* field_in,
         if (n_tau_out) *n_tau_out = n_tau;
         if (n_tau > max_tau) fail(CSL_ERR_CAPACITY, "max_tau is smaller than the number of delays");
         const size_t A = h->n_atoms;
+        std::vector<double> zero(2 * A * 3, 0.0);
+        // W1 guard (lattice_md.cpp:729-736, max_dSdt_norm_no_drive :236-280):
+        // L_inf norm of dS/dt at the ground state with the drive off.
+        bool w1 = false;
+        if (reuse_m0_for_m1) {
+            const size_t nd = h->N * 3;
+            int rc = csl_set_spins(h, ground, nd);
+            if (rc == CSL_OK) rc = csl_set_pulse(h, zero.data(), 0.0, width, freq, 0.0, 0.0);
+            std::vector<double> d(nd);
+            if (rc == CSL_OK) rc = csl_rhs(h, 0.0, d.data(), nd);
+            if (rc != CSL_OK) fail(rc, g_err);
+            double mx = 0.0;
+            for (double v : d) mx = std::max(mx, std::abs(v));
+            if (max_dsdt) *max_dsdt = mx;
+            w1 = mx <= stationarity_tol;
+        }
+        if (used_w1) *used_w1 = w1 ? 1 : 0;
         // Lattice::set_pulse: field_drive[p] = R_atom * field_in (lattice_md.cpp:215-216)
         std::vector<double> fd(2 * A * 3, 0.0);
         for (size_t a = 0; a < A; ++a)
@@ -935,16 +964,20 @@ int csl_pump_probe(csl_lattice* h, const double* ground, const double* field_in,
             }
         const int rc = csl_set_pulse(h, fd.data(), amp, width, freq, 0.0, 0.0);
         if (rc != CSL_OK) fail(rc, g_err);
-        const size_t B = 1 + 2 * n_tau;
+        // replicas: M0, [M1(tau) unless W1], M01(tau)
+        const size_t n1 = w1 ? 0 : n_tau;
+        const size_t B = 1 + n1 + n_tau;
         std::vector<double> tp(2 * B, 0.0), am(2 * B, 0.0), taus(n_tau);
         for (size_t i = 0; i < n_tau; ++i) taus[i] = tau_start + double(i) * tau_step;
         am[0] = amp;  // M0: pulse at 0 only
         for (size_t i = 0; i < n_tau; ++i) {
-            tp[2 * (1 + i)] = taus[i];  // M1(tau): pulse at tau only
-            am[2 * (1 + i)] = amp;
-            tp[2 * (1 + n_tau + i) + 1] = taus[i];  // M01(tau): pulses at 0 and tau
-            am[2 * (1 + n_tau + i)] = amp;
-            am[2 * (1 + n_tau + i) + 1] = amp;
+            if (!w1) {
+                tp[2 * (1 + i)] = taus[i];  // M1(tau): pulse at tau only
+                am[2 * (1 + i)] = amp;
+            }
+            tp[2 * (1 + n1 + i) + 1] = taus[i];  // M01(tau): pulses at 0 and tau
+            am[2 * (1 + n1 + i)] = amp;
+            am[2 * (1 + n1 + i) + 1] = amp;
         }
         size_t n_steps = 0, n_rec = 0;
         {
@@ -958,7 +991,6 @@ int csl_pump_probe(csl_lattice* h, const double* ground, const double* field_in,
                                               CSL_TGRID_BOOST, 0, n_rec, tt.data(), mg.data(), nullptr, nullptr,
                                               &nr);
         // reset the pulse as the reference drivers do (lattice_md.cpp:578-583)
-        std::vector<double> zero(2 * A * 3, 0.0);
         (void)csl_set_pulse(h, zero.data(), 0.0, width, freq, 0.0, 0.0);
         if (rc2 != CSL_OK) fail(rc2, g_err);
         // CPU observer gate (lattice_md.cpp:532-554)
@@ -971,16 +1003,54 @@ int csl_pump_probe(csl_lattice* h, const double* ground, const double* field_in,
                 last = t;
             }
         }
-        if (n_rec_out) *n_rec_out = keep.size();
-        if (keep.size() > max_rec) fail(CSL_ERR_CAPACITY, "max_rec is smaller than the number of records");
-        for (size_t j = 0; j < keep.size(); ++j) {
+        const size_t nk = keep.size();
+        if (n_rec_out) *n_rec_out = nk;
+        if (nk > max_rec) fail(CSL_ERR_CAPACITY, "max_rec is smaller than the number of records");
+        for (size_t j = 0; j < nk; ++j) {
             const size_t r = keep[j];
             if (times) times[j] = tt[r];
             if (M0) std::memcpy(M0 + j * 9, &mg[(r * B + 0) * 9], 9 * sizeof(double));
             for (size_t i = 0; i < n_tau; ++i) {
-                if (M1) std::memcpy(M1 + (i * max_rec + j) * 9, &mg[(r * B + 1 + i) * 9], 9 * sizeof(double));
+                if (M1 && !w1)
+                    std::memcpy(M1 + (i * max_rec + j) * 9, &mg[(r * B + 1 + i) * 9], 9 * sizeof(double));
                 if (M01)
-                    std::memcpy(M01 + (i * max_rec + j) * 9, &mg[(r * B + 1 + n_tau + i) * 9], 9 * sizeof(double));
+                    std::memcpy(M01 + (i * max_rec + j) * 9, &mg[(r * B + 1 + n1 + i) * 9], 9 * sizeof(double));
+            }
+        }
+        if (w1 && M1 && nk > 0) {
+            // synthesize_M1_from_M0 (lattice_md.cpp:283-317): M1(t_i) = M_ground
+            // before tau + T_start_M0, else M0 at index lround((t_i - tau - T_start_M0) / T_step)
+            // clamped to the record range.  M_ground (lattice_md.cpp:737-742) is
+            // {magnetization_local_antiferro, magnetization_local,
+            // magnetization_global} of the ground state: alternating sign (-1)^i
+            // over the site index, plain mean, and mean of R_atom S (lattice.h:
+            // 4430-4460, 4885-4902), a 9-value reduction of the host input.
+            double Mg[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+            for (size_t i = 0; i < h->N; ++i) {
+                const double* S = ground + i * 3;
+                const double sgn = (i & 1) ? -1.0 : 1.0;
+                const double* R = &h->h_frames[(i % A) * 9];
+                for (int mu = 0; mu < 3; ++mu) {
+                    Mg[mu] += sgn * S[mu];
+                    Mg[3 + mu] += S[mu];
+                    Mg[6 + mu] += R[mu * 3 + 0] * S[0] + R[mu * 3 + 1] * S[1] + R[mu * 3 + 2] * S[2];
+                }
+            }
+            for (double& v : Mg) v /= double(h->N);
+            const double t_first = tt[keep[0]];
+            for (size_t i = 0; i < n_tau; ++i) {
+                const double thr = taus[i] + t_first;
+                for (size_t j = 0; j < nk; ++j) {
+                    const double ti = tt[keep[j]];
+                    const double* src = Mg;
+                    if (!(ti < thr)) {
+                        long long idx = std::llround((ti - taus[i] - t_first) / T_step);
+                        if (idx < 0) idx = 0;
+                        if (idx >= (long long)nk) idx = (long long)nk - 1;
+                        src = &mg[(keep[size_t(idx)] * B + 0) * 9];
+                    }
+                    std::memcpy(M1 + (i * max_rec + j) * 9, src, 9 * sizeof(double));
+                }
             }
         }
         if (tau_values) std::memcpy(tau_values, taus.data(), n_tau * sizeof(double));

@@ -250,8 +250,10 @@ class GPULattice:
         return ms.value, n.value
 
     def pump_probe(self, ground, field_in, amp, width, freq, tau_start, tau_end, tau_step,
-                   T0, T1, T_step):
-        """csl_pump_probe: M0, M1(tau), M01(tau) in one batched launch."""
+                   T0, T1, T_step, reuse_m0_for_m1=False, stationarity_tol=1e-6):
+        """csl_pump_probe_w1: M0, M1(tau), M01(tau) in one batched launch;
+        with reuse_m0_for_m1 and a stationary ground state M1 is the time
+        shift of M0 (W1).  The result carries w1 (bool) and max_dsdt."""
         g = f64(ground).reshape(-1)
         fi = f64(field_in).reshape(self.A, 3)
         n_tau_max = int(abs((tau_end - tau_start) / tau_step)) + 2
@@ -263,13 +265,17 @@ class GPULattice:
         M01 = np.empty(n_tau_max * max_rec * 9)
         taus = np.empty(n_tau_max)
         nr, nt = C.c_size_t(), C.c_size_t()
-        check(lib().csl_pump_probe(self.h, dptr(g), dptr(fi.reshape(-1)), amp, width, freq, tau_start,
-                                   tau_end, tau_step, T0, T1, T_step, max_rec, n_tau_max, C.byref(nr),
-                                   C.byref(nt), dptr(times), dptr(M0), dptr(M1), dptr(M01), dptr(taus)))
+        w1, mx = C.c_int(0), C.c_double(float("nan"))
+        check(lib().csl_pump_probe_w1(self.h, dptr(g), dptr(fi.reshape(-1)), amp, width, freq, tau_start,
+                                      tau_end, tau_step, T0, T1, T_step, int(bool(reuse_m0_for_m1)),
+                                      float(stationarity_tol), max_rec, n_tau_max, C.byref(nr), C.byref(nt),
+                                      dptr(times), dptr(M0), dptr(M1), dptr(M01), dptr(taus), C.byref(w1),
+                                      C.byref(mx)))
         r, t = nr.value, nt.value
         return {"times": times[:r], "M0": M0.reshape(max_rec, 3, 3)[:r],
                 "M1": M1.reshape(n_tau_max, max_rec, 3, 3)[:t, :r],
-                "M01": M01.reshape(n_tau_max, max_rec, 3, 3)[:t, :r], "tau": taus[:t]}
+                "M01": M01.reshape(n_tau_max, max_rec, 3, 3)[:t, :r], "tau": taus[:t],
+                "w1": bool(w1.value), "max_dsdt": mx.value}
 
 
 # ---------------------------------------------------------------------------
@@ -350,18 +356,22 @@ def pump_probe_spectroscopy(gl: GPULattice, ground, field_in, pulse_amp, pulse_w
                             tau_start, tau_end, tau_step, T_start, T_end, T_step,
                             dir_name: Optional[str] = None, method: str = "rk4",
                             Temp_start: float = 0.0, Temp_end: float = 0.0, n_anneal: int = 0,
-                            T_zero_quench: bool = False, quench_sweeps: int = 0):
+                            T_zero_quench: bool = False, quench_sweeps: int = 0,
+                            reuse_m0_for_m1: bool = False, stationarity_tol: float = 1e-6):
     """Lattice::pump_probe_spectroscopy (lattice_md.cpp:683-926): M0, M1(tau)
     and M01(tau) for every delay, all integrated in one batched launch.
     With dir_name the result goes to dir_name/pump_probe_spectroscopy.h5 in
     the HDF5PumpProbeWriter layout (hdf5_io.h:729-1000; libcsl_h5.so), with
     the ground state's E/N and |M_local| and the annealing parameters (which
     only describe how the ground state was made) as metadata.
-    Returns dict(times, M0 [n,3,3], M1 [n_tau,n,3,3], M01, tau)."""
+    reuse_m0_for_m1 / stationarity_tol are the reference's W1 option
+    (lattice_md.cpp:729-752): M1(tau) synthesised from M0 when max|dS/dt| of
+    the undriven ground state is within the tolerance.
+    Returns dict(times, M0 [n,3,3], M1 [n_tau,n,3,3], M01, tau, w1, max_dsdt)."""
     if method != "rk4":
         raise _capi.CslError(_capi.CSL_ERR_UNSUPPORTED, "only method 'rk4' is implemented")
     res = gl.pump_probe(ground, field_in, pulse_amp, pulse_width, pulse_freq, tau_start, tau_end,
-                        tau_step, T_start, T_end, T_step)
+                        tau_step, T_start, T_end, T_step, reuse_m0_for_m1, stationarity_tol)
     if dir_name is not None:
         from . import h5io
         lat = gl.lattice

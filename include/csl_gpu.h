@@ -202,6 +202,25 @@ int csl_pump_probe(csl_lattice* h, const double* ground, const double* field_in,
                    double* times, double* M0, double* M1, double* M01,
                    double* tau_values);
 
+/* csl_pump_probe with the reference's W1 option (reuse_m0_for_m1,
+ * lattice_md.cpp:729-752): when reuse_m0_for_m1 is set and the L_inf norm of
+ * dS/dt at the ground state with the drive off (max_dSdt_norm_no_drive,
+ * :236-280, evaluated on the device) is <= stationarity_tol, the M1(tau)
+ * replicas are not integrated; M1 is synthesised from M0 by the time shift of
+ * synthesize_M1_from_M0 (:283-317), with the ground-state magnetisations
+ * before tau.  *used_w1 (may be NULL) receives 1 when that happened,
+ * *max_dsdt (may be NULL) the norm when it was evaluated.  The handle's spins
+ * hold the ground state afterwards when reuse_m0_for_m1 is set.  Same
+ * outputs as csl_pump_probe. */
+int csl_pump_probe_w1(csl_lattice* h, const double* ground, const double* field_in,
+                      double amp, double width, double freq,
+                      double tau_start, double tau_end, double tau_step,
+                      double T0, double T1, double T_step,
+                      int reuse_m0_for_m1, double stationarity_tol,
+                      size_t max_rec, size_t max_tau, size_t* n_rec, size_t* n_tau,
+                      double* times, double* M0, double* M1, double* M01,
+                      double* tau_values, int* used_w1, double* max_dsdt);
+
 /* ---- autotuner ------------------------------------------------------- */
 
 /* Times the persistent kernel for B trajectories over n_probe_steps at each

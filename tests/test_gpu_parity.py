@@ -171,6 +171,38 @@ def test_pump_probe_2dcs_small(po):
         assert np.max(np.abs(res[k] - ref[k])) <= TOL_TRAJ, k
 
 
+def test_pump_probe_w1_reuse_m0(po):
+    """W1 (reuse_m0_for_m1): on a stationary ground state (3000 greedy
+    sweeps, max|dS/dt| ~ 1e-13) M1(tau) is the time shift of M0, matching the
+    oracle's synthesize_M1_from_M0 and, to integration accuracy, the
+    integrated M1; on a non-stationary state the guard falls back."""
+    uc, lat, o = _pair(po, 6, 6, h=0.1)
+    g = seeding.quench(lat, random_state(lat, 3), sweeps=3000)
+    f = unit_x_field(lat)
+    args = (0.5, 0.5, 0.0, 0.0, 1.0, 0.25, -1.0, 2.005, 0.01)
+    ref = o.pump_probe(g.reshape(-1), f, *args, reuse_m0=True, stat_tol=1e-6, n_outer=1)
+    assert ref["used_w1"]
+    with GPULattice(lat) as gl:
+        res = G.pump_probe_spectroscopy(gl, g, f, *args, reuse_m0_for_m1=True, stationarity_tol=1e-6)
+        full = G.pump_probe_spectroscopy(gl, g, f, *args)
+        tilted = seeding.tilt(g, 0.01, seed=9)
+        res2 = G.pump_probe_spectroscopy(gl, tilted, f, *args, reuse_m0_for_m1=True, stationarity_tol=1e-6)
+        full2 = G.pump_probe_spectroscopy(gl, tilted, f, *args)
+    assert res["w1"] and res["max_dsdt"] <= 1e-10
+    assert abs(res["max_dsdt"] - o.max_dsdt_no_drive(g.reshape(-1))) <= 1e-15
+    assert np.array_equal(res["times"], ref["times"])
+    for k in ("M0", "M1", "M01"):
+        assert np.max(np.abs(res[k] - ref[k])) <= TOL_TRAJ, k
+    assert np.max(np.abs(res["M1"] - full["M1"])) <= 1e-8
+    # batch sizes differ (1 + n_tau vs 1 + 2 n_tau replicas): launch shapes,
+    # hence the order of the record reductions, may differ
+    for k in ("M0", "M01"):
+        assert np.max(np.abs(res[k] - full[k])) <= 1e-13, k
+    assert not res2["w1"] and res2["max_dsdt"] > 1e-6
+    for k in ("M0", "M1", "M01"):
+        assert np.max(np.abs(res2[k] - full2[k])) <= 1e-13, k
+
+
 def test_general_tables_non_uniform(po):
     """Per-bond distinct matrices and per-site fields take the general
     (non-compressed) path; checked against the oracle built from the same CSR."""
