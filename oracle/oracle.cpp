@@ -34,12 +34,20 @@
  *                             state is stationary (W1 identity); serial and
  *                             OpenMP tau loops agree
  *   tangent linear .......... central finite differences of the trajectory
+ *   Dormand-Prince 5(4) ..... tableau order conditions (exact rationals);
+ *                             5th-order global convergence and 4th-order
+ *                             embedded error on y' = l y + cos(w t) (closed
+ *                             form); tolerance-proportional error vs a
+ *                             converged solution; the accept/reject
+ *                             sequence of Boost.Odeint's controller is
+ *                             parity unpinned (Boost is not in the tree)
  */
 #include "oracle.h"
 
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <functional>
 #include <limits>
 #include <vector>
 
@@ -371,6 +379,147 @@ void set_pulse(OracleLattice& L, const double* f1, double t1, const double* f2, 
 
 }  // namespace
 
+// ---- Dormand-Prince 5(4) -------------------------------------------------
+//
+// Tableau of Dormand & Prince, J. Comput. Appl. Math. 6 (1980) 19-26, the one
+// Boost.Odeint's runge_kutta_dopri5 implements (FSAL: the derivative at the
+// accepted x_new starts the next step).  The controller restates Boost.Odeint's
+// controlled_runge_kutta with default_error_checker (a_x = a_dxdt = 1) and
+// default_step_adjuster; Boost is a third-party dependency absent from the
+// reference tree, so the accept/reject sequence is parity unpinned against
+// Boost and pinned instead by the tableau's order conditions, the 5th-order
+// convergence of the propagated solution and the 4th-order embedded error on
+// closed-form problems, and convergence to a tight-tolerance solution.
+namespace {
+
+struct DP5 {
+    static constexpr double c2 = 1.0 / 5.0, c3 = 3.0 / 10.0, c4 = 4.0 / 5.0, c5 = 8.0 / 9.0;
+    static constexpr double a21 = 1.0 / 5.0;
+    static constexpr double a31 = 3.0 / 40.0, a32 = 9.0 / 40.0;
+    static constexpr double a41 = 44.0 / 45.0, a42 = -56.0 / 15.0, a43 = 32.0 / 9.0;
+    static constexpr double a51 = 19372.0 / 6561.0, a52 = -25360.0 / 2187.0, a53 = 64448.0 / 6561.0,
+                            a54 = -212.0 / 729.0;
+    static constexpr double a61 = 9017.0 / 3168.0, a62 = -355.0 / 33.0, a63 = 46732.0 / 5247.0,
+                            a64 = 49.0 / 176.0, a65 = -5103.0 / 18656.0;
+    static constexpr double b1 = 35.0 / 384.0, b3 = 500.0 / 1113.0, b4 = 125.0 / 192.0,
+                            b5 = -2187.0 / 6784.0, b6 = 11.0 / 84.0;
+    // b - b* (4th-order weights 5179/57600, 7571/16695, 393/640, -92097/339200, 187/2100, 1/40)
+    static constexpr double e1 = 35.0 / 384.0 - 5179.0 / 57600.0, e3 = 500.0 / 1113.0 - 7571.0 / 16695.0,
+                            e4 = 125.0 / 192.0 - 393.0 / 640.0, e5 = -2187.0 / 6784.0 + 92097.0 / 339200.0,
+                            e6 = 11.0 / 84.0 - 187.0 / 2100.0, e7 = -1.0 / 40.0;
+};
+
+struct DP5Work {
+    std::vector<double> k2, k3, k4, k5, k6, k7, tmp, xnew, xerr;
+    explicit DP5Work(size_t n) : k2(n), k3(n), k4(n), k5(n), k6(n), k7(n), tmp(n), xnew(n), xerr(n) {}
+};
+
+// One attempt from (x, k1 = f(x, t)): xnew, k7 = f(xnew, t + dt) and the
+// embedded error xerr.  Stage sums are evaluated left to right.
+template <class F>
+void dp5_attempt(F&& f, const double* x, const double* k1, double t, double dt, size_t n, DP5Work& w) {
+    using C = DP5;
+    for (size_t i = 0; i < n; ++i) w.tmp[i] = x[i] + dt * C::a21 * k1[i];
+    f(w.tmp.data(), w.k2.data(), t + C::c2 * dt);
+    for (size_t i = 0; i < n; ++i) w.tmp[i] = x[i] + dt * (C::a31 * k1[i] + C::a32 * w.k2[i]);
+    f(w.tmp.data(), w.k3.data(), t + C::c3 * dt);
+    for (size_t i = 0; i < n; ++i)
+        w.tmp[i] = x[i] + dt * (C::a41 * k1[i] + C::a42 * w.k2[i] + C::a43 * w.k3[i]);
+    f(w.tmp.data(), w.k4.data(), t + C::c4 * dt);
+    for (size_t i = 0; i < n; ++i)
+        w.tmp[i] = x[i] + dt * (C::a51 * k1[i] + C::a52 * w.k2[i] + C::a53 * w.k3[i] + C::a54 * w.k4[i]);
+    f(w.tmp.data(), w.k5.data(), t + C::c5 * dt);
+    for (size_t i = 0; i < n; ++i)
+        w.tmp[i] = x[i] + dt * (C::a61 * k1[i] + C::a62 * w.k2[i] + C::a63 * w.k3[i] + C::a64 * w.k4[i] +
+                                C::a65 * w.k5[i]);
+    f(w.tmp.data(), w.k6.data(), t + dt);
+    for (size_t i = 0; i < n; ++i)
+        w.xnew[i] = x[i] + dt * (C::b1 * k1[i] + C::b3 * w.k3[i] + C::b4 * w.k4[i] + C::b5 * w.k5[i] +
+                                 C::b6 * w.k6[i]);
+    f(w.xnew.data(), w.k7.data(), t + dt);
+    for (size_t i = 0; i < n; ++i)
+        w.xerr[i] = dt * (C::e1 * k1[i] + C::e3 * w.k3[i] + C::e4 * w.k4[i] + C::e5 * w.k5[i] +
+                          C::e6 * w.k6[i] + C::e7 * w.k7[i]);
+}
+
+// default_error_checker::error: max_i |xerr_i| / (abs + rel (|x_i| + |dt| |dxdt_i|)).
+double dp5_error(const double* x, const double* k1, const double* xerr, double dt, double abs_tol,
+                 double rel_tol, size_t n) {
+    double m = 0.0;
+    for (size_t i = 0; i < n; ++i) {
+        const double r = std::abs(xerr[i]) / (abs_tol + rel_tol * (std::abs(x[i]) + std::abs(dt) * std::abs(k1[i])));
+        m = std::max(m, r);
+    }
+    return m;
+}
+
+// default_step_adjuster: shrink after a rejection (error order 4), grow after
+// an accepted step with err < 1/2 (stepper order 5, at most x5).
+double dp5_decrease(double dt, double err) {
+    return dt * std::max(0.9 * std::pow(err, -1.0 / 3.0), 0.2);
+}
+double dp5_increase(double dt, double err) {
+    if (err < 0.5) {
+        err = std::max(std::pow(5.0, -5.0), err);
+        dt = dt * (0.9 * std::pow(err, -1.0 / 5.0));
+    }
+    return dt;
+}
+
+// boost::numeric::odeint::detail::less_with_sign for dt > 0.
+bool less_with_sign(double t1, double t2) { return t2 - t1 > std::numeric_limits<double>::epsilon(); }
+
+constexpr int kMaxRejections = 500;  // failed_step_checker
+
+// Controlled DP5 state: x and its FSAL derivative k1 at time t.
+struct DP5Run {
+    std::function<void(const double*, double*, double)> f;
+    double* x;
+    size_t n;
+    std::vector<double> k1;
+    DP5Work w;
+    double abs_tol, rel_tol;
+    size_t accepted = 0, rejected = 0;
+    DP5Run(std::function<void(const double*, double*, double)> f_, double* x_, size_t n_, double t,
+           double a, double r)
+        : f(std::move(f_)), x(x_), n(n_), k1(n_), w(n_), abs_tol(a), rel_tol(r) {
+        f(x, k1.data(), t);
+    }
+    // controlled try_step: true when the step was accepted (t, dt advanced)
+    bool try_step(double& t, double& dt) {
+        dp5_attempt(f, x, k1.data(), t, dt, n, w);
+        const double err = dp5_error(x, k1.data(), w.xerr.data(), dt, abs_tol, rel_tol, n);
+        if (!(err <= 1.0)) {  // NaN counts as a failure
+            dt = dp5_decrease(dt, err);
+            ++rejected;
+            return false;
+        }
+        std::memcpy(x, w.xnew.data(), n * sizeof(double));
+        std::memcpy(k1.data(), w.k7.data(), n * sizeof(double));
+        t += dt;
+        dt = dp5_increase(dt, err);
+        ++accepted;
+        return true;
+    }
+    // integrate_adaptive (controlled_stepper_tag) from t to t_end; obs(t) at
+    // the start and after every accepted step when with_obs.  Returns false
+    // after kMaxRejections rejections in a row.
+    template <class Obs>
+    bool adaptive(double t, double t_end, double dt, Obs&& obs, bool with_obs) {
+        while (less_with_sign(t, t_end)) {
+            if (with_obs) obs(t);
+            if (less_with_sign(t_end, t + dt)) dt = t_end - t;
+            int fails = 0;
+            while (!try_step(t, dt))
+                if (++fails >= kMaxRejections) return false;
+        }
+        if (with_obs) obs(t);
+        return true;
+    }
+};
+
+}  // namespace
+
 extern "C" {
 
 OracleLattice* orc_lattice_new(int spin_dim, int n_atoms, int d1, int d2, int d3, int n_bonds,
@@ -543,6 +692,105 @@ size_t orc_integrate_md(const OracleLattice* L, double* state, double T0, double
     auto stp = [&](double t, double h) { rk4_step_generic(f, state, n, t, h, w); };
     integrate_const(stp, T0, T1, dt, obs);
     return nrec;
+}
+
+long orc_integrate_dopri5(const OracleLattice* L, double* state, double T0, double T1, double dt,
+                          int control, double abs_tol, double rel_tol, size_t save_interval,
+                          size_t max_rec, double* times, double* out_mags, double* energy,
+                          size_t* n_accepted, size_t* n_rejected) {
+    const size_t n = L->N * L->D;
+    size_t step_count = 0, nrec = 0;
+    std::vector<double> m(3 * L->D);
+    auto obs = [&](double t) {  // MD observer (lattice_md.cpp:383-436)
+        if (step_count % save_interval == 0) {
+            if (nrec < max_rec) {
+                mags(*L, state, m.data());
+                if (times) times[nrec] = t;
+                if (out_mags) std::memcpy(out_mags + nrec * 3 * L->D, m.data(), sizeof(double) * 3 * L->D);
+                if (energy) energy[nrec] = orc_energy(L, state) / double(L->N);
+            }
+            ++nrec;
+        }
+        ++step_count;
+    };
+    auto f = [L](const double* a, double* b, double tt) { rhs(*L, a, b, tt); };
+    bool ok = true;
+    size_t acc = 0, rej = 0;
+    if (control == 0) {
+        // fixed steps on the integrate_const grid, no error control
+        DP5Work w(n);
+        std::vector<double> k1(n);
+        auto stp = [&](double t, double h) {
+            f(state, k1.data(), t);
+            dp5_attempt(f, state, k1.data(), t, h, n, w);
+            std::memcpy(state, w.xnew.data(), n * sizeof(double));
+            ++acc;
+        };
+        integrate_const(stp, T0, T1, dt, obs);
+    } else {
+        DP5Run run(f, state, n, T0, abs_tol, rel_tol);
+        if (control == 1) {
+            ok = run.adaptive(T0, T1, dt, obs, true);  // integrate_adaptive
+        } else {
+            // integrate_const with a controlled stepper: observe on the grid
+            // t_k = T0 + k dt, adaptive steps inside each interval, each
+            // interval starting from the initial dt again.
+            double t = T0;
+            size_t k = 0;
+            while (ok && less_eq_with_sign(t + dt, T1)) {
+                obs(t);
+                const double t_next = T0 + double(k + 1) * dt;
+                ok = run.adaptive(t, t_next, dt, obs, false);
+                ++k;
+                t = t_next;
+            }
+            if (ok) obs(t);
+        }
+        acc = run.accepted;
+        rej = run.rejected;
+    }
+    if (n_accepted) *n_accepted = acc;
+    if (n_rejected) *n_rejected = rej;
+    return ok ? long(nrec) : -1;
+}
+
+/* Scalar test problem for the Dormand-Prince pinning tests:
+ * y' = lambda*y + cos(omega*t), integrated with the same stepper code. */
+long orc_dopri5_scalar(double lambda, double omega, double y0, double T0, double T1, double dt,
+                       int control, double abs_tol, double rel_tol, double* y_end, size_t* n_accepted,
+                       size_t* n_rejected, double* err_first_step) {
+    double y = y0;
+    auto f = [lambda, omega](const double* a, double* b, double tt) { b[0] = lambda * a[0] + std::cos(omega * tt); };
+    if (err_first_step) {  // embedded error estimate of one step of size dt from (T0, y0)
+        DP5Work w(1);
+        double k1;
+        f(&y, &k1, T0);
+        dp5_attempt(f, &y, &k1, T0, dt, 1, w);
+        err_first_step[0] = w.xerr[0];
+        err_first_step[1] = w.xnew[0];
+    }
+    size_t acc = 0, rej = 0;
+    bool ok = true;
+    if (control == 0) {
+        DP5Work w(1);
+        double k1;
+        auto stp = [&](double t, double h) {
+            f(&y, &k1, t);
+            dp5_attempt(f, &y, &k1, t, h, 1, w);
+            y = w.xnew[0];
+            ++acc;
+        };
+        integrate_const(stp, T0, T1, dt, [](double) {});
+    } else {
+        DP5Run run(f, &y, 1, T0, abs_tol, rel_tol);
+        ok = run.adaptive(T0, T1, dt, [](double) {}, false);
+        acc = run.accepted;
+        rej = run.rejected;
+    }
+    *y_end = y;
+    if (n_accepted) *n_accepted = acc;
+    if (n_rejected) *n_rejected = rej;
+    return ok ? 0 : -1;
 }
 
 size_t orc_single_pulse_drive(OracleLattice* L, const double* init, const double* field_in,

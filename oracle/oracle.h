@@ -77,6 +77,28 @@ size_t orc_boost_num_steps(double t0, double t1, double dt);
 size_t orc_integrate_md(const OracleLattice* L, double* state, double T0, double T1,
                         double dt, size_t save_interval, size_t max_rec,
                         double* times, double* mags, double* energy_per_site);
+/* Dormand-Prince 5(4) (Boost.Odeint runge_kutta_dopri5, make_controlled(abs,
+ * rel)) as integrate_ode_system uses it (lattice.h:4072-4085):
+ *   control 0: fixed steps on the integrate_const grid (no error control,
+ *              the reference GPU stepper's "dopri5", integrator.cuh:186-228);
+ *   control 1: integrate_adaptive, observer after every accepted step
+ *              (molecular_dynamics_cpu, lattice_md.cpp:452-453);
+ *   control 2: integrate_const with the controlled stepper, observer on the
+ *              grid T0 + k dt (pulse drivers, lattice_md.cpp:570-575).
+ * Records as orc_integrate_md.  Returns the number of records, or -1 after
+ * 500 rejections in a row. */
+long   orc_integrate_dopri5(const OracleLattice* L, double* state, double T0, double T1,
+                            double dt, int control, double abs_tol, double rel_tol,
+                            size_t save_interval, size_t max_rec, double* times,
+                            double* mags, double* energy_per_site, size_t* n_accepted,
+                            size_t* n_rejected);
+/* The same stepper on y' = lambda*y + cos(omega*t) (pinning tests):
+ * y(T1), step counts, and for one step of size dt from (T0, y0) the embedded
+ * error estimate and the propagated value in err_first_step[0..1]. */
+long   orc_dopri5_scalar(double lambda, double omega, double y0, double T0, double T1,
+                         double dt, int control, double abs_tol, double rel_tol,
+                         double* y_end, size_t* n_accepted, size_t* n_rejected,
+                         double* err_first_step);
 /* single_pulse_drive / double_pulse_drive, CPU branch, method "rk4"
  * (lattice_md.cpp:493-586, :589-680).  Returns the number of records. */
 size_t orc_single_pulse_drive(OracleLattice* L, const double* init, const double* field_in,

@@ -39,6 +39,8 @@ _SIG = {
     "orc_single_pulse_drive": (_sz, [_vp, _pd, _pd, _d, _d, _d, _d, _d, _d, _d, _i, _sz, _pd, _pd]),
     "orc_double_pulse_drive": (_sz, [_vp, _pd, _pd, _d, _pd, _d, _d, _d, _d, _d, _d, _d, _i, _sz, _pd, _pd]),
     "orc_max_dsdt_no_drive": (_d, [_vp, _pd]),
+    "orc_integrate_dopri5": (C.c_long, [_vp, _pd, _d, _d, _d, _i, _d, _d, _sz, _sz, _pd, _pd, _pd, _psz, _psz]),
+    "orc_dopri5_scalar": (C.c_long, [_d, _d, _d, _d, _d, _d, _i, _d, _d, _pd, _psz, _psz, _pd]),
     "orc_pump_probe": (_i, [_vp, _pd, _pd, _d, _d, _d, _d, _d, _d, _d, _d, _d, _i, _d, _i, _i, _sz, _sz,
                             _psz, _pd, _pd, _pd, _pd, _pd, _pi]),
     "orc_build_pulse_segments": (_sz, [_d, _d, _pd, _sz, _d, _d, _d, _sz, _pd]),
@@ -179,6 +181,28 @@ class OracleLattice:
         e = np.zeros(mx)
         nr = lib().orc_integrate_md(self.h, _p(s), T0, T1, dt, save_interval, mx, _p(t), _p(m), _p(e))
         return {"times": t[:nr], "mags": m.reshape(mx, 3, self.D)[:nr], "energy": e[:nr], "final": s}
+
+    def integrate_dopri5(self, state, T0, T1, dt, control=1, abs_tol=1e-6, rel_tol=1e-6, save_interval=1,
+                         max_rec=None):
+        """Dormand-Prince 5(4): control 0 fixed grid steps, 1 integrate_adaptive,
+        2 integrate_const with the controlled stepper."""
+        s = _f(state).copy()
+        mx = max_rec or 1000
+        while True:
+            t = np.zeros(mx)
+            m = np.zeros(mx * 3 * self.D)
+            e = np.zeros(mx)
+            na, nr_ = C.c_size_t(), C.c_size_t()
+            s = _f(state).copy()
+            nr = lib().orc_integrate_dopri5(self.h, _p(s), T0, T1, dt, control, abs_tol, rel_tol, save_interval,
+                                            mx, _p(t), _p(m), _p(e), C.byref(na), C.byref(nr_))
+            if nr < 0:
+                raise RuntimeError("step size control failed (500 rejections in a row)")
+            if nr <= mx:
+                break
+            mx = nr
+        return {"times": t[:nr], "mags": m.reshape(mx, 3, self.D)[:nr], "energy": e[:nr], "final": s,
+                "accepted": na.value, "rejected": nr_.value}
 
     def single_pulse_drive(self, init, field_in, t_B, amp, width, freq, T0, T1, step, chunking=False):
         n = lib().orc_boost_num_steps(T0, T1, step) + 4
@@ -348,3 +372,16 @@ def pulse_segments(T0, T1, centers, window, T_step, free_dt_factor):
 
 def max_threads():
     return lib().orc_max_threads()
+
+
+def dopri5_scalar(lam, omega, y0, T0, T1, dt, control=1, abs_tol=1e-6, rel_tol=1e-6):
+    """The oracle's Dormand-Prince stepper on y' = lam*y + cos(omega*t).
+    Returns (y(T1), accepted, rejected, (err estimate, y1) of one step of size dt)."""
+    y = C.c_double()
+    na, nr = C.c_size_t(), C.c_size_t()
+    e = np.zeros(2)
+    rc = lib().orc_dopri5_scalar(lam, omega, y0, T0, T1, dt, control, abs_tol, rel_tol, C.byref(y), C.byref(na),
+                                 C.byref(nr), _p(e))
+    if rc < 0:
+        raise RuntimeError("step size control failed")
+    return y.value, na.value, nr.value, (e[0], e[1])
