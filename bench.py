@@ -215,6 +215,69 @@ def bench_md(args):
     print(json.dumps(line), flush=True)
 
 
+DP_T1, DP_TOL, DP_B = 100.0, 1e-6, 256
+
+
+def bench_md_dopri5(args):
+    """MD with the reference's default integrator (md_integrator "dopri5",
+    md_abs_tol = md_rel_tol = 1e-6, spin_config.h:93,111-112):
+    integrate_adaptive of the Kitaev 24x24x1 lattice (1152 sites) from t = 0
+    to 100 (the horizon of the RK4 MD workload), M(t) and E(t) at every
+    accepted step.  value = trajectories/s of one trajectory on the
+    Dormand-Prince kernel; a batch of 256 trajectories (one workgroup per CU,
+    seeded starts) is reported beside it."""
+    from classical_mcmc_spin_amd import GPULattice, model, seeding
+    uc = model.build_kitaev_honeycomb(K=-1.0, Gamma=0.25, Gammap=-0.02)
+    lat = model.Lattice(uc, MD_L, MD_L, 1)
+    s0 = seeding.random_spins(lat.lattice_size, 1234)
+    gl = GPULattice(lat)
+    run = lambda init: gl.integrate_dopri5(init, 0.0, DP_T1, MD_DT, control=1, abs_tol=DP_TOL, rel_tol=DP_TOL)
+    for _ in range(max(args.warmup, 1)):
+        r = run(s0)
+    t = time.perf_counter()
+    for _ in range(args.steps):
+        r = run(s0)
+    el = (time.perf_counter() - t) / args.steps
+    kms = gl.last_kernel_ms()[0]
+    acc, rej = r["accepted"][0], r["rejected"][0]
+    assert np.all(np.isfinite(r["mags"][0]))
+    inits = np.stack([seeding.random_spins(lat.lattice_size, 5000 + b) for b in range(DP_B)])
+    rb = run(inits)
+    t = time.perf_counter()
+    rb = run(inits)
+    elb = time.perf_counter() - t
+    kmsb = gl.last_kernel_ms()[0]
+    gl.close()
+    v = 1.0 / el
+    line = {
+        "metric": METRIC, "value": v, "unit": "trajectories/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": 1e3 * el, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "md_kitaev_honeycomb_24x24x1_1traj_t100_dopri5_adaptive_tol1e-6",
+                   "sites": lat.lattice_size, "T": DP_T1, "abs_tol": DP_TOL, "rel_tol": DP_TOL,
+                   "record_every": "accepted step"},
+        "accepted_steps": acc, "rejected_steps": rej, "kernel_ms": kms,
+        "site_steps_per_s": acc * lat.lattice_size / el,
+        "batched": {"trajectories": DP_B, "traj_per_s": DP_B / elb, "kernel_ms": kmsb,
+                    "accepted_steps_mean": float(np.mean(rb["accepted"])),
+                    "rejected_steps_mean": float(np.mean(rb["rejected"]))},
+    }
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import pyoracle
+        o = pyoracle.OracleLattice(uc, MD_L, MD_L, 1)
+        T = 5.0
+        t = time.perf_counter()
+        ro = o.integrate_dopri5(s0.reshape(-1), 0.0, T, MD_DT, control=1, abs_tol=DP_TOL, rel_tol=DP_TOL)
+        elc = time.perf_counter() - t
+        line["cpu_baseline"] = {
+            "value": T / DP_T1 / elc, "unit": "trajectories/s", "cores": pyoracle.max_threads(), "kind": "port",
+            "sample": f"t = 0..{T:g} of the 0..{DP_T1:g} run ({ro['accepted']} accepted steps, {elc:.1f} s, OpenMP "
+                      f"over sites inside each RHS)",
+            "site_steps_per_s": ro["accepted"] * lat.lattice_size / elc}
+    print(json.dumps(line), flush=True)
+
+
 ADJ_L, ADJ_STEPS, ADJ_B = 24, 2000, 256
 
 
@@ -388,7 +451,7 @@ def main():
     ap.add_argument("--block", type=int, default=0)
     ap.add_argument("--launch", default="", help="fixed persistent launch shape 'block,spt,dbuf' "
                                                    "instead of the autotuner's choice")
-    ap.add_argument("--workload", default="2dcs", choices=["2dcs", "pyrochlore", "mixed", "md", "adjoint"],
+    ap.add_argument("--workload", default="2dcs", choices=["2dcs", "pyrochlore", "mixed", "md", "md_dopri5", "adjoint"],
                     help="2dcs: the batched 2DCS workload (default); pyrochlore: 64 samples of the "
                          "65 536-site pyrochlore lattice; mixed: 257-trajectory SU(2)+SU(3) 2DCS batch "
                          "(both on the grid-wide stage kernels)")
@@ -408,6 +471,8 @@ def main():
         return bench_mixed(args)
     if args.workload == "md":
         return bench_md(args)
+    if args.workload == "md_dopri5":
+        return bench_md_dopri5(args)
     if args.workload == "adjoint":
         return bench_adjoint(args)
     from classical_mcmc_spin_amd import GPULattice

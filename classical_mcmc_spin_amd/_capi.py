@@ -21,6 +21,7 @@ CSL_ERR_NOSTATE = -4
 CSL_ERR_HIP = -5
 CSL_ERR_CAPACITY = -6
 CSL_ERR_NONFINITE = -7
+CSL_ERR_STEP = -8
 
 TGRID = {"gpu": 0, "batched": 1, "boost": 2}
 
@@ -59,8 +60,10 @@ SIGNATURES = {
     "csl_count_records": (_i, [_d, _d, _d, _sz, _i, _psz, _psz]),
     "csl_pump_probe": (_i, [_vp, _pd, _pd, _d, _d, _d, _d, _d, _d, _d, _d, _d, _sz, _sz, _psz, _psz,
                             _pd, _pd, _pd, _pd, _pd]),
-    "csl_pump_probe_w1": (_i, [_vp, _pd, _pd, _d, _d, _d, _d, _d, _d, _d, _d, _d, _i, _d, _sz, _sz, _psz,
-                               _psz, _pd, _pd, _pd, _pd, _pd, _pi, _pd]),
+    "csl_pump_probe_ex": (_i, [_vp, _pd, _pd, _d, _d, _d, _d, _d, _d, _d, _d, _d, _i, _d, _cs, _d, _d, _sz, _sz,
+                               _psz, _psz, _pd, _pd, _pd, _pd, _pd, _pi, _pd]),
+    "csl_integrate_dopri5": (_i, [_vp, _sz, _pd, _i, _pd, _pd, _d, _d, _d, _i, _d, _d, _sz, _i, _sz, _sz,
+                                  _pd, _pd, _pd, _pd, _psz, _psz, _psz]),
     "csl_autotune_batched": (_i, [_vp, _sz, _sz, _pi, _pd]),
     "csl_last_kernel_ms": (_i, [_vp, _pd, _psz]),
     "csl_last_launch_config": (_i, [_vp, _pi, _pi, _pi]),

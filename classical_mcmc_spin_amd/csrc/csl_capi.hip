@@ -63,6 +63,8 @@ struct csl_lattice {
     // workspaces
     DBuf<double> X, ACC, Y1, Y2, partials, fac, tgrid, rec_mags, rec_energy, rec_states, aux;
     DBuf<double> bt_init, bt_tp, bt_amp, bt_final;
+    DBuf<double> dp_times;        // Dormand-Prince records: times [B][max_rec]
+    DBuf<long long> dp_counters;  // [B][4]
     // CSR -> device slot mapping of the handle's tables (for per-replica tables)
     std::vector<size_t> h_off, h_slot_of, h_par;
     // per-trajectory Hamiltonians (csl_set_replica_hamiltonians)
@@ -908,24 +910,149 @@ int csl_integrate_batched(csl_lattice* h, size_t B, const double* init, int init
     });
 }
 
+int csl_integrate_dopri5(csl_lattice* h, size_t B, const double* init, int init_per_replica,
+                         const double* t_pulse, const double* pulse_amp, double T0, double T1, double dt,
+                         int control, double abs_tol, double rel_tol, size_t save_interval, int tgrid,
+                         size_t max_attempts, size_t max_rec, double* times, double* mags, double* energy,
+                         double* final_states, size_t* n_rec_out, size_t* n_acc_out, size_t* n_rej_out) {
+    return guarded([&] {
+        check_handle(h);
+        if (B == 0) fail(CSL_ERR_ARG, "batch size must be >= 1");
+        if (!init) fail(CSL_ERR_NULL, "null init");
+        if (B > 65535) fail(CSL_ERR_ARG, "at most 65535 trajectories per call");
+        if (control < 0 || control > 2) fail(CSL_ERR_ARG, "control must be 0, 1 or 2");
+        if (save_interval == 0 || save_interval > size_t(std::numeric_limits<int>::max()))
+            fail(CSL_ERR_ARG, "save_interval must be >= 1");
+        if (!(dt > 0.0) || !std::isfinite(dt) || !std::isfinite(T0) || !std::isfinite(T1))
+            fail(CSL_ERR_ARG, "need finite T0, T1 and dt > 0");
+        if (control != 0 && (!(abs_tol >= 0.0) || !(rel_tol >= 0.0) || !(abs_tol + rel_tol > 0.0)))
+            fail(CSL_ERR_ARG, "tolerances must be >= 0 and not both 0");
+        if (max_rec == 0 || max_rec > size_t(std::numeric_limits<int>::max()))
+            fail(CSL_ERR_ARG, "max_rec must be in [1, 2^31)");
+        const int block = csl::dopri5_block(h->L);
+        if (block == 0 || csl::dopri5_lds_bytes(h->L, block) > h->lds_limit)
+            fail(CSL_ERR_UNSUPPORTED, "the Dormand-Prince kernel holds at most 2048 sites per trajectory");
+        std::vector<double> grid;
+        if (control != 1) {
+            grid = make_grid(T0, T1, dt, tgrid);
+            if (grid.size() > size_t(std::numeric_limits<int>::max())) fail(CSL_ERR_ARG, "too many grid points");
+        }
+        const size_t nd = h->N * 3;
+        const size_t n_init = init_per_replica ? B * nd : nd;
+        check_finite(init, n_init, "initial state");
+        hipStream_t s = h->stream;
+        h->bt_init.upload(init, n_init, s);
+        std::vector<double> tp(2 * B, 0.0), am(2 * B, h->amp);
+        if (t_pulse) {
+            check_finite(t_pulse, 2 * B, "t_pulse");
+            tp.assign(t_pulse, t_pulse + 2 * B);
+        }
+        if (pulse_amp) {
+            check_finite(pulse_amp, 2 * B, "pulse_amp");
+            am.assign(pulse_amp, pulse_amp + 2 * B);
+        }
+        h->bt_tp.upload(tp.data(), tp.size(), s);
+        h->bt_amp.upload(am.data(), am.size(), s);
+        if (!grid.empty()) h->tgrid.upload(grid.data(), grid.size(), s);
+        h->dp_times.ensure(B * max_rec);
+        h->rec_mags.ensure(B * max_rec * 9);
+        if (energy) h->rec_energy.ensure(B * max_rec);
+        if (final_states) h->bt_final.ensure(B * nd);
+        h->dp_counters.ensure(B * 4);
+        csl::Dopri5Args A{};
+        A.B = int(B);
+        A.init = h->bt_init.p;
+        A.init_stride = init_per_replica ? nd : 0;
+        A.t_pulse = h->bt_tp.p;
+        A.amp = h->bt_amp.p;
+        A.control = control;
+        A.tgrid = grid.empty() ? nullptr : h->tgrid.p;
+        A.n_grid = int(grid.size());
+        A.T0 = T0;
+        A.T1 = T1;
+        A.dt = dt;
+        A.abs_tol = abs_tol;
+        A.rel_tol = rel_tol;
+        A.save_interval = int(save_interval);
+        A.max_rec = int(max_rec);
+        A.max_attempts = max_attempts ? (long long)std::min<size_t>(max_attempts, size_t(1) << 62) : 50000000LL;
+        A.times = h->dp_times.p;
+        A.mags = h->rec_mags.p;
+        A.energy = energy ? h->rec_energy.p : nullptr;
+        A.final_states = final_states ? h->bt_final.p : nullptr;
+        A.counters = h->dp_counters.p;
+        csl::LatDev L = h->L;
+        if (h->rep_B) {
+            if (B != h->rep_B) fail(CSL_ERR_ARG, "batch size differs from the per-replica Hamiltonians set");
+            L.mats = h->rep_mats.p;
+            L.field = h->rep_field.p;
+            A.mats_stride = h->rep_mats_stride;
+            A.field_stride = h->rep_field_stride;
+        }
+        hipEvent_t e0, e1;
+        hip_check(hipEventCreate(&e0), "event");
+        hip_check(hipEventCreate(&e1), "event");
+        hip_check(hipEventRecord(e0, s), "event record");
+        hip_check(csl::launch_dopri5(L, A, s), "k_dopri5 launch");
+        hip_check(hipEventRecord(e1, s), "event record");
+        hip_check(hipEventSynchronize(e1), "k_dopri5");
+        float ms = 0.f;
+        hip_check(hipEventElapsedTime(&ms, e0, e1), "elapsed");
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        h->last_ms = ms;
+        h->last_block = block;
+        h->last_spt = csl::dopri5_sites_per_thread(h->L);
+        h->last_dbuf = 0;
+        h->last_launches = 1;
+        std::vector<long long> cnt(B * 4);
+        hip_check(hipMemcpyAsync(cnt.data(), h->dp_counters.p, cnt.size() * sizeof(long long),
+                                 hipMemcpyDeviceToHost, s), "D2H");
+        copy_back(times, h->dp_times.p, B * max_rec, s);
+        copy_back(mags, h->rec_mags.p, B * max_rec * 9, s);
+        copy_back(energy, h->rec_energy.p, B * max_rec, s);
+        copy_back(final_states, h->bt_final.p, B * nd, s);
+        hip_check(hipStreamSynchronize(s), "integrate_dopri5");
+        bool over = false;
+        for (size_t b = 0; b < B; ++b) {
+            if (n_rec_out) n_rec_out[b] = size_t(cnt[b * 4 + 0]);
+            if (n_acc_out) n_acc_out[b] = size_t(cnt[b * 4 + 1]);
+            if (n_rej_out) n_rej_out[b] = size_t(cnt[b * 4 + 2]);
+            over |= size_t(cnt[b * 4 + 0]) > max_rec;
+        }
+        for (size_t b = 0; b < B; ++b) {
+            const long long st = cnt[b * 4 + 3];
+            if (st == csl::kDopri5StepFailed)
+                fail(CSL_ERR_STEP, "trajectory " + std::to_string(b) + ": step size control failed (500 rejections)");
+            if (st == csl::kDopri5AttemptCap)
+                fail(CSL_ERR_STEP, "trajectory " + std::to_string(b) + ": more than max_attempts attempts");
+            if (st == csl::kDopri5BadStep)
+                fail(CSL_ERR_STEP, "trajectory " + std::to_string(b) + ": step size no longer advances time");
+        }
+        if (over) fail(CSL_ERR_CAPACITY, "max_rec is smaller than the number of records of some trajectory");
+    });
+}
+
 int csl_pump_probe(csl_lattice* h, const double* ground, const double* field_in, double amp,
                    double width, double freq, double tau_start, double tau_end, double tau_step,
                    double T0, double T1, double T_step, size_t max_rec, size_t max_tau,
                    size_t* n_rec_out, size_t* n_tau_out, double* times, double* M0, double* M1,
                    double* M01, double* tau_values) {
-    return csl_pump_probe_w1(h, ground, field_in, amp, width, freq, tau_start, tau_end, tau_step, T0, T1, T_step,
-                             0, 0.0, max_rec, max_tau, n_rec_out, n_tau_out, times, M0, M1, M01, tau_values,
-                             nullptr, nullptr);
+    return csl_pump_probe_ex(h, ground, field_in, amp, width, freq, tau_start, tau_end, tau_step, T0, T1, T_step,
+                             0, 0.0, "rk4", 0.0, 0.0, max_rec, max_tau, n_rec_out, n_tau_out, times, M0, M1, M01,
+                             tau_values, nullptr, nullptr);
 }
 
-int csl_pump_probe_w1(csl_lattice* h, const double* ground, const double* field_in, double amp,
+int csl_pump_probe_ex(csl_lattice* h, const double* ground, const double* field_in, double amp,
                       double width, double freq, double tau_start, double tau_end, double tau_step,
                       double T0, double T1, double T_step, int reuse_m0_for_m1, double stationarity_tol,
-                      size_t max_rec, size_t max_tau, size_t* n_rec_out, size_t* n_tau_out, double* times,
-                      double* M0, double* M1, double* M01, double* tau_values, int* used_w1,
-                      double* max_dsdt) {
+                      const char* method, double abs_tol, double rel_tol, size_t max_rec, size_t max_tau,
+                      size_t* n_rec_out, size_t* n_tau_out, double* times, double* M0, double* M1, double* M01,
+                      double* tau_values, int* used_w1, double* max_dsdt) {
     return guarded([&] {
         check_handle(h);
+        const bool dp5 = method && std::strcmp(method, "dopri5") == 0;
+        if (!dp5 && !method_is_rk4(method)) fail(CSL_ERR_UNSUPPORTED, "pump-probe implements \"rk4\" and \"dopri5\"");
         if (!ground || !field_in) fail(CSL_ERR_NULL, "null ground state / field_in");
         if (!(tau_step != 0.0) || !std::isfinite(tau_step)) fail(CSL_ERR_ARG, "tau_step must be non-zero");
         if (reuse_m0_for_m1 && !(stationarity_tol >= 0.0)) fail(CSL_ERR_ARG, "stationarity_tol must be >= 0");
@@ -987,9 +1114,28 @@ int csl_pump_probe_w1(csl_lattice* h, const double* ground, const double* field_
         }
         std::vector<double> tt(n_rec), mg(n_rec * B * 9);
         size_t nr = 0;
-        const int rc2 = csl_integrate_batched(h, B, ground, 0, tp.data(), am.data(), T0, T1, T_step, 1, "rk4",
-                                              CSL_TGRID_BOOST, 0, n_rec, tt.data(), mg.data(), nullptr, nullptr,
-                                              &nr);
+        int rc2 = CSL_OK;
+        if (dp5) {
+            // integrate_const with the controlled Dormand-Prince stepper
+            // (single_pulse_drive / double_pulse_drive, lattice_md.cpp:570-575),
+            // one workgroup and step-size sequence per replica
+            std::vector<double> td(B * n_rec), md(B * n_rec * 9);
+            std::vector<size_t> nrb(B);
+            rc2 = csl_integrate_dopri5(h, B, ground, 0, tp.data(), am.data(), T0, T1, T_step, 2, abs_tol, rel_tol, 1,
+                                       CSL_TGRID_BOOST, 0, n_rec, td.data(), md.data(), nullptr, nullptr, nrb.data(),
+                                       nullptr, nullptr);
+            if (rc2 == CSL_OK) {
+                nr = nrb[0];
+                for (size_t r = 0; r < nr; ++r) {
+                    tt[r] = td[r];
+                    for (size_t b = 0; b < B; ++b)
+                        std::memcpy(&mg[(r * B + b) * 9], &md[(b * n_rec + r) * 9], 9 * sizeof(double));
+                }
+            }
+        } else {
+            rc2 = csl_integrate_batched(h, B, ground, 0, tp.data(), am.data(), T0, T1, T_step, 1, "rk4",
+                                        CSL_TGRID_BOOST, 0, n_rec, tt.data(), mg.data(), nullptr, nullptr, &nr);
+        }
         // reset the pulse as the reference drivers do (lattice_md.cpp:578-583)
         (void)csl_set_pulse(h, zero.data(), 0.0, width, freq, 0.0, 0.0);
         if (rc2 != CSL_OK) fail(rc2, g_err);

@@ -52,6 +52,43 @@ struct AdjArgs {
     size_t field_stride;
 };
 
+// Dormand-Prince 5(4) integration (csl_dopri5.hip): one workgroup per trajectory.
+struct Dopri5Args {
+    int B;
+    const double* init;        // AoS reference order; trajectory b at init + b*init_stride
+    size_t init_stride;
+    const double* t_pulse;     // [B*2]
+    const double* amp;         // [B*2]
+    int control;               // 0 fixed grid steps, 1 integrate_adaptive, 2 integrate_const + controller
+    const double* tgrid;       // control 0/2: observation grid [n_grid]
+    int n_grid;
+    double T0, T1, dt;
+    double abs_tol, rel_tol;
+    int save_interval;
+    int max_rec;
+    long long max_attempts;    // attempts per trajectory before giving up (status 2)
+    double* times;             // [B][max_rec]
+    double* mags;              // [B][max_rec][9] or null
+    double* energy;            // [B][max_rec] or null
+    double* final_states;      // [B][N*3] AoS or null
+    long long* counters;       // [B][4]: observer saves, accepted, rejected, status
+    size_t mats_stride;        // per-trajectory Hamiltonians (0 = shared)
+    size_t field_stride;
+};
+
+// status codes in Dopri5Args::counters[b*4 + 3]
+constexpr long long kDopri5Ok = 0;
+constexpr long long kDopri5StepFailed = 1;   // 500 rejections in a row
+constexpr long long kDopri5AttemptCap = 2;   // max_attempts reached
+constexpr long long kDopri5BadStep = 3;      // step size no longer advances t or is not finite
+
+// Sites per thread and workgroup size of the Dormand-Prince kernel for a
+// lattice (0 when the lattice does not fit one workgroup).
+int dopri5_sites_per_thread(const LatDev& L);
+int dopri5_block(const LatDev& L);
+size_t dopri5_lds_bytes(const LatDev& L, int block);
+hipError_t launch_dopri5(const LatDev& L, const Dopri5Args& A, hipStream_t st);
+
 // zero-temperature relaxation (csl_quench.hip)
 hipError_t launch_quench_class(const LatDev& L, double* x, const int* color, int c, const int* frozen, double len,
                                int B, hipStream_t st);

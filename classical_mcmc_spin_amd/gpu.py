@@ -250,10 +250,12 @@ class GPULattice:
         return ms.value, n.value
 
     def pump_probe(self, ground, field_in, amp, width, freq, tau_start, tau_end, tau_step,
-                   T0, T1, T_step, reuse_m0_for_m1=False, stationarity_tol=1e-6):
-        """csl_pump_probe_w1: M0, M1(tau), M01(tau) in one batched launch;
-        with reuse_m0_for_m1 and a stationary ground state M1 is the time
-        shift of M0 (W1).  The result carries w1 (bool) and max_dsdt."""
+                   T0, T1, T_step, reuse_m0_for_m1=False, stationarity_tol=1e-6, method="rk4",
+                   abs_tol=1e-8, rel_tol=1e-8):
+        """csl_pump_probe_ex: M0, M1(tau), M01(tau) in one batched launch
+        (method "rk4" on the Boost grid, or "dopri5" with step control per
+        replica); with reuse_m0_for_m1 and a stationary ground state M1 is the
+        time shift of M0 (W1).  The result carries w1 (bool) and max_dsdt."""
         g = f64(ground).reshape(-1)
         fi = f64(field_in).reshape(self.A, 3)
         n_tau_max = int(abs((tau_end - tau_start) / tau_step)) + 2
@@ -266,16 +268,63 @@ class GPULattice:
         taus = np.empty(n_tau_max)
         nr, nt = C.c_size_t(), C.c_size_t()
         w1, mx = C.c_int(0), C.c_double(float("nan"))
-        check(lib().csl_pump_probe_w1(self.h, dptr(g), dptr(fi.reshape(-1)), amp, width, freq, tau_start,
+        check(lib().csl_pump_probe_ex(self.h, dptr(g), dptr(fi.reshape(-1)), amp, width, freq, tau_start,
                                       tau_end, tau_step, T0, T1, T_step, int(bool(reuse_m0_for_m1)),
-                                      float(stationarity_tol), max_rec, n_tau_max, C.byref(nr), C.byref(nt),
-                                      dptr(times), dptr(M0), dptr(M1), dptr(M01), dptr(taus), C.byref(w1),
-                                      C.byref(mx)))
+                                      float(stationarity_tol), method.encode(), float(abs_tol), float(rel_tol),
+                                      max_rec, n_tau_max, C.byref(nr), C.byref(nt), dptr(times), dptr(M0),
+                                      dptr(M1), dptr(M01), dptr(taus), C.byref(w1), C.byref(mx)))
         r, t = nr.value, nt.value
         return {"times": times[:r], "M0": M0.reshape(max_rec, 3, 3)[:r],
                 "M1": M1.reshape(n_tau_max, max_rec, 3, 3)[:t, :r],
                 "M01": M01.reshape(n_tau_max, max_rec, 3, 3)[:t, :r], "tau": taus[:t],
                 "w1": bool(w1.value), "max_dsdt": mx.value}
+
+    def integrate_dopri5(self, init, T0, T1, dt, control=1, abs_tol=1e-6, rel_tol=1e-6, save_interval=1,
+                         tgrid="boost", t_pulse=None, pulse_amp=None, want_energy=True, want_final=True,
+                         max_rec=None, max_attempts=0):
+        """csl_integrate_dopri5: Dormand-Prince 5(4) for B trajectories
+        (init [N,3] -> B from t_pulse, or [B,N,3]); control 0 fixed grid
+        steps, 1 integrate_adaptive, 2 integrate_const with step control.
+        Returns dict(times, mags, energy: lists of per-trajectory arrays,
+        final [B,N,3], n_rec, accepted, rejected [B]); a too small record
+        buffer is enlarged and the (deterministic) run repeated."""
+        init = f64(init)
+        per = 1 if init.ndim == 3 else 0
+        tp = None if t_pulse is None else f64(t_pulse).reshape(-1, 2)
+        B = init.shape[0] if per else (1 if tp is None else tp.shape[0])
+        if tp is not None and tp.shape[0] != B:
+            raise ValueError("t_pulse has a different batch size than init")
+        amp = None if pulse_amp is None else f64(pulse_amp).reshape(B, 2)
+        if max_rec is None:
+            ns, nr = self.count_records(T0, T1, dt, 1, tgrid)
+            max_rec = nr + 1 if control != 1 else max(64, 2 * nr)
+        while True:
+            times = np.empty(B * max_rec)
+            mags = np.empty(B * max_rec * 9)
+            energy = np.empty(B * max_rec) if want_energy else None
+            final = np.empty(B * self.N * 3) if want_final else None
+            nrec = np.zeros(B, dtype=np.uintp)
+            nacc = np.zeros(B, dtype=np.uintp)
+            nrej = np.zeros(B, dtype=np.uintp)
+            rc = lib().csl_integrate_dopri5(self.h, B, dptr(init.reshape(-1)), per,
+                                            dptr(None if tp is None else tp.reshape(-1)),
+                                            dptr(None if amp is None else amp.reshape(-1)), T0, T1, dt, control,
+                                            abs_tol, rel_tol, save_interval, _capi.TGRID[tgrid], max_attempts,
+                                            max_rec, dptr(times), dptr(mags), dptr(energy), dptr(final),
+                                            szptr(nrec), szptr(nacc), szptr(nrej))
+            if rc == _capi.CSL_ERR_CAPACITY and int(nrec.max()) > max_rec:
+                max_rec = int(nrec.max())
+                continue
+            check(rc)
+            break
+        n = [int(v) for v in nrec]
+        t2 = times.reshape(B, max_rec)
+        m2 = mags.reshape(B, max_rec, 3, 3)
+        e2 = None if energy is None else energy.reshape(B, max_rec)
+        return {"times": [t2[b, :n[b]] for b in range(B)], "mags": [m2[b, :n[b]] for b in range(B)],
+                "energy": None if e2 is None else [e2[b, :n[b]] for b in range(B)],
+                "final": None if final is None else final.reshape(B, self.N, 3),
+                "n_rec": n, "accepted": [int(v) for v in nacc], "rejected": [int(v) for v in nrej]}
 
 
 # ---------------------------------------------------------------------------
@@ -298,15 +347,28 @@ def _gate(times, T_start, T_end, step):
 
 
 def molecular_dynamics(gl: GPULattice, spins, T_start, T_end, dt, save_interval=1, method="rk4",
-                       alpha_gilbert=0.0, engine=0, out_dir: Optional[str] = None):
-    """molecular_dynamics_cpu with method "rk4" (lattice_md.cpp:341-453): the
-    observer records M_antiferro, M_local, M_global and E/N every save_interval
-    integrate_const calls.  With out_dir the records and the spin snapshots
-    go to out_dir/trajectory.h5 in the HDF5MDWriter layout
-    (lattice_md.cpp:363-373, hdf5_io.h:83-300; libcsl_h5.so).
-    Returns dict(times, mags, energy[, states])."""
+                       alpha_gilbert=0.0, engine=0, out_dir: Optional[str] = None,
+                       abs_tol: float = 1e-6, rel_tol: float = 1e-6):
+    """molecular_dynamics_cpu (lattice_md.cpp:341-453): the observer records
+    M_antiferro, M_local, M_global and E/N every save_interval observer
+    calls.  method "rk4": integrate_const on the Boost grid; "dopri5" (the
+    reference's default md_integrator, spin_config.h:93): integrate_adaptive
+    with make_controlled(abs_tol, rel_tol), observer after every accepted
+    step (lattice_md.cpp:444-453), on the Dormand-Prince kernel.  With
+    out_dir the records and the spin snapshots go to out_dir/trajectory.h5 in
+    the HDF5MDWriter layout (lattice_md.cpp:363-373, hdf5_io.h:83-300;
+    libcsl_h5.so).  Returns dict(times, mags, energy[, states]); dopri5 runs
+    add accepted / rejected step counts."""
     gl.set_damping(alpha_gilbert)
     gl.set_pulse(np.zeros(2 * gl.A * 3), 0.0, 1.0, 0.0, 0.0, 0.0)
+    if method == "dopri5":
+        if out_dir is not None:
+            raise _capi.CslError(_capi.CSL_ERR_UNSUPPORTED, "trajectory.h5 snapshots are written for rk4 runs")
+        r = gl.integrate_dopri5(spins, T_start, T_end, dt, control=1, abs_tol=abs_tol, rel_tol=rel_tol,
+                                save_interval=save_interval)
+        gl.set_spins(r["final"][0])
+        return {"times": r["times"][0], "mags": r["mags"][0], "energy": r["energy"][0], "states": None,
+                "accepted": r["accepted"][0], "rejected": r["rejected"][0]}
     gl.set_spins(spins)
     res = gl.integrate(T_start, T_end, dt, save_interval, method, tgrid="boost", engine=engine,
                        want_states=out_dir is not None)
@@ -321,35 +383,45 @@ def molecular_dynamics(gl: GPULattice, spins, T_start, T_end, dt, save_interval=
     return res
 
 
-def _pulse_run(gl: GPULattice, spins, fd, t1, t2, amp, width, freq, T_start, T_end, step):
+def _pulse_run(gl: GPULattice, spins, fd, t1, t2, amp, width, freq, T_start, T_end, step, method="rk4",
+               abs_tol=1e-8, rel_tol=1e-8):
     gl.set_pulse(fd, amp, width, freq, t1, t2)
-    gl.set_spins(spins)
-    res = gl.integrate(T_start, T_end, step, 1, "rk4", tgrid="boost", want_energy=False)
+    if method == "dopri5":
+        # integrate_const with the controlled stepper (lattice_md.cpp:570-575)
+        r = gl.integrate_dopri5(spins, T_start, T_end, step, control=2, abs_tol=abs_tol, rel_tol=rel_tol,
+                                t_pulse=[[t1, t2]], pulse_amp=[[amp, amp]], want_energy=False)
+        times, mags = r["times"][0], r["mags"][0]
+    elif method == "rk4":
+        gl.set_spins(spins)
+        res = gl.integrate(T_start, T_end, step, 1, "rk4", tgrid="boost", want_energy=False)
+        times, mags = res["times"], res["mags"]
+    else:
+        raise _capi.CslError(_capi.CSL_ERR_UNSUPPORTED, "methods: 'rk4', 'dopri5'")
     gl.set_pulse(np.zeros_like(fd), 0.0, width, freq, 0.0, 0.0)  # reset (lattice_md.cpp:578-583)
-    keep = _gate(res["times"], T_start, T_end, step)
-    return res["times"][keep], res["mags"][keep]
+    keep = _gate(times, T_start, T_end, step)
+    return times[keep], mags[keep]
 
 
 def single_pulse_drive(gl: GPULattice, spins, field_in, t_B, pulse_amp, pulse_width, pulse_freq,
-                       T_start, T_end, step_size, method="rk4"):
-    """Lattice::single_pulse_drive (lattice_md.cpp:493-586).  Returns
-    (times [n], mags [n, 3(M_antiferro, M_local, M_global), 3])."""
-    if method != "rk4":
-        raise _capi.CslError(_capi.CSL_ERR_UNSUPPORTED, "only method 'rk4' is implemented")
+                       T_start, T_end, step_size, method="rk4", abs_tol=1e-8, rel_tol=1e-8):
+    """Lattice::single_pulse_drive (lattice_md.cpp:493-586), method "rk4" or
+    "dopri5" (controlled, tolerances as the reference's pump-probe defaults
+    kDefaultPumpProbeAbsTol/RelTol = 1e-8).  Returns (times [n],
+    mags [n, 3(M_antiferro, M_local, M_global), 3])."""
     lat = gl.lattice
     fd = np.concatenate([_drive_dirs(lat, field_in).reshape(-1), np.zeros(lat.N_atoms * 3)])
-    return _pulse_run(gl, spins, fd, t_B, 0.0, pulse_amp, pulse_width, pulse_freq, T_start, T_end, step_size)
+    return _pulse_run(gl, spins, fd, t_B, 0.0, pulse_amp, pulse_width, pulse_freq, T_start, T_end, step_size,
+                      method, abs_tol, rel_tol)
 
 
 def double_pulse_drive(gl: GPULattice, spins, field_in_1, t_B_1, field_in_2, t_B_2, pulse_amp,
-                       pulse_width, pulse_freq, T_start, T_end, step_size, method="rk4"):
+                       pulse_width, pulse_freq, T_start, T_end, step_size, method="rk4", abs_tol=1e-8,
+                       rel_tol=1e-8):
     """Lattice::double_pulse_drive (lattice_md.cpp:589-680)."""
-    if method != "rk4":
-        raise _capi.CslError(_capi.CSL_ERR_UNSUPPORTED, "only method 'rk4' is implemented")
     lat = gl.lattice
     fd = np.concatenate([_drive_dirs(lat, field_in_1).reshape(-1), _drive_dirs(lat, field_in_2).reshape(-1)])
     return _pulse_run(gl, spins, fd, t_B_1, t_B_2, pulse_amp, pulse_width, pulse_freq, T_start, T_end,
-                      step_size)
+                      step_size, method, abs_tol, rel_tol)
 
 
 def pump_probe_spectroscopy(gl: GPULattice, ground, field_in, pulse_amp, pulse_width, pulse_freq,
@@ -357,7 +429,8 @@ def pump_probe_spectroscopy(gl: GPULattice, ground, field_in, pulse_amp, pulse_w
                             dir_name: Optional[str] = None, method: str = "rk4",
                             Temp_start: float = 0.0, Temp_end: float = 0.0, n_anneal: int = 0,
                             T_zero_quench: bool = False, quench_sweeps: int = 0,
-                            reuse_m0_for_m1: bool = False, stationarity_tol: float = 1e-6):
+                            reuse_m0_for_m1: bool = False, stationarity_tol: float = 1e-6,
+                            abs_tol: float = 1e-8, rel_tol: float = 1e-8):
     """Lattice::pump_probe_spectroscopy (lattice_md.cpp:683-926): M0, M1(tau)
     and M01(tau) for every delay, all integrated in one batched launch.
     With dir_name the result goes to dir_name/pump_probe_spectroscopy.h5 in
@@ -368,10 +441,9 @@ def pump_probe_spectroscopy(gl: GPULattice, ground, field_in, pulse_amp, pulse_w
     (lattice_md.cpp:729-752): M1(tau) synthesised from M0 when max|dS/dt| of
     the undriven ground state is within the tolerance.
     Returns dict(times, M0 [n,3,3], M1 [n_tau,n,3,3], M01, tau, w1, max_dsdt)."""
-    if method != "rk4":
-        raise _capi.CslError(_capi.CSL_ERR_UNSUPPORTED, "only method 'rk4' is implemented")
     res = gl.pump_probe(ground, field_in, pulse_amp, pulse_width, pulse_freq, tau_start, tau_end,
-                        tau_step, T_start, T_end, T_step, reuse_m0_for_m1, stationarity_tol)
+                        tau_step, T_start, T_end, T_step, reuse_m0_for_m1, stationarity_tol, method, abs_tol,
+                        rel_tol)
     if dir_name is not None:
         from . import h5io
         lat = gl.lattice

@@ -44,6 +44,7 @@ extern "C" {
 #define CSL_ERR_HIP         -5   /* HIP runtime error                        */
 #define CSL_ERR_CAPACITY    -6   /* output buffer too small                  */
 #define CSL_ERR_NONFINITE   -7   /* NaN / Inf in an input state              */
+#define CSL_ERR_STEP        -8   /* adaptive step-size control failed        */
 
 /* time-grid policies for the integrators (the host computes the step times):
  *   CSL_TGRID_GPU     t += dt while t < T1                (integrator.cuh:470-487)
@@ -202,24 +203,64 @@ int csl_pump_probe(csl_lattice* h, const double* ground, const double* field_in,
                    double* times, double* M0, double* M1, double* M01,
                    double* tau_values);
 
-/* csl_pump_probe with the reference's W1 option (reuse_m0_for_m1,
- * lattice_md.cpp:729-752): when reuse_m0_for_m1 is set and the L_inf norm of
- * dS/dt at the ground state with the drive off (max_dSdt_norm_no_drive,
- * :236-280, evaluated on the device) is <= stationarity_tol, the M1(tau)
- * replicas are not integrated; M1 is synthesised from M0 by the time shift of
- * synthesize_M1_from_M0 (:283-317), with the ground-state magnetisations
- * before tau.  *used_w1 (may be NULL) receives 1 when that happened,
+/* csl_pump_probe with the reference driver's method and W1 options
+ * (pump_probe_spectroscopy(..., method, ..., reuse_m0_for_m1,
+ * stationarity_tol, ..., abs_tol, rel_tol), lattice_md.cpp:683-697).
+ * method "rk4" integrates on the Boost grid as csl_pump_probe; "dopri5" runs
+ * every replica with the controlled Dormand-Prince stepper between the
+ * observation points (integrate_const with make_controlled(abs_tol, rel_tol),
+ * lattice_md.cpp:570-575) through csl_integrate_dopri5.  When
+ * reuse_m0_for_m1 is set and the L_inf norm of dS/dt at the ground state
+ * with the drive off (max_dSdt_norm_no_drive, :236-280, evaluated on the
+ * device) is <= stationarity_tol, the M1(tau) replicas are not integrated;
+ * M1 is synthesised from M0 by the time shift of synthesize_M1_from_M0
+ * (:283-317), with the ground-state magnetisations before tau (W1,
+ * :729-752).  *used_w1 (may be NULL) receives 1 when that happened,
  * *max_dsdt (may be NULL) the norm when it was evaluated.  The handle's spins
  * hold the ground state afterwards when reuse_m0_for_m1 is set.  Same
  * outputs as csl_pump_probe. */
-int csl_pump_probe_w1(csl_lattice* h, const double* ground, const double* field_in,
+int csl_pump_probe_ex(csl_lattice* h, const double* ground, const double* field_in,
                       double amp, double width, double freq,
                       double tau_start, double tau_end, double tau_step,
                       double T0, double T1, double T_step,
                       int reuse_m0_for_m1, double stationarity_tol,
+                      const char* method, double abs_tol, double rel_tol,
                       size_t max_rec, size_t max_tau, size_t* n_rec, size_t* n_tau,
                       double* times, double* M0, double* M1, double* M01,
                       double* tau_values, int* used_w1, double* max_dsdt);
+
+/* ---- Dormand-Prince 5(4) --------------------------------------------- */
+
+/* Integrates B trajectories with the Dormand-Prince 5(4) pair, one
+ * workgroup per trajectory with its own step-size sequence (lattices of up
+ * to 2048 sites; CSL_ERR_UNSUPPORTED beyond).  control selects how the
+ * reference uses the stepper (integrate_ode_system, lattice.h:4072-4085):
+ *   0  fixed steps dt on the time grid of `tgrid`, no error control (the
+ *      reference GPU stepper's "dopri5", integrator.cuh:186-228);
+ *   1  integrate_adaptive with make_controlled(abs_tol, rel_tol): observer at
+ *      T0 and after every accepted step, the last step cut to end at T1
+ *      (molecular_dynamics_cpu, lattice_md.cpp:452-453);
+ *   2  integrate_const with the controlled stepper: observer on the
+ *      integrate_const grid T0 + k dt, adaptive steps inside each interval
+ *      (the pulse drivers, lattice_md.cpp:570-575).
+ * Observer call c is recorded when c % save_interval == 0 (lattice_md.cpp:
+ * 399).  init / t_pulse / pulse_amp as csl_integrate_batched (t_pulse may be
+ * NULL: zeros; pulse_amp NULL: the handle's amplitude); per-replica
+ * Hamiltonians apply.  Outputs per trajectory b: times [b*max_rec + r],
+ * mags [(b*max_rec + r)*9 + j], energy [b*max_rec + r] (E/N, may be NULL),
+ * final_states [b*N*3] (may be NULL), n_rec[b] (records the run produced),
+ * n_accepted[b], n_rejected[b] (may be NULL).  When some n_rec[b] exceeds
+ * max_rec the first max_rec records are written and CSL_ERR_CAPACITY is
+ * returned, so the caller can repeat with a larger buffer (the run is
+ * deterministic).  CSL_ERR_STEP: 500 rejections in a row, a step size that no
+ * longer advances t, or more than max_attempts attempts (0: 5e7). */
+int csl_integrate_dopri5(csl_lattice* h, size_t B, const double* init, int init_per_replica,
+                         const double* t_pulse, const double* pulse_amp,
+                         double T0, double T1, double dt, int control,
+                         double abs_tol, double rel_tol, size_t save_interval, int tgrid,
+                         size_t max_attempts, size_t max_rec, double* times, double* mags,
+                         double* energy, double* final_states, size_t* n_rec,
+                         size_t* n_accepted, size_t* n_rejected);
 
 /* ---- autotuner ------------------------------------------------------- */
 

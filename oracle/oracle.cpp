@@ -460,7 +460,7 @@ double dp5_decrease(double dt, double err) {
 }
 double dp5_increase(double dt, double err) {
     if (err < 0.5) {
-        err = std::max(std::pow(5.0, -5.0), err);
+        err = std::max(1.0 / 3125.0, err);  // 5^-5
         dt = dt * (0.9 * std::pow(err, -1.0 / 5.0));
     }
     return dt;
