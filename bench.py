@@ -266,15 +266,19 @@ def bench_md_dopri5(args):
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import pyoracle
         o = pyoracle.OracleLattice(uc, MD_L, MD_L, 1)
-        T = 5.0
-        t = time.perf_counter()
-        ro = o.integrate_dopri5(s0.reshape(-1), 0.0, T, MD_DT, control=1, abs_tol=DP_TOL, rel_tol=DP_TOL)
-        elc = time.perf_counter() - t
+        runs, t = 0, time.perf_counter()
+        while True:  # whole trajectories, repeated for about 3 s
+            ro = o.integrate_dopri5(s0.reshape(-1), 0.0, DP_T1, MD_DT, control=1, abs_tol=DP_TOL, rel_tol=DP_TOL,
+                                    max_rec=4096)
+            runs += 1
+            elc = time.perf_counter() - t
+            if elc >= 3.0:
+                break
         line["cpu_baseline"] = {
-            "value": T / DP_T1 / elc, "unit": "trajectories/s", "cores": pyoracle.max_threads(), "kind": "port",
-            "sample": f"t = 0..{T:g} of the 0..{DP_T1:g} run ({ro['accepted']} accepted steps, {elc:.1f} s, OpenMP "
-                      f"over sites inside each RHS)",
-            "site_steps_per_s": ro["accepted"] * lat.lattice_size / elc}
+            "value": runs / elc, "unit": "trajectories/s", "cores": pyoracle.max_threads(), "kind": "port",
+            "sample": f"{runs} whole trajectories t = 0..{DP_T1:g} ({ro['accepted']} accepted steps each, {elc:.1f} s, "
+                      f"OpenMP over sites inside each RHS)",
+            "site_steps_per_s": runs * ro["accepted"] * lat.lattice_size / elc}
     print(json.dumps(line), flush=True)
 
 
