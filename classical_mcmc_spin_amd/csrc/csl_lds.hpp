@@ -34,11 +34,19 @@ __device__ __forceinline__ LdsState lds_state(double* base, int NP) {
     return LdsState{reinterpret_cast<double2*>(base), base + 2 * NP};
 }
 
-// Local field of site p (no drive) gathering neighbours from LDS.
-template <int ZMAX, bool UNI>
+// Local field of site p (no drive) gathering neighbours from LDS; with
+// FIELD_ONLY the on-site part is the sublattice field alone.
+template <int ZMAX, bool UNI, bool FIELD_ONLY = false>
 __device__ __forceinline__ void local_field_lds(const LatDev& L, const LdsState& y, int p, int atom,
                                                 const int* nb, int cnt, const double S[3], double H[3]) {
-    field_onsite(L, atom, p, S, H);
+    if (FIELD_ONLY) {
+        cptr<double> B = as_const(L.field) + atom * 3;
+        H[0] = -B[0];
+        H[1] = -B[1];
+        H[2] = -B[2];
+    } else {
+        field_onsite(L, atom, p, S, H);
+    }
 #pragma unroll
     for (int k = 0; k < ZMAX; ++k) {
         if (k < cnt) {

@@ -156,10 +156,11 @@ class GPULattice:
 
     def integrate_batched(self, init, t_pulse, pulse_amp=None, T0=0.0, T1=1.0, dt=0.01,
                           save_interval=1, method="rk4", tgrid="batched", block=0,
-                          want_energy=True, want_final=False):
+                          want_energy=True, want_final=False, want_mags=True):
         """Generalised gpu::integrate_gpu_batched (lattice_gpu_api.h:200-214).
         init: [N,3] shared or [B,N,3]; t_pulse, pulse_amp: [B,2].
-        Returns dict(times, mags [n,B,3,3], energy [n,B], final [B,N,3])."""
+        Returns dict(times, mags [n,B,3,3], energy [n,B], final [B,N,3]);
+        with want_mags=False the records stay in device memory (mags None)."""
         tp = f64(t_pulse).reshape(-1, 2)
         B = tp.shape[0]
         init = f64(init)
@@ -169,7 +170,7 @@ class GPULattice:
         amp = None if pulse_amp is None else f64(pulse_amp).reshape(B, 2)
         _, nr = self.count_records(T0, T1, dt, save_interval, tgrid)
         times = np.empty(nr)
-        mags = np.empty(nr * B * 9)
+        mags = np.empty(nr * B * 9) if want_mags else None
         energy = np.empty(nr * B) if want_energy else None
         final = np.empty(B * self.N * 3) if want_final else None
         n = C.c_size_t()
@@ -177,7 +178,7 @@ class GPULattice:
                                           dptr(None if amp is None else amp.reshape(-1)), T0, T1, dt,
                                           save_interval, method.encode(), _capi.TGRID[tgrid], block, nr,
                                           dptr(times), dptr(mags), dptr(energy), dptr(final), C.byref(n)))
-        return {"times": times, "mags": mags.reshape(nr, B, 3, 3),
+        return {"times": times, "mags": None if mags is None else mags.reshape(nr, B, 3, 3),
                 "energy": None if energy is None else energy.reshape(nr, B),
                 "final": None if final is None else final.reshape(B, self.N, 3)}
 
