@@ -242,6 +242,9 @@ void build_tables(csl_lattice* h, size_t N, size_t D, size_t n_atoms, const doub
     L.uniform = uni ? 1 : 0;
     L.site_uniform = site_uni ? 1 : 0;
     L.has_onsite = has_onsite ? 1 : 0;
+    bool full = uni && zmax > 0;
+    for (size_t a = 0; a < n_atoms; ++a) full = full && cnt_atom[a] == zmax;
+    L.full_coord = full ? 1 : 0;
     L.col = h->col.p;
     L.cnt = h->cnt.p;
     L.cnt_atom = h->cnt_atom.p;

@@ -50,6 +50,7 @@ struct LatDev {
     int uniform;    // bond matrices per (sublattice, slot)
     int site_uniform;  // field / onsite per sublattice
     int has_onsite;    // any non-zero onsite matrix (the tables hold 2A)
+    int full_coord;    // uniform tables and every site has zmax bonds
     const int* __restrict__ col;        // [zmax*NP]
     const int* __restrict__ cnt;        // [NP] bonds of each site
     const int* __restrict__ cnt_atom;   // [n_atoms] (uniform handles)

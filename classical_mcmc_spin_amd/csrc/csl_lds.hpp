@@ -35,7 +35,8 @@ __device__ __forceinline__ LdsState lds_state(double* base, int NP) {
 }
 
 // Local field of site p (no drive) gathering neighbours from LDS; with
-// FIELD_ONLY the on-site part is the sublattice field alone.
+// FIELD_ONLY the on-site part is the sublattice field alone and every site
+// has ZMAX bonds.
 template <int ZMAX, bool UNI, bool FIELD_ONLY = false>
 __device__ __forceinline__ void local_field_lds(const LatDev& L, const LdsState& y, int p, int atom,
                                                 const int* nb, int cnt, const double S[3], double H[3]) {
@@ -49,7 +50,7 @@ __device__ __forceinline__ void local_field_lds(const LatDev& L, const LdsState&
     }
 #pragma unroll
     for (int k = 0; k < ZMAX; ++k) {
-        if (k < cnt) {
+        if (FIELD_ONLY || k < cnt) {  // FIELD_ONLY tables have zmax bonds at every site
             double Sp[3];
             y.load(nb[k], Sp);
             if (UNI)

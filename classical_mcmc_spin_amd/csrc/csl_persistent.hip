@@ -12,11 +12,12 @@
 // (lattice_gpu.cu:455-577).
 //
 // FIELD_ONLY: the on-site part of the local field is a per-sublattice field
-// with no anisotropy matrix (uniform tables, site_uniform, !has_onsite).  The
-// runtime branches between per-site / per-sublattice fields and the onsite
-// product then disappear at compile time, which frees the scalar registers
-// they held (11 % faster on the 36x36 Kitaev batch).  Chosen by the launcher
-// for the double-buffered, stage-unrolled shapes.
+// with no anisotropy matrix and every site has ZMAX bonds (uniform tables,
+// site_uniform, !has_onsite, full_coord).  The runtime branches between
+// per-site / per-sublattice fields, the onsite product and the per-bond
+// count tests then disappear at compile time, with the scalar registers and
+// lane masks they held (319 -> 251 ms on the 36x36 Kitaev batch).  Chosen by
+// the launcher for the double-buffered, stage-unrolled shapes.
 //
 // LDS state layout (one buffer; two when DBUF): the x,y components of every
 // site packed as a 16-byte pair, then the z plane, so a neighbour gather is
@@ -301,7 +302,7 @@ template <int SPT, int ZMAX, bool UNI>
 hipError_t launch_p_db(const LatDev& L, const PersistArgs& A, int block, int flags, size_t lds,
                        hipStream_t st) {
     const bool dbuf = flags & kPersistDoubleBuffer, unroll = flags & kPersistUnrollStages;
-    if (unroll && dbuf && UNI && L.site_uniform && !L.has_onsite)
+    if (unroll && dbuf && UNI && L.site_uniform && !L.has_onsite && L.full_coord && L.zmax == ZMAX)
         return launch_p<SPT, ZMAX, UNI, true, true, true>(L, A, block, lds, st);
     if (unroll)
         return dbuf ? launch_p<SPT, ZMAX, UNI, true, true>(L, A, block, lds, st)
