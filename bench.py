@@ -284,16 +284,17 @@ def bench_md_dopri5(args):
     print(json.dumps(line), flush=True)
 
 
-ADJ_L, ADJ_STEPS, ADJ_B = 24, 2000, 256
+ADJ_L, ADJ_STEPS, ADJ_B, ADJ_B_FULL = 24, 2000, 32, 256
 
 
 def bench_adjoint(args):
     """Adjoint gradients: Kitaev 24x24 (1152 sites), one pulse (A = 0.5,
     w = 0.5 at t = 1), Gilbert alpha = 0.02, loss on M_global over 2000 RK4
     steps (dt = 0.01) against a target run at (K, G, G', J, h) =
-    (-1, 0.25, -0.02, 0, 0.05); 256 parameter starts within +-10 % (one
-    workgroup per start, one per CU: the LDS state of 1152 sites takes 83 KB).
-    value = gradients (starts) per second."""
+    (-1, 0.25, -0.02, 0, 0.05); 32 parameter starts within +-10 %.
+    value = gradients (starts) per second.  One workgroup per start and one
+    per CU (the LDS state of 1152 sites takes 83 KB), so 32 starts use 32 of
+    the 256 CUs; a batch of 256 starts is reported beside it."""
     from classical_mcmc_spin_amd import adjoint as AD
     from classical_mcmc_spin_amd import GPULattice, model, seeding
     lat = model.Lattice(model.build_kitaev_honeycomb(), ADJ_L, ADJ_L, 1)
@@ -312,7 +313,8 @@ def bench_adjoint(args):
                                   tgrid="boost", want_energy=False)
     target = tr["mags"][:, 0].reshape(-1, 9)
     rng = np.random.default_rng(8)
-    thetas = star * (1.0 + 0.1 * rng.uniform(-1, 1, size=(ADJ_B, 5)))
+    thetas_full = star * (1.0 + 0.1 * rng.uniform(-1, 1, size=(ADJ_B_FULL, 5)))
+    thetas = thetas_full[:ADJ_B]
     fit = AD.KitaevFit(ADJ_L, ADJ_L, init, target, 0.0, T1a, dt, pulse=(0.5, 0.5, 0.0, 1.0), alpha=0.02)
     for _ in range(args.warmup):
         fit(thetas)
@@ -321,8 +323,14 @@ def bench_adjoint(args):
         loss, grad = fit(thetas)
     el = (time.perf_counter() - t) / args.steps
     kms = fit.gl.last_kernel_ms()[0]
-    fit.close()
     assert np.all(np.isfinite(grad))
+    fit(thetas_full)
+    t = time.perf_counter()
+    _, grad_full = fit(thetas_full)
+    el_full = time.perf_counter() - t
+    kms_full = fit.gl.last_kernel_ms()[0]
+    fit.close()
+    assert np.all(np.isfinite(grad_full))
     v = ADJ_B / el
     # forward-equivalent site-steps: forward + recomputation + transposed steps
     line = {
@@ -337,6 +345,7 @@ def bench_adjoint(args):
                      "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                      "frac": v * ADJ_STEPS * lat.lattice_size * BYTES_PER_SITE_STEP / HBM_PEAK, "traffic": None,
                      "kernel": "k_adjoint"},
+        "starts_256": {"gradients_per_s": ADJ_B_FULL / el_full, "kernel_ms": kms_full},
     }
     if not args.no_cpu:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -408,8 +417,18 @@ def bench_mixed(args):
         kernel_ms.append(gm.last_kernel_ms()[0])
     el = time.perf_counter() - t
     assert res["records"].shape[0] == MIX_STEPS + 1 and np.all(np.isfinite(res["records"]))
-    gm.close()
     kms = float(np.mean(kernel_ms))
+    # d loss / d (Fe-Tm coupling scale) for every trajectory of the batch:
+    # forward and tangent-linear run in the same stage launches; loss on the
+    # 33 magnetisation entries of the records against the M0 records x 1.01
+    target = res["records"][:, 0] * 1.01
+    w = np.r_[np.ones(33), 0.0]
+    t = time.perf_counter()
+    gr = gm.scale_gradient(g, tp, target, w, sc, T0, T1m, MIX_DT)
+    el_g = time.perf_counter() - t
+    kms_g = gm.last_kernel_ms()[0]
+    assert np.all(np.isfinite(gr["grad"]))
+    gm.close()
     traj_s = B * args.steps / el
     n_sites = ml.N2 + ml.N3
     # one stage: SU(2) sites read+write 3 doubles, SU(3) sites 8 doubles
@@ -427,6 +446,9 @@ def bench_mixed(args):
                      "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                      "frac": bytes_launch / (kms * 1e-3) / HBM_PEAK, "traffic": None,
                      "kernel": "k_mixed_stage", "kernel_ms": kms, "algorithmic_bytes_per_launch": bytes_launch},
+        "scale_gradient": {"gradients_per_s": B / el_g, "ms_per_batch": 1e3 * el_g,
+                           "kernel_ms_per_launch": kms_g, "forward_plus_tangent_over_forward": el_g / (el / args.steps),
+                           "grad_first_last": [float(gr["grad"][0]), float(gr["grad"][-1])]},
     }
     if not args.no_cpu:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -444,6 +466,10 @@ def bench_mixed(args):
             "sample": f"1 trajectory x {n} RK4 steps of the same lattice ({el:.1f} s, OpenMP over sites "
                       f"inside each RHS as the reference's mixed MD path)",
             "site_steps_per_s": n * n_sites / el}
+        # the CPU gradient by central differences: two forward runs per trajectory
+        line["scale_gradient"]["cpu_baseline"] = {
+            "value": n / el / MIX_STEPS / 2.0, "unit": "gradients/s", "cores": pyoracle.max_threads(),
+            "kind": "port", "sample": "central difference = 2 forward runs, timed on the sample above"}
     print(json.dumps(line), flush=True)
 
 

@@ -70,6 +70,13 @@ struct MixStageArgs {
     const double* scale;       // [B][2] pulse scales or null (1)
     int rec;                   // stage 0 of a recorded step: write partials
     double* partials;          // [B][nblk][34]
+    // tangent-linear run (k_mixed_stage<true>): z = d state / d theta for a
+    // common scale theta of both Fe-Tm bond kinds (e23, e32), at theta = 1
+    double* zx;                // [B][SZ] tangent at step start
+    double* zacc;              // tangent RK4 accumulator
+    const double* zcur;        // tangent stage input
+    double* znext;             // tangent next stage input
+    double* tpartials;         // [B][nblk][34] record partials of z (entry 33 stays 0)
 };
 
 constexpr int kRec = CSL_MIXED_RECORD;
@@ -146,6 +153,32 @@ __device__ __forceinline__ void field_onsite(const double* __restrict__ field, c
     }
 }
 
+// 2A dS of one site (the linear part of field_onsite; tables hold 2A)
+template <int D>
+__device__ __forceinline__ void onsite_lin(const double* __restrict__ onsite, int site_uniform, int has_onsite,
+                                           int atom, int p, const double* dS, double* dH) {
+    if (!has_onsite) return;
+    if (site_uniform) {
+        const cptr<double> A = as_const(onsite) + size_t(atom) * D * D;
+#pragma unroll
+        for (int a = 0; a < D; ++a) {
+            double r = 0.0;
+#pragma unroll
+            for (int e = 0; e < D; ++e) r += A[a * D + e] * dS[e];
+            dH[a] += r;
+        }
+    } else {
+        const double* A = onsite + size_t(p) * D * D;
+#pragma unroll
+        for (int a = 0; a < D; ++a) {
+            double r = 0.0;
+#pragma unroll
+            for (int e = 0; e < D; ++e) r += A[a * D + e] * dS[e];
+            dH[a] += r;
+        }
+    }
+}
+
 // SU(2) local field without drive at device site p of one trajectory y
 __device__ __forceinline__ void field_su2(const MixDev& M, const double* __restrict__ y, int atom, int p,
                                           const double* S, double* H) {
@@ -204,7 +237,31 @@ __device__ __forceinline__ void mix_block(int& bx, int& by) {
     }
 }
 
-// One RK4 stage of every SU(2) and SU(3) site of every trajectory.
+// RK4 update of one state component (stage s) from its derivative kd
+__device__ __forceinline__ void rk_update(int s, size_t q, double Sd, double kd, double bdt, double cdt,
+                                          double* x, double* acc, double* nxt) {
+    if (s == 0) {
+        acc[q] = Sd + bdt * kd;
+        nxt[q] = Sd + cdt * kd;
+    } else if (s < 3) {
+        const double xd = x[q];
+        acc[q] = acc[q] + bdt * kd;
+        nxt[q] = xd + cdt * kd;
+    } else {
+        x[q] = acc[q] + bdt * kd;
+    }
+}
+
+// One RK4 stage of every SU(2) and SU(3) site of every trajectory.  With TL
+// the same launch also advances the tangent z = d state / d theta (theta the
+// scale of both Fe-Tm bond kinds): dk = J_f(y) z + d f / d theta, with
+//   SU(2)  dH = 2A dS + sum J22 dS_p + sum J23 dl_p + sum J23 l_p,
+//          dk = dH x S + H x dS (+ the linearised Gilbert term)
+//   SU(3)  dH = 2A dl + sum J33 dl_p + sum J32 dS_p + sum J32 S_p,
+//          dk = f(dH, l) + f(H, dl) - Gamma dl
+// and the same RK4 coefficients, i.e. the exact derivative of the discrete
+// scheme (what a discrete adjoint computes for one parameter).
+template <bool TL>
 __global__ void __launch_bounds__(256) k_mixed_stage(MixDev M, MixStageArgs A) {
     __shared__ double fac[4];
     __shared__ double red[4][kRec];
@@ -237,9 +294,10 @@ __global__ void __launch_bounds__(256) k_mixed_stage(MixDev M, MixStageArgs A) {
     const double dt = A.dt;
     const double cdt = (s == 2) ? 1.0 * dt : 0.5 * dt;
     const double bdt = (s == 0 || s == 3) ? (1.0 / 6.0) * dt : (1.0 / 3.0) * dt;
-    double part[kRec];
+    double part[kRec], tpart[kRec];
 #pragma unroll
-    for (int j = 0; j < kRec; ++j) part[j] = 0.0;
+    for (int j = 0; j < kRec; ++j) part[j] = tpart[j] = 0.0;
+    const double* zc = TL ? A.zcur + off : nullptr;
     if (!is3) {
         const int p = bx * 256 + tid;
         const int pw = uniform(bx * 256 + wave * kWave);
@@ -280,6 +338,47 @@ __global__ void __launch_bounds__(256) k_mixed_stage(MixDev M, MixStageArgs A) {
                 const double g = (S2 > 0.0) ? M.alpha / sqrt(S2) : 0.0;
 #pragma unroll
                 for (int a = 0; a < 3; ++a) k[a] += g * (S[a] * SH - H[a] * S2);
+            }
+            if (TL) {
+                const double dS[3] = {zc[p], zc[NP + p], zc[2 * NP + p]};
+                double dH[3] = {0.0, 0.0, 0.0};
+                onsite_lin<3>(M.onsite2, M.site_uniform2, M.has_onsite2, atom, p, dS, dH);
+                gather_bonds<3, 3>(M.e22, zc, M.NP2, M.NP2, atom, p, dH);
+                gather_bonds<3, 8>(M.e23, zc + size_t(3) * M.NP2, M.NP2, M.NP3, atom, p, dH);
+                gather_bonds<3, 8>(M.e23, cur + size_t(3) * M.NP2, M.NP2, M.NP3, atom, p, dH);  // d H / d theta
+                double dk[3] = {dH[1] * S[2] - dH[2] * S[1] + (H[1] * dS[2] - H[2] * dS[1]),
+                                dH[2] * S[0] - dH[0] * S[2] + (H[2] * dS[0] - H[0] * dS[2]),
+                                dH[0] * S[1] - dH[1] * S[0] + (H[0] * dS[1] - H[1] * dS[0])};
+                if (M.alpha != 0.0) {
+                    const double S2 = S[0] * S[0] + S[1] * S[1] + S[2] * S[2];
+                    if (S2 > 0.0) {
+                        const double SH = S[0] * H[0] + S[1] * H[1] + S[2] * H[2];
+                        const double g = M.alpha / sqrt(S2);
+                        const double SdS = S[0] * dS[0] + S[1] * dS[1] + S[2] * dS[2];
+                        const double dSH = (dS[0] * H[0] + dS[1] * H[1] + dS[2] * H[2]) +
+                                           (S[0] * dH[0] + S[1] * dH[1] + S[2] * dH[2]);
+                        const double dg = -g * SdS / S2;
+#pragma unroll
+                        for (int a = 0; a < 3; ++a)
+                            dk[a] += dg * (S[a] * SH - H[a] * S2) +
+                                     g * (dS[a] * SH + S[a] * dSH - dH[a] * S2 - H[a] * (2.0 * SdS));
+                    }
+                }
+#pragma unroll
+                for (int d = 0; d < 3; ++d)
+                    rk_update(s, size_t(d) * NP + p, dS[d], dk[d], bdt, cdt, A.zx + off, A.zacc + off,
+                              A.znext + off);
+                if (A.rec) {
+                    const cptr<double> R = as_const(M.frames2) + size_t(atom) * 9;
+                    const double sg = M.afm2[atom];
+#pragma unroll
+                    for (int mu = 0; mu < 3; ++mu) {
+                        const double g = R[mu * 3] * dS[0] + R[mu * 3 + 1] * dS[1] + R[mu * 3 + 2] * dS[2];
+                        tpart[mu] = sg * g;
+                        tpart[3 + mu] = dS[mu];
+                        tpart[6 + mu] = g;
+                    }
+                }
             }
             if (s == 0) {
 #pragma unroll
@@ -350,6 +449,44 @@ __global__ void __launch_bounds__(256) k_mixed_stage(MixDev M, MixStageArgs A) {
                 for (int a = 0; a < 8; ++a)
                     if (M.gamma[a] != 0.0) k[a] -= M.gamma[a] * (S[a] - M.eq3[size_t(a) * NP + p]);
             }
+            if (TL) {
+                double dl[8];
+#pragma unroll
+                for (int a = 0; a < 8; ++a) dl[a] = zc[o3 + size_t(a) * NP + p];
+                double dH[8];
+#pragma unroll
+                for (int a = 0; a < 8; ++a) dH[a] = 0.0;
+                onsite_lin<8>(M.onsite3, M.site_uniform3, M.has_onsite3, atom, p, dl, dH);
+                gather_bonds<8, 8>(M.e33, zc + o3, M.NP3, M.NP3, atom, p, dH);
+                gather_bonds<8, 3>(M.e32, zc, M.NP3, M.NP2, atom, p, dH);
+                gather_bonds<8, 3>(M.e32, cur, M.NP3, M.NP2, atom, p, dH);  // d H / d theta
+                double dk[8], t2[8];
+                su3_cross(dH, S, dk);
+                su3_cross(H, dl, t2);
+#pragma unroll
+                for (int a = 0; a < 8; ++a) dk[a] += t2[a];
+                if (M.has_bloch) {
+#pragma unroll
+                    for (int a = 0; a < 8; ++a) dk[a] -= M.gamma[a] * dl[a];
+                }
+#pragma unroll
+                for (int d = 0; d < 8; ++d)
+                    rk_update(s, o3 + size_t(d) * NP + p, dl[d], dk[d], bdt, cdt, A.zx + off, A.zacc + off,
+                              A.znext + off);
+                if (A.rec) {
+                    const cptr<double> R = as_const(M.frames3) + size_t(atom) * 64;
+                    const double sg = ((cell * M.A3 + atom) & 1) ? -1.0 : 1.0;
+#pragma unroll
+                    for (int mu = 0; mu < 8; ++mu) {
+                        double g = 0.0;
+#pragma unroll
+                        for (int nu = 0; nu < 8; ++nu) g += R[mu * 8 + nu] * dl[nu];
+                        tpart[9 + mu] = sg * dl[mu];
+                        tpart[17 + mu] = dl[mu];
+                        tpart[25 + mu] = g;
+                    }
+                }
+            }
             if (s == 0) {
 #pragma unroll
                 for (int d = 0; d < 8; ++d) {
@@ -390,6 +527,61 @@ __global__ void __launch_bounds__(256) k_mixed_stage(MixDev M, MixStageArgs A) {
             const double v = mine ? (red[0][tid] + red[1][tid]) + (red[2][tid] + red[3][tid]) : 0.0;
             A.partials[(size_t(b) * gridDim.x + bx) * kRec + tid] = v;
         }
+        if (TL) {
+            __syncthreads();
+#pragma unroll
+            for (int j = 0; j < kRec; ++j) {
+                if (j >= j0 && j < j1) {
+                    const double v = wave_sum(tpart[j]);
+                    if (lane == 0) red[wave][j] = v;
+                }
+            }
+            __syncthreads();
+            if (tid < kRec) {
+                const bool mine = tid >= j0 && tid < j1;
+                const double v = mine ? (red[0][tid] + red[1][tid]) + (red[2][tid] + red[3][tid]) : 0.0;
+                A.tpartials[(size_t(b) * gridDim.x + bx) * kRec + tid] = v;
+            }
+        }
+    }
+}
+
+// Loss sum_t sum_j w_j (rec_j(t) - target_j(t))^2 of each trajectory and its
+// derivative 2 sum_t sum_j w_j (rec_j - target_j) trec_j from the records and
+// the tangent records ([n_rec][B][34]); target [n_rec][34] shared or
+// [n_rec][B][34].  One workgroup per trajectory.
+struct Weights34 {
+    double w[kRec];
+};
+__global__ void __launch_bounds__(256) k_mixed_loss_grad(const double* __restrict__ recs,
+                                                         const double* __restrict__ trecs,
+                                                         const double* __restrict__ target, int per_replica,
+                                                         int n_rec, int B, Weights34 W, double* __restrict__ loss,
+                                                         double* __restrict__ grad) {
+    __shared__ double red[2][4];
+    const int b = blockIdx.x;
+    const int tid = threadIdx.x;
+    double l = 0.0, g = 0.0;
+    for (long i = tid; i < long(n_rec) * kRec; i += 256) {
+        const int r = int(i / kRec), j = int(i % kRec);
+        const double w = W.w[j];
+        if (w == 0.0) continue;
+        const size_t q = (size_t(r) * B + b) * kRec + j;
+        const double tg = per_replica ? target[q] : target[size_t(r) * kRec + j];
+        const double d = recs[q] - tg;
+        l += w * d * d;
+        g += 2.0 * w * d * trecs[q];
+    }
+    l = wave_sum(l);
+    g = wave_sum(g);
+    if ((tid & (kWave - 1)) == 0) {
+        red[0][tid >> 6] = l;
+        red[1][tid >> 6] = g;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        loss[b] = (red[0][0] + red[0][1]) + (red[0][2] + red[0][3]);
+        grad[b] = (red[1][0] + red[1][1]) + (red[1][2] + red[1][3]);
     }
 }
 
@@ -637,6 +829,7 @@ struct csl_mixed {
     DBuf<double> state;                      // SoA [SZ]
     bool has_state = false;
     DBuf<double> X, ACC, Y1, Y2, partials, recs, tp, tp3, scale, init, flat, aux;
+    DBuf<double> ZX, ZACC, ZY1, ZY2, tpartials, trecs, zaux, target, loss, grad;  // tangent-linear runs
     double h_tp2[2] = {0.0, 0.0}, h_tp3[2] = {0.0, 0.0};  // single-trajectory pulse centres
     double last_ms = 0.0;
     size_t last_launches = 0;
@@ -711,8 +904,10 @@ void to_flat(csl_mixed* h, const double* d_soa, double* d_flat, size_t B) {
 
 // Integrate B trajectories whose SoA states are in h->X; records into h->recs
 // ([n_rec][B][34]); optional state snapshots (flat, B = 1) into d_states.
+// With tl the tangent z = d state / d theta (h->ZX, zero at the start) is
+// advanced in the same launches and its records go to h->trecs.
 void run_mixed(csl_mixed* h, size_t B, const double* d_tp, const double* d_tp3, const double* d_scale,
-               const std::vector<double>& grid, double dt, size_t si, double* d_states) {
+               const std::vector<double>& grid, double dt, size_t si, double* d_states, bool tl = false) {
     const size_t n_steps = grid.size() - 1;
     const size_t n_rec = records_for(n_steps, si);
     hipStream_t s = h->stream;
@@ -724,6 +919,13 @@ void run_mixed(csl_mixed* h, size_t B, const double* d_tp, const double* d_tp3, 
     const int nblk = nblk2 + nblk3;
     h->partials.ensure(B * size_t(nblk) * CSL_MIXED_RECORD);
     h->recs.ensure(n_rec * B * CSL_MIXED_RECORD);
+    if (tl) {
+        h->ZACC.ensure(plane);
+        h->ZY1.ensure(plane);
+        h->ZY2.ensure(plane);
+        h->tpartials.ensure(B * size_t(nblk) * CSL_MIXED_RECORD);
+        h->trecs.ensure(n_rec * B * CSL_MIXED_RECORD);
+    }
     const bool drive = d_tp && (h->M.amp2 != 0.0 || h->M.amp3 != 0.0);
     const double inv2 = 1.0 / double(h->N2), inv3 = 1.0 / double(h->N3), invE = 1.0 / double(h->N2 + h->N3);
     hipEvent_t e0, e1;
@@ -747,7 +949,16 @@ void run_mixed(csl_mixed* h, size_t B, const double* d_tp, const double* d_tp3, 
         A.scale = drive ? d_scale : nullptr;
         A.rec = rec;
         A.partials = h->partials.p;
-        hipLaunchKernelGGL(csl::k_mixed_stage, grid_dim, dim3(256), 0, s, h->M, A);
+        if (tl) {
+            A.zx = h->ZX.p;
+            A.zacc = h->ZACC.p;
+            A.zcur = st == 0 ? h->ZX.p : (st == 2 ? h->ZY2.p : h->ZY1.p);
+            A.znext = st == 0 ? h->ZY1.p : (st == 1 ? h->ZY2.p : (st == 2 ? h->ZY1.p : h->ZY2.p));
+            A.tpartials = h->tpartials.p;
+            hipLaunchKernelGGL(csl::k_mixed_stage<true>, grid_dim, dim3(256), 0, s, h->M, A);
+        } else {
+            hipLaunchKernelGGL(csl::k_mixed_stage<false>, grid_dim, dim3(256), 0, s, h->M, A);
+        }
         hip_check(hipGetLastError(), "k_mixed_stage");
         ++launches;
     };
@@ -755,6 +966,11 @@ void run_mixed(csl_mixed* h, size_t B, const double* d_tp, const double* d_tp3, 
         hipLaunchKernelGGL(csl::k_mixed_reduce, dim3(unsigned(B)), dim3(256), 0, s, h->partials.p, nblk, int(B),
                            int(ri), inv2, inv3, invE, h->recs.p);
         hip_check(hipGetLastError(), "k_mixed_reduce");
+        if (tl) {
+            hipLaunchKernelGGL(csl::k_mixed_reduce, dim3(unsigned(B)), dim3(256), 0, s, h->tpartials.p, nblk,
+                               int(B), int(ri), inv2, inv3, invE, h->trecs.p);
+            hip_check(hipGetLastError(), "k_mixed_reduce");
+        }
     };
     for (size_t n = 0; n < n_steps; ++n) {
         const bool rec = (n % si) == 0;
@@ -783,7 +999,17 @@ void run_mixed(csl_mixed* h, size_t B, const double* d_tp, const double* d_tp3, 
         A.dt = dt;
         A.rec = 1;
         A.partials = h->partials.p;
-        hipLaunchKernelGGL(csl::k_mixed_stage, grid_dim, dim3(256), 0, s, h->M, A);
+        if (tl) {
+            h->zaux.ensure(plane);
+            A.zx = h->ZX.p;
+            A.zacc = h->zaux.p;
+            A.zcur = h->ZX.p;
+            A.znext = h->zaux.p;
+            A.tpartials = h->tpartials.p;
+            hipLaunchKernelGGL(csl::k_mixed_stage<true>, grid_dim, dim3(256), 0, s, h->M, A);
+        } else {
+            hipLaunchKernelGGL(csl::k_mixed_stage<false>, grid_dim, dim3(256), 0, s, h->M, A);
+        }
         hip_check(hipGetLastError(), "k_mixed_stage");
         reduce(n_rec - 1);
     }
@@ -1160,6 +1386,64 @@ int csl_mixed_integrate_batched(csl_mixed* h, size_t B, const double* init, int 
         hip_check(hipStreamSynchronize(s), "integrate_batched");
         if (times)
             for (size_t r = 0; r < n_rec; ++r) times[r] = grid[std::min(r * save_interval, n_steps)];
+    });
+}
+
+int csl_mixed_scale_gradient(csl_mixed* h, size_t B, const double* init, int init_per_replica,
+                             const double* t_pulse, const double* pulse_scale, double T0, double T1, double dt,
+                             size_t save_interval, int tgrid, const double* target, int target_per_replica,
+                             const double* weights, double* loss, double* grad, size_t max_rec, double* recs,
+                             double* trecs, size_t* n_rec_out) {
+    return guarded([&] {
+        check_mixed(h);
+        if (B == 0) fail(CSL_ERR_ARG, "batch size must be >= 1");
+        if (B > 65535) fail(CSL_ERR_ARG, "at most 65535 trajectories per call");
+        if (!init || !t_pulse || !target || !weights || !loss || !grad)
+            fail(CSL_ERR_NULL, "null init / t_pulse / target / weights / loss / grad");
+        if (save_interval == 0) fail(CSL_ERR_ARG, "save_interval must be >= 1");
+        if (weights[CSL_MIXED_RECORD - 1] != 0.0)
+            fail(CSL_ERR_ARG, "the energy entry of the record cannot carry a loss weight");
+        check_finite(weights, CSL_MIXED_RECORD, "weights");
+        auto grid = make_grid(T0, T1, dt, tgrid);
+        const size_t n_steps = grid.size() - 1;
+        const size_t n_rec = records_for(n_steps, save_interval);
+        if (n_rec_out) *n_rec_out = n_rec;
+        if ((recs || trecs) && max_rec < n_rec) fail(CSL_ERR_CAPACITY, "max_rec is smaller than the number of records");
+        if (n_rec > size_t(std::numeric_limits<int>::max() / CSL_MIXED_RECORD)) fail(CSL_ERR_ARG, "too many records");
+        const size_t nf = h->nflat();
+        const size_t n_init = init_per_replica ? B * nf : nf;
+        check_finite(init, n_init, "initial state");
+        check_finite(t_pulse, 2 * B, "t_pulse");
+        const size_t n_tgt = (target_per_replica ? B : 1) * n_rec * CSL_MIXED_RECORD;
+        check_finite(target, n_tgt, "target");
+        hipStream_t s = h->stream;
+        h->init.upload(init, n_init, s);
+        h->tp.upload(t_pulse, 2 * B, s);
+        std::vector<double> sc(2 * B, 1.0);
+        if (pulse_scale) {
+            check_finite(pulse_scale, 2 * B, "pulse_scale");
+            sc.assign(pulse_scale, pulse_scale + 2 * B);
+        }
+        h->scale.upload(sc.data(), sc.size(), s);
+        h->target.upload(target, n_tgt, s);
+        h->X.ensure(B * h->sz());
+        h->ZX.ensure(B * h->sz());
+        hip_check(hipMemsetAsync(h->X.p, 0, B * h->sz() * sizeof(double), s), "memset");
+        hip_check(hipMemsetAsync(h->ZX.p, 0, B * h->sz() * sizeof(double), s), "memset");
+        to_soa(h, h->init.p, init_per_replica ? nf : 0, h->X.p, B);
+        run_mixed(h, B, h->tp.p, h->tp.p, h->scale.p, grid, dt, save_interval, nullptr, true);
+        csl::Weights34 W{};
+        std::memcpy(W.w, weights, sizeof(W.w));
+        h->loss.ensure(B);
+        h->grad.ensure(B);
+        hipLaunchKernelGGL(csl::k_mixed_loss_grad, dim3(unsigned(B)), dim3(256), 0, s, h->recs.p, h->trecs.p,
+                           h->target.p, target_per_replica ? 1 : 0, int(n_rec), int(B), W, h->loss.p, h->grad.p);
+        hip_check(hipGetLastError(), "k_mixed_loss_grad");
+        copy_back(loss, h->loss.p, B, s);
+        copy_back(grad, h->grad.p, B, s);
+        copy_back(recs, h->recs.p, n_rec * B * CSL_MIXED_RECORD, s);
+        copy_back(trecs, h->trecs.p, n_rec * B * CSL_MIXED_RECORD, s);
+        hip_check(hipStreamSynchronize(s), "scale_gradient");
     });
 }
 

@@ -170,6 +170,44 @@ class GPUMixedLattice:
         return {"times": times, "records": recs.reshape(nr, B, REC),
                 "final": None if final is None else final.reshape(B, self.n_state)}
 
+    def scale_gradient(self, init, t_pulse, target, weights, pulse_scale=None, T0=0.0, T1=1.0, dt=0.01,
+                       save_interval=1, tgrid="boost", want_records=False):
+        """csl_mixed_scale_gradient: per trajectory the loss
+        sum_t sum_j w_j (rec_j(t) - target_j(t))^2 and its derivative with
+        respect to a common scale of the Fe-Tm couplings (at 1), exact for
+        the discrete RK4 scheme.  target [n_rec, 34] shared or [n_rec, B, 34];
+        weights [34] (entry 33, the energy, must be 0).  Returns dict(loss [B],
+        grad [B], times[, records, trecords [n_rec, B, 34]])."""
+        tp = f64(t_pulse).reshape(-1, 2)
+        B = tp.shape[0]
+        init = f64(init)
+        per = 1 if init.ndim == 2 else 0
+        if per and init.shape[0] != B:
+            raise ValueError("init has a different batch size than t_pulse")
+        sc = None if pulse_scale is None else f64(pulse_scale).reshape(B, 2)
+        ns, nr = _count(T0, T1, dt, save_interval, tgrid)
+        tgt = f64(target)
+        per_t = 1 if tgt.ndim == 3 else 0
+        if tgt.shape != ((nr, B, REC) if per_t else (nr, REC)):
+            raise ValueError(f"target must be [{nr}, 34] or [{nr}, {B}, 34]")
+        w = f64(weights).reshape(REC)
+        loss, grad = np.empty(B), np.empty(B)
+        recs = np.empty(nr * B * REC) if want_records else None
+        trecs = np.empty(nr * B * REC) if want_records else None
+        n = C.c_size_t()
+        check(lib().csl_mixed_scale_gradient(self.h, B, dptr(init.reshape(-1)), per, dptr(tp.reshape(-1)),
+                                             dptr(None if sc is None else sc.reshape(-1)), T0, T1, dt, save_interval,
+                                             _capi.TGRID[tgrid], dptr(tgt.reshape(-1)), per_t, dptr(w), dptr(loss),
+                                             dptr(grad), nr, dptr(recs), dptr(trecs), C.byref(n)))
+        # record times on the Boost grid t_k = T0 + k dt (other grids: None)
+        times = (np.array([T0 + min(r * save_interval, ns) * dt for r in range(nr)]) if tgrid == "boost"
+                 else None)
+        out = {"loss": loss, "grad": grad, "times": times}
+        if want_records:
+            out["records"] = recs.reshape(nr, B, REC)
+            out["trecords"] = trecs.reshape(nr, B, REC)
+        return out
+
     def last_kernel_ms(self):
         ms, n = C.c_double(), C.c_size_t()
         check(lib().csl_mixed_last_kernel_ms(self.h, C.byref(ms), C.byref(n)))

@@ -121,6 +121,24 @@ int csl_mixed_integrate_batched(csl_mixed* h, size_t B, const double* init, int 
                                 int tgrid, size_t max_rec, double* times, double* recs,
                                 double* final_states, size_t* n_rec);
 
+/* Gradient of the loss on the records w.r.t. a common scale theta of the
+ * Fe-Tm (SU(2)-SU(3)) couplings, at theta = 1, for B trajectories set up as
+ * in csl_mixed_integrate_batched: loss_b = sum_t sum_j w_j (rec_j(t) -
+ * target_j(t))^2 over the records of trajectory b, grad_b = d loss_b /
+ * d theta.  The derivative is the exact one of the discrete RK4 scheme,
+ * computed forward (tangent-linear: d state / d theta advanced with the state
+ * in the same stage launches; for one parameter it equals what the discrete
+ * adjoint gives, without checkpoints).  target is [n_rec][34] shared
+ * (target_per_replica 0) or [n_rec][B][34]; weights [34] with weights[33]
+ * (energy) = 0.  Optional outputs recs and trecs (the tangent records,
+ * d rec / d theta) are [max_rec][B][34].  No counterpart in the reference
+ * (which has no gradient path); forward model as csl_mixed_integrate_batched. */
+int csl_mixed_scale_gradient(csl_mixed* h, size_t B, const double* init, int init_per_replica,
+                             const double* t_pulse, const double* pulse_scale, double T0, double T1, double dt,
+                             size_t save_interval, int tgrid, const double* target, int target_per_replica,
+                             const double* weights, double* loss, double* grad, size_t max_rec, double* recs,
+                             double* trecs, size_t* n_rec);
+
 /* Average time of one stage launch of the last integration on this handle
  * (HIP events on the handle's stream) and the number of stage launches. */
 int csl_mixed_last_kernel_ms(csl_mixed* h, double* ms, size_t* launches);
