@@ -60,10 +60,11 @@ SIGNATURES = {
     "csl_count_records": (_i, [_d, _d, _d, _sz, _i, _psz, _psz]),
     "csl_pump_probe": (_i, [_vp, _pd, _pd, _d, _d, _d, _d, _d, _d, _d, _d, _d, _sz, _sz, _psz, _psz,
                             _pd, _pd, _pd, _pd, _pd]),
-    "csl_pump_probe_ex": (_i, [_vp, _pd, _pd, _d, _d, _d, _d, _d, _d, _d, _d, _d, _i, _d, _cs, _d, _d, _sz, _sz,
-                               _psz, _psz, _pd, _pd, _pd, _pd, _pd, _pi, _pd]),
-    "csl_integrate_dopri5": (_i, [_vp, _sz, _pd, _i, _pd, _pd, _d, _d, _d, _i, _d, _d, _sz, _i, _sz, _sz,
-                                  _pd, _pd, _pd, _pd, _psz, _psz, _psz]),
+    "csl_pump_probe_ex": (_i, [_vp, _pd, _pd, _d, _d, _d, _d, _d, _d, _d, _d, _d, _i, _d, _i, _cs, _d, _d, _sz,
+                               _sz, _psz, _psz, _pd, _pd, _pd, _pd, _pd, _pi, _pd]),
+    "csl_integrate_dopri5": (_i, [_vp, _sz, _pd, _i, _pd, _pd, _d, _d, _d, _i, _d, _d, _sz, _i, _pd, _psz, _sz,
+                                  _sz, _sz, _pd, _pd, _pd, _pd, _psz, _psz, _psz]),
+    "csl_pulse_segments": (_i, [_d, _d, _pd, _sz, _d, _d, _d, _sz, _pd, _psz]),
     "csl_autotune_batched": (_i, [_vp, _sz, _sz, _pi, _pd]),
     "csl_last_kernel_ms": (_i, [_vp, _pd, _psz]),
     "csl_last_launch_config": (_i, [_vp, _pi, _pi, _pi]),

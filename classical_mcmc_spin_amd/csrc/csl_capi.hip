@@ -65,6 +65,8 @@ struct csl_lattice {
     DBuf<double> bt_init, bt_tp, bt_amp, bt_final;
     DBuf<double> dp_times;        // Dormand-Prince records: times [B][max_rec]
     DBuf<long long> dp_counters;  // [B][4]
+    DBuf<double> dp_segs;         // control-2 segments [B][max_seg][2]
+    DBuf<int> dp_nseg;            // [B]
     // CSR -> device slot mapping of the handle's tables (for per-replica tables)
     std::vector<size_t> h_off, h_slot_of, h_par;
     // per-trajectory Hamiltonians (csl_set_replica_hamiltonians)
@@ -662,6 +664,73 @@ int csl_get_spins(csl_lattice* h, double* spins, size_t n) {
     });
 }
 
+// Pulse-window segments of the pulse drivers (W3, build_pulse_segments,
+// pulse_chunking.h:82-161): every window [c - window, c + window] is snapped
+// to the grid T0 + k*T_step (nearest point) and clipped to [T0, Te], where
+// Te is the last grid point <= T1; overlapping or touching windows merge;
+// the stretches between them are free segments with the step hint
+// min(max(T_step, free_dt_factor*T_step), length).  Output rows are
+// (t0, t1, dt_init), contiguous from T0 to Te; empty when Te <= T0.
+static std::vector<std::array<double, 3>> pulse_segments(double T0, double T1, const double* centers, size_t nc,
+                                                         double window, double T_step, double free_dt_factor) {
+    std::vector<std::array<double, 3>> out;
+    if (!(T1 > T0) || !(T_step > 0.0)) return out;
+    const double Te = T0 + std::floor((T1 - T0) / T_step + 1e-9) * T_step;
+    if (!(Te > T0)) return out;
+    auto on_grid = [&](double t) { return T0 + std::round((t - T0) / T_step) * T_step; };
+    std::vector<std::pair<double, double>> win;
+    for (size_t i = 0; i < nc; ++i) {
+        const double lo = std::max(T0, on_grid(centers[i] - window));
+        const double hi = std::min(Te, on_grid(centers[i] + window));
+        if (lo < hi) win.emplace_back(lo, hi);
+    }
+    std::sort(win.begin(), win.end());
+    std::vector<std::pair<double, double>> active;
+    for (const auto& w : win) {
+        if (!active.empty() && w.first <= active.back().second)
+            active.back().second = std::max(active.back().second, w.second);
+        else
+            active.push_back(w);
+    }
+    const double free_hint = std::max(T_step, free_dt_factor * T_step);
+    double at = T0;
+    for (const auto& w : active) {
+        if (at < w.first) out.push_back({at, w.first, std::min(free_hint, w.first - at)});
+        out.push_back({w.first, w.second, T_step});
+        at = w.second;
+    }
+    if (at < Te) out.push_back({at, Te, std::min(free_hint, Te - at)});
+    return out;
+}
+
+// half-width of a pulse window in units of the width, and the free-segment
+// step-hint factor (pulse_chunking.h:173,178)
+constexpr double kPulseWindowWidths = 9.0;
+constexpr double kFreeStepFactor = 20.0;
+
+// observer calls of one integrate_const run over [s0, s1] with step dt
+static size_t const_calls(double s0, double s1, double dt) {
+    size_t k = 0;
+    while ((s0 + double(k) * dt + dt) - s1 <= 2.220446049250313e-16) ++k;
+    return k + 1;
+}
+
+int csl_pulse_segments(double T_start, double T_end, const double* centers, size_t n_centers, double window,
+                       double T_step, double free_dt_factor, size_t max_out, double* out, size_t* n_out) {
+    return guarded([&] {
+        if (n_centers && !centers) fail(CSL_ERR_NULL, "null centers");
+        if (!std::isfinite(T_start) || !std::isfinite(T_end) || !std::isfinite(window) || !std::isfinite(T_step))
+            fail(CSL_ERR_ARG, "non-finite time argument");
+        check_finite(centers, n_centers, "centers");
+        const auto segs = pulse_segments(T_start, T_end, centers, n_centers, window, T_step, free_dt_factor);
+        if (n_out) *n_out = segs.size();
+        if (segs.size() > max_out) fail(CSL_ERR_CAPACITY, "max_out is smaller than the number of segments");
+        if (segs.size() && !out) fail(CSL_ERR_NULL, "null out");
+        for (size_t i = 0; i < segs.size(); ++i)
+            for (int j = 0; j < 3; ++j) out[3 * i + j] = segs[i][j];
+    });
+}
+
 int csl_count_records(double T0, double T1, double dt, size_t save_interval, int tgrid,
                       size_t* n_steps, size_t* n_rec) {
     return guarded([&] {
@@ -916,6 +985,7 @@ int csl_integrate_batched(csl_lattice* h, size_t B, const double* init, int init
 int csl_integrate_dopri5(csl_lattice* h, size_t B, const double* init, int init_per_replica,
                          const double* t_pulse, const double* pulse_amp, double T0, double T1, double dt,
                          int control, double abs_tol, double rel_tol, size_t save_interval, int tgrid,
+                         const double* segments, const size_t* n_segments, size_t max_segments,
                          size_t max_attempts, size_t max_rec, double* times, double* mags, double* energy,
                          double* final_states, size_t* n_rec_out, size_t* n_acc_out, size_t* n_rej_out) {
     return guarded([&] {
@@ -936,9 +1006,27 @@ int csl_integrate_dopri5(csl_lattice* h, size_t B, const double* init, int init_
         if (block == 0 || csl::dopri5_lds_bytes(h->L, block) > h->lds_limit)
             fail(CSL_ERR_UNSUPPORTED, "the Dormand-Prince kernel holds at most 2048 sites per trajectory");
         std::vector<double> grid;
-        if (control != 1) {
+        if (control == 0) {
             grid = make_grid(T0, T1, dt, tgrid);
             if (grid.size() > size_t(std::numeric_limits<int>::max())) fail(CSL_ERR_ARG, "too many grid points");
+        }
+        std::vector<double> sg;
+        std::vector<int> nsg;
+        if (segments) {
+            if (control != 2) fail(CSL_ERR_ARG, "segments apply to control 2");
+            if (!n_segments) fail(CSL_ERR_NULL, "null n_segments");
+            if (max_segments == 0 || max_segments > 4096) fail(CSL_ERR_ARG, "max_segments must be in [1, 4096]");
+            check_finite(segments, B * max_segments * 2, "segments");
+            sg.assign(segments, segments + B * max_segments * 2);
+            nsg.resize(B);
+            for (size_t b = 0; b < B; ++b) {
+                if (n_segments[b] == 0 || n_segments[b] > max_segments)
+                    fail(CSL_ERR_ARG, "every trajectory needs 1..max_segments segments");
+                nsg[b] = int(n_segments[b]);
+                for (size_t j = 0; j < n_segments[b]; ++j)
+                    if (!(sg[(b * max_segments + j) * 2 + 1] > sg[(b * max_segments + j) * 2]))
+                        fail(CSL_ERR_ARG, "a segment must end after it starts");
+            }
         }
         const size_t nd = h->N * 3;
         const size_t n_init = init_per_replica ? B * nd : nd;
@@ -957,6 +1045,10 @@ int csl_integrate_dopri5(csl_lattice* h, size_t B, const double* init, int init_
         h->bt_tp.upload(tp.data(), tp.size(), s);
         h->bt_amp.upload(am.data(), am.size(), s);
         if (!grid.empty()) h->tgrid.upload(grid.data(), grid.size(), s);
+        if (!sg.empty()) {
+            h->dp_segs.upload(sg.data(), sg.size(), s);
+            h->dp_nseg.upload(nsg.data(), nsg.size(), s);
+        }
         h->dp_times.ensure(B * max_rec);
         h->rec_mags.ensure(B * max_rec * 9);
         if (energy) h->rec_energy.ensure(B * max_rec);
@@ -971,6 +1063,9 @@ int csl_integrate_dopri5(csl_lattice* h, size_t B, const double* init, int init_
         A.control = control;
         A.tgrid = grid.empty() ? nullptr : h->tgrid.p;
         A.n_grid = int(grid.size());
+        A.segs = sg.empty() ? nullptr : h->dp_segs.p;
+        A.n_segs = sg.empty() ? nullptr : h->dp_nseg.p;
+        A.max_seg = int(max_segments);
         A.T0 = T0;
         A.T1 = T1;
         A.dt = dt;
@@ -1042,16 +1137,16 @@ int csl_pump_probe(csl_lattice* h, const double* ground, const double* field_in,
                    size_t* n_rec_out, size_t* n_tau_out, double* times, double* M0, double* M1,
                    double* M01, double* tau_values) {
     return csl_pump_probe_ex(h, ground, field_in, amp, width, freq, tau_start, tau_end, tau_step, T0, T1, T_step,
-                             0, 0.0, "rk4", 0.0, 0.0, max_rec, max_tau, n_rec_out, n_tau_out, times, M0, M1, M01,
-                             tau_values, nullptr, nullptr);
+                             0, 0.0, 1, "rk4", 0.0, 0.0, max_rec, max_tau, n_rec_out, n_tau_out, times, M0, M1,
+                             M01, tau_values, nullptr, nullptr);
 }
 
 int csl_pump_probe_ex(csl_lattice* h, const double* ground, const double* field_in, double amp,
                       double width, double freq, double tau_start, double tau_end, double tau_step,
                       double T0, double T1, double T_step, int reuse_m0_for_m1, double stationarity_tol,
-                      const char* method, double abs_tol, double rel_tol, size_t max_rec, size_t max_tau,
-                      size_t* n_rec_out, size_t* n_tau_out, double* times, double* M0, double* M1, double* M01,
-                      double* tau_values, int* used_w1, double* max_dsdt) {
+                      int pulse_window_chunking, const char* method, double abs_tol, double rel_tol,
+                      size_t max_rec, size_t max_tau, size_t* n_rec_out, size_t* n_tau_out, double* times,
+                      double* M0, double* M1, double* M01, double* tau_values, int* used_w1, double* max_dsdt) {
     return guarded([&] {
         check_handle(h);
         const bool dp5 = method && std::strcmp(method, "dopri5") == 0;
@@ -1109,49 +1204,97 @@ int csl_pump_probe_ex(csl_lattice* h, const double* ground, const double* field_
             am[2 * (1 + n1 + i)] = amp;
             am[2 * (1 + n1 + i) + 1] = amp;
         }
-        size_t n_steps = 0, n_rec = 0;
-        {
-            auto g = make_grid(T0, T1, T_step, CSL_TGRID_BOOST);
-            n_steps = g.size() - 1;
-            n_rec = records_for(n_steps, 1);
-        }
-        std::vector<double> tt(n_rec), mg(n_rec * B * 9);
-        size_t nr = 0;
+        // CPU observer gate (lattice_md.cpp:532-554) over one trajectory's records
+        auto gate = [&](const double* t, size_t n) {
+            std::vector<size_t> k;
+            double last = T0;
+            for (size_t r = 0; r < n; ++r)
+                if (t[r] - last >= T_step - 1e-10 || t[r] >= T1 - 1e-10) {
+                    k.push_back(r);
+                    last = t[r];
+                }
+            return k;
+        };
+        std::vector<double> tt, mg;  // gated records: times [nk], mags [nk][B][9]
+        std::vector<size_t> keep;
         int rc2 = CSL_OK;
         if (dp5) {
             // integrate_const with the controlled Dormand-Prince stepper
-            // (single_pulse_drive / double_pulse_drive, lattice_md.cpp:570-575),
-            // one workgroup and step-size sequence per replica
-            std::vector<double> td(B * n_rec), md(B * n_rec * 9);
-            std::vector<size_t> nrb(B);
-            rc2 = csl_integrate_dopri5(h, B, ground, 0, tp.data(), am.data(), T0, T1, T_step, 2, abs_tol, rel_tol, 1,
-                                       CSL_TGRID_BOOST, 0, n_rec, td.data(), md.data(), nullptr, nullptr, nrb.data(),
-                                       nullptr, nullptr);
-            if (rc2 == CSL_OK) {
-                nr = nrb[0];
-                for (size_t r = 0; r < nr; ++r) {
-                    tt[r] = td[r];
-                    for (size_t b = 0; b < B; ++b)
-                        std::memcpy(&mg[(r * B + b) * 9], &md[(b * n_rec + r) * 9], 9 * sizeof(double));
+            // (single_pulse_drive / double_pulse_drive, lattice_md.cpp:562-575),
+            // one workgroup and step-size sequence per replica; with W3 chunking
+            // one integrate_const call per pulse-window segment of the replica
+            // (windows of 9 widths around its pulse centres)
+            std::vector<double> sg;
+            std::vector<size_t> nsg(B, 1);
+            size_t max_seg = 1, cap = 0;
+            bool empty = false;
+            if (pulse_window_chunking) {
+                std::vector<std::vector<std::array<double, 3>>> per(B);
+                for (size_t b = 0; b < B; ++b) {
+                    // pulse centres: M0 {0}, M1(tau) {tau}, M01(tau) {0, tau}
+                    const double c[2] = {tp[2 * b], tp[2 * b + 1]};
+                    const size_t nc = b < 1 + n1 ? 1 : 2;
+                    per[b] = pulse_segments(T0, T1, c, nc, kPulseWindowWidths * width, T_step, kFreeStepFactor);
+                    empty |= per[b].empty();
+                    max_seg = std::max(max_seg, per[b].size());
+                }
+                if (!empty) {
+                    sg.assign(B * max_seg * 2, 0.0);
+                    for (size_t b = 0; b < B; ++b) {
+                        nsg[b] = per[b].size();
+                        size_t calls = 0;
+                        for (size_t j = 0; j < per[b].size(); ++j) {
+                            sg[(b * max_seg + j) * 2] = per[b][j][0];
+                            sg[(b * max_seg + j) * 2 + 1] = per[b][j][1];
+                            calls += const_calls(per[b][j][0], per[b][j][1], T_step);
+                        }
+                        cap = std::max(cap, calls);
+                    }
+                }
+            } else {
+                cap = const_calls(T0, T1, T_step);
+            }
+            if (!empty) {
+                std::vector<double> td(B * cap), md(B * cap * 9);
+                std::vector<size_t> nrb(B);
+                rc2 = csl_integrate_dopri5(h, B, ground, 0, tp.data(), am.data(), T0, T1, T_step, 2, abs_tol,
+                                           rel_tol, 1, CSL_TGRID_BOOST, sg.empty() ? nullptr : sg.data(),
+                                           sg.empty() ? nullptr : nsg.data(), max_seg, 0, cap, td.data(),
+                                           md.data(), nullptr, nullptr, nrb.data(), nullptr, nullptr);
+                if (rc2 == CSL_OK) {
+                    // each replica through its own observer gate; the reference's
+                    // segments keep every trajectory on the same sample count
+                    std::vector<std::vector<size_t>> kb(B);
+                    for (size_t b = 0; b < B; ++b) kb[b] = gate(&td[b * cap], nrb[b]);
+                    const size_t nk = kb[0].size();
+                    for (size_t b = 1; b < B; ++b)
+                        if (kb[b].size() != nk) fail(CSL_ERR_ARG, "replicas differ in their sample counts");
+                    tt.resize(nk);
+                    mg.resize(nk * B * 9);
+                    for (size_t j = 0; j < nk; ++j) {
+                        tt[j] = td[kb[0][j]];
+                        for (size_t b = 0; b < B; ++b)
+                            std::memcpy(&mg[(j * B + b) * 9], &md[(b * cap + kb[b][j]) * 9], 9 * sizeof(double));
+                    }
+                    keep.resize(nk);
+                    for (size_t j = 0; j < nk; ++j) keep[j] = j;
                 }
             }
         } else {
+            // RK4: chunked segments share the step and the grid, so the run is
+            // the unchunked one up to the rounding of the grid times
+            auto g = make_grid(T0, T1, T_step, CSL_TGRID_BOOST);
+            const size_t n_rec = records_for(g.size() - 1, 1);
+            tt.resize(n_rec);
+            mg.resize(n_rec * B * 9);
+            size_t nr = 0;
             rc2 = csl_integrate_batched(h, B, ground, 0, tp.data(), am.data(), T0, T1, T_step, 1, "rk4",
                                         CSL_TGRID_BOOST, 0, n_rec, tt.data(), mg.data(), nullptr, nullptr, &nr);
+            if (rc2 == CSL_OK) keep = gate(tt.data(), nr);
         }
         // reset the pulse as the reference drivers do (lattice_md.cpp:578-583)
         (void)csl_set_pulse(h, zero.data(), 0.0, width, freq, 0.0, 0.0);
         if (rc2 != CSL_OK) fail(rc2, g_err);
-        // CPU observer gate (lattice_md.cpp:532-554)
-        std::vector<size_t> keep;
-        double last = T0;
-        for (size_t r = 0; r < nr; ++r) {
-            const double t = tt[r];
-            if (t - last >= T_step - 1e-10 || t >= T1 - 1e-10) {
-                keep.push_back(r);
-                last = t;
-            }
-        }
         const size_t nk = keep.size();
         if (n_rec_out) *n_rec_out = nk;
         if (nk > max_rec) fail(CSL_ERR_CAPACITY, "max_rec is smaller than the number of records");

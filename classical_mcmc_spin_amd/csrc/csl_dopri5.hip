@@ -14,8 +14,11 @@
 //              step, the last step cut to land on T1 (molecular_dynamics_cpu,
 //              lattice_md.cpp:452-453),
 //   control 2  integrate_const with the controlled stepper: observer on the
-//              grid, adaptive steps inside each interval, each interval
-//              starting again from dt (pulse drivers, lattice_md.cpp:570-575).
+//              grid, adaptive steps inside each interval, the step size
+//              carried from interval to interval (pulse drivers,
+//              lattice_md.cpp:570-575); optionally one integrate_const call
+//              per pulse-window segment of the trajectory, each starting
+//              from dt again (W3 chunking, lattice_md.cpp:562-571).
 // Error norm (Boost.Odeint default_error_checker, a_x = a_dxdt = 1):
 //   max_i |xerr_i| / (abs + rel (|x_i| + |dt| |dxdt_i|));
 // step control (default_step_adjuster): reject if err > 1 and scale dt by
@@ -57,8 +60,15 @@ constexpr double kE1 = 35.0 / 384.0 - 5179.0 / 57600.0, kE3 = 500.0 / 1113.0 - 7
 constexpr double kEps = 2.220446049250313e-16;  // numeric_limits<double>::epsilon()
 constexpr int kMaxRejections = 500;
 
-// Boost.Odeint less_with_sign for dt > 0
+// Boost.Odeint less_with_sign / less_eq_with_sign for dt > 0
 __device__ __forceinline__ bool less_with_sign(double t1, double t2) { return t2 - t1 > kEps; }
+__device__ __forceinline__ bool less_eq_with_sign(double t1, double t2) { return t1 - t2 <= kEps; }
+// integrate_const's observation time start + k * dt, rounded after the
+// product as on the host (no fused multiply-add)
+__device__ __forceinline__ double grid_time(double start, long long k, double dt) {
+#pragma clang fp contract(off)
+    return start + double(k) * dt;
+}
 // std::max(a, b) semantics (b when a < b, else a)
 __device__ __forceinline__ double smax(double a, double b) { return a < b ? b : a; }
 
@@ -331,23 +341,44 @@ __global__ void __launch_bounds__(dopri5_max_block<SPT>()) k_dopri5(LatDev Lk, D
             }
     };
 
-    // observer at T0 (call 0 is always saved) and the first derivative
-    {
-        const double fl = drive_on && lane < 2 ? lane_drive(A.T0) : 0.0;
-        rhs(bufA, drive_on ? readlane_f64(fl, 0) : 0.0, drive_on ? readlane_f64(fl, 1) : 0.0, k1, true);
-    }
-    __syncthreads();
-    publish(0, A.T0);
-    long long obs_calls = 1, saved = 1, n_acc = 0, n_rej = 0, attempts = 0;
+    long long obs_calls = 0, saved = 0, n_acc = 0, n_rej = 0, attempts = 0;
     long long status = kDopri5Ok;
 
     // One loop for the three controls so that the attempt is expanded once.
-    // control 1 runs from T0 to T1; controls 0 and 2 walk the observation
-    // grid interval by interval (gi), control 2 restarting from dt in each.
-    double t = A.T0, h = A.dt;
+    // control 1 runs from T0 to T1.  control 0 walks the observation grid
+    // interval by interval (gi).  control 2 makes one integrate_const call
+    // per segment [s0, s1] (the whole window, or the trajectory's own list):
+    // the observer at s0 and a fresh stepper there (k1 = f(x, s0), step hint
+    // dt), then interval k from s0 + k dt to (s0 + k dt) + dt, observed at
+    // s0 + (k + 1) dt, with the step size carried from interval to interval
+    // and k1 taken again at the interval start (a new stepper copy per
+    // interval; FSAL's value is kept when the accepted time already equals
+    // it or the drive is off, where both are the same number).
+    const int nseg = control == 2 ? (A.segs ? A.n_segs[b] : 1) : 0;
+    int sg = 0;
+    double s0 = A.T0, s1 = A.T1;
+    if (control == 2 && A.segs) {
+        s0 = A.segs[(size_t(b) * A.max_seg) * 2];
+        s1 = A.segs[(size_t(b) * A.max_seg) * 2 + 1];
+    }
+    double t = s0, h = A.dt, tcur = s0, tend = A.T1;
+    long long kk = 0;  // control 2: interval index within the segment
     int gi = 0, fails = 0;
-    bool fresh = true;  // control 2: at the start of interval gi
+    // pending evaluation k1 = f(x, te) of the state in bufA, with an
+    // observer call when eobs
+    bool need_eval = true, eobs = true;
+    double te = s0;
+    bool fresh = true;  // control 2: at the start of interval kk
     while (true) {
+        if (need_eval) {
+            const bool ob = eobs && (obs_calls % si) == 0;
+            const double fl = drive_on && lane < 2 ? lane_drive(te) : 0.0;
+            rhs(bufA, drive_on ? readlane_f64(fl, 0) : 0.0, drive_on ? readlane_f64(fl, 1) : 0.0, k1, ob);
+            __syncthreads();
+            if (ob) publish(saved++, te);
+            if (eobs) ++obs_calls;
+            need_eval = false;
+        }
         double t7, tobs;
         bool ends;
         if (control == 0) {
@@ -358,34 +389,38 @@ __global__ void __launch_bounds__(dopri5_max_block<SPT>()) k_dopri5(LatDev Lk, D
             tobs = t7;
             ends = true;
         } else {
-            double tend;
             if (control == 1) {
                 if (!less_with_sign(t, A.T1)) break;
                 tend = A.T1;
-            } else {
-                if (gi + 1 >= A.n_grid) break;
-                tend = A.tgrid[gi + 1];
-                if (fresh) {
-                    t = A.tgrid[gi];
+            } else if (fresh) {
+                if (!less_eq_with_sign(tcur + A.dt, s1)) {
+                    // segment done: the next integrate_const call
+                    if (++sg >= nseg) break;
+                    s0 = A.segs[(size_t(b) * A.max_seg + sg) * 2];
+                    s1 = A.segs[(size_t(b) * A.max_seg + sg) * 2 + 1];
+                    kk = 0;
+                    t = tcur = te = s0;
                     h = A.dt;
-                    fresh = false;
-                    fails = 0;
-                }
-                if (!less_with_sign(t, tend)) {
-                    // empty interval: observe the current state (bufA holds x)
-                    if ((obs_calls % si) == 0) {
-                        double kd[SPT][3];
-                        const double fl = drive_on && lane < 2 ? lane_drive(tend) : 0.0;
-                        rhs(bufA, drive_on ? readlane_f64(fl, 0) : 0.0, drive_on ? readlane_f64(fl, 1) : 0.0, kd,
-                            true);
-                        __syncthreads();
-                        publish(saved++, tend);
-                    }
-                    ++obs_calls;
-                    ++gi;
-                    fresh = true;
+                    need_eval = eobs = true;
                     continue;
                 }
+                tend = tcur + A.dt;
+                if (!less_with_sign(tcur, tend)) {
+                    // an interval too short to step: observed unchanged
+                    ++kk;
+                    t = tcur = te = grid_time(s0, kk, A.dt);
+                    need_eval = eobs = true;
+                    continue;
+                }
+                fresh = false;
+                fails = 0;
+                if (drive_on && t != tcur) {
+                    t = te = tcur;
+                    need_eval = true;
+                    eobs = false;
+                    continue;
+                }
+                t = tcur;
             }
             if (fails == 0 && less_with_sign(tend, t + h)) h = tend - t;
             if (!(h > 0.0) || !(t + h > t)) {
@@ -393,7 +428,7 @@ __global__ void __launch_bounds__(dopri5_max_block<SPT>()) k_dopri5(LatDev Lk, D
                 break;
             }
             t7 = t + h;
-            tobs = control == 1 ? t7 : tend;
+            tobs = control == 1 ? t7 : grid_time(s0, kk + 1, A.dt);
             // the observer fires when this step, if accepted, ends the run / interval
             ends = control == 1 || !less_with_sign(t + h, tend);
         }
@@ -423,8 +458,10 @@ __global__ void __launch_bounds__(dopri5_max_block<SPT>()) k_dopri5(LatDev Lk, D
         if (ends) {
             if (obs) publish(saved++, tobs);
             ++obs_calls;
-            if (control != 1) {
-                ++gi;
+            if (control == 0) ++gi;
+            if (control == 2) {
+                ++kk;
+                tcur = tobs;
                 fresh = true;
             }
         }

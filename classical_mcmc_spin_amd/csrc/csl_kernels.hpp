@@ -60,7 +60,10 @@ struct Dopri5Args {
     const double* t_pulse;     // [B*2]
     const double* amp;         // [B*2]
     int control;               // 0 fixed grid steps, 1 integrate_adaptive, 2 integrate_const + controller
-    const double* tgrid;       // control 0/2: observation grid [n_grid]
+    const double* tgrid;       // control 0: observation grid [n_grid]
+    const double* segs;        // control 2: [B][max_seg] segments (t0, t1), or null for [T0, T1]
+    const int* n_segs;         // [B] segments per trajectory (>= 1)
+    int max_seg;
     int n_grid;
     double T0, T1, dt;
     double abs_tol, rel_tol;

@@ -252,11 +252,12 @@ class GPULattice:
 
     def pump_probe(self, ground, field_in, amp, width, freq, tau_start, tau_end, tau_step,
                    T0, T1, T_step, reuse_m0_for_m1=False, stationarity_tol=1e-6, method="rk4",
-                   abs_tol=1e-8, rel_tol=1e-8):
+                   abs_tol=1e-8, rel_tol=1e-8, pulse_window_chunking=True):
         """csl_pump_probe_ex: M0, M1(tau), M01(tau) in one batched launch
         (method "rk4" on the Boost grid, or "dopri5" with step control per
-        replica); with reuse_m0_for_m1 and a stationary ground state M1 is the
-        time shift of M0 (W1).  The result carries w1 (bool) and max_dsdt."""
+        replica, by pulse-window segments with pulse_window_chunking); with
+        reuse_m0_for_m1 and a stationary ground state M1 is the time shift of
+        M0 (W1).  The result carries w1 (bool) and max_dsdt."""
         g = f64(ground).reshape(-1)
         fi = f64(field_in).reshape(self.A, 3)
         n_tau_max = int(abs((tau_end - tau_start) / tau_step)) + 2
@@ -271,7 +272,8 @@ class GPULattice:
         w1, mx = C.c_int(0), C.c_double(float("nan"))
         check(lib().csl_pump_probe_ex(self.h, dptr(g), dptr(fi.reshape(-1)), amp, width, freq, tau_start,
                                       tau_end, tau_step, T0, T1, T_step, int(bool(reuse_m0_for_m1)),
-                                      float(stationarity_tol), method.encode(), float(abs_tol), float(rel_tol),
+                                      float(stationarity_tol), int(bool(pulse_window_chunking)), method.encode(),
+                                      float(abs_tol), float(rel_tol),
                                       max_rec, n_tau_max, C.byref(nr), C.byref(nt), dptr(times), dptr(M0),
                                       dptr(M1), dptr(M01), dptr(taus), C.byref(w1), C.byref(mx)))
         r, t = nr.value, nt.value
@@ -282,13 +284,15 @@ class GPULattice:
 
     def integrate_dopri5(self, init, T0, T1, dt, control=1, abs_tol=1e-6, rel_tol=1e-6, save_interval=1,
                          tgrid="boost", t_pulse=None, pulse_amp=None, want_energy=True, want_final=True,
-                         max_rec=None, max_attempts=0):
+                         max_rec=None, max_attempts=0, segments=None):
         """csl_integrate_dopri5: Dormand-Prince 5(4) for B trajectories
         (init [N,3] -> B from t_pulse, or [B,N,3]); control 0 fixed grid
-        steps, 1 integrate_adaptive, 2 integrate_const with step control.
-        Returns dict(times, mags, energy: lists of per-trajectory arrays,
-        final [B,N,3], n_rec, accepted, rejected [B]); a too small record
-        buffer is enlarged and the (deterministic) run repeated."""
+        steps, 1 integrate_adaptive, 2 integrate_const with step control,
+        with segments (B sequences of (t0, t1[, dt_init]) rows) one
+        integrate_const call per segment.  Returns dict(times, mags, energy:
+        lists of per-trajectory arrays, final [B,N,3], n_rec, accepted,
+        rejected [B]); a too small record buffer is enlarged and the
+        (deterministic) run repeated."""
         init = f64(init)
         per = 1 if init.ndim == 3 else 0
         tp = None if t_pulse is None else f64(t_pulse).reshape(-1, 2)
@@ -296,9 +300,23 @@ class GPULattice:
         if tp is not None and tp.shape[0] != B:
             raise ValueError("t_pulse has a different batch size than init")
         amp = None if pulse_amp is None else f64(pulse_amp).reshape(B, 2)
+        sg = nsg = None
+        ms = 0
+        if segments is not None:
+            rows = [np.asarray(r, dtype=float).reshape(-1, np.shape(r)[-1])[:, :2] for r in segments]
+            if len(rows) != B:
+                raise ValueError("segments needs one sequence per trajectory")
+            ms = max(1, max(len(r) for r in rows))
+            sg = np.zeros((B, ms, 2))
+            for b, r in enumerate(rows):
+                sg[b, :len(r)] = r
+            nsg = np.array([len(r) for r in rows], dtype=np.uintp)
         if max_rec is None:
-            ns, nr = self.count_records(T0, T1, dt, 1, tgrid)
-            max_rec = nr + 1 if control != 1 else max(64, 2 * nr)
+            if sg is not None:
+                max_rec = max(sum(_const_calls(t0, t1, dt) for t0, t1 in r) for r in rows)
+            else:
+                ns, nr = self.count_records(T0, T1, dt, 1, tgrid)
+                max_rec = nr + 1 if control != 1 else max(64, 2 * nr)
         while True:
             times = np.empty(B * max_rec)
             mags = np.empty(B * max_rec * 9)
@@ -310,7 +328,8 @@ class GPULattice:
             rc = lib().csl_integrate_dopri5(self.h, B, dptr(init.reshape(-1)), per,
                                             dptr(None if tp is None else tp.reshape(-1)),
                                             dptr(None if amp is None else amp.reshape(-1)), T0, T1, dt, control,
-                                            abs_tol, rel_tol, save_interval, _capi.TGRID[tgrid], max_attempts,
+                                            abs_tol, rel_tol, save_interval, _capi.TGRID[tgrid],
+                                            dptr(None if sg is None else sg.reshape(-1)), szptr(nsg), ms, max_attempts,
                                             max_rec, dptr(times), dptr(mags), dptr(energy), dptr(final),
                                             szptr(nrec), szptr(nacc), szptr(nrej))
             if rc == _capi.CSL_ERR_CAPACITY and int(nrec.max()) > max_rec:
@@ -335,6 +354,33 @@ def _drive_dirs(lattice: Lattice, field_in) -> np.ndarray:
     """Lattice::set_pulse frame rotation field_drive = R_atom * B (lattice_md.cpp:215-216)."""
     fi = np.asarray(field_in, dtype=float).reshape(lattice.N_atoms, lattice.spin_dim)
     return np.einsum("aij,aj->ai", lattice.frames, fi)
+
+
+def _const_calls(t0, t1, dt):
+    """Observer calls of Boost integrate_const over [t0, t1] with step dt."""
+    k = 0
+    while (t0 + k * dt + dt) - t1 <= 2.220446049250313e-16:
+        k += 1
+    return k + 1
+
+
+# half-width of a pulse window in pulse widths and the free-segment step-hint
+# factor of the reference's W3 chunking (pulse_chunking.h:173,178)
+PULSE_WINDOW_WIDTHS = 9.0
+FREE_DT_FACTOR = 20.0
+
+
+def pulse_segments(T_start, T_end, centers, window, T_step, free_dt_factor=FREE_DT_FACTOR):
+    """build_pulse_segments (pulse_chunking.h:82-161) through
+    csl_pulse_segments: rows (t0, t1, dt_init) of the free and pulse-active
+    segments covering [T_start, last grid point <= T_end]."""
+    c = f64(np.atleast_1d(np.asarray(centers, dtype=float)))
+    cap = 2 * c.size + 2
+    out = np.empty(3 * cap)
+    n = C.c_size_t()
+    check(lib().csl_pulse_segments(float(T_start), float(T_end), dptr(c), c.size, float(window), float(T_step),
+                                   float(free_dt_factor), cap, dptr(out), C.byref(n)))
+    return out[:3 * n.value].reshape(-1, 3)
 
 
 def _gate(times, T_start, T_end, step):
@@ -385,14 +431,24 @@ def molecular_dynamics(gl: GPULattice, spins, T_start, T_end, dt, save_interval=
 
 
 def _pulse_run(gl: GPULattice, spins, fd, t1, t2, amp, width, freq, T_start, T_end, step, method="rk4",
-               abs_tol=1e-8, rel_tol=1e-8):
-    gl.set_pulse(fd, amp, width, freq, t1, t2)
+               abs_tol=1e-8, rel_tol=1e-8, centers=(), chunking=True):
     if method == "dopri5":
-        # integrate_const with the controlled stepper (lattice_md.cpp:570-575)
+        # integrate_const with the controlled stepper (lattice_md.cpp:562-575),
+        # one call per pulse-window segment with chunking (W3)
+        segs = None
+        if chunking:
+            segs = pulse_segments(T_start, T_end, centers, PULSE_WINDOW_WIDTHS * width, step)
+            if len(segs) == 0:
+                return np.empty(0), np.empty((0, 3, 3))
+            segs = [segs]
+        gl.set_pulse(fd, amp, width, freq, t1, t2)
         r = gl.integrate_dopri5(spins, T_start, T_end, step, control=2, abs_tol=abs_tol, rel_tol=rel_tol,
-                                t_pulse=[[t1, t2]], pulse_amp=[[amp, amp]], want_energy=False)
+                                t_pulse=[[t1, t2]], pulse_amp=[[amp, amp]], want_energy=False, segments=segs)
         times, mags = r["times"][0], r["mags"][0]
     elif method == "rk4":
+        # chunked RK4 takes the same steps on the same grid (up to the
+        # rounding of the grid times), so the run is not split
+        gl.set_pulse(fd, amp, width, freq, t1, t2)
         gl.set_spins(spins)
         res = gl.integrate(T_start, T_end, step, 1, "rk4", tgrid="boost", want_energy=False)
         times, mags = res["times"], res["mags"]
@@ -404,25 +460,28 @@ def _pulse_run(gl: GPULattice, spins, fd, t1, t2, amp, width, freq, T_start, T_e
 
 
 def single_pulse_drive(gl: GPULattice, spins, field_in, t_B, pulse_amp, pulse_width, pulse_freq,
-                       T_start, T_end, step_size, method="rk4", abs_tol=1e-8, rel_tol=1e-8):
+                       T_start, T_end, step_size, method="rk4", abs_tol=1e-8, rel_tol=1e-8,
+                       pulse_window_chunking=True):
     """Lattice::single_pulse_drive (lattice_md.cpp:493-586), method "rk4" or
     "dopri5" (controlled, tolerances as the reference's pump-probe defaults
-    kDefaultPumpProbeAbsTol/RelTol = 1e-8).  Returns (times [n],
-    mags [n, 3(M_antiferro, M_local, M_global), 3])."""
+    kDefaultPumpProbeAbsTol/RelTol = 1e-8; with pulse_window_chunking one
+    integrate_const call per segment of pulse_segments around t_B).
+    Returns (times [n], mags [n, 3(M_antiferro, M_local, M_global), 3])."""
     lat = gl.lattice
     fd = np.concatenate([_drive_dirs(lat, field_in).reshape(-1), np.zeros(lat.N_atoms * 3)])
     return _pulse_run(gl, spins, fd, t_B, 0.0, pulse_amp, pulse_width, pulse_freq, T_start, T_end, step_size,
-                      method, abs_tol, rel_tol)
+                      method, abs_tol, rel_tol, (t_B,), pulse_window_chunking)
 
 
 def double_pulse_drive(gl: GPULattice, spins, field_in_1, t_B_1, field_in_2, t_B_2, pulse_amp,
                        pulse_width, pulse_freq, T_start, T_end, step_size, method="rk4", abs_tol=1e-8,
-                       rel_tol=1e-8):
-    """Lattice::double_pulse_drive (lattice_md.cpp:589-680)."""
+                       rel_tol=1e-8, pulse_window_chunking=True):
+    """Lattice::double_pulse_drive (lattice_md.cpp:589-680); the segments
+    surround both pulse centres."""
     lat = gl.lattice
     fd = np.concatenate([_drive_dirs(lat, field_in_1).reshape(-1), _drive_dirs(lat, field_in_2).reshape(-1)])
     return _pulse_run(gl, spins, fd, t_B_1, t_B_2, pulse_amp, pulse_width, pulse_freq, T_start, T_end,
-                      step_size, method, abs_tol, rel_tol)
+                      step_size, method, abs_tol, rel_tol, (t_B_1, t_B_2), pulse_window_chunking)
 
 
 def pump_probe_spectroscopy(gl: GPULattice, ground, field_in, pulse_amp, pulse_width, pulse_freq,
@@ -431,7 +490,7 @@ def pump_probe_spectroscopy(gl: GPULattice, ground, field_in, pulse_amp, pulse_w
                             Temp_start: float = 0.0, Temp_end: float = 0.0, n_anneal: int = 0,
                             T_zero_quench: bool = False, quench_sweeps: int = 0,
                             reuse_m0_for_m1: bool = False, stationarity_tol: float = 1e-6,
-                            abs_tol: float = 1e-8, rel_tol: float = 1e-8):
+                            abs_tol: float = 1e-8, rel_tol: float = 1e-8, pulse_window_chunking: bool = True):
     """Lattice::pump_probe_spectroscopy (lattice_md.cpp:683-926): M0, M1(tau)
     and M01(tau) for every delay, all integrated in one batched launch.
     With dir_name the result goes to dir_name/pump_probe_spectroscopy.h5 in
@@ -440,11 +499,12 @@ def pump_probe_spectroscopy(gl: GPULattice, ground, field_in, pulse_amp, pulse_w
     only describe how the ground state was made) as metadata.
     reuse_m0_for_m1 / stationarity_tol are the reference's W1 option
     (lattice_md.cpp:729-752): M1(tau) synthesised from M0 when max|dS/dt| of
-    the undriven ground state is within the tolerance.
+    the undriven ground state is within the tolerance; pulse_window_chunking
+    is the reference's W3 option for "dopri5" (lattice_md.cpp:562-571).
     Returns dict(times, M0 [n,3,3], M1 [n_tau,n,3,3], M01, tau, w1, max_dsdt)."""
     res = gl.pump_probe(ground, field_in, pulse_amp, pulse_width, pulse_freq, tau_start, tau_end,
                         tau_step, T_start, T_end, T_step, reuse_m0_for_m1, stationarity_tol, method, abs_tol,
-                        rel_tol)
+                        rel_tol, pulse_window_chunking)
     if dir_name is not None:
         from . import h5io
         lat = gl.lattice

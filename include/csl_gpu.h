@@ -183,6 +183,20 @@ int csl_set_replica_hamiltonians(csl_lattice* h, size_t B, const double* J, cons
 int csl_count_records(double T0, double T1, double dt, size_t save_interval,
                       int tgrid, size_t* n_steps, size_t* n_rec);
 
+/* build_pulse_segments (pulse_chunking.h:82-161): the free and
+ * pulse-active segments covering [T_start, T_end'] where T_end' is the last
+ * point of the grid T_start + k*T_step not after T_end.  Each window
+ * [c - window, c + window] around a centre is snapped to the nearest grid
+ * points and clipped; overlapping or touching windows merge.  Rows
+ * out[3*i + {0, 1, 2}] = (t0, t1, dt_init): contiguous, on the grid;
+ * dt_init is T_step for active segments and min(max(T_step,
+ * free_dt_factor*T_step), t1 - t0) for free ones.  *n_out receives the
+ * count (0 when T_end' <= T_start); CSL_ERR_CAPACITY when it exceeds
+ * max_out.  Host-only: needs no device. */
+int csl_pulse_segments(double T_start, double T_end, const double* centers, size_t n_centers,
+                       double window, double T_step, double free_dt_factor, size_t max_out,
+                       double* out, size_t* n_out);
+
 /* ---- 2DCS / pump-probe driver --------------------------------------- */
 
 /* The work of pump_probe_spectroscopy (lattice_md.cpp:683-926) and
@@ -205,11 +219,15 @@ int csl_pump_probe(csl_lattice* h, const double* ground, const double* field_in,
 
 /* csl_pump_probe with the reference driver's method and W1 options
  * (pump_probe_spectroscopy(..., method, ..., reuse_m0_for_m1,
- * stationarity_tol, ..., abs_tol, rel_tol), lattice_md.cpp:683-697).
- * method "rk4" integrates on the Boost grid as csl_pump_probe; "dopri5" runs
- * every replica with the controlled Dormand-Prince stepper between the
- * observation points (integrate_const with make_controlled(abs_tol, rel_tol),
- * lattice_md.cpp:570-575) through csl_integrate_dopri5.  When
+ * stationarity_tol, ..., pulse_window_chunking, abs_tol, rel_tol),
+ * lattice_md.cpp:683-697).  method "rk4" integrates on the Boost grid as
+ * csl_pump_probe; "dopri5" runs every replica with the controlled
+ * Dormand-Prince stepper between the observation points (integrate_const
+ * with make_controlled(abs_tol, rel_tol), lattice_md.cpp:562-575) through
+ * csl_integrate_dopri5, with pulse_window_chunking (the reference default)
+ * one integrate_const call per segment of csl_pulse_segments around the
+ * replica's pulse centres (window 9 widths, lattice_md.cpp:562-571; for
+ * "rk4" the chunked and unchunked runs take the same steps).  When
  * reuse_m0_for_m1 is set and the L_inf norm of dS/dt at the ground state
  * with the drive off (max_dSdt_norm_no_drive, :236-280, evaluated on the
  * device) is <= stationarity_tol, the M1(tau) replicas are not integrated;
@@ -223,7 +241,7 @@ int csl_pump_probe_ex(csl_lattice* h, const double* ground, const double* field_
                       double amp, double width, double freq,
                       double tau_start, double tau_end, double tau_step,
                       double T0, double T1, double T_step,
-                      int reuse_m0_for_m1, double stationarity_tol,
+                      int reuse_m0_for_m1, double stationarity_tol, int pulse_window_chunking,
                       const char* method, double abs_tol, double rel_tol,
                       size_t max_rec, size_t max_tau, size_t* n_rec, size_t* n_tau,
                       double* times, double* M0, double* M1, double* M01,
@@ -241,8 +259,15 @@ int csl_pump_probe_ex(csl_lattice* h, const double* ground, const double* field_
  *      T0 and after every accepted step, the last step cut to end at T1
  *      (molecular_dynamics_cpu, lattice_md.cpp:452-453);
  *   2  integrate_const with the controlled stepper: observer on the
- *      integrate_const grid T0 + k dt, adaptive steps inside each interval
- *      (the pulse drivers, lattice_md.cpp:570-575).
+ *      integrate_const grid T0 + k dt, adaptive steps inside each interval,
+ *      the step size carried from one interval to the next (the pulse
+ *      drivers, lattice_md.cpp:570-575).  With segments (may be NULL) each
+ *      trajectory b makes one integrate_const call per row j <
+ *      n_segments[b] of segments[(b*max_segments + j)*2 + {0, 1}] = (t0, t1),
+ *      each starting again from the step hint dt with the observer at t0
+ *      (W3 chunking, lattice_md.cpp:562-571; the seams are observed twice,
+ *      as the reference's observer sees them).
+ * tgrid applies to control 0; control 2 uses the integrate_const grid.
  * Observer call c is recorded when c % save_interval == 0 (lattice_md.cpp:
  * 399).  init / t_pulse / pulse_amp as csl_integrate_batched (t_pulse may be
  * NULL: zeros; pulse_amp NULL: the handle's amplitude); per-replica
@@ -258,6 +283,7 @@ int csl_integrate_dopri5(csl_lattice* h, size_t B, const double* init, int init_
                          const double* t_pulse, const double* pulse_amp,
                          double T0, double T1, double dt, int control,
                          double abs_tol, double rel_tol, size_t save_interval, int tgrid,
+                         const double* segments, const size_t* n_segments, size_t max_segments,
                          size_t max_attempts, size_t max_rec, double* times, double* mags,
                          double* energy, double* final_states, size_t* n_rec,
                          size_t* n_accepted, size_t* n_rejected);

@@ -501,11 +501,15 @@ struct DP5Run {
         ++accepted;
         return true;
     }
-    // integrate_adaptive (controlled_stepper_tag) from t to t_end; obs(t) at
-    // the start and after every accepted step when with_obs.  Returns false
-    // after kMaxRejections rejections in a row.
+    // a new copy of the controlled stepper: the first try_step takes the
+    // derivative at (x, t) again
+    void restart(double t) { f(x, k1.data(), t); }
+    // integrate_adaptive (controlled_stepper_tag) from t to t_end with the
+    // step size dt (updated in place, as Boost's reference argument); obs(t)
+    // at the start and after every accepted step when with_obs.  Returns
+    // false after kMaxRejections rejections in a row.
     template <class Obs>
-    bool adaptive(double t, double t_end, double dt, Obs&& obs, bool with_obs) {
+    bool adaptive(double t, double t_end, double& dt, Obs&& obs, bool with_obs) {
         while (less_with_sign(t, t_end)) {
             if (with_obs) obs(t);
             if (less_with_sign(t_end, t + dt)) dt = t_end - t;
@@ -696,8 +700,8 @@ size_t orc_integrate_md(const OracleLattice* L, double* state, double T0, double
 
 long orc_integrate_dopri5(const OracleLattice* L, double* state, double T0, double T1, double dt,
                           int control, double abs_tol, double rel_tol, size_t save_interval,
-                          size_t max_rec, double* times, double* out_mags, double* energy,
-                          size_t* n_accepted, size_t* n_rejected) {
+                          const double* segs, size_t n_seg, size_t max_rec, double* times,
+                          double* out_mags, double* energy, size_t* n_accepted, size_t* n_rejected) {
     const size_t n = L->N * L->D;
     size_t step_count = 0, nrec = 0;
     std::vector<double> m(3 * L->D);
@@ -730,21 +734,32 @@ long orc_integrate_dopri5(const OracleLattice* L, double* state, double T0, doub
     } else {
         DP5Run run(f, state, n, T0, abs_tol, rel_tol);
         if (control == 1) {
-            ok = run.adaptive(T0, T1, dt, obs, true);  // integrate_adaptive
+            double h = dt;
+            ok = run.adaptive(T0, T1, h, obs, true);  // integrate_adaptive
         } else {
-            // integrate_const with a controlled stepper: observe on the grid
-            // t_k = T0 + k dt, adaptive steps inside each interval, each
-            // interval starting from the initial dt again.
-            double t = T0;
-            size_t k = 0;
-            while (ok && less_eq_with_sign(t + dt, T1)) {
-                obs(t);
-                const double t_next = T0 + double(k + 1) * dt;
-                ok = run.adaptive(t, t_next, dt, obs, false);
-                ++k;
-                t = t_next;
+            // integrate_const with a controlled stepper (Boost's
+            // controlled_stepper_tag loop): observe at time_k = s0 + k dt,
+            // integrate_adaptive from time_k to time_k + dt with a copy of
+            // the stepper made at the call (so the derivative is taken again
+            // at time_k) and the step size carried over from the previous
+            // interval; the step hint is dt at the start of each call.
+            auto integrate_const_ctl = [&](double s0, double s1) {
+                double time = s0, h = dt;
+                size_t k = 0;
+                while (ok && less_eq_with_sign(time + dt, s1)) {
+                    obs(time);
+                    run.restart(time);
+                    ok = run.adaptive(time, time + dt, h, obs, false);
+                    ++k;
+                    time = s0 + double(k) * dt;
+                }
+                if (ok) obs(time);
+            };
+            if (n_seg == 0) {
+                integrate_const_ctl(T0, T1);
+            } else {
+                for (size_t j = 0; ok && j < n_seg; ++j) integrate_const_ctl(segs[2 * j], segs[2 * j + 1]);
             }
-            if (ok) obs(t);
         }
         acc = run.accepted;
         rej = run.rejected;
@@ -783,7 +798,8 @@ long orc_dopri5_scalar(double lambda, double omega, double y0, double T0, double
         integrate_const(stp, T0, T1, dt, [](double) {});
     } else {
         DP5Run run(f, &y, 1, T0, abs_tol, rel_tol);
-        ok = run.adaptive(T0, T1, dt, [](double) {}, false);
+        double h = dt;
+        ok = run.adaptive(T0, T1, h, [](double) {}, false);
         acc = run.accepted;
         rej = run.rejected;
     }

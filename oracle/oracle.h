@@ -84,14 +84,16 @@ size_t orc_integrate_md(const OracleLattice* L, double* state, double T0, double
  *   control 1: integrate_adaptive, observer after every accepted step
  *              (molecular_dynamics_cpu, lattice_md.cpp:452-453);
  *   control 2: integrate_const with the controlled stepper, observer on the
- *              grid T0 + k dt (pulse drivers, lattice_md.cpp:570-575).
+ *              grid T0 + k dt (pulse drivers, lattice_md.cpp:570-575); with
+ *              n_seg > 0 one integrate_const call per row (t0, t1) of segs
+ *              (W3 chunking, lattice_md.cpp:562-571).
  * Records as orc_integrate_md.  Returns the number of records, or -1 after
  * 500 rejections in a row. */
 long   orc_integrate_dopri5(const OracleLattice* L, double* state, double T0, double T1,
                             double dt, int control, double abs_tol, double rel_tol,
-                            size_t save_interval, size_t max_rec, double* times,
-                            double* mags, double* energy_per_site, size_t* n_accepted,
-                            size_t* n_rejected);
+                            size_t save_interval, const double* segs, size_t n_seg,
+                            size_t max_rec, double* times, double* mags,
+                            double* energy_per_site, size_t* n_accepted, size_t* n_rejected);
 /* The same stepper on y' = lambda*y + cos(omega*t) (pinning tests):
  * y(T1), step counts, and for one step of size dt from (T0, y0) the embedded
  * error estimate and the propagated value in err_first_step[0..1]. */

@@ -39,7 +39,8 @@ _SIG = {
     "orc_single_pulse_drive": (_sz, [_vp, _pd, _pd, _d, _d, _d, _d, _d, _d, _d, _i, _sz, _pd, _pd]),
     "orc_double_pulse_drive": (_sz, [_vp, _pd, _pd, _d, _pd, _d, _d, _d, _d, _d, _d, _d, _i, _sz, _pd, _pd]),
     "orc_max_dsdt_no_drive": (_d, [_vp, _pd]),
-    "orc_integrate_dopri5": (C.c_long, [_vp, _pd, _d, _d, _d, _i, _d, _d, _sz, _sz, _pd, _pd, _pd, _psz, _psz]),
+    "orc_integrate_dopri5": (C.c_long, [_vp, _pd, _d, _d, _d, _i, _d, _d, _sz, _pd, _sz, _sz, _pd, _pd, _pd,
+                                        _psz, _psz]),
     "orc_dopri5_scalar": (C.c_long, [_d, _d, _d, _d, _d, _d, _i, _d, _d, _pd, _psz, _psz, _pd]),
     "orc_pump_probe": (_i, [_vp, _pd, _pd, _d, _d, _d, _d, _d, _d, _d, _d, _d, _i, _d, _i, _i, _sz, _sz,
                             _psz, _pd, _pd, _pd, _pd, _pd, _pi]),
@@ -183,9 +184,11 @@ class OracleLattice:
         return {"times": t[:nr], "mags": m.reshape(mx, 3, self.D)[:nr], "energy": e[:nr], "final": s}
 
     def integrate_dopri5(self, state, T0, T1, dt, control=1, abs_tol=1e-6, rel_tol=1e-6, save_interval=1,
-                         max_rec=None):
+                         max_rec=None, segments=None):
         """Dormand-Prince 5(4): control 0 fixed grid steps, 1 integrate_adaptive,
-        2 integrate_const with the controlled stepper."""
+        2 integrate_const with the controlled stepper (one call per row
+        (t0, t1[, dt_init]) of segments when given)."""
+        sg = None if segments is None else _f(np.asarray(segments, dtype=float).reshape(len(segments), -1)[:, :2])
         s = _f(state).copy()
         mx = max_rec or 1000
         while True:
@@ -195,6 +198,7 @@ class OracleLattice:
             na, nr_ = C.c_size_t(), C.c_size_t()
             s = _f(state).copy()
             nr = lib().orc_integrate_dopri5(self.h, _p(s), T0, T1, dt, control, abs_tol, rel_tol, save_interval,
+                                            None if sg is None else _p(sg), 0 if sg is None else len(sg),
                                             mx, _p(t), _p(m), _p(e), C.byref(na), C.byref(nr_))
             if nr < 0:
                 raise RuntimeError("step size control failed (500 rejections in a row)")

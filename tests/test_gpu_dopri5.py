@@ -109,6 +109,33 @@ def test_batched_replicas_own_step_sequences(po):
     o.clear_pulse()
 
 
+def test_pulse_window_segments_vs_oracle(po):
+    """control 2 by pulse-window segments (W3 chunking): every trajectory
+    with its own pulse centres and segment list, one integrate_const call
+    per segment; the seams are observed twice as in the reference's runs."""
+    lat, o = _pair(po, 6, 6)
+    f = unit_x_field(lat)
+    fd = np.einsum("aij,aj->ai", lat.frames, f).reshape(-1)
+    width, T0, T1, st = 0.05, -0.5, 3.0, 0.02
+    tp = np.array([[0.0, 0.0], [0.8, 0.0], [0.0, 0.3], [0.0, 2.9]])
+    am = np.array([[0.6, 0.0], [0.6, 0.0], [0.6, 0.6], [0.6, 0.6]])
+    centers = [[0.0], [0.8], [0.0, 0.3], [0.0, 2.9]]
+    segs = [G.pulse_segments(T0, T1, c, 9 * width, st) for c in centers]
+    assert [len(q) for q in segs] == [3, 3, 3, 4]
+    inits = np.stack([random_state(lat, 60 + b) for b in range(len(tp))])
+    with GPULattice(lat) as gl:
+        gl.set_pulse(np.concatenate([fd, fd]), 1.0, width, 3.0, 0.0, 0.0)
+        r = gl.integrate_dopri5(inits, T0, T1, st, control=2, abs_tol=1e-9, rel_tol=1e-9, t_pulse=tp,
+                                pulse_amp=am, segments=segs, max_attempts=MAX_ATTEMPTS)
+    for b in range(len(tp)):
+        print(f"segments, trajectory {b}", flush=True)
+        o.set_pulse(f * am[b, 0], tp[b, 0], f * am[b, 1], tp[b, 1], 1.0, width, 3.0)
+        ref = o.integrate_dopri5(inits[b].reshape(-1), T0, T1, st, control=2, abs_tol=1e-9, rel_tol=1e-9,
+                                 segments=segs[b])
+        _cmp(r, ref, b, adaptive=False)
+    o.clear_pulse()
+
+
 def test_record_buffer_retry_and_limits(po):
     lat, o = _pair(po, 4, 4)
     s = random_state(lat, 9)
@@ -130,9 +157,10 @@ def test_record_buffer_retry_and_limits(po):
 
 def test_drivers_dopri5(po):
     """molecular_dynamics and the pump-probe driver with method "dopri5":
-    the MD run against the oracle, the pump-probe M0 / M1 / M01 against
-    oracle runs of the controlled integrate_const, and the tight-tolerance
-    MD run against converged RK4."""
+    the MD run against the oracle, the pump-probe M0 / M1 / M01 (with and
+    without pulse-window chunking) and single_pulse_drive against oracle runs
+    of the controlled integrate_const, and the tight-tolerance MD run
+    against converged RK4."""
     lat, o = _pair(po, 6, 6)
     s = random_state(lat, 13)
     ref = o.integrate_dopri5(s.reshape(-1), 0.0, 2.0, 0.01, control=1, abs_tol=1e-6, rel_tol=1e-6)
@@ -145,16 +173,38 @@ def test_drivers_dopri5(po):
         tight = G.molecular_dynamics(gl, s, 0.0, 2.0, 0.01, method="dopri5", abs_tol=1e-11, rel_tol=1e-11)
         conv = o.integrate_md(s.reshape(-1), 0.0, 2.0 + 1e-4, 1e-3)
         assert np.max(np.abs(tight["mags"][-1] - conv["mags"][-1])) <= 1e-8
-        pp = G.pump_probe_spectroscopy(gl, g, f, 0.5, 0.5, 0.0, 0.0, 0.5, 0.25, -0.5, 1.005, 0.01,
-                                       method="dopri5", abs_tol=1e-8, rel_tol=1e-8)
+        # pulse width 0.05: windows of 0.45 around the centres leave free
+        # segments in [-0.5, 1.005]
+        pp = {c: G.pump_probe_spectroscopy(gl, g, f, 0.5, 0.05, 2.0, 0.0, 0.5, 0.25, -0.5, 1.005, 0.01,
+                                           method="dopri5", abs_tol=1e-8, rel_tol=1e-8, pulse_window_chunking=c)
+              for c in (True, False)}
+        t_sp, m_sp = G.single_pulse_drive(gl, g, f, 0.25, 0.5, 0.05, 2.0, -0.5, 1.005, 0.01, method="dopri5")
     fd = np.asarray(f, dtype=float)
-    for name, t1, t2, a2 in (("M0", 0.0, 0.0, 0.0), ("M1", 0.25, 0.0, 0.0), ("M01", 0.0, 0.25, 0.5)):
-        if name == "M1":
-            o.set_pulse(fd, 0.25, fd * 0.0, 0.0, 0.5, 0.5, 0.0)
-        else:
-            o.set_pulse(fd, 0.0, fd * (1.0 if a2 else 0.0), t2, 0.5, 0.5, 0.0)
-        r = o.integrate_dopri5(g.reshape(-1), -0.5, 1.005, 0.01, control=2, abs_tol=1e-8, rel_tol=1e-8)
-        keep = G._gate(r["times"], -0.5, 1.005, 0.01)
-        got = pp[name] if name == "M0" else pp[name][1]
-        assert np.max(np.abs(got - r["mags"][keep])) <= TOL, name
+    for chunk in (True, False):
+        for name, t1, t2, a2 in (("M0", 0.0, 0.0, 0.0), ("M1", 0.25, 0.0, 0.0), ("M01", 0.0, 0.25, 0.5)):
+            print(f"pump-probe dopri5 {name}, chunking {chunk}", flush=True)
+            if name == "M1":
+                o.set_pulse(fd, 0.25, fd * 0.0, 0.0, 0.5, 0.05, 2.0)
+                c = [0.25]
+            else:
+                o.set_pulse(fd, 0.0, fd * (1.0 if a2 else 0.0), t2, 0.5, 0.05, 2.0)
+                c = [0.0, 0.25] if a2 else [0.0]
+            segs = pyoracle_segments(c) if chunk else None
+            r = o.integrate_dopri5(g.reshape(-1), -0.5, 1.005, 0.01, control=2, abs_tol=1e-8, rel_tol=1e-8,
+                                   segments=segs)
+            keep = G._gate(r["times"], -0.5, 1.005, 0.01)
+            res = pp[chunk]
+            got = res[name] if name == "M0" else res[name][1]
+            assert len(got) == len(keep), name
+            assert np.max(np.abs(got - r["mags"][keep])) <= TOL, name
+            if name == "M0":
+                assert np.max(np.abs(res["times"] - r["times"][keep])) <= 1e-12
+            if name == "M1" and chunk:  # single_pulse_drive at t_B = 0.25, chunked by default
+                # its grid times start from its own segments: equal to M0's up to rounding
+                assert np.max(np.abs(t_sp - res["times"])) <= 1e-12 and np.max(np.abs(m_sp - got)) <= TOL
     o.clear_pulse()
+
+
+def pyoracle_segments(centers):
+    import pyoracle
+    return pyoracle.pulse_segments(-0.5, 1.005, centers, 9 * 0.05, 0.01, 20.0)

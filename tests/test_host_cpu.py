@@ -211,3 +211,27 @@ def test_gpu_grid_policy_matches_reference_loops():
             n += 1
         _capi.check(L.csl_count_records(T0, T1, dt, 1, 1, C.byref(ns), C.byref(nr)))
         assert ns.value == n
+
+
+def test_pulse_segments_product_equals_oracle():
+    """csl_pulse_segments (host code of the product) against the oracle's
+    build_pulse_segments restatement, on the cases the oracle is pinned with
+    (tests/test_oracle_physics.py) and on random windows: identical rows."""
+    import pyoracle as po
+    from classical_mcmc_spin_amd import gpu as G
+    cases = [(-100.0, 100.0, [0.0, 50.0], 4.5, 0.01), (-10.0, 10.0, [0.0, 1.0], 4.5, 0.01),
+             (-10.0, 10.0, [0.0, 5.0], 9 * 0.45, 0.1), (-10.0, 10.0, [0.0, 0.55], 3.6, 0.1),
+             (0.0, 10.013, [0.0, 5.0], 3.6, 0.1), (0.0, 10.0, [100.0], 3.6, 0.1), (0.0, 0.0, [0.0], 1.0, 0.1),
+             (1.0, 0.0, [0.0], 1.0, 0.1), (0.0, 1.0, [0.0], 0.1, 0.0), (-5.0, 20.0, [], 1.0, 0.05)]
+    rng = np.random.default_rng(3)
+    for _ in range(200):
+        T0 = rng.uniform(-20, 0)
+        T1 = T0 + rng.uniform(0.5, 40)
+        cases.append((T0, T1, list(rng.uniform(T0 - 5, T1 + 5, size=rng.integers(1, 4))),
+                      rng.uniform(0.01, 6), rng.choice([0.01, 0.05, 0.1, 0.013])))
+    for T0, T1, c, w, st in cases:
+        a = G.pulse_segments(T0, T1, c, w, st, 20.0)
+        b = po.pulse_segments(T0, T1, c, w, st, 20.0) if c else po.pulse_segments(T0, T1, [1e300], w, st, 20.0)
+        assert a.shape == b.shape and np.array_equal(a, b), (T0, T1, c, w, st)
+    with pytest.raises(_capi.CslError):
+        G.pulse_segments(0.0, float("nan"), [0.0], 1.0, 0.1)

@@ -119,3 +119,80 @@ def test_lattice_controlled_const_grid(lat_pair):
     assert r0["accepted"] == len(r0["times"]) - 1 and r0["rejected"] == 0
     assert np.max(np.abs(r0["mags"] - ref_on_grid)) <= 1e-6
     o.clear_pulse()
+
+
+def test_controlled_const_carries_step_size(lat_pair):
+    """integrate_const with the controlled stepper hands the step size of one
+    observation interval to the next (Boost passes dt by reference through
+    its integrate_adaptive calls): with an interval four times the step the
+    tolerance allows, a controller restarted at dt would reject at least once
+    per interval; the carried step rejects only a few times in total."""
+    lat, o = lat_pair
+    s = random_state(lat, 8)
+    n_int = 40
+    r = o.integrate_dopri5(s.reshape(-1), 0.0, 0.4 * n_int + 1e-3, 0.4, control=2, abs_tol=1e-10, rel_tol=1e-10)
+    assert len(r["times"]) == n_int + 1
+    assert r["accepted"] >= 3 * n_int  # several steps per interval
+    assert r["rejected"] <= n_int // 4
+
+
+def test_single_segment_equals_window(lat_pair):
+    """One segment covering the window is the unchunked integrate_const call."""
+    lat, o = lat_pair
+    s = random_state(lat, 9)
+    f = np.zeros((2, 3))
+    f[:, 1] = 1.0
+    o.set_pulse(f, 0.4, f * 0.0, 0.0, 0.6, 0.25, 2.0)
+    a = o.integrate_dopri5(s.reshape(-1), -0.5, 1.0, 0.05, control=2, abs_tol=1e-8, rel_tol=1e-8)
+    b = o.integrate_dopri5(s.reshape(-1), -0.5, 1.0, 0.05, control=2, abs_tol=1e-8, rel_tol=1e-8,
+                           segments=[[-0.5, 1.0]])
+    o.clear_pulse()
+    for k in ("times", "mags", "final"):
+        assert np.array_equal(a[k], b[k])
+    assert (a["accepted"], a["rejected"]) == (b["accepted"], b["rejected"])
+
+
+def _gate(times, T0, T1, step):
+    keep, last = [], T0
+    for i, t in enumerate(times):
+        if t - last >= step - 1e-10 or t >= T1 - 1e-10:
+            keep.append(i)
+            last = t
+    return np.array(keep)
+
+
+def test_chunked_pulse_drive_regression(lat_pair):
+    """W3 chunking with the controlled stepper (pulse_chunking.h; the
+    reference's test_pulse_drive_tolerance.cpp on the Kitaev lattice): the
+    chunked runs keep the unchunked sample grid, every seam is observed
+    twice before the observer gate, the chunked and unchunked trajectories
+    agree to the tolerance level, and tolerance 1e-8 agrees with 1e-10 to
+    the reference's 5e-6 ceiling at every sample."""
+    lat, o = lat_pair
+    s = random_state(lat, 10)
+    f = np.zeros((2, 3))
+    f[:, 0] = 1.0
+    t_B, amp, width, freq = 1.0, 0.5, 0.2, 4.0
+    T0, T1, st = -1.0, 8.0, 0.05
+    o.set_pulse(f, t_B, f * 0.0, 0.0, amp, width, freq)
+    segs = pyoracle.pulse_segments(T0, T1, [t_B], 9 * width, st, 20.0)
+    assert len(segs) == 3
+
+    def run(tol, chunk):
+        r = o.integrate_dopri5(s.reshape(-1), T0, T1, st, control=2, abs_tol=tol, rel_tol=tol,
+                               segments=segs if chunk else None)
+        k = _gate(r["times"], T0, T1, st)
+        return r, r["times"][k], r["mags"][k]
+
+    r_c, t_c, m_c = run(1e-10, True)
+    r_u, t_u, m_u = run(1e-10, False)
+    r_8, t_8, m_8 = run(1e-8, True)
+    o.clear_pulse()
+    assert len(r_c["times"]) == len(r_u["times"]) + len(segs) - 1
+    # samples -0.95 .. 7.95: the step to 8.0 fails Boost's epsilon test
+    # (-1 + 179 * 0.05) + 0.05 - 8 <= eps in both runs
+    assert len(t_c) == len(t_u) == len(t_8) == 179
+    assert np.max(np.abs(t_c - t_u)) <= 1e-12 and np.max(np.abs(t_8 - t_c)) <= 1e-12
+    assert np.max(np.abs(m_c - m_u)) <= 1e-8
+    d = np.max(np.abs(m_8 - m_c))
+    assert 0.0 < d <= 5e-6
