@@ -132,7 +132,7 @@ def bench_pyrochlore(args):
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": 1e3 * el / args.steps,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
         "data": "synthetic",
-        "config": {"workload": "pyrochlore_16x16x16_conventional_64samples_4096steps_rk4_dt0.02",
+        "config": {"workload": f"pyrochlore_16x16x16_conventional_64samples_{PYRO_STEPS}steps_rk4_dt0.02",
                    "sites": N, "trajectories": PYRO_B, "rk4_steps": PYRO_STEPS, "record_every": 1},
         "site_steps_per_s": samples_s * N * PYRO_STEPS,
         "roofline": {"bound": "hbm", "achieved": bytes_launch / (kms * 1e-3) / 1e9,
@@ -438,7 +438,7 @@ def bench_mixed(args):
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": 1e3 * el / args.steps,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
         "data": "synthetic",
-        "config": {"workload": "2dcs_tmfeo3_mixed_8x8x8_257traj_8192steps_rk4_dt0.005",
+        "config": {"workload": f"2dcs_tmfeo3_mixed_8x8x8_257traj_{MIX_STEPS}steps_rk4_dt0.005",
                    "sites_su2": ml.N2, "sites_su3": ml.N3, "trajectories": B, "rk4_steps": MIX_STEPS,
                    "record_every": 1},
         "site_steps_per_s": traj_s * n_sites * MIX_STEPS,
@@ -487,6 +487,9 @@ def main():
                     help="2dcs: the batched 2DCS workload (default); pyrochlore: 64 samples of the "
                          "65 536-site pyrochlore lattice; mixed: 257-trajectory SU(2)+SU(3) 2DCS batch "
                          "(both on the grid-wide stage kernels)")
+    ap.add_argument("--rk-steps", type=int, default=0,
+                    help="RK4 steps of the pyrochlore / mixed integrations (default 4096 / 8192); "
+                         "shorter runs keep counter-collection passes within a few hundred dispatches")
     ap.add_argument("--profile", action="store_true",
                     help="only the timed batched launches (for rocprofv3 runs): no autotune sweep, "
                          "plain paths, parity check or CPU baseline")
@@ -497,6 +500,9 @@ def main():
               "part of this build.", file=sys.stderr)
         sys.exit(2)
 
+    global PYRO_STEPS, MIX_STEPS
+    if args.rk_steps > 0:
+        PYRO_STEPS = MIX_STEPS = args.rk_steps
     if args.workload == "pyrochlore":
         return bench_pyrochlore(args)
     if args.workload == "mixed":
