@@ -94,6 +94,8 @@ SIGNATURES = {
     "csl_mixed_integrate": (_i, [_vp, _d, _d, _d, _sz, _cs, _i, _sz, _pd, _pd, _pd, _psz]),
     "csl_mixed_integrate_batched": (_i, [_vp, _sz, _pd, _i, _pd, _pd, _d, _d, _d, _sz, _cs, _i, _sz, _pd,
                                          _pd, _pd, _psz]),
+    "csl_mixed_integrate_dopri5": (_i, [_vp, _sz, _pd, _i, _pd, _pd, _d, _d, _d, _i, _d, _d, _sz, _i, _pd, _psz,
+                                        _sz, _sz, _sz, _pd, _pd, _pd, _psz, _psz, _psz]),
     "csl_mixed_scale_gradient": (_i, [_vp, _sz, _pd, _i, _pd, _pd, _d, _d, _d, _sz, _i, _pd, _i, _pd, _pd, _pd,
                                       _sz, _pd, _pd, _psz]),
     "csl_mixed_last_kernel_ms": (_i, [_vp, _pd, _psz]),

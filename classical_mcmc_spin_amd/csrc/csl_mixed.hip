@@ -707,6 +707,514 @@ __global__ void k_mixed_normalize(MixDev M, double* __restrict__ y, double len2,
     }
 }
 
+// ---------------------------------------------------------------------------
+// Dormand-Prince 5(4) on the mixed lattice (MixedLattice::integrate_ode_system
+// with "dopri5", mixed_lattice.h:3960-4010).  The state exceeds one
+// workgroup's LDS, so every attempt is a sequence of grid-wide launches over
+// all trajectories: stages 2..6 (k_s of the stage input, then the next stage
+// input of the owned site), stage 7 (k7 of the candidate, the per-workgroup
+// error maximum and record partials) and one decision workgroup per
+// trajectory (k_mixdp_decide), which accepts or rejects, publishes the
+// observer records and sets up the trajectory's next attempt or derivative
+// evaluation.  Each trajectory keeps its own step-size sequence; its x and
+// k1 live in one of two buffers (par), so accepting a step is a flip.
+
+// Dormand-Prince tableau (Dormand & Prince 1980)
+namespace dp {
+constexpr double c2 = 1.0 / 5.0, c3 = 3.0 / 10.0, c4 = 4.0 / 5.0, c5 = 8.0 / 9.0;
+constexpr double a21 = 1.0 / 5.0;
+constexpr double a31 = 3.0 / 40.0, a32 = 9.0 / 40.0;
+constexpr double a41 = 44.0 / 45.0, a42 = -56.0 / 15.0, a43 = 32.0 / 9.0;
+constexpr double a51 = 19372.0 / 6561.0, a52 = -25360.0 / 2187.0, a53 = 64448.0 / 6561.0, a54 = -212.0 / 729.0;
+constexpr double a61 = 9017.0 / 3168.0, a62 = -355.0 / 33.0, a63 = 46732.0 / 5247.0, a64 = 49.0 / 176.0,
+                 a65 = -5103.0 / 18656.0;
+constexpr double b1 = 35.0 / 384.0, b3 = 500.0 / 1113.0, b4 = 125.0 / 192.0, b5 = -2187.0 / 6784.0,
+                 b6 = 11.0 / 84.0;
+constexpr double e1 = 35.0 / 384.0 - 5179.0 / 57600.0, e3 = 500.0 / 1113.0 - 7571.0 / 16695.0,
+                 e4 = 125.0 / 192.0 - 393.0 / 640.0, e5 = -2187.0 / 6784.0 + 92097.0 / 339200.0,
+                 e6 = 11.0 / 84.0 - 187.0 / 2100.0, e7 = -1.0 / 40.0;
+constexpr double eps = 2.220446049250313e-16;  // numeric_limits<double>::epsilon()
+}  // namespace dp
+
+__device__ __forceinline__ double wave_max_f64(double v) {
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) v = fmax(v, __shfl_xor(v, o, kWave));
+    return v;
+}
+__device__ __forceinline__ bool dp_less(double t1, double t2) { return t2 - t1 > dp::eps; }
+__device__ __forceinline__ bool dp_less_eq(double t1, double t2) { return t1 - t2 <= dp::eps; }
+__device__ __forceinline__ double dp_max(double a, double b) { return a < b ? b : a; }
+__device__ __forceinline__ double dp_grid_time(double start, long long k, double dt) {
+#pragma clang fp contract(off)
+    return start + double(k) * dt;
+}
+
+// Per-trajectory controller state, device resident.
+struct MixDpCtl {
+    int mode;        // 0 finished, 1 attempt, 2 derivative evaluation k1 = f(x, te)
+    int par;         // x and k1 are X[par], K1[par]; the candidate goes to X[1 - par]
+    int obs;         // this iteration's state is observed (attempt: if accepted)
+    int eobs;        // the evaluation is an observer call
+    int ends;        // an accepted attempt ends the run / interval
+    int fresh;       // control 2: at the start of interval kk
+    int fails, status, sg, gi;
+    long long kk, obs_calls, saved, n_acc, n_rej, attempts;
+    double t, h, t7, tobs, te, tend, s0, s1, tcur;
+};
+
+struct MixDpArgs {
+    int stage;                    // 2..7
+    int nblk2;
+    double *X0, *X1, *K10, *K11;  // x and k1, two buffers each
+    double* K[5];                 // k2..k6
+    double *Y0, *Y1;              // stage inputs y3..y6
+    const double* tp;             // [B][2] pulse centres (both species)
+    const double* scale;          // [B][2]
+    double abs_tol, rel_tol;
+    int control;
+    const MixDpCtl* ctl;
+    double* errp;                 // [B][nblk]
+    double* partials;             // [B][nblk][34]
+};
+
+// Stage input of trajectory b: a[i], or a[i] + c b[i] (y2 = x + h a21 k1
+// formed on the fly from x and k1 of the owned site and of the partners).
+struct DpSrc {
+    const double* a;
+    const double* k;
+    double c;
+    __device__ __forceinline__ double operator()(size_t i) const { return k ? a[i] + c * k[i] : a[i]; }
+};
+
+template <int DA, int DB>
+__device__ __forceinline__ void gather_src(const Ell& E, const DpSrc& src, size_t base, int NPsrc, int NPpart,
+                                           int atom, int p, double* H) {
+    const int c = E.uniform ? E.cnt_atom[atom] : E.cnt[p];
+    for (int k = 0; k < c; ++k) {
+        const int q = E.col[size_t(k) * NPsrc + p];
+        double s[DB];
+#pragma unroll
+        for (int e = 0; e < DB; ++e) s[e] = src(base + size_t(e) * NPpart + q);
+        const double* J = E.uniform ? E.mats + (size_t(atom) * E.zmax + k) * (DA * DB)
+                                    : E.mats + (size_t(k) * NPsrc + p) * (DA * DB);
+#pragma unroll
+        for (int a = 0; a < DA; ++a) {
+            double r = 0.0;
+#pragma unroll
+            for (int e = 0; e < DB; ++e) r += J[a * DB + e] * s[e];
+            H[a] += r;
+        }
+    }
+}
+
+__global__ void __launch_bounds__(256) k_mixdp_stage(MixDev M, MixDpArgs A) {
+    __shared__ double fac[4];
+    __shared__ double red[4][kRec];
+    __shared__ double emx[4];
+    int bx, b;
+    mix_block(bx, b);
+    const MixDpCtl& C = A.ctl[b];
+    const int mode = C.mode;
+    const int S = A.stage;
+    if (mode == 0 || (mode == 2 && S != 7)) return;  // uniform per workgroup
+    const bool eval = mode == 2;
+    const int tid = threadIdx.x;
+    const int lane = tid & (kWave - 1);
+    const int wave = uniform(tid >> 6);
+    const bool is3 = bx >= A.nblk2;
+    const int par = C.par;
+    const double h = C.h;
+    const bool rec = C.obs != 0 && S == 7;
+    const size_t off = size_t(b) * mix_stride(M);
+    const double* X = (par ? A.X1 : A.X0) + off;
+    double* XN = (par ? A.X0 : A.X1) + off;
+    double* K1 = (par ? A.K11 : A.K10) + off;
+    double* K7 = (par ? A.K10 : A.K11) + off;
+    double ts;
+    switch (S) {
+        case 2: ts = C.t + dp::c2 * h; break;
+        case 3: ts = C.t + dp::c3 * h; break;
+        case 4: ts = C.t + dp::c4 * h; break;
+        case 5: ts = C.t + dp::c5 * h; break;
+        case 6: ts = C.t + h; break;
+        default: ts = eval ? C.te : C.t7; break;
+    }
+    if (tid < 4) {  // drive envelopes of this trajectory at the stage time
+        const int sp = tid >> 1, k = tid & 1;
+        const double amp = sp ? M.amp3 : M.amp2;
+        double f = 0.0;
+        if (amp != 0.0) {
+            f = envelope(amp, ts, A.tp[size_t(b) * 2 + k], sp ? M.width3 : M.width2, sp ? M.freq3 : M.freq2);
+            f *= A.scale[size_t(b) * 2 + k];
+        }
+        fac[tid] = f;
+    }
+    __syncthreads();
+    DpSrc src;
+    src.k = nullptr;
+    src.c = 0.0;
+    if (S == 2) {
+        src.a = X;
+        src.k = K1;
+        src.c = h * dp::a21;
+    } else if (S <= 6) {
+        src.a = ((S & 1) ? A.Y1 : A.Y0) + off;
+    } else {
+        src.a = eval ? X : XN;
+    }
+    double* ynext = S <= 5 ? (((S + 1) & 1) ? A.Y1 : A.Y0) + off : XN;
+    const double* Kb[5];
+#pragma unroll
+    for (int j = 0; j < 5; ++j) Kb[j] = A.K[j] + off;
+    double part[kRec];
+#pragma unroll
+    for (int j = 0; j < kRec; ++j) part[j] = 0.0;
+    double emax = 0.0;
+    const double ah = fabs(h);
+    // one derivative component kd of plane index q: store it and form the
+    // next stage input / candidate, or (stage 7) the error ratio
+    auto finish = [&](size_t q, double kd) {
+        if (S == 7) {
+            if (eval) {
+                K1[q] = kd;
+            } else {
+                K7[q] = kd;
+                const double k1 = K1[q];
+                const double xerr = h * (dp::e1 * k1 + dp::e3 * Kb[1][q] + dp::e4 * Kb[2][q] + dp::e5 * Kb[3][q] +
+                                         dp::e6 * Kb[4][q] + dp::e7 * kd);
+                const double r = fabs(xerr) / (A.abs_tol + A.rel_tol * (fabs(X[q]) + ah * fabs(k1)));
+                emax = fmax(emax, r);
+            }
+            return;
+        }
+        const double x = X[q], k1 = K1[q];
+        double y;
+        if (S == 2) {
+            y = x + h * (dp::a31 * k1 + dp::a32 * kd);
+        } else if (S == 3) {
+            y = x + h * (dp::a41 * k1 + dp::a42 * Kb[0][q] + dp::a43 * kd);
+        } else if (S == 4) {
+            y = x + h * (dp::a51 * k1 + dp::a52 * Kb[0][q] + dp::a53 * Kb[1][q] + dp::a54 * kd);
+        } else if (S == 5) {
+            y = x + h * (dp::a61 * k1 + dp::a62 * Kb[0][q] + dp::a63 * Kb[1][q] + dp::a64 * Kb[2][q] + dp::a65 * kd);
+        } else {
+            y = x + h * (dp::b1 * k1 + dp::b3 * Kb[1][q] + dp::b4 * Kb[2][q] + dp::b5 * Kb[3][q] + dp::b6 * kd);
+        }
+        A.K[S - 2][off + q] = kd;
+        ynext[q] = y;
+    };
+    const size_t o3 = size_t(3) * M.NP2;
+    if (!is3) {
+        const int p = bx * 256 + tid;
+        const int pw = uniform(bx * 256 + wave * kWave);
+        const int atom = uniform(pw < M.NP2 ? pw / M.npad : 0);
+        if (p < M.NP2 && (p - atom * M.npad) < M.n_cells) {
+            const int NP = M.NP2;
+            const double S3[3] = {src(p), src(size_t(NP) + p), src(size_t(2) * NP + p)};
+            double H[3];
+            field_onsite<3>(M.field2, M.onsite2, M.site_uniform2, M.has_onsite2, atom, p, S3, H);
+            gather_src<3, 3>(M.e22, src, 0, M.NP2, M.NP2, atom, p, H);
+            gather_src<3, 8>(M.e23, src, o3, M.NP2, M.NP3, atom, p, H);
+            if (rec) {
+                const cptr<double> R = as_const(M.frames2) + size_t(atom) * 9;
+                const double sg = M.afm2[atom];
+                const double B0 = M.site_uniform2 ? M.field2[atom * 3 + 0] : M.field2[size_t(p) * 3 + 0];
+                const double B1 = M.site_uniform2 ? M.field2[atom * 3 + 1] : M.field2[size_t(p) * 3 + 1];
+                const double B2 = M.site_uniform2 ? M.field2[atom * 3 + 2] : M.field2[size_t(p) * 3 + 2];
+#pragma unroll
+                for (int mu = 0; mu < 3; ++mu) {
+                    const double g = R[mu * 3] * S3[0] + R[mu * 3 + 1] * S3[1] + R[mu * 3 + 2] * S3[2];
+                    part[mu] = sg * g;
+                    part[3 + mu] = S3[mu];
+                    part[6 + mu] = g;
+                }
+                part[33] = 0.5 * (S3[0] * (H[0] - B0) + S3[1] * (H[1] - B1) + S3[2] * (H[2] - B2));
+            }
+            const double f1 = fac[0], f2 = fac[1];
+            if (f1 != 0.0 || f2 != 0.0) {
+                const cptr<double> d0 = as_const(M.fdrive2) + size_t(atom) * 3;
+                const cptr<double> d1 = as_const(M.fdrive2) + size_t(M.A2 + atom) * 3;
+#pragma unroll
+                for (int a = 0; a < 3; ++a) H[a] -= d0[a] * f1 + d1[a] * f2;
+            }
+            double k[3] = {H[1] * S3[2] - H[2] * S3[1], H[2] * S3[0] - H[0] * S3[2], H[0] * S3[1] - H[1] * S3[0]};
+            if (M.alpha != 0.0) {  // mixed_lattice_md.cpp:570-578
+                const double S2 = S3[0] * S3[0] + S3[1] * S3[1] + S3[2] * S3[2];
+                const double SH = S3[0] * H[0] + S3[1] * H[1] + S3[2] * H[2];
+                const double g = (S2 > 0.0) ? M.alpha / sqrt(S2) : 0.0;
+#pragma unroll
+                for (int a = 0; a < 3; ++a) k[a] += g * (S3[a] * SH - H[a] * S2);
+            }
+#pragma unroll
+            for (int d = 0; d < 3; ++d) finish(size_t(d) * NP + p, k[d]);
+        }
+    } else {
+        const int p = (bx - A.nblk2) * 256 + tid;
+        const int pw = uniform((bx - A.nblk2) * 256 + wave * kWave);
+        const int atom = uniform(pw < M.NP3 ? pw / M.npad : 0);
+        const int cell = p - atom * M.npad;
+        if (p < M.NP3 && cell < M.n_cells) {
+            const int NP = M.NP3;
+            double S8[8];
+#pragma unroll
+            for (int a = 0; a < 8; ++a) S8[a] = src(o3 + size_t(a) * NP + p);
+            double H[8];
+            field_onsite<8>(M.field3, M.onsite3, M.site_uniform3, M.has_onsite3, atom, p, S8, H);
+            gather_src<8, 8>(M.e33, src, o3, M.NP3, M.NP3, atom, p, H);
+            gather_src<8, 3>(M.e32, src, 0, M.NP3, M.NP2, atom, p, H);
+            if (rec) {
+                const cptr<double> R = as_const(M.frames3) + size_t(atom) * 64;
+                const double sg = ((cell * M.A3 + atom) & 1) ? -1.0 : 1.0;
+                double e = 0.0;
+#pragma unroll
+                for (int mu = 0; mu < 8; ++mu) {
+                    double g = 0.0;
+#pragma unroll
+                    for (int nu = 0; nu < 8; ++nu) g += R[mu * 8 + nu] * S8[nu];
+                    part[9 + mu] = sg * S8[mu];
+                    part[17 + mu] = S8[mu];
+                    part[25 + mu] = g;
+                    const double Bm = M.site_uniform3 ? M.field3[atom * 8 + mu] : M.field3[size_t(p) * 8 + mu];
+                    e += S8[mu] * (H[mu] - Bm);
+                }
+                part[33] = 0.5 * e;
+            }
+            const double f1 = fac[2], f2 = fac[3];
+            if (f1 != 0.0 || f2 != 0.0) {
+                const cptr<double> d0 = as_const(M.fdrive3) + size_t(atom) * 8;
+                const cptr<double> d1 = as_const(M.fdrive3) + size_t(M.A3 + atom) * 8;
+#pragma unroll
+                for (int a = 0; a < 8; ++a) H[a] -= d0[a] * f1 + d1[a] * f2;
+            }
+            double k[8];
+            su3_cross(H, S8, k);
+            if (M.has_bloch) {  // mixed_lattice_md.cpp:596-605
+#pragma unroll
+                for (int a = 0; a < 8; ++a)
+                    if (M.gamma[a] != 0.0) k[a] -= M.gamma[a] * (S8[a] - M.eq3[size_t(a) * NP + p]);
+            }
+#pragma unroll
+            for (int d = 0; d < 8; ++d) finish(o3 + size_t(d) * NP + p, k[d]);
+        }
+    }
+    if (S != 7) return;  // uniform
+    const size_t nblk = gridDim.x;
+    if (!eval) {
+        const double e = wave_max_f64(emax);
+        if (lane == 0) emx[wave] = e;
+    }
+    if (rec) {
+        const int j0 = is3 ? 9 : 0, j1 = is3 ? 33 : 9;
+#pragma unroll
+        for (int j = 0; j < kRec; ++j) {
+            if ((j >= j0 && j < j1) || j == 33) {
+                const double v = wave_sum(part[j]);
+                if (lane == 0) red[wave][j] = v;
+            }
+        }
+    }
+    __syncthreads();
+    if (!eval && tid == 0) A.errp[size_t(b) * nblk + bx] = fmax(fmax(emx[0], emx[1]), fmax(emx[2], emx[3]));
+    if (rec && tid < kRec) {
+        const int j0 = is3 ? 9 : 0, j1 = is3 ? 33 : 9;
+        const bool mine = (tid >= j0 && tid < j1) || tid == 33;
+        const double v = mine ? (red[0][tid] + red[1][tid]) + (red[2][tid] + red[3][tid]) : 0.0;
+        A.partials[(size_t(b) * nblk + bx) * kRec + tid] = v;
+    }
+}
+
+struct MixDpDecideArgs {
+    int nblk, control, save_interval, n_grid, max_seg, max_rec;
+    long long max_attempts;
+    double T0, T1, dt, inv2, inv3, invE;
+    const double* tgrid;      // control 0
+    const double* segs;       // control 2: [B][max_seg][2] or null (one segment [T0, T1])
+    const int* n_segs;
+    const double* tp;         // [B][2]
+    const double* scale;      // [B][2]
+    double amp2, amp3;
+    MixDpCtl* ctl;
+    const double* errp;
+    const double* partials;
+    double* recs;             // [B][max_rec][34]
+    double* times;            // [B][max_rec]
+};
+
+// Decision of one trajectory after an iteration, then the set-up of its next
+// attempt or evaluation (the loop of k_dopri5 split at the launches).
+__global__ void __launch_bounds__(256) k_mixdp_decide(MixDpDecideArgs D) {
+    __shared__ double red[kRec][4];
+    __shared__ double emx[4];
+    __shared__ long long pub_ri;
+    __shared__ double pub_t;
+    const int b = blockIdx.x;
+    const int tid = threadIdx.x;
+    const int lane = tid & (kWave - 1);
+    const int wave = tid >> 6;
+    MixDpCtl* cp = D.ctl + b;
+    const int mode = cp->mode;
+    if (mode == 0) return;  // uniform
+    const bool attempt = mode == 1;
+    const bool ob = cp->obs != 0;
+    // workgroup maximum of the error ratios, sums of the record partials
+    double e = 0.0;
+    if (attempt && D.control != 0)
+        for (int k = tid; k < D.nblk; k += 256) e = fmax(e, D.errp[size_t(b) * D.nblk + k]);
+    e = wave_max_f64(e);
+    if (lane == 0) emx[wave] = e;
+    if (ob) {
+        double v[kRec];
+#pragma unroll
+        for (int j = 0; j < kRec; ++j) v[j] = 0.0;
+        for (int k = tid; k < D.nblk; k += 256) {
+            const double* q = D.partials + (size_t(b) * D.nblk + k) * kRec;
+#pragma unroll
+            for (int j = 0; j < kRec; ++j) v[j] += q[j];
+        }
+#pragma unroll
+        for (int j = 0; j < kRec; ++j) {
+            v[j] = wave_sum(v[j]);
+            if (lane == 0) red[j][wave] = v[j];
+        }
+    }
+    __syncthreads();
+    if (tid == 0) {
+        MixDpCtl c = *cp;
+        const double err = fmax(fmax(emx[0], emx[1]), fmax(emx[2], emx[3]));
+        long long ri = -1;
+        double rt = 0.0;
+        bool done = false;
+        const double dt = D.dt;
+        const double* sg = D.segs ? D.segs + size_t(b) * D.max_seg * 2 : nullptr;
+        const int nseg = D.control == 2 ? (D.segs ? D.n_segs[b] : 1) : 0;
+        const bool drive_on = (D.amp2 != 0.0 || D.amp3 != 0.0) &&
+                              (D.scale[size_t(b) * 2] != 0.0 || D.scale[size_t(b) * 2 + 1] != 0.0);
+        // the iteration that just ran
+        if (!attempt) {
+            if (c.eobs) {
+                if (ob) {
+                    ri = c.saved++;
+                    rt = c.te;
+                }
+                ++c.obs_calls;
+            }
+        } else if (D.control != 0 && !(err <= 1.0)) {
+            c.h = c.h * dp_max(0.9 * pow(err, -1.0 / 3.0), 0.2);
+            ++c.n_rej;
+            if (++c.fails >= 500) {
+                c.status = 1;
+                done = true;
+            }
+        } else {
+            c.fails = 0;
+            c.par ^= 1;
+            if (D.control != 0) {
+                c.t = c.t + c.h;
+                if (err < 0.5) c.h = c.h * (0.9 * pow(dp_max(1.0 / 3125.0, err), -1.0 / 5.0));
+            }
+            ++c.n_acc;
+            if (c.ends) {
+                if (c.obs) {
+                    ri = c.saved++;
+                    rt = c.tobs;
+                }
+                ++c.obs_calls;
+                if (D.control == 0) ++c.gi;
+                if (D.control == 2) {
+                    ++c.kk;
+                    c.tcur = c.tobs;
+                    c.fresh = 1;
+                }
+            }
+        }
+        // the next attempt or evaluation
+        c.mode = 0;
+        while (!done) {
+            if (D.control == 0) {
+                if (c.gi + 1 >= D.n_grid) break;
+                c.t = D.tgrid[c.gi];
+                c.h = dt;
+                c.t7 = D.tgrid[c.gi + 1];
+                c.tobs = c.t7;
+                c.ends = 1;
+            } else {
+                if (D.control == 1) {
+                    if (!dp_less(c.t, D.T1)) break;
+                    c.tend = D.T1;
+                } else if (c.fresh) {
+                    if (!dp_less_eq(c.tcur + dt, c.s1)) {
+                        // segment done: the next integrate_const call
+                        if (++c.sg >= nseg) break;
+                        c.s0 = sg[size_t(c.sg) * 2];
+                        c.s1 = sg[size_t(c.sg) * 2 + 1];
+                        c.kk = 0;
+                        c.t = c.tcur = c.te = c.s0;
+                        c.h = dt;
+                        c.mode = 2;
+                        c.eobs = 1;
+                        break;
+                    }
+                    c.tend = c.tcur + dt;
+                    if (!dp_less(c.tcur, c.tend)) {
+                        ++c.kk;
+                        c.t = c.tcur = c.te = dp_grid_time(c.s0, c.kk, dt);
+                        c.mode = 2;
+                        c.eobs = 1;
+                        break;
+                    }
+                    c.fresh = 0;
+                    c.fails = 0;
+                    if (drive_on && c.t != c.tcur) {
+                        c.t = c.te = c.tcur;
+                        c.mode = 2;
+                        c.eobs = 0;
+                        break;
+                    }
+                    c.t = c.tcur;
+                }
+                if (c.fails == 0 && dp_less(c.tend, c.t + c.h)) c.h = c.tend - c.t;
+                if (!(c.h > 0.0) || !(c.t + c.h > c.t)) {
+                    c.status = 3;
+                    break;
+                }
+                c.t7 = c.t + c.h;
+                c.tobs = D.control == 1 ? c.t7 : dp_grid_time(c.s0, c.kk + 1, dt);
+                c.ends = (D.control == 1 || !dp_less(c.t + c.h, c.tend)) ? 1 : 0;
+            }
+            if (c.attempts >= D.max_attempts) {
+                c.status = 2;
+                break;
+            }
+            ++c.attempts;
+            c.mode = 1;
+            break;
+        }
+        if (c.mode == 1) c.obs = (c.ends && (c.obs_calls % D.save_interval) == 0) ? 1 : 0;
+        if (c.mode == 2) c.obs = (c.eobs && (c.obs_calls % D.save_interval) == 0) ? 1 : 0;
+        *cp = c;
+        pub_ri = (ri >= 0 && ri < D.max_rec) ? ri : -1;
+        pub_t = rt;
+    }
+    __syncthreads();
+    const long long ri = pub_ri;
+    if (ri >= 0 && tid < kRec) {
+        const double tot = (red[tid][0] + red[tid][1]) + (red[tid][2] + red[tid][3]);
+        const double sc = tid < 9 ? D.inv2 : (tid < 33 ? D.inv3 : D.invE);
+        const size_t r = size_t(b) * D.max_rec + size_t(ri);
+        D.recs[r * kRec + tid] = tot * sc;
+        if (tid == 0) D.times[r] = pub_t;
+    }
+}
+
+// X0 <- X1 for the trajectories whose final state is in X1
+__global__ void k_mixdp_final(const MixDpCtl* __restrict__ ctl, const double* __restrict__ X1,
+                              double* __restrict__ X0, size_t sz) {
+    const int b = blockIdx.y;
+    if (ctl[b].par == 0) return;
+    const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < sz) X0[size_t(b) * sz + i] = X1[size_t(b) * sz + i];
+}
+
 }  // namespace csl
 
 using namespace cslh;
@@ -830,6 +1338,11 @@ struct csl_mixed {
     bool has_state = false;
     DBuf<double> X, ACC, Y1, Y2, partials, recs, tp, tp3, scale, init, flat, aux;
     DBuf<double> ZX, ZACC, ZY1, ZY2, tpartials, trecs, zaux, target, loss, grad;  // tangent-linear runs
+    // Dormand-Prince runs: x / k1 pairs, k2..k6, stage inputs, controller state
+    DBuf<double> dpX0, dpX1, dpK10, dpK11, dpK2, dpK3, dpK4, dpK5, dpK6, dpY0, dpY1, dp_errp, dp_recs, dp_times,
+        dp_segs, dp_grid;
+    DBuf<int> dp_nseg;
+    DBuf<csl::MixDpCtl> dp_ctl;
     double h_tp2[2] = {0.0, 0.0}, h_tp3[2] = {0.0, 0.0};  // single-trajectory pulse centres
     double last_ms = 0.0;
     size_t last_launches = 0;
@@ -1386,6 +1899,217 @@ int csl_mixed_integrate_batched(csl_mixed* h, size_t B, const double* init, int 
         hip_check(hipStreamSynchronize(s), "integrate_batched");
         if (times)
             for (size_t r = 0; r < n_rec; ++r) times[r] = grid[std::min(r * save_interval, n_steps)];
+    });
+}
+
+int csl_mixed_integrate_dopri5(csl_mixed* h, size_t B, const double* init, int init_per_replica,
+                               const double* t_pulse, const double* pulse_scale, double T0, double T1, double dt,
+                               int control, double abs_tol, double rel_tol, size_t save_interval, int tgrid,
+                               const double* segments, const size_t* n_segments, size_t max_segments,
+                               size_t max_attempts, size_t max_rec, double* times, double* recs,
+                               double* final_states, size_t* n_rec_out, size_t* n_acc_out, size_t* n_rej_out) {
+    return guarded([&] {
+        check_mixed(h);
+        if (B == 0) fail(CSL_ERR_ARG, "batch size must be >= 1");
+        if (B > 65535) fail(CSL_ERR_ARG, "at most 65535 trajectories per call");
+        if (!init || !t_pulse) fail(CSL_ERR_NULL, "null init / t_pulse");
+        if (control < 0 || control > 2) fail(CSL_ERR_ARG, "control must be 0, 1 or 2");
+        if (save_interval == 0 || save_interval > size_t(std::numeric_limits<int>::max()))
+            fail(CSL_ERR_ARG, "save_interval must be >= 1");
+        if (!(dt > 0.0) || !std::isfinite(dt) || !std::isfinite(T0) || !std::isfinite(T1))
+            fail(CSL_ERR_ARG, "need finite T0, T1 and dt > 0");
+        if (control != 0 && (!(abs_tol >= 0.0) || !(rel_tol >= 0.0) || !(abs_tol + rel_tol > 0.0)))
+            fail(CSL_ERR_ARG, "tolerances must be >= 0 and not both 0");
+        if (max_rec == 0 || max_rec > size_t(std::numeric_limits<int>::max()))
+            fail(CSL_ERR_ARG, "max_rec must be in [1, 2^31)");
+        std::vector<double> grid;
+        if (control == 0) {
+            grid = make_grid(T0, T1, dt, tgrid);
+            if (grid.size() > size_t(std::numeric_limits<int>::max())) fail(CSL_ERR_ARG, "too many grid points");
+        }
+        std::vector<double> sg;
+        std::vector<int> nsg;
+        if (segments) {
+            if (control != 2) fail(CSL_ERR_ARG, "segments apply to control 2");
+            if (!n_segments) fail(CSL_ERR_NULL, "null n_segments");
+            if (max_segments == 0 || max_segments > 4096) fail(CSL_ERR_ARG, "max_segments must be in [1, 4096]");
+            check_finite(segments, B * max_segments * 2, "segments");
+            sg.assign(segments, segments + B * max_segments * 2);
+            nsg.resize(B);
+            for (size_t b = 0; b < B; ++b) {
+                if (n_segments[b] == 0 || n_segments[b] > max_segments)
+                    fail(CSL_ERR_ARG, "every trajectory needs 1..max_segments segments");
+                nsg[b] = int(n_segments[b]);
+                for (size_t j = 0; j < n_segments[b]; ++j)
+                    if (!(sg[(b * max_segments + j) * 2 + 1] > sg[(b * max_segments + j) * 2]))
+                        fail(CSL_ERR_ARG, "a segment must end after it starts");
+            }
+        }
+        const size_t nf = h->nflat();
+        const size_t n_init = init_per_replica ? B * nf : nf;
+        check_finite(init, n_init, "initial state");
+        check_finite(t_pulse, 2 * B, "t_pulse");
+        hipStream_t s = h->stream;
+        h->init.upload(init, n_init, s);
+        h->tp.upload(t_pulse, 2 * B, s);
+        std::vector<double> sc(2 * B, 1.0);
+        if (pulse_scale) {
+            check_finite(pulse_scale, 2 * B, "pulse_scale");
+            sc.assign(pulse_scale, pulse_scale + 2 * B);
+        }
+        h->scale.upload(sc.data(), sc.size(), s);
+        if (!grid.empty()) h->dp_grid.upload(grid.data(), grid.size(), s);
+        if (!sg.empty()) {
+            h->dp_segs.upload(sg.data(), sg.size(), s);
+            h->dp_nseg.upload(nsg.data(), nsg.size(), s);
+        }
+        const size_t plane = B * h->sz();
+        for (DBuf<double>* q : {&h->dpX0, &h->dpX1, &h->dpK10, &h->dpK11, &h->dpK2, &h->dpK3, &h->dpK4, &h->dpK5,
+                                &h->dpK6, &h->dpY0, &h->dpY1})
+            q->ensure(plane);
+        // padding entries of the SoA planes stay zero in every buffer
+        for (DBuf<double>* q : {&h->dpX0, &h->dpX1, &h->dpY0, &h->dpY1})
+            hip_check(hipMemsetAsync(q->p, 0, plane * sizeof(double), s), "memset");
+        to_soa(h, h->init.p, init_per_replica ? nf : 0, h->dpX0.p, B);
+        const int nblk2 = blocks_of(h->M.NP2), nblk3 = blocks_of(h->M.NP3);
+        const int nblk = nblk2 + nblk3;
+        h->dp_errp.ensure(B * size_t(nblk));
+        h->partials.ensure(B * size_t(nblk) * CSL_MIXED_RECORD);
+        h->dp_recs.ensure(B * max_rec * CSL_MIXED_RECORD);
+        h->dp_times.ensure(B * max_rec);
+        // initial controller state: the derivative at the start with the
+        // first observer call
+        std::vector<csl::MixDpCtl> ctl(B);
+        for (size_t b = 0; b < B; ++b) {
+            csl::MixDpCtl& c = ctl[b];
+            std::memset(&c, 0, sizeof(c));
+            c.s0 = sg.empty() ? T0 : sg[b * max_segments * 2];
+            c.s1 = sg.empty() ? T1 : sg[b * max_segments * 2 + 1];
+            if (control != 2) c.s0 = T0;
+            c.t = c.tcur = c.te = c.s0;
+            c.tend = T1;
+            c.h = dt;
+            c.mode = 2;
+            c.eobs = 1;
+            c.obs = 1;
+            c.fresh = 1;
+        }
+        h->dp_ctl.upload(ctl.data(), B, s);
+        csl::MixDpArgs A{};
+        A.nblk2 = nblk2;
+        A.X0 = h->dpX0.p;
+        A.X1 = h->dpX1.p;
+        A.K10 = h->dpK10.p;
+        A.K11 = h->dpK11.p;
+        A.K[0] = h->dpK2.p;
+        A.K[1] = h->dpK3.p;
+        A.K[2] = h->dpK4.p;
+        A.K[3] = h->dpK5.p;
+        A.K[4] = h->dpK6.p;
+        A.Y0 = h->dpY0.p;
+        A.Y1 = h->dpY1.p;
+        A.tp = h->tp.p;
+        A.scale = h->scale.p;
+        A.abs_tol = abs_tol;
+        A.rel_tol = rel_tol;
+        A.control = control;
+        A.ctl = h->dp_ctl.p;
+        A.errp = h->dp_errp.p;
+        A.partials = h->partials.p;
+        csl::MixDpDecideArgs D{};
+        D.nblk = nblk;
+        D.control = control;
+        D.save_interval = int(save_interval);
+        D.n_grid = int(grid.size());
+        D.max_seg = int(max_segments);
+        D.max_rec = int(max_rec);
+        D.max_attempts = max_attempts ? (long long)std::min<size_t>(max_attempts, size_t(1) << 62) : 50000000LL;
+        D.T0 = T0;
+        D.T1 = T1;
+        D.dt = dt;
+        D.inv2 = 1.0 / double(h->N2);
+        D.inv3 = 1.0 / double(h->N3);
+        D.invE = 1.0 / double(h->N2 + h->N3);
+        D.tgrid = grid.empty() ? nullptr : h->dp_grid.p;
+        D.segs = sg.empty() ? nullptr : h->dp_segs.p;
+        D.n_segs = sg.empty() ? nullptr : h->dp_nseg.p;
+        D.tp = h->tp.p;
+        D.scale = h->scale.p;
+        D.amp2 = h->M.amp2;
+        D.amp3 = h->M.amp3;
+        D.ctl = h->dp_ctl.p;
+        D.errp = h->dp_errp.p;
+        D.partials = h->partials.p;
+        D.recs = h->dp_recs.p;
+        D.times = h->dp_times.p;
+        const dim3 grid_dim{unsigned(nblk), unsigned(B), 1u};
+        // every iteration advances each unfinished trajectory by one attempt
+        // or one evaluation; each attempt counts against max_attempts and
+        // each evaluation belongs to an observer interval or a segment, so
+        // the loop is bounded by the slowest trajectory's work
+        const double n_int = std::abs(T1 - T0) / dt;
+        const size_t it_cap = size_t(D.max_attempts) + 2 * size_t(std::min(n_int, 1e12)) + 2 * max_segments + 16;
+        hipEvent_t e0, e1;
+        hip_check(hipEventCreate(&e0), "event");
+        hip_check(hipEventCreate(&e1), "event");
+        hip_check(hipEventRecord(e0, s), "event record");
+        std::vector<csl::MixDpCtl> back(B);
+        size_t it = 0, launches = 0;
+        bool running = true;
+        while (running) {
+            const size_t batch = it < 64 ? 8 : 64;  // iterations between host checks
+            for (size_t k = 0; k < batch; ++k) {
+                for (int S = 2; S <= 7; ++S) {
+                    A.stage = S;
+                    hipLaunchKernelGGL(csl::k_mixdp_stage, grid_dim, dim3(256), 0, s, h->M, A);
+                }
+                hipLaunchKernelGGL(csl::k_mixdp_decide, dim3(unsigned(B)), dim3(256), 0, s, D);
+                launches += 7;
+            }
+            hip_check(hipGetLastError(), "k_mixdp launch");
+            it += batch;
+            hip_check(hipMemcpyAsync(back.data(), h->dp_ctl.p, B * sizeof(csl::MixDpCtl), hipMemcpyDeviceToHost, s),
+                      "D2H");
+            hip_check(hipStreamSynchronize(s), "mixed dopri5");
+            running = false;
+            for (size_t b = 0; b < B; ++b) running |= back[b].mode != 0;
+            if (running && it > it_cap) fail(CSL_ERR_STEP, "the Dormand-Prince iterations did not finish");
+        }
+        hip_check(hipEventRecord(e1, s), "event record");
+        {
+            dim3 g(unsigned((h->sz() + 255) / 256), unsigned(B));
+            hipLaunchKernelGGL(csl::k_mixdp_final, g, dim3(256), 0, s, h->dp_ctl.p, h->dpX1.p, h->dpX0.p, h->sz());
+            hip_check(hipGetLastError(), "k_mixdp_final");
+        }
+        copy_back(times, h->dp_times.p, B * max_rec, s);
+        copy_back(recs, h->dp_recs.p, B * max_rec * CSL_MIXED_RECORD, s);
+        if (final_states) {
+            h->flat.ensure(B * nf);
+            to_flat(h, h->dpX0.p, h->flat.p, B);
+            copy_back(final_states, h->flat.p, B * nf, s);
+        }
+        hip_check(hipStreamSynchronize(s), "mixed dopri5");
+        float ms = 0.f;
+        hip_check(hipEventElapsedTime(&ms, e0, e1), "elapsed");
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        h->last_ms = launches ? ms / double(launches) : 0.0;
+        h->last_launches = launches;
+        bool over = false;
+        for (size_t b = 0; b < B; ++b) {
+            const csl::MixDpCtl& c = back[b];
+            if (n_rec_out) n_rec_out[b] = size_t(c.saved);
+            if (n_acc_out) n_acc_out[b] = size_t(c.n_acc);
+            if (n_rej_out) n_rej_out[b] = size_t(c.n_rej);
+            over |= size_t(c.saved) > max_rec;
+        }
+        for (size_t b = 0; b < B; ++b) {
+            const int st = back[b].status;
+            if (st == 1) fail(CSL_ERR_STEP, "trajectory " + std::to_string(b) + ": step size control failed (500 rejections)");
+            if (st == 2) fail(CSL_ERR_STEP, "trajectory " + std::to_string(b) + ": more than max_attempts attempts");
+            if (st == 3) fail(CSL_ERR_STEP, "trajectory " + std::to_string(b) + ": step size no longer advances time");
+        }
+        if (over) fail(CSL_ERR_CAPACITY, "max_rec is smaller than the number of records of some trajectory");
     });
 }
 

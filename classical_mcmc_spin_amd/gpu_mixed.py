@@ -4,7 +4,7 @@ MixedLattice simulator entry points on it.
 GPUMixedLattice wraps one csl_mixed handle (include/csl_mixed.h).  The
 drivers mirror src/core/mixed_lattice_md.cpp and mixed_lattice.h:
 
-  molecular_dynamics        MixedLattice::molecular_dynamics_cpu, "rk4"
+  molecular_dynamics        MixedLattice::molecular_dynamics_cpu, "rk4" or "dopri5"
                             (mixed_lattice_md.cpp:1208-1333)
   single_pulse_drive        MixedLattice::single_pulse_drive (mixed_lattice.h:4754-4881)
   double_pulse_drive        MixedLattice::double_pulse_drive (mixed_lattice.h:4883-)
@@ -16,7 +16,8 @@ Records are the observer's [M_SU2 (antiferro, local, global), M_SU3
 with dt = step_size and the pulse drivers' observer gate (the t = T_start
 sample is dropped, mixed_lattice.h:4797-4799).  The reference's pulse-window
 chunking integrates grid-snapped segments with the same step, so for RK4 it
-gives the same samples as one unchunked run.
+gives the same samples as one unchunked run; with "dopri5" every segment is
+its own integrate_const call of the controlled stepper.
 """
 from __future__ import annotations
 
@@ -60,6 +61,7 @@ class GPUMixedLattice:
         args = [szptr(a) if a.dtype == np.uintp else dptr(a) for a in keep]
         check(lib().csl_mixed_create(self.A2, self.A3, n_cells, *args, C.byref(h)))
         self.h = h
+        self.pulse_centres = (0.0, 0.0)
         check(lib().csl_mixed_set_frames(self.h, dptr(f64(s2.frames)), dptr(f64(s2.afm_signs)),
                                          dptr(f64(s3.frames))))
 
@@ -103,6 +105,7 @@ class GPUMixedLattice:
     def set_pulse(self, species: int, drive, amp, width, freq, t1, t2):
         """set_gpu_pulse_SU2 / SU3: drive [2, n_atoms, D] already F^T B."""
         check(lib().csl_mixed_set_pulse(self.h, species, dptr(f64(drive).reshape(-1)), amp, width, freq, t1, t2))
+        self.pulse_centres = (float(t1), float(t2))
 
     def set_damping(self, alpha=0.0, gamma=None, eq=None):
         check(lib().csl_mixed_set_damping(self.h, alpha, dptr(None if gamma is None else f64(gamma)),
@@ -170,6 +173,64 @@ class GPUMixedLattice:
         return {"times": times, "records": recs.reshape(nr, B, REC),
                 "final": None if final is None else final.reshape(B, self.n_state)}
 
+    def integrate_dopri5(self, init, t_pulse, pulse_scale=None, T0=0.0, T1=1.0, dt=0.01, control=1,
+                         abs_tol=1e-6, rel_tol=1e-6, save_interval=1, tgrid="boost", segments=None,
+                         max_rec=None, max_attempts=0, want_final=False):
+        """csl_mixed_integrate_dopri5: Dormand-Prince 5(4) for B trajectories
+        (each its own step-size sequence), controls as
+        GPULattice.integrate_dopri5; segments: B sequences of (t0, t1[, dt])
+        rows for control 2.  Returns dict(times, records: lists of
+        per-trajectory arrays [n_b], [n_b, 34], final [B, n_state], n_rec,
+        accepted, rejected)."""
+        from .gpu import _const_calls
+        tp = f64(t_pulse).reshape(-1, 2)
+        B = tp.shape[0]
+        init = f64(init)
+        per = 1 if init.ndim == 2 else 0
+        if per and init.shape[0] != B:
+            raise ValueError("init has a different batch size than t_pulse")
+        sc = None if pulse_scale is None else f64(pulse_scale).reshape(B, 2)
+        sg = nsg = None
+        ms = 0
+        if segments is not None:
+            rows = [np.asarray(r, dtype=float).reshape(-1, np.shape(r)[-1])[:, :2] for r in segments]
+            if len(rows) != B:
+                raise ValueError("segments needs one sequence per trajectory")
+            ms = max(1, max(len(r) for r in rows))
+            sg = np.zeros((B, ms, 2))
+            for b, r in enumerate(rows):
+                sg[b, :len(r)] = r
+            nsg = np.array([len(r) for r in rows], dtype=np.uintp)
+        if max_rec is None:
+            if sg is not None:
+                max_rec = max(sum(_const_calls(t0, t1, dt) for t0, t1 in r) for r in rows)
+            else:
+                _, nr = _count(T0, T1, dt, 1, tgrid)
+                max_rec = nr + 1 if control != 1 else max(64, 2 * nr)
+        while True:
+            times = np.empty(B * max_rec)
+            recs = np.empty(B * max_rec * REC)
+            final = np.empty(B * self.n_state) if want_final else None
+            nrec = np.zeros(B, dtype=np.uintp)
+            nacc = np.zeros(B, dtype=np.uintp)
+            nrej = np.zeros(B, dtype=np.uintp)
+            rc = lib().csl_mixed_integrate_dopri5(
+                self.h, B, dptr(init.reshape(-1)), per, dptr(tp.reshape(-1)),
+                dptr(None if sc is None else sc.reshape(-1)), T0, T1, dt, control, abs_tol, rel_tol, save_interval,
+                _capi.TGRID[tgrid], dptr(None if sg is None else sg.reshape(-1)), szptr(nsg), ms, max_attempts,
+                max_rec, dptr(times), dptr(recs), dptr(final), szptr(nrec), szptr(nacc), szptr(nrej))
+            if rc == _capi.CSL_ERR_CAPACITY and int(nrec.max()) > max_rec:
+                max_rec = int(nrec.max())
+                continue
+            check(rc)
+            break
+        n = [int(v) for v in nrec]
+        t2 = times.reshape(B, max_rec)
+        r2 = recs.reshape(B, max_rec, REC)
+        return {"times": [t2[b, :n[b]] for b in range(B)], "records": [r2[b, :n[b]] for b in range(B)],
+                "final": None if final is None else final.reshape(B, self.n_state), "n_rec": n,
+                "accepted": [int(v) for v in nacc], "rejected": [int(v) for v in nrej]}
+
     def scale_gradient(self, init, t_pulse, target, weights, pulse_scale=None, T0=0.0, T1=1.0, dt=0.01,
                        save_interval=1, tgrid="boost", want_records=False):
         """csl_mixed_scale_gradient: per trajectory the loss
@@ -232,12 +293,26 @@ def _reset(gm: GPUMixedLattice):
     gm.set_pulse(3, np.zeros((2, gm.A3, 8)), 0.0, 1.0, 0.0, 0.0, 0.0)
 
 
-def molecular_dynamics(gm: GPUMixedLattice, flat, T_start, T_end, dt, save_interval=1, method="rk4"):
-    """MixedLattice::molecular_dynamics_cpu with "rk4": a record every
-    save_interval integrate_const calls (HDF5 output omitted)."""
-    if method != "rk4":
-        raise _capi.CslError(_capi.CSL_ERR_UNSUPPORTED, "only method 'rk4' is implemented")
+def molecular_dynamics(gm: GPUMixedLattice, flat, T_start, T_end, dt, save_interval=1, method="rk4",
+                       abs_tol=-1.0, rel_tol=-1.0):
+    """MixedLattice::molecular_dynamics_cpu (mixed_lattice_md.cpp:1208-1333)
+    without the HDF5 file: "rk4" records every save_interval integrate_const
+    calls; "dopri5" (the reference default) runs integrate_adaptive with
+    the controlled Dormand-Prince stepper and records every save_interval-th
+    accepted step; tolerances <= 0 fall back to 1e-6
+    (get_integration_tolerances, mixed_lattice.h:4296-4301)."""
     _reset(gm)
+    if method == "dopri5":
+        a = abs_tol if abs_tol > 0.0 else 1e-6
+        r_ = rel_tol if rel_tol > 0.0 else 1e-6
+        r = gm.integrate_dopri5(np.asarray(flat, float).reshape(-1), [[0.0, 0.0]], None, T_start, T_end, dt,
+                                control=1, abs_tol=a, rel_tol=r_, save_interval=save_interval)
+        out = split_records(r["records"][0])
+        out["times"] = r["times"][0]
+        out["accepted"], out["rejected"] = r["accepted"][0], r["rejected"][0]
+        return out
+    if method != "rk4":
+        raise _capi.CslError(_capi.CSL_ERR_UNSUPPORTED, "methods: 'rk4', 'dopri5'")
     gm.set_spins(flat)
     r = gm.integrate(T_start, T_end, dt, save_interval, method)
     out = split_records(r["records"])
@@ -246,28 +321,57 @@ def molecular_dynamics(gm: GPUMixedLattice, flat, T_start, T_end, dt, save_inter
 
 
 def single_pulse_drive(gm, flat, field_in_SU2, field_in_SU3, t_B, amp2, w2, fr2, amp3, w3, fr3,
-                       T_start, T_end, step_size):
-    """MixedLattice::single_pulse_drive: returns dict(times, M2, M3, energy)."""
+                       T_start, T_end, step_size, method="rk4", pulse_window_chunking=True,
+                       abs_tol=1e-8, rel_tol=1e-8):
+    """MixedLattice::single_pulse_drive (mixed_lattice.h:4754-4881): returns
+    dict(times, M2, M3, energy).  "dopri5": integrate_const with the
+    controlled stepper, by pulse-window segments (window 9 max(w2, w3))
+    with pulse_window_chunking."""
     z2, z3 = np.zeros((gm.A2, 3)), np.zeros((gm.A3, 8))
     _set_pulses(gm, field_in_SU2, z2, field_in_SU3, z3, t_B, 0.0, amp2, w2, fr2, amp3, w3, fr3)
-    return _gated_run(gm, flat, T_start, T_end, step_size)
+    return _gated_run(gm, flat, T_start, T_end, step_size, method, (t_B,), max(w2, w3), pulse_window_chunking,
+                      abs_tol, rel_tol)
 
 
 def double_pulse_drive(gm, flat, field_in_SU2_1, field_in_SU3_1, t_B_1, field_in_SU2_2, field_in_SU3_2, t_B_2,
-                       amp2, w2, fr2, amp3, w3, fr3, T_start, T_end, step_size):
-    """MixedLattice::double_pulse_drive: returns dict(times, M2, M3, energy)."""
+                       amp2, w2, fr2, amp3, w3, fr3, T_start, T_end, step_size, method="rk4",
+                       pulse_window_chunking=True, abs_tol=1e-8, rel_tol=1e-8):
+    """MixedLattice::double_pulse_drive: returns dict(times, M2, M3, energy);
+    the segments surround both pulse centres."""
     _set_pulses(gm, field_in_SU2_1, field_in_SU2_2, field_in_SU3_1, field_in_SU3_2, t_B_1, t_B_2, amp2, w2, fr2,
                 amp3, w3, fr3)
-    return _gated_run(gm, flat, T_start, T_end, step_size)
+    return _gated_run(gm, flat, T_start, T_end, step_size, method, (t_B_1, t_B_2), max(w2, w3),
+                      pulse_window_chunking, abs_tol, rel_tol)
 
 
-def _gated_run(gm, flat, T_start, T_end, step):
-    gm.set_spins(flat)
-    r = gm.integrate(T_start, T_end, step, 1, "rk4")
+def _gated_run(gm, flat, T_start, T_end, step, method="rk4", centers=(), width=1.0, chunking=True,
+               abs_tol=1e-8, rel_tol=1e-8):
+    if method == "dopri5":
+        from .gpu import PULSE_WINDOW_WIDTHS, pulse_segments
+        segs = None
+        if chunking:
+            segs = pulse_segments(T_start, T_end, centers, PULSE_WINDOW_WIDTHS * width, step)
+            if len(segs) == 0:
+                _reset(gm)
+                out = split_records(np.empty((0, REC)))
+                out["times"] = np.empty(0)
+                return out
+            segs = [segs]
+        # the handle's pulse centres are its own t_pulse (set by _set_pulses)
+        tp = [[gm.pulse_centres[0], gm.pulse_centres[1]]]
+        r = gm.integrate_dopri5(np.asarray(flat, float).reshape(-1), tp, None, T_start, T_end, step, control=2,
+                                abs_tol=abs_tol, rel_tol=rel_tol, segments=segs)
+        times, recs = r["times"][0], r["records"][0]
+    elif method == "rk4":
+        gm.set_spins(flat)
+        r = gm.integrate(T_start, T_end, step, 1, "rk4")
+        times, recs = r["times"], r["records"]
+    else:
+        raise _capi.CslError(_capi.CSL_ERR_UNSUPPORTED, "methods: 'rk4', 'dopri5'")
     _reset(gm)
-    keep = _gate(r["times"], T_start, T_end, step)
-    out = split_records(r["records"][keep])
-    out["times"] = r["times"][keep]
+    keep = _gate(times, T_start, T_end, step)
+    out = split_records(recs[keep])
+    out["times"] = times[keep]
     return out
 
 

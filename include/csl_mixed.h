@@ -121,6 +121,27 @@ int csl_mixed_integrate_batched(csl_mixed* h, size_t B, const double* init, int 
                                 int tgrid, size_t max_rec, double* times, double* recs,
                                 double* final_states, size_t* n_rec);
 
+/* Dormand-Prince 5(4) for B mixed trajectories (MixedLattice::
+ * integrate_ode_system with "dopri5", mixed_lattice.h:3960-4010), each with
+ * its own step-size sequence; inputs as csl_mixed_integrate_batched.
+ * control, tolerances, save_interval, tgrid (control 0), segments /
+ * n_segments / max_segments (control 2, W3 chunking of single_pulse_drive,
+ * mixed_lattice.h:4841-4870) and max_attempts as csl_integrate_dopri5 of
+ * the SU(2) path: 0 fixed steps, 1 integrate_adaptive (molecular_dynamics,
+ * mixed_lattice_md.cpp:1314), 2 integrate_const with step control (pulse
+ * drivers).  Outputs per trajectory b: times [b*max_rec + r], recs
+ * [(b*max_rec + r)*34], final_states [b*(N2*3+N3*8)] (may be NULL), n_rec,
+ * n_accepted, n_rejected [B] (the last two may be NULL).  CSL_ERR_CAPACITY
+ * when some trajectory has more than max_rec records (the first max_rec
+ * are written); CSL_ERR_STEP when the step control fails. */
+int csl_mixed_integrate_dopri5(csl_mixed* h, size_t B, const double* init, int init_per_replica,
+                               const double* t_pulse, const double* pulse_scale, double T0, double T1,
+                               double dt, int control, double abs_tol, double rel_tol,
+                               size_t save_interval, int tgrid, const double* segments,
+                               const size_t* n_segments, size_t max_segments, size_t max_attempts,
+                               size_t max_rec, double* times, double* recs, double* final_states,
+                               size_t* n_rec, size_t* n_accepted, size_t* n_rejected);
+
 /* Gradient of the loss on the records w.r.t. a common scale theta of the
  * Fe-Tm (SU(2)-SU(3)) couplings, at theta = 1, for B trajectories set up as
  * in csl_mixed_integrate_batched: loss_b = sum_t sum_j w_j (rec_j(t) -

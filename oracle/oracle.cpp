@@ -522,6 +522,63 @@ struct DP5Run {
     }
 };
 
+// The three ways the reference drives the Dormand-Prince stepper
+// (integrate_ode_system, lattice.h:4072-4085) over a state of n doubles with
+// right-hand side f: control 0 fixed steps on the integrate_const grid,
+// 1 integrate_adaptive, 2 integrate_const with the controlled stepper (one
+// call per segment row (t0, t1) of segs when n_seg > 0).  obs(t) is called
+// for every observer call; false after kMaxRejections rejections in a row.
+template <class F, class Obs>
+bool dp5_controls(F&& f, double* state, size_t n, double T0, double T1, double dt, int control, double abs_tol,
+                  double rel_tol, const double* segs, size_t n_seg, Obs&& obs, size_t& acc, size_t& rej) {
+    bool ok = true;
+    if (control == 0) {
+        // fixed steps on the integrate_const grid, no error control
+        DP5Work w(n);
+        std::vector<double> k1(n);
+        auto stp = [&](double t, double h) {
+            f(state, k1.data(), t);
+            dp5_attempt(f, state, k1.data(), t, h, n, w);
+            std::memcpy(state, w.xnew.data(), n * sizeof(double));
+            ++acc;
+        };
+        integrate_const(stp, T0, T1, dt, obs);
+    } else {
+        DP5Run run(f, state, n, T0, abs_tol, rel_tol);
+        if (control == 1) {
+            double h = dt;
+            ok = run.adaptive(T0, T1, h, obs, true);  // integrate_adaptive
+        } else {
+            // integrate_const with a controlled stepper (Boost's
+            // controlled_stepper_tag loop): observe at time_k = s0 + k dt,
+            // integrate_adaptive from time_k to time_k + dt with a copy of
+            // the stepper made at the call (so the derivative is taken again
+            // at time_k) and the step size carried over from the previous
+            // interval; the step hint is dt at the start of each call.
+            auto integrate_const_ctl = [&](double s0, double s1) {
+                double time = s0, h = dt;
+                size_t k = 0;
+                while (ok && less_eq_with_sign(time + dt, s1)) {
+                    obs(time);
+                    run.restart(time);
+                    ok = run.adaptive(time, time + dt, h, obs, false);
+                    ++k;
+                    time = s0 + double(k) * dt;
+                }
+                if (ok) obs(time);
+            };
+            if (n_seg == 0) {
+                integrate_const_ctl(T0, T1);
+            } else {
+                for (size_t j = 0; ok && j < n_seg; ++j) integrate_const_ctl(segs[2 * j], segs[2 * j + 1]);
+            }
+        }
+        acc = run.accepted;
+        rej = run.rejected;
+    }
+    return ok;
+}
+
 }  // namespace
 
 extern "C" {
@@ -718,52 +775,8 @@ long orc_integrate_dopri5(const OracleLattice* L, double* state, double T0, doub
         ++step_count;
     };
     auto f = [L](const double* a, double* b, double tt) { rhs(*L, a, b, tt); };
-    bool ok = true;
     size_t acc = 0, rej = 0;
-    if (control == 0) {
-        // fixed steps on the integrate_const grid, no error control
-        DP5Work w(n);
-        std::vector<double> k1(n);
-        auto stp = [&](double t, double h) {
-            f(state, k1.data(), t);
-            dp5_attempt(f, state, k1.data(), t, h, n, w);
-            std::memcpy(state, w.xnew.data(), n * sizeof(double));
-            ++acc;
-        };
-        integrate_const(stp, T0, T1, dt, obs);
-    } else {
-        DP5Run run(f, state, n, T0, abs_tol, rel_tol);
-        if (control == 1) {
-            double h = dt;
-            ok = run.adaptive(T0, T1, h, obs, true);  // integrate_adaptive
-        } else {
-            // integrate_const with a controlled stepper (Boost's
-            // controlled_stepper_tag loop): observe at time_k = s0 + k dt,
-            // integrate_adaptive from time_k to time_k + dt with a copy of
-            // the stepper made at the call (so the derivative is taken again
-            // at time_k) and the step size carried over from the previous
-            // interval; the step hint is dt at the start of each call.
-            auto integrate_const_ctl = [&](double s0, double s1) {
-                double time = s0, h = dt;
-                size_t k = 0;
-                while (ok && less_eq_with_sign(time + dt, s1)) {
-                    obs(time);
-                    run.restart(time);
-                    ok = run.adaptive(time, time + dt, h, obs, false);
-                    ++k;
-                    time = s0 + double(k) * dt;
-                }
-                if (ok) obs(time);
-            };
-            if (n_seg == 0) {
-                integrate_const_ctl(T0, T1);
-            } else {
-                for (size_t j = 0; ok && j < n_seg; ++j) integrate_const_ctl(segs[2 * j], segs[2 * j + 1]);
-            }
-        }
-        acc = run.accepted;
-        rej = run.rejected;
-    }
+    const bool ok = dp5_controls(f, state, n, T0, T1, dt, control, abs_tol, rel_tol, segs, n_seg, obs, acc, rej);
     if (n_accepted) *n_accepted = acc;
     if (n_rejected) *n_rejected = rej;
     return ok ? long(nrec) : -1;
@@ -1443,6 +1456,34 @@ size_t orc_mixed_integrate_md(const MixedOracle* M, double* state, double T0, do
     auto stp = [&](double t, double h) { rk4_step_generic(F, state, n, t, h, w); };
     integrate_const(stp, T0, T1, dt, obs);
     return nrec;
+}
+
+// Dormand-Prince 5(4) over the flat mixed state (MixedLattice::
+// integrate_ode_system with "dopri5", mixed_lattice.h:3960-4010): the
+// controls of orc_integrate_dopri5, a record (34 doubles) every
+// save_interval-th observer call.
+long orc_mixed_integrate_dopri5(const MixedOracle* M, double* state, double T0, double T1, double dt,
+                                int control, double abs_tol, double rel_tol, size_t save_interval,
+                                const double* segs, size_t n_seg, size_t max_rec, double* times, double* recs,
+                                size_t* n_accepted, size_t* n_rejected) {
+    const size_t n = M->N2 * 3 + M->N3 * 8;
+    size_t calls = 0, nrec = 0;
+    auto obs = [&](double t) {
+        if (calls % save_interval == 0) {
+            if (nrec < max_rec) {
+                if (times) times[nrec] = t;
+                if (recs) mixed_record(*M, state, recs + nrec * 34);
+            }
+            ++nrec;
+        }
+        ++calls;
+    };
+    auto F = [M](const double* x, double* dx, double t) { mixed_rhs(*M, x, dx, t); };
+    size_t acc = 0, rej = 0;
+    const bool ok = dp5_controls(F, state, n, T0, T1, dt, control, abs_tol, rel_tol, segs, n_seg, obs, acc, rej);
+    if (n_accepted) *n_accepted = acc;
+    if (n_rejected) *n_rejected = rej;
+    return ok ? long(nrec) : -1;
 }
 
 }  // extern "C"

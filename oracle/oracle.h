@@ -181,6 +181,15 @@ void   orc_mixed_record(const MixedOracle* M, const double* x, double* out34);
 /* Boost RK4 + integrate_const over [T0, T1]; records every step. */
 size_t orc_mixed_integrate_md(const MixedOracle* M, double* state, double T0, double T1,
                               double dt, size_t max_rec, double* times, double* recs);
+/* Dormand-Prince 5(4) over the flat mixed state with the controls of
+ * orc_integrate_dopri5 (mixed_lattice.h:3960-4010); records of 34 doubles
+ * every save_interval-th observer call.  Returns the number of records, or
+ * -1 after 500 rejections in a row. */
+long   orc_mixed_integrate_dopri5(const MixedOracle* M, double* state, double T0, double T1,
+                                  double dt, int control, double abs_tol, double rel_tol,
+                                  size_t save_interval, const double* segs, size_t n_seg,
+                                  size_t max_rec, double* times, double* recs,
+                                  size_t* n_accepted, size_t* n_rejected);
 
 #ifdef __cplusplus
 }

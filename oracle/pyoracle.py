@@ -59,6 +59,8 @@ _SIG = {
     "orc_mixed_energy": (_d, [_vp, _pd]),
     "orc_mixed_record": (None, [_vp, _pd, _pd]),
     "orc_mixed_integrate_md": (_sz, [_vp, _pd, _d, _d, _d, _sz, _pd, _pd]),
+    "orc_mixed_integrate_dopri5": (C.c_long, [_vp, _pd, _d, _d, _d, _i, _d, _d, _sz, _pd, _sz, _sz, _pd, _pd,
+                                              _psz, _psz]),
 }
 
 _lib = None
@@ -361,6 +363,29 @@ class MixedOracle:
         r = np.zeros(n * self.REC)
         nr = lib().orc_mixed_integrate_md(self.h, _p(s), T0, T1, dt, n, _p(t), _p(r))
         return {"times": t[:nr], "records": r.reshape(n, self.REC)[:nr], "final": s}
+
+    def integrate_dopri5(self, state, T0, T1, dt, control=1, abs_tol=1e-6, rel_tol=1e-6, save_interval=1,
+                         segments=None):
+        """Dormand-Prince 5(4) over the flat state with the controls of
+        OracleLattice.integrate_dopri5; records of 34 values."""
+        sg = None if segments is None else _f(np.asarray(segments, dtype=float).reshape(len(segments), -1)[:, :2])
+        mx = 1000
+        while True:
+            s = _f(state).copy()
+            t = np.zeros(mx)
+            r = np.zeros(mx * self.REC)
+            na, nr_ = C.c_size_t(), C.c_size_t()
+            nr = lib().orc_mixed_integrate_dopri5(self.h, _p(s), T0, T1, dt, control, abs_tol, rel_tol,
+                                                  save_interval, None if sg is None else _p(sg),
+                                                  0 if sg is None else len(sg), mx, _p(t), _p(r), C.byref(na),
+                                                  C.byref(nr_))
+            if nr < 0:
+                raise RuntimeError("step size control failed (500 rejections in a row)")
+            if nr <= mx:
+                break
+            mx = nr
+        return {"times": t[:nr], "records": r.reshape(mx, self.REC)[:nr], "final": s,
+                "accepted": na.value, "rejected": nr_.value}
 
 
 def boost_num_steps(T0, T1, dt):
