@@ -374,6 +374,91 @@ def bench_adjoint(args):
 MIX_L, MIX_NTAU, MIX_STEPS, MIX_DT = 8, 128, 8192, 0.005
 
 
+MIXDP_INTERVALS, MIXDP_TOL = 2048, 1e-8
+
+
+def bench_mixed_dopri5(args):
+    """The mixed 2DCS batch of bench_mixed (257 trajectories, 8x8x8 cells)
+    with the reference's default pulse-drive integrator: Dormand-Prince 5(4)
+    through integrate_const with step control at T_step = 0.005, tolerance
+    1e-8 (kDefaultPumpProbeAbsTol / RelTol), W3 pulse-window segments per
+    trajectory (windows of 9 widths around its pulse centres), over 2048
+    observation intervals from t = -1.  Every trajectory runs its own step
+    sequence; value = trajectories/s."""
+    from classical_mcmc_spin_amd import gpu as G
+    from classical_mcmc_spin_amd import gpu_mixed as GM
+    from classical_mcmc_spin_amd import mixed_model as mm
+    from classical_mcmc_spin_amd import seeding
+    muc = mm.build_tmfeo3_mixed()
+    ml = mm.MixedLattice(muc, MIX_L, MIX_L, MIX_L)
+    g = ml.flat_state(seeding.random_spins(ml.N2, 1234), mm.random_su3_states(ml.N3, 4321))
+    taus = 0.05 * np.arange(MIX_NTAU)
+    B = 1 + 2 * MIX_NTAU
+    tp = np.zeros((B, 2))
+    sc = np.zeros((B, 2))
+    sc[0, 0] = 1.0
+    tp[1:1 + MIX_NTAU, 0] = taus
+    sc[1:1 + MIX_NTAU, 0] = 1.0
+    tp[1 + MIX_NTAU:, 1] = taus
+    sc[1 + MIX_NTAU:, :] = 1.0
+    centers = [[0.0]] + [[t] for t in taus] + [[0.0, t] for t in taus]
+    f2 = np.zeros((4, 3))
+    f2[:, 0] = 1.0
+    f3 = np.zeros((4, 8))
+    f3[:, 1] = 1.0
+    T0 = -1.0
+    T1 = T0 + (MIXDP_INTERVALS + 0.5) * MIX_DT
+    segs = [G.pulse_segments(T0, T1, c, 9.0 * WIDTH, MIX_DT) for c in centers]
+    gm = GM.GPUMixedLattice(ml)
+    GM._set_pulses(gm, f2, f2, f3, f3, 0.0, 0.0, AMP, WIDTH, 0.0, AMP, WIDTH, 0.0)
+
+    def one_step():
+        return gm.integrate_dopri5(g, tp, sc, T0, T1, MIX_DT, control=2, abs_tol=MIXDP_TOL, rel_tol=MIXDP_TOL,
+                                   segments=segs)
+
+    for _ in range(args.warmup):
+        one_step()
+    t = time.perf_counter()
+    for _ in range(args.steps):
+        r = one_step()
+    el = (time.perf_counter() - t) / args.steps
+    kms, launches = gm.last_kernel_ms()
+    gm.close()
+    acc = np.array(r["accepted"])
+    assert all(np.all(np.isfinite(x)) for x in r["records"])
+    n_sites = ml.N2 + ml.N3
+    line = {
+        "metric": METRIC, "value": B / el, "unit": "trajectories/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": 1e3 * el, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"2dcs_tmfeo3_mixed_8x8x8_257traj_{MIXDP_INTERVALS}x0.005_dopri5_tol1e-8_w3",
+                   "sites_su2": ml.N2, "sites_su3": ml.N3, "trajectories": B, "intervals": MIXDP_INTERVALS,
+                   "abs_tol": MIXDP_TOL, "rel_tol": MIXDP_TOL, "segments_max": max(len(q) for q in segs)},
+        "accepted_steps_mean": float(acc.mean()), "rejected_steps_mean": float(np.mean(r["rejected"])),
+        "launch_ms_mean": kms, "launches": launches,
+        "site_steps_per_s": float(acc.sum()) * n_sites / el,
+    }
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import pyoracle
+        o = pyoracle.MixedOracle(muc, MIX_L, MIX_L, MIX_L)
+        b = 1 + MIX_NTAU + 20  # an M01 trajectory
+        o.set_pulse(2, f2, tp[b, 0], f2, tp[b, 1], AMP, WIDTH, 0.0)
+        o.set_pulse(3, f3, tp[b, 0], f3, tp[b, 1], AMP, WIDTH, 0.0)
+        n = 200
+        Tc = T0 + (n + 0.5) * MIX_DT
+        seg_c = G.pulse_segments(T0, Tc, centers[b], 9.0 * WIDTH, MIX_DT)
+        t = time.perf_counter()
+        ro = o.integrate_dopri5(g, T0, Tc, MIX_DT, control=2, abs_tol=MIXDP_TOL, rel_tol=MIXDP_TOL, segments=seg_c)
+        elc = time.perf_counter() - t
+        line["cpu_baseline"] = {
+            "value": n / elc / MIXDP_INTERVALS, "unit": "trajectories/s", "cores": pyoracle.max_threads(),
+            "kind": "port",
+            "sample": f"1 trajectory x {n} of the {MIXDP_INTERVALS} intervals ({ro['accepted']} accepted steps, "
+                      f"{elc:.1f} s, OpenMP over sites inside each RHS)"}
+    print(json.dumps(line), flush=True)
+
+
 def bench_mixed(args):
     """Mixed SU(2)+SU(3) 2DCS batch: TmFeO3-like 8x8x8 cells (2048 Fe SU(2)
     + 2048 Tm SU(3) sites), pulses A = 0.5, w = 0.5 along x (Fe) and
@@ -483,7 +568,8 @@ def main():
     ap.add_argument("--block", type=int, default=0)
     ap.add_argument("--launch", default="", help="fixed persistent launch shape 'block,spt,dbuf' "
                                                    "instead of the autotuner's choice")
-    ap.add_argument("--workload", default="2dcs", choices=["2dcs", "pyrochlore", "mixed", "md", "md_dopri5", "adjoint"],
+    ap.add_argument("--workload", default="2dcs",
+                    choices=["2dcs", "pyrochlore", "mixed", "mixed_dopri5", "md", "md_dopri5", "adjoint"],
                     help="2dcs: the batched 2DCS workload (default); pyrochlore: 64 samples of the "
                          "65 536-site pyrochlore lattice; mixed: 257-trajectory SU(2)+SU(3) 2DCS batch "
                          "(both on the grid-wide stage kernels)")
@@ -507,6 +593,8 @@ def main():
         return bench_pyrochlore(args)
     if args.workload == "mixed":
         return bench_mixed(args)
+    if args.workload == "mixed_dopri5":
+        return bench_mixed_dopri5(args)
     if args.workload == "md":
         return bench_md(args)
     if args.workload == "md_dopri5":

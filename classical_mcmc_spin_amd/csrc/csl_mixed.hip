@@ -757,13 +757,14 @@ struct MixDpCtl {
     int eobs;        // the evaluation is an observer call
     int ends;        // an accepted attempt ends the run / interval
     int fresh;       // control 2: at the start of interval kk
+    int refresh;     // the attempt first takes k1 = f(x, te) again (stage 1)
     int fails, status, sg, gi;
     long long kk, obs_calls, saved, n_acc, n_rej, attempts;
     double t, h, t7, tobs, te, tend, s0, s1, tcur;
 };
 
 struct MixDpArgs {
-    int stage;                    // 2..7
+    int stage;                    // 1..7 (1: k1 refresh before an attempt)
     int nblk2;
     double *X0, *X1, *K10, *K11;  // x and k1, two buffers each
     double* K[5];                 // k2..k6
@@ -816,8 +817,10 @@ __global__ void __launch_bounds__(256) k_mixdp_stage(MixDev M, MixDpArgs A) {
     const MixDpCtl& C = A.ctl[b];
     const int mode = C.mode;
     const int S = A.stage;
-    if (mode == 0 || (mode == 2 && S != 7)) return;  // uniform per workgroup
-    const bool eval = mode == 2;
+    // uniform per workgroup: finished trajectories, evaluations outside
+    // stage 7, stage 1 without a refresh
+    if (mode == 0 || (mode == 2 && S != 7) || (S == 1 && (mode != 1 || C.refresh == 0))) return;
+    const bool eval = mode == 2 || S == 1;
     const int tid = threadIdx.x;
     const int lane = tid & (kWave - 1);
     const int wave = uniform(tid >> 6);
@@ -832,6 +835,7 @@ __global__ void __launch_bounds__(256) k_mixdp_stage(MixDev M, MixDpArgs A) {
     double* K7 = (par ? A.K10 : A.K11) + off;
     double ts;
     switch (S) {
+        case 1: ts = C.te; break;
         case 2: ts = C.t + dp::c2 * h; break;
         case 3: ts = C.t + dp::c3 * h; break;
         case 4: ts = C.t + dp::c4 * h; break;
@@ -857,7 +861,7 @@ __global__ void __launch_bounds__(256) k_mixdp_stage(MixDev M, MixDpArgs A) {
         src.a = X;
         src.k = K1;
         src.c = h * dp::a21;
-    } else if (S <= 6) {
+    } else if (S >= 3 && S <= 6) {
         src.a = ((S & 1) ? A.Y1 : A.Y0) + off;
     } else {
         src.a = eval ? X : XN;
@@ -874,7 +878,7 @@ __global__ void __launch_bounds__(256) k_mixdp_stage(MixDev M, MixDpArgs A) {
     // one derivative component kd of plane index q: store it and form the
     // next stage input / candidate, or (stage 7) the error ratio
     auto finish = [&](size_t q, double kd) {
-        if (S == 7) {
+        if (S == 1 || S == 7) {
             if (eval) {
                 K1[q] = kd;
             } else {
@@ -1129,6 +1133,7 @@ __global__ void __launch_bounds__(256) k_mixdp_decide(MixDpDecideArgs D) {
         }
         // the next attempt or evaluation
         c.mode = 0;
+        c.refresh = 0;
         while (!done) {
             if (D.control == 0) {
                 if (c.gi + 1 >= D.n_grid) break;
@@ -1164,13 +1169,11 @@ __global__ void __launch_bounds__(256) k_mixdp_decide(MixDpDecideArgs D) {
                     }
                     c.fresh = 0;
                     c.fails = 0;
-                    if (drive_on && c.t != c.tcur) {
-                        c.t = c.te = c.tcur;
-                        c.mode = 2;
-                        c.eobs = 0;
-                        break;
-                    }
-                    c.t = c.tcur;
+                    // a new stepper copy takes the derivative at tcur; it
+                    // differs from the carried one only if the accepted
+                    // time is not tcur itself and the drive depends on t
+                    c.refresh = (drive_on && c.t != c.tcur) ? 1 : 0;
+                    c.t = c.te = c.tcur;
                 }
                 if (c.fails == 0 && dp_less(c.tend, c.t + c.h)) c.h = c.tend - c.t;
                 if (!(c.h > 0.0) || !(c.t + c.h > c.t)) {
@@ -2059,12 +2062,12 @@ int csl_mixed_integrate_dopri5(csl_mixed* h, size_t B, const double* init, int i
         while (running) {
             const size_t batch = it < 64 ? 8 : 64;  // iterations between host checks
             for (size_t k = 0; k < batch; ++k) {
-                for (int S = 2; S <= 7; ++S) {
+                for (int S = 1; S <= 7; ++S) {
                     A.stage = S;
                     hipLaunchKernelGGL(csl::k_mixdp_stage, grid_dim, dim3(256), 0, s, h->M, A);
                 }
                 hipLaunchKernelGGL(csl::k_mixdp_decide, dim3(unsigned(B)), dim3(256), 0, s, D);
-                launches += 7;
+                launches += 8;
             }
             hip_check(hipGetLastError(), "k_mixdp launch");
             it += batch;
