@@ -125,6 +125,18 @@ __device__ __forceinline__ void add_bond(P J, const double Sp[3], double H[3]) {
     }
 }
 
+// The same bond term accumulated straight into H (one FMA chain per
+// component, no per-row partial sum): fewer FP64 instructions for the
+// LDS-resident kernels; the sum is associated differently from add_bond
+// (rounding-level difference).
+template <class P>
+__device__ __forceinline__ void add_bond_fused(P J, const double Sp[3], double H[3]) {
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+#pragma unroll
+        for (int e = 0; e < 3; ++e) H[d] = fma(J[d * 3 + e], Sp[e], H[d]);
+}
+
 // dS/dt = H x S, plus Gilbert damping -alpha/|S| S x (H x S)
 // (lattice_md.cpp:80-101)
 __device__ __forceinline__ void ll_su2(const double H[3], const double S[3], double alpha,

@@ -54,9 +54,9 @@ __device__ __forceinline__ void local_field_lds(const LatDev& L, const LdsState&
             double Sp[3];
             y.load(nb[k], Sp);
             if (UNI)
-                add_bond(as_const(L.mats) + (atom * L.zmax + k) * 9, Sp, H);
+                add_bond_fused(as_const(L.mats) + (atom * L.zmax + k) * 9, Sp, H);
             else
-                add_bond(L.mats + (size_t(k) * L.NP + p) * 9, Sp, H);
+                add_bond_fused(L.mats + (size_t(k) * L.NP + p) * 9, Sp, H);
         }
     }
 }
