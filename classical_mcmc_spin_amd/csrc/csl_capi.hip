@@ -7,6 +7,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -64,6 +65,12 @@ struct csl_lattice {
     DBuf<double> X, ACC, Y1, Y2, partials, fac, tgrid, rec_mags, rec_energy, rec_states, aux;
     DBuf<double> bt_init, bt_tp, bt_amp, bt_final;
     DBuf<double> dp_times;        // Dormand-Prince records: times [B][max_rec]
+    // persistent-kernel work split (batches larger than the co-resident workgroups)
+    DBuf<int> ps_sched;
+    DBuf<double> ps_saved;
+    DBuf<unsigned> ps_flags, ps_status;
+    int split_batches = 1;
+    int last_grid = 0;
     DBuf<long long> dp_counters;  // [B][4]
     DBuf<double> dp_segs;         // control-2 segments [B][max_seg][2]
     DBuf<int> dp_nseg;            // [B]
@@ -351,6 +358,61 @@ void run_persistent(csl_lattice* h, size_t B, const double* d_init, size_t init_
         A.mats_stride = h->rep_mats_stride;
         A.field_stride = h->rep_field_stride;
     }
+    // More trajectories than workgroups that fit the GPU at once: B
+    // workgroups would run in rounds, the last one mostly idle (513 = 2*256 + 1
+    // takes three rounds).  Instead G co-resident workgroups share the B * n
+    // trajectory-steps evenly (McNaughton's wrap-around): workgroup g takes
+    // the contiguous range [g*W, (g+1)*W) of the trajectory-major step order,
+    // W = ceil(B*n/G) >= n, so at most one trajectory is cut between g and
+    // g+1.  g runs the first part of that trajectory before anything else and
+    // saves the state; g+1 runs the second part last, after its own whole
+    // trajectories, by which time the state is long there.
+    const int occ = h->split_batches ? csl::persistent_occupancy(L, tv.block, tv.spt, tv.flags) : 0;
+    const size_t G = size_t(std::max(occ, 0)) * size_t(h->n_cu);
+    A.sched = nullptr;
+    h->last_grid = int(B);
+    if (G > 0 && B > G && n_steps > 0 && G <= size_t(std::numeric_limits<int>::max())) {
+        const size_t n = n_steps, W = B * n, per = (W + G - 1) / G;
+        std::vector<std::vector<std::array<int, 5>>> pc(G);
+        size_t max_p = 1;
+        for (size_t g = 0; g < G; ++g) {
+            const size_t a = g * per, e = std::min(W, (g + 1) * per);
+            std::vector<std::array<int, 5>> lay;
+            for (size_t u = a; u < e;) {
+                const size_t j = u / n, s0 = u - j * n, s1 = std::min(n, e - j * n);
+                lay.push_back({int(j), int(s0), int(s1), -1, -1});
+                u = j * n + s1;
+            }
+            auto& out = pc[g];
+            if (!lay.empty() && lay.back()[1] == 0 && size_t(lay.back()[2]) < n) {
+                lay.back()[4] = int(g);  // the first part of a trajectory that g+1 finishes
+                out.push_back(lay.back());
+                lay.pop_back();
+            }
+            const bool tail = !lay.empty() && lay.front()[1] > 0;
+            if (tail) lay.front()[3] = int(g) - 1;  // finished here, begun by g-1
+            for (size_t i = tail ? 1 : 0; i < lay.size(); ++i) out.push_back(lay[i]);
+            if (tail) out.push_back(lay.front());
+            max_p = std::max(max_p, out.size());
+        }
+        std::vector<int> sched(G * max_p * 5, -1);
+        for (size_t g = 0; g < G; ++g)
+            for (size_t i = 0; i < pc[g].size(); ++i)
+                for (int k = 0; k < 5; ++k) sched[(g * max_p + i) * 5 + k] = pc[g][i][k];
+        h->ps_sched.upload(sched.data(), sched.size(), s);
+        h->ps_saved.ensure(G * 3 * size_t(h->L.NP));
+        h->ps_flags.ensure(G);
+        h->ps_status.ensure(1);
+        hip_check(hipMemsetAsync(h->ps_flags.p, 0, G * sizeof(unsigned), s), "memset");
+        hip_check(hipMemsetAsync(h->ps_status.p, 0, sizeof(unsigned), s), "memset");
+        A.sched = h->ps_sched.p;
+        A.max_pieces = int(max_p);
+        A.grid = int(G);
+        A.saved = h->ps_saved.p;
+        A.flags = h->ps_flags.p;
+        A.status = h->ps_status.p;
+        h->last_grid = int(G);
+    }
     hipEvent_t e0, e1;
     hip_check(hipEventCreate(&e0), "event");
     hip_check(hipEventCreate(&e1), "event");
@@ -358,6 +420,11 @@ void run_persistent(csl_lattice* h, size_t B, const double* d_init, size_t init_
     hip_check(csl::launch_persistent(L, A, tv.block, tv.spt, tv.flags, s), "k_persistent launch");
     hip_check(hipEventRecord(e1, s), "event record");
     hip_check(hipEventSynchronize(e1), "k_persistent");
+    if (A.sched) {
+        unsigned st = 0;
+        hip_check(hipMemcpy(&st, h->ps_status.p, sizeof(unsigned), hipMemcpyDeviceToHost), "D2H");
+        if (st) fail(CSL_ERR_HIP, "a workgroup of the split batch waited too long for a trajectory's state");
+    }
     float ms = 0.f;
     hip_check(hipEventElapsedTime(&ms, e0, e1), "elapsed");
     (void)hipEventDestroy(e0);
@@ -1758,6 +1825,20 @@ int csl_set_launch_config(csl_lattice* h, size_t B, int block, int sites_per_thr
                 return;
             }
         fail(CSL_ERR_ARG, "launch shape does not fit this lattice");
+    });
+}
+
+int csl_set_batch_split(csl_lattice* h, int enable) {
+    return guarded([&] {
+        check_handle(h);
+        h->split_batches = enable ? 1 : 0;
+    });
+}
+
+int csl_last_grid(csl_lattice* h, int* workgroups) {
+    return guarded([&] {
+        check_handle(h);
+        if (workgroups) *workgroups = h->last_grid;
     });
 }
 

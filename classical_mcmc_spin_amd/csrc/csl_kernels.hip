@@ -331,18 +331,32 @@ size_t persistent_lds_bytes(const LatDev& L, int block, bool dbuf) {
     return sizeof(double) * (size_t(dbuf ? 6 : 3) * L.NP + 12 + size_t(block / kWave) * 10);
 }
 
-hipError_t launch_persistent(const LatDev& L, const PersistArgs& A, int block, int spt, int flags,
-                             hipStream_t st) {
+namespace {
+hipError_t persistent_dispatch(const LatDev& L, const PersistArgs& A, int block, int spt, int flags,
+                               hipStream_t st, int* occ) {
     const bool dbuf = flags & kPersistDoubleBuffer;
     const size_t lds = persistent_lds_bytes(L, block, dbuf);
     switch (spt) {
-        case 1: return launch_persistent_spt<1>(L, A, block, flags, lds, st);
-        case 2: return launch_persistent_spt<2>(L, A, block, flags, lds, st);
-        case 3: return launch_persistent_spt<3>(L, A, block, flags, lds, st);
-        case 4: return launch_persistent_spt<4>(L, A, block, flags, lds, st);
-        case 6: return launch_persistent_spt<6>(L, A, block, flags, lds, st);
+        case 1: return launch_persistent_spt<1>(L, A, block, flags, lds, st, occ);
+        case 2: return launch_persistent_spt<2>(L, A, block, flags, lds, st, occ);
+        case 3: return launch_persistent_spt<3>(L, A, block, flags, lds, st, occ);
+        case 4: return launch_persistent_spt<4>(L, A, block, flags, lds, st, occ);
+        case 6: return launch_persistent_spt<6>(L, A, block, flags, lds, st, occ);
         default: return hipErrorInvalidValue;
     }
+}
+}  // namespace
+
+hipError_t launch_persistent(const LatDev& L, const PersistArgs& A, int block, int spt, int flags,
+                             hipStream_t st) {
+    return persistent_dispatch(L, A, block, spt, flags, st, nullptr);
+}
+
+int persistent_occupancy(const LatDev& L, int block, int spt, int flags) {
+    int occ = 0;
+    PersistArgs A{};
+    if (persistent_dispatch(L, A, block, spt, flags, nullptr, &occ) != hipSuccess) return 0;
+    return occ;
 }
 
 bool persistent_spt_supported(int spt) {

@@ -25,6 +25,19 @@ struct PersistArgs {
     // and L.field + b*field_stride (0 = shared tables)
     size_t mats_stride;
     size_t field_stride;
+    // Work split for batches larger than the co-resident workgroups: the
+    // launch has `grid` workgroups and workgroup g runs the pieces
+    // sched[(g * max_pieces + i) * 5 + {0..4}] = (trajectory, first step, end
+    // step, slot to load the state from (-1: the initial state), slot to save
+    // the state to (-1: none)); trajectory -1 marks an unused piece.  A piece
+    // that loads a slot waits until the piece that saves it has set its flag.
+    // sched == null: workgroup g integrates trajectory g from 0 to n_steps.
+    const int* sched;
+    int max_pieces;
+    int grid;
+    double* saved;           // [slots][NP][3] device-order states at the seams
+    unsigned* flags;         // [slots], zero before the launch
+    unsigned* status;        // set to 1 if a wait for a slot gave up
 };
 
 // Discrete-adjoint gradient run (csl_adjoint.hip): one workgroup per trajectory.
@@ -138,18 +151,23 @@ constexpr int persistent_max_block() {
 constexpr int kPersistDoubleBuffer = 1;  // two LDS state buffers, one barrier per stage
 constexpr int kPersistUnrollStages = 2;  // the four RK stages specialised at compile time
 
+// Launches the persistent kernel, or with occupancy != null only stores its
+// co-resident workgroups per CU for this shape.
 template <int SPT>
 hipError_t launch_persistent_spt(const LatDev& L, const PersistArgs& A, int block, int flags, size_t lds,
-                                 hipStream_t st);
-template <> hipError_t launch_persistent_spt<1>(const LatDev&, const PersistArgs&, int, int, size_t, hipStream_t);
-template <> hipError_t launch_persistent_spt<2>(const LatDev&, const PersistArgs&, int, int, size_t, hipStream_t);
-template <> hipError_t launch_persistent_spt<3>(const LatDev&, const PersistArgs&, int, int, size_t, hipStream_t);
-template <> hipError_t launch_persistent_spt<4>(const LatDev&, const PersistArgs&, int, int, size_t, hipStream_t);
-template <> hipError_t launch_persistent_spt<6>(const LatDev&, const PersistArgs&, int, int, size_t, hipStream_t);
+                                 hipStream_t st, int* occupancy);
+template <> hipError_t launch_persistent_spt<1>(const LatDev&, const PersistArgs&, int, int, size_t, hipStream_t, int*);
+template <> hipError_t launch_persistent_spt<2>(const LatDev&, const PersistArgs&, int, int, size_t, hipStream_t, int*);
+template <> hipError_t launch_persistent_spt<3>(const LatDev&, const PersistArgs&, int, int, size_t, hipStream_t, int*);
+template <> hipError_t launch_persistent_spt<4>(const LatDev&, const PersistArgs&, int, int, size_t, hipStream_t, int*);
+template <> hipError_t launch_persistent_spt<6>(const LatDev&, const PersistArgs&, int, int, size_t, hipStream_t, int*);
 
 size_t persistent_lds_bytes(const LatDev& L, int block, bool dbuf);
 bool persistent_spt_supported(int spt);
 int persistent_block_limit(int spt);
+// Co-resident workgroups per CU of the persistent kernel for this launch
+// shape (0 when it cannot be determined).
+int persistent_occupancy(const LatDev& L, int block, int spt, int flags);
 hipError_t launch_persistent(const LatDev& L, const PersistArgs& A, int block, int spt, int flags,
                              hipStream_t st);
 int stage_blocks(const LatDev& L);

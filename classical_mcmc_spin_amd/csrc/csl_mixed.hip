@@ -85,7 +85,8 @@ __device__ __forceinline__ size_t mix_stride(const MixDev& M) {
     return size_t(3) * M.NP2 + size_t(8) * M.NP3;
 }
 
-// y_p = sum_k J_k x_partner(k), J m = DA x DB, accumulated per bond row by row
+// y_p += sum_k J_k x_partner(k), J m = DA x DB, accumulated straight into y
+// (one FMA chain per component)
 template <int DA, int DB>
 __device__ __forceinline__ void gather_bonds(const Ell& E, const double* __restrict__ src, int NPsrc,
                                              int NPpart, int atom, int p, double* H) {
@@ -98,21 +99,15 @@ __device__ __forceinline__ void gather_bonds(const Ell& E, const double* __restr
         if (E.uniform) {
             const cptr<double> J = as_const(E.mats) + (size_t(atom) * E.zmax + k) * (DA * DB);
 #pragma unroll
-            for (int a = 0; a < DA; ++a) {
-                double r = 0.0;
+            for (int a = 0; a < DA; ++a)
 #pragma unroll
-                for (int e = 0; e < DB; ++e) r += J[a * DB + e] * s[e];
-                H[a] += r;
-            }
+                for (int e = 0; e < DB; ++e) H[a] = fma(J[a * DB + e], s[e], H[a]);
         } else {
             const double* J = E.mats + (size_t(k) * NPsrc + p) * (DA * DB);
 #pragma unroll
-            for (int a = 0; a < DA; ++a) {
-                double r = 0.0;
+            for (int a = 0; a < DA; ++a)
 #pragma unroll
-                for (int e = 0; e < DB; ++e) r += J[a * DB + e] * s[e];
-                H[a] += r;
-            }
+                for (int e = 0; e < DB; ++e) H[a] = fma(J[a * DB + e], s[e], H[a]);
         }
     }
 }
@@ -799,12 +794,9 @@ __device__ __forceinline__ void gather_src(const Ell& E, const DpSrc& src, size_
         const double* J = E.uniform ? E.mats + (size_t(atom) * E.zmax + k) * (DA * DB)
                                     : E.mats + (size_t(k) * NPsrc + p) * (DA * DB);
 #pragma unroll
-        for (int a = 0; a < DA; ++a) {
-            double r = 0.0;
+        for (int a = 0; a < DA; ++a)
 #pragma unroll
-            for (int e = 0; e < DB; ++e) r += J[a * DB + e] * s[e];
-            H[a] += r;
-        }
+            for (int e = 0; e < DB; ++e) H[a] = fma(J[a * DB + e], s[e], H[a]);
     }
 }
 

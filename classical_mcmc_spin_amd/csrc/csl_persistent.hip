@@ -45,46 +45,21 @@ __device__ __forceinline__ double site_energy_atom(const LatDev& L, int atom, co
 
 template <int SPT, int ZMAX, bool UNI, bool DBUF, bool UNROLL, bool FIELD_ONLY>
 __global__ void __launch_bounds__(persistent_max_block<SPT>()) k_persistent(LatDev Lk, PersistArgs A) {
-    // this trajectory's Hamiltonian tables
-    LatDev L0 = Lk;
-    L0.mats += size_t(blockIdx.x) * A.mats_stride;
-    L0.field += size_t(blockIdx.x) * A.field_stride;
-    const LatDev& L = L0;
     extern __shared__ double smem[];
-    const int NP = L.NP;
+    const int NP = Lk.NP;
     const LdsState bufA = lds_state(smem, NP);
     const LdsState bufB = DBUF ? lds_state(smem + 3 * NP, NP) : bufA;
     double* drv = smem + (DBUF ? 6 : 3) * NP;  // [2 parity][3 stage times][2 pulses]
     double* red = drv + 12;                    // [nwaves][10]
 
-    const int b = blockIdx.x;
     const int T = blockDim.x;
     const int tid = threadIdx.x;
     const int wave = uniform(tid >> 6);
     const int lane = tid & (kWave - 1);
     const int nwaves = T >> 6;
-
-    const double tp1 = A.t_pulse[2 * b], tp2 = A.t_pulse[2 * b + 1];
-    const double am1 = A.amp[2 * b], am2 = A.amp[2 * b + 1];
-    const bool drive_on = (am1 != 0.0) || (am2 != 0.0);
     const double dt = A.dt;
-    const double alpha = L.alpha;
 
-    // initial state -> LDS (device order)
-    const double* init = A.init + size_t(b) * A.init_stride;
-    for (int p = tid; p < NP; p += T) {
-        const int atom = p / L.npad, cell = p - atom * L.npad;
-        double v[3] = {0.0, 0.0, 0.0};
-        if (cell < L.n_cells) {
-            const size_t i = size_t(cell) * L.n_atoms + atom;
-            v[0] = init[3 * i];
-            v[1] = init[3 * i + 1];
-            v[2] = init[3 * i + 2];
-        }
-        bufA.store(p, v);
-    }
-
-    // per-thread ownership: sites p = tid + m*T
+    // per-thread ownership: sites p = tid + m*T (the same for every trajectory)
     double x[SPT][3], acc[SPT][3];
     int nb[SPT][ZMAX];
     int cnt[SPT], atm[SPT];
@@ -93,14 +68,75 @@ __global__ void __launch_bounds__(persistent_max_block<SPT>()) k_persistent(LatD
     for (int m = 0; m < SPT; ++m) {
         const int p = tid + m * T;
         const int pw = wave * kWave + m * T;  // wave-uniform
-        atm[m] = uniform(pw < NP ? pw / L.npad : 0);
-        act[m] = active_site(L, p);
-        const int c = UNI ? uniform(pw < NP ? as_const(L.cnt_atom)[atm[m]] : 0) : (p < NP ? L.cnt[p] : 0);
+        atm[m] = uniform(pw < NP ? pw / Lk.npad : 0);
+        act[m] = active_site(Lk, p);
+        const int c = UNI ? uniform(pw < NP ? as_const(Lk.cnt_atom)[atm[m]] : 0) : (p < NP ? Lk.cnt[p] : 0);
         cnt[m] = c;
 #pragma unroll
-        for (int k = 0; k < ZMAX; ++k) nb[m][k] = (act[m] && k < c) ? L.col[size_t(k) * NP + p] : 0;
-        acc[m][0] = acc[m][1] = acc[m][2] = 0.0;
+        for (int k = 0; k < ZMAX; ++k) nb[m][k] = (act[m] && k < c) ? Lk.col[size_t(k) * NP + p] : 0;
     }
+
+    const int n_pieces = A.sched ? A.max_pieces : 1;
+    for (int piece = 0; piece < n_pieces; ++piece) {
+    int b = blockIdx.x, s_begin = 0, s_end = A.n_steps, slot_in = -1, slot_out = -1;
+    if (A.sched) {
+        const int* r = A.sched + (size_t(blockIdx.x) * A.max_pieces + piece) * 5;
+        b = r[0];
+        s_begin = r[1];
+        s_end = r[2];
+        slot_in = r[3];
+        slot_out = r[4];
+        if (b < 0) continue;  // uniform
+    }
+    // this trajectory's Hamiltonian tables
+    LatDev L0 = Lk;
+    L0.mats += size_t(b) * A.mats_stride;
+    L0.field += size_t(b) * A.field_stride;
+    const LatDev& L = L0;
+
+    const double tp1 = A.t_pulse[2 * b], tp2 = A.t_pulse[2 * b + 1];
+    const double am1 = A.amp[2 * b], am2 = A.amp[2 * b + 1];
+    const bool drive_on = (am1 != 0.0) || (am2 != 0.0);
+    const double alpha = L.alpha;
+
+    __syncthreads();  // the previous piece is done with the LDS buffers
+    if (slot_in < 0) {
+        // initial state -> LDS (device order)
+        const double* init = A.init + size_t(b) * A.init_stride;
+        for (int p = tid; p < NP; p += T) {
+            const int atom = p / L.npad, cell = p - atom * L.npad;
+            double v[3] = {0.0, 0.0, 0.0};
+            if (cell < L.n_cells) {
+                const size_t i = size_t(cell) * L.n_atoms + atom;
+                v[0] = init[3 * i];
+                v[1] = init[3 * i + 1];
+                v[2] = init[3 * i + 2];
+            }
+            bufA.store(p, v);
+        }
+    } else {
+        // the state at step s_begin, saved by the workgroup that ran the
+        // trajectory's first steps (it does so before anything else, so the
+        // wait is short); a bounded wait that gives up flags the launch
+        if (tid == 0) {
+            unsigned ok = 0;
+            for (long it = 0; it < (1L << 24); ++it) {
+                ok = __hip_atomic_load(A.flags + slot_in, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+                if (ok) break;
+                __builtin_amdgcn_s_sleep(8);
+            }
+            if (!ok) __hip_atomic_store(A.status, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        __syncthreads();
+        __atomic_thread_fence(__ATOMIC_ACQUIRE);
+        const double* sv = A.saved + size_t(slot_in) * 3 * NP;
+        for (int p = tid; p < NP; p += T) {
+            const double v[3] = {sv[3 * p], sv[3 * p + 1], sv[3 * p + 2]};
+            bufA.store(p, v);
+        }
+    }
+#pragma unroll
+    for (int m = 0; m < SPT; ++m) acc[m][0] = acc[m][1] = acc[m][2] = 0.0;
     __syncthreads();
 #pragma unroll
     for (int m = 0; m < SPT; ++m) {
@@ -170,7 +206,7 @@ __global__ void __launch_bounds__(persistent_max_block<SPT>()) k_persistent(LatD
         }
     };
 
-    write_drive(0);
+    write_drive(s_begin);
     __syncthreads();
 
     // One RK4 stage of step n.  Called with a runtime stage index (one copy of
@@ -236,11 +272,11 @@ __global__ void __launch_bounds__(persistent_max_block<SPT>()) k_persistent(LatD
         }
     };
 
-    for (int n = 0; n < A.n_steps; ++n) {
+    for (int n = s_begin; n < s_end; ++n) {
         const int par = n & 1;
         const bool rec = (n % A.save_interval) == 0;
         const int ri = n / A.save_interval;
-        if (n + 1 < A.n_steps) write_drive(n + 1);
+        if (n + 1 < s_end) write_drive(n + 1);
         if constexpr (UNROLL) {
             stage(std::integral_constant<int, 0>{}, par, rec, ri);
             stage(std::integral_constant<int, 1>{}, par, rec, ri);
@@ -252,8 +288,23 @@ __global__ void __launch_bounds__(persistent_max_block<SPT>()) k_persistent(LatD
         }
     }
 
+    if (slot_out >= 0) {
+        // hand the state at s_end to the piece that continues the trajectory
+        double* sv = A.saved + size_t(slot_out) * 3 * NP;
+#pragma unroll
+        for (int m = 0; m < SPT; ++m) {
+            const int p = tid + m * T;
+            if (p < NP) {
+#pragma unroll
+                for (int d = 0; d < 3; ++d) sv[3 * p + d] = act[m] ? x[m][d] : 0.0;
+            }
+        }
+        __threadfence();
+        __syncthreads();
+        if (tid == 0) __hip_atomic_store(A.flags + slot_out, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    }
     // final record: state after the last step (lattice_gpu.cu:567-568)
-    {
+    if (s_end == A.n_steps) {
         const LatDev L = refresh(L0);
         double e[SPT];
 #pragma unroll
@@ -284,48 +335,50 @@ __global__ void __launch_bounds__(persistent_max_block<SPT>()) k_persistent(LatD
             }
         }
     }
+    }  // pieces
 }
 
 namespace {
 
 template <int SPT, int ZMAX, bool UNI, bool DBUF, bool UNROLL, bool FIELD_ONLY = false>
-hipError_t launch_p(const LatDev& L, const PersistArgs& A, int block, size_t lds, hipStream_t st) {
+hipError_t launch_p(const LatDev& L, const PersistArgs& A, int block, size_t lds, hipStream_t st, int* occ) {
     auto kern = k_persistent<SPT, ZMAX, UNI, DBUF, UNROLL, FIELD_ONLY>;
     hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
                                        hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(kern, dim3(A.B), dim3(block), lds, st, L, A);
+    if (occ) return hipOccupancyMaxActiveBlocksPerMultiprocessor(occ, kern, block, lds);
+    hipLaunchKernelGGL(kern, dim3(A.sched ? A.grid : A.B), dim3(block), lds, st, L, A);
     return hipGetLastError();
 }
 
 template <int SPT, int ZMAX, bool UNI>
 hipError_t launch_p_db(const LatDev& L, const PersistArgs& A, int block, int flags, size_t lds,
-                       hipStream_t st) {
+                       hipStream_t st, int* occ) {
     const bool dbuf = flags & kPersistDoubleBuffer, unroll = flags & kPersistUnrollStages;
     if (unroll && dbuf && UNI && L.site_uniform && !L.has_onsite && L.full_coord && L.zmax == ZMAX)
-        return launch_p<SPT, ZMAX, UNI, true, true, true>(L, A, block, lds, st);
+        return launch_p<SPT, ZMAX, UNI, true, true, true>(L, A, block, lds, st, occ);
     if (unroll)
-        return dbuf ? launch_p<SPT, ZMAX, UNI, true, true>(L, A, block, lds, st)
-                    : launch_p<SPT, ZMAX, UNI, false, true>(L, A, block, lds, st);
-    return dbuf ? launch_p<SPT, ZMAX, UNI, true, false>(L, A, block, lds, st)
-                : launch_p<SPT, ZMAX, UNI, false, false>(L, A, block, lds, st);
+        return dbuf ? launch_p<SPT, ZMAX, UNI, true, true>(L, A, block, lds, st, occ)
+                    : launch_p<SPT, ZMAX, UNI, false, true>(L, A, block, lds, st, occ);
+    return dbuf ? launch_p<SPT, ZMAX, UNI, true, false>(L, A, block, lds, st, occ)
+                : launch_p<SPT, ZMAX, UNI, false, false>(L, A, block, lds, st, occ);
 }
 
 template <int SPT, int ZMAX>
 hipError_t launch_p_uni(const LatDev& L, const PersistArgs& A, int block, int flags, size_t lds,
-                        hipStream_t st) {
-    return L.uniform ? launch_p_db<SPT, ZMAX, true>(L, A, block, flags, lds, st)
-                     : launch_p_db<SPT, ZMAX, false>(L, A, block, flags, lds, st);
+                        hipStream_t st, int* occ) {
+    return L.uniform ? launch_p_db<SPT, ZMAX, true>(L, A, block, flags, lds, st, occ)
+                     : launch_p_db<SPT, ZMAX, false>(L, A, block, flags, lds, st, occ);
 }
 
 }  // namespace
 
 template <>
 hipError_t launch_persistent_spt<CSL_SPT>(const LatDev& L, const PersistArgs& A, int block, int flags,
-                                          size_t lds, hipStream_t st) {
-    if (L.zmax <= 3) return launch_p_uni<CSL_SPT, 3>(L, A, block, flags, lds, st);
-    if (L.zmax <= 6) return launch_p_uni<CSL_SPT, 6>(L, A, block, flags, lds, st);
-    if (L.zmax <= 12) return launch_p_uni<CSL_SPT, 12>(L, A, block, flags, lds, st);
+                                          size_t lds, hipStream_t st, int* occ) {
+    if (L.zmax <= 3) return launch_p_uni<CSL_SPT, 3>(L, A, block, flags, lds, st, occ);
+    if (L.zmax <= 6) return launch_p_uni<CSL_SPT, 6>(L, A, block, flags, lds, st, occ);
+    if (L.zmax <= 12) return launch_p_uni<CSL_SPT, 12>(L, A, block, flags, lds, st, occ);
     return hipErrorInvalidValue;
 }
 

@@ -239,6 +239,16 @@ class GPULattice:
         """flags: bit 0 two LDS state buffers, bit 1 compile-time RK stages."""
         check(lib().csl_set_launch_config(self.h, B, block, sites_per_thread, int(flags)))
 
+    def set_batch_split(self, enable: bool):
+        """csl_set_batch_split: share the trajectory-steps of a batch larger
+        than the co-resident workgroups evenly over them (default on)."""
+        check(lib().csl_set_batch_split(self.h, int(bool(enable))))
+
+    def last_grid(self) -> int:
+        g = C.c_int()
+        check(lib().csl_last_grid(self.h, C.byref(g)))
+        return g.value
+
     def last_launch_config(self):
         b, s, d = C.c_int(), C.c_int(), C.c_int()
         check(lib().csl_last_launch_config(self.h, C.byref(b), C.byref(s), C.byref(d)))

@@ -315,6 +315,19 @@ int csl_last_launch_config(csl_lattice* h, int* block, int* sites_per_thread, in
  * class (as the autotuner would); rejects shapes that do not fit. */
 int csl_set_launch_config(csl_lattice* h, size_t B, int block, int sites_per_thread, int flags);
 
+/* Work split of the one-workgroup-per-trajectory kernel (on by default):
+ * when a batch holds more trajectories than workgroups fit the GPU at once,
+ * that many workgroups share the trajectory-steps evenly, a trajectory cut
+ * between two of them continuing from the state the first one saves (same
+ * arithmetic, bit-identical results).  Off: one workgroup per trajectory,
+ * run in rounds.  No counterpart in the reference, whose batched GPU path
+ * launches grid-wide kernels per RK stage. */
+int csl_set_batch_split(csl_lattice* h, int enable);
+
+/* Workgroups of the last persistent launch on this handle (the batch size
+ * unless the work was split). */
+int csl_last_grid(csl_lattice* h, int* workgroups);
+
 /* Zero-temperature relaxation of B states in place (states [B*N*3], AoS):
  * Lattice::greedy_quench (lattice.h:3352-3368) — sweeps that set every spin
  * antiparallel to its local field, S = -H/|H| * spin_length (skipped when
