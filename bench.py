@@ -650,6 +650,7 @@ def main():
     site_steps_launch = B * N * N_STEPS
     achieved = site_steps_launch * BYTES_PER_SITE_STEP / (kms * 1e-3)
     launch_cfg = gl.last_launch_config()
+    launch_cfg["workgroups"] = gl.last_grid()  # < B when the batch is split over the co-resident workgroups
     if args.profile:
         print(json.dumps({"profile": True, "kernel_ms": kms, "ms_per_step": ms_step, "launch": launch_cfg,
                           "traj_per_s": traj_s, "pcie_inclusive_traj_per_s": pcie_traj_s}), flush=True)
