@@ -43,7 +43,12 @@ __device__ __forceinline__ double site_energy_atom(const LatDev& L, int atom, co
 }
 }  // namespace
 
-template <int SPT, int ZMAX, bool UNI, bool DBUF, bool UNROLL, bool FIELD_ONLY>
+// ATOM_MAJOR (FIELD_ONLY shapes with npad = waves_per_atom * 64 * SPT and
+// n_atoms * waves_per_atom waves): every wave owns SPT * 64 cells of one
+// sublattice, site m of lane l of wave w at atom(w) * npad + (w_in_atom *
+// SPT + m) * 64 + l, so all owned sites share one atom and its bond matrices
+// are loaded once per bond for all of them (instead of once per site slot).
+template <int SPT, int ZMAX, bool UNI, bool DBUF, bool UNROLL, bool FIELD_ONLY, bool ATOM_MAJOR = false>
 __global__ void __launch_bounds__(persistent_max_block<SPT>()) k_persistent(LatDev Lk, PersistArgs A) {
     extern __shared__ double smem[];
     const int NP = Lk.NP;
@@ -59,16 +64,21 @@ __global__ void __launch_bounds__(persistent_max_block<SPT>()) k_persistent(LatD
     const int nwaves = T >> 6;
     const double dt = A.dt;
 
-    // per-thread ownership: sites p = tid + m*T (the same for every trajectory)
+    // per-thread ownership: sites p = tid + m*T, or the atom-major mapping
+    // (the same for every trajectory)
+    const int wpa = ATOM_MAJOR ? Lk.npad / (kWave * SPT) : 1;  // waves per atom
+    const int atom_w = ATOM_MAJOR ? wave / wpa : 0;
+    const int pbase = ATOM_MAJOR ? atom_w * Lk.npad + (wave - atom_w * wpa) * (SPT * kWave) + lane : tid;
+    auto site = [&](int m) { return ATOM_MAJOR ? pbase + m * kWave : tid + m * T; };
     double x[SPT][3], acc[SPT][3];
     int nb[SPT][ZMAX];
     int cnt[SPT], atm[SPT];
     bool act[SPT];
 #pragma unroll
     for (int m = 0; m < SPT; ++m) {
-        const int p = tid + m * T;
-        const int pw = wave * kWave + m * T;  // wave-uniform
-        atm[m] = uniform(pw < NP ? pw / Lk.npad : 0);
+        const int p = site(m);
+        const int pw = ATOM_MAJOR ? atom_w * Lk.npad : wave * kWave + m * T;  // wave-uniform
+        atm[m] = ATOM_MAJOR ? atom_w : uniform(pw < NP ? pw / Lk.npad : 0);
         act[m] = active_site(Lk, p);
         const int c = UNI ? uniform(pw < NP ? as_const(Lk.cnt_atom)[atm[m]] : 0) : (p < NP ? Lk.cnt[p] : 0);
         cnt[m] = c;
@@ -140,7 +150,7 @@ __global__ void __launch_bounds__(persistent_max_block<SPT>()) k_persistent(LatD
     __syncthreads();
 #pragma unroll
     for (int m = 0; m < SPT; ++m) {
-        const int p = tid + m * T;
+        const int p = site(m);
         if (act[m]) {
             bufA.load(p, x[m]);
         } else {
@@ -175,8 +185,9 @@ __global__ void __launch_bounds__(persistent_max_block<SPT>()) k_persistent(LatD
 #pragma unroll
         for (int m = 0; m < SPT; ++m) {
             if (act[m]) {
-                cptr<double> R = as_const(Lr.frames) + atm[m] * 9;
-                const double sg = as_const(Lr.afm)[atm[m]];
+                const int am = ATOM_MAJOR ? atom_w : atm[m];
+                cptr<double> R = as_const(Lr.frames) + am * 9;
+                const double sg = as_const(Lr.afm)[am];
 #pragma unroll
                 for (int d = 0; d < 3; ++d) {
                     const double g = R[d * 3 + 0] * x[m][0] + R[d * 3 + 1] * x[m][1] + R[d * 3 + 2] * x[m][2];
@@ -228,10 +239,10 @@ __global__ void __launch_bounds__(persistent_max_block<SPT>()) k_persistent(LatD
             double yn[DBUF ? 1 : SPT][3];
 #pragma unroll
             for (int m = 0; m < SPT; ++m) {
-                const int p = tid + m * T;
+                const int p = site(m);
                 e[m] = 0.0;
                 if (act[m]) {
-                    const int atom = atm[m];
+                    const int atom = ATOM_MAJOR ? atom_w : atm[m];
                     double S[3];
                     cur.load(p, S);
                     double H[3];
@@ -263,7 +274,7 @@ __global__ void __launch_bounds__(persistent_max_block<SPT>()) k_persistent(LatD
                 __syncthreads();
 #pragma unroll
                 for (int m = 0; m < SPT; ++m) {
-                    const int p = tid + m * T;
+                    const int p = site(m);
                     if (act[m]) nxt.store(p, yn[DBUF ? 0 : m]);
                 }
             }
@@ -293,7 +304,7 @@ __global__ void __launch_bounds__(persistent_max_block<SPT>()) k_persistent(LatD
         double* sv = A.saved + size_t(slot_out) * 3 * NP;
 #pragma unroll
         for (int m = 0; m < SPT; ++m) {
-            const int p = tid + m * T;
+            const int p = site(m);
             if (p < NP) {
 #pragma unroll
                 for (int d = 0; d < 3; ++d) sv[3 * p + d] = act[m] ? x[m][d] : 0.0;
@@ -309,7 +320,7 @@ __global__ void __launch_bounds__(persistent_max_block<SPT>()) k_persistent(LatD
         double e[SPT];
 #pragma unroll
         for (int m = 0; m < SPT; ++m) {
-            const int p = tid + m * T;
+            const int p = site(m);
             e[m] = 0.0;
             if (act[m]) {
                 double H[3];
@@ -324,7 +335,7 @@ __global__ void __launch_bounds__(persistent_max_block<SPT>()) k_persistent(LatD
             double* out = A.final_states + size_t(b) * L.N * 3;
 #pragma unroll
             for (int m = 0; m < SPT; ++m) {
-                const int p = tid + m * T;
+                const int p = site(m);
                 if (act[m]) {
                     const int cell = p - atm[m] * L.npad;
                     const size_t i = size_t(cell) * L.n_atoms + atm[m];
@@ -340,9 +351,9 @@ __global__ void __launch_bounds__(persistent_max_block<SPT>()) k_persistent(LatD
 
 namespace {
 
-template <int SPT, int ZMAX, bool UNI, bool DBUF, bool UNROLL, bool FIELD_ONLY = false>
+template <int SPT, int ZMAX, bool UNI, bool DBUF, bool UNROLL, bool FIELD_ONLY = false, bool ATOM_MAJOR = false>
 hipError_t launch_p(const LatDev& L, const PersistArgs& A, int block, size_t lds, hipStream_t st, int* occ) {
-    auto kern = k_persistent<SPT, ZMAX, UNI, DBUF, UNROLL, FIELD_ONLY>;
+    auto kern = k_persistent<SPT, ZMAX, UNI, DBUF, UNROLL, FIELD_ONLY, ATOM_MAJOR>;
     hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
                                        hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));
     if (e != hipSuccess) return e;
@@ -355,8 +366,11 @@ template <int SPT, int ZMAX, bool UNI>
 hipError_t launch_p_db(const LatDev& L, const PersistArgs& A, int block, int flags, size_t lds,
                        hipStream_t st, int* occ) {
     const bool dbuf = flags & kPersistDoubleBuffer, unroll = flags & kPersistUnrollStages;
-    if (unroll && dbuf && UNI && L.site_uniform && !L.has_onsite && L.full_coord && L.zmax == ZMAX)
+    if (unroll && dbuf && UNI && L.site_uniform && !L.has_onsite && L.full_coord && L.zmax == ZMAX) {
+        const bool atom_major = L.npad % (kWave * SPT) == 0 && block == L.n_atoms * (L.npad / (kWave * SPT)) * kWave;
+        if (atom_major) return launch_p<SPT, ZMAX, UNI, true, true, true, true>(L, A, block, lds, st, occ);
         return launch_p<SPT, ZMAX, UNI, true, true, true>(L, A, block, lds, st, occ);
+    }
     if (unroll)
         return dbuf ? launch_p<SPT, ZMAX, UNI, true, true>(L, A, block, lds, st, occ)
                     : launch_p<SPT, ZMAX, UNI, false, true>(L, A, block, lds, st, occ);
