@@ -119,7 +119,7 @@ __global__ void __launch_bounds__(adjoint_max_block<SPT>()) k_adjoint(LatDev Lk,
             if (q < cnt[m]) {
                 double Sp[3];
                 y.load(nb[m][q], Sp);
-                add_bond(as_const(L.mats) + (atm[m] * L.zmax + q) * 9, Sp, H);
+                add_bond_fused(as_const(L.mats) + (atm[m] * L.zmax + q) * 9, Sp, H);
             }
         if (drive) sub_drive(L, atm[m], f1, f2, H);
         ll_su2(H, S, alpha, k);
@@ -297,7 +297,7 @@ __global__ void __launch_bounds__(adjoint_max_block<SPT>()) k_adjoint(LatDev Lk,
                 for (int q = 0; q < ZMAX; ++q)
                     if (q < cnt[m]) {
                         yb.load(nb[m][q], Sp[q]);
-                        add_bond(as_const(L.mats) + (atm[m] * L.zmax + q) * 9, Sp[q], H);
+                        add_bond_fused(as_const(L.mats) + (atm[m] * L.zmax + q) * 9, Sp[q], H);
                     }
                 if (drive) sub_drive(L, atm[m], f1, f2, H);
                 double cc[3], w[3];
@@ -329,7 +329,7 @@ __global__ void __launch_bounds__(adjoint_max_block<SPT>()) k_adjoint(LatDev Lk,
 #pragma unroll
                     for (int z = 0; z < ZMAX; ++z)
                         if (z < cnt[m])
-                            add_bond(as_const(A.dmats) + ((size_t(q) * L.n_atoms + atm[m]) * L.zmax + z) * 9, Sp[z],
+                            add_bond_fused(as_const(A.dmats) + ((size_t(q) * L.n_atoms + atm[m]) * L.zmax + z) * 9, Sp[z],
                                      dH);
                     gp[q] += u[0] * dH[0] + u[1] * dH[1] + u[2] * dH[2];
                     }
@@ -359,7 +359,7 @@ __global__ void __launch_bounds__(adjoint_max_block<SPT>()) k_adjoint(LatDev Lk,
                     if (q < cnt[m]) {
                         double up[3];
                         bufU.load(nb[m][q], up);
-                        add_bond(as_const(L.mats) + (atm[m] * L.zmax + q) * 9, up, r);
+                        add_bond_fused(as_const(L.mats) + (atm[m] * L.zmax + q) * 9, up, r);
                     }
                 mu[m][0] = r[0];
                 mu[m][1] = r[1];
