@@ -97,7 +97,11 @@ __device__ __forceinline__ double wave_max(double v) {
 
 }  // namespace
 
-template <int SPT, int ZMAX, bool UNI>
+// ATOM_MAJOR: every wave owns SPT * 64 cells of one sublattice (site m of
+// lane l of wave w at atom(w) * npad + (w_in_atom * SPT + m) * 64 + l) when
+// the block tiles the sublattices, so a wave's sites share one atom's bond
+// matrices.
+template <int SPT, int ZMAX, bool UNI, bool ATOM_MAJOR = false>
 __global__ void __launch_bounds__(dopri5_max_block<SPT>()) k_dopri5(LatDev Lk, Dopri5Args A) {
     LatDev L0 = Lk;
     L0.mats += size_t(blockIdx.x) * A.mats_stride;
@@ -138,6 +142,10 @@ __global__ void __launch_bounds__(dopri5_max_block<SPT>()) k_dopri5(LatDev Lk, D
         bufA.store(p, v);
     }
 
+    const int wpa = ATOM_MAJOR ? L0.npad / (kWave * SPT) : 1;  // waves per atom
+    const int atom_w = ATOM_MAJOR ? wave / wpa : 0;
+    const int pbase = ATOM_MAJOR ? atom_w * L0.npad + (wave - atom_w * wpa) * (SPT * kWave) + lane : tid;
+    auto site = [&](int m) { return ATOM_MAJOR ? pbase + m * kWave : tid + m * T; };
     double x[SPT][3], k1[SPT][3], k2[SPT][3], k3[SPT][3], k4[SPT][3], k5[SPT][3];
     double xn[SPT][3], k7[SPT][3];
     int nb[SPT][ZMAX];
@@ -145,9 +153,9 @@ __global__ void __launch_bounds__(dopri5_max_block<SPT>()) k_dopri5(LatDev Lk, D
     bool act[SPT];
 #pragma unroll
     for (int m = 0; m < SPT; ++m) {
-        const int p = tid + m * T;
-        const int pw = wave * kWave + m * T;  // wave-uniform
-        atm[m] = uniform(pw < NP ? pw / L0.npad : 0);
+        const int p = site(m);
+        const int pw = ATOM_MAJOR ? atom_w * L0.npad : wave * kWave + m * T;  // wave-uniform
+        atm[m] = ATOM_MAJOR ? atom_w : uniform(pw < NP ? pw / L0.npad : 0);
         act[m] = active_site(L0, p);
         const int c = UNI ? uniform(pw < NP ? as_const(L0.cnt_atom)[atm[m]] : 0) : (p < NP ? L0.cnt[p] : 0);
         cnt[m] = c;
@@ -157,7 +165,7 @@ __global__ void __launch_bounds__(dopri5_max_block<SPT>()) k_dopri5(LatDev Lk, D
     __syncthreads();
 #pragma unroll
     for (int m = 0; m < SPT; ++m) {
-        const int p = tid + m * T;
+        const int p = site(m);
         if (act[m]) {
             bufA.load(p, x[m]);
         } else {
@@ -181,10 +189,10 @@ __global__ void __launch_bounds__(dopri5_max_block<SPT>()) k_dopri5(LatDev Lk, D
         for (int j = 0; j < 10; ++j) part[j] = 0.0;
 #pragma unroll
         for (int m = 0; m < SPT; ++m) {
-            const int p = tid + m * T + zz;
+            const int p = site(m) + zz;
             k[m][0] = k[m][1] = k[m][2] = 0.0;
             if (act[m]) {
-                const int atom = atm[m];
+                const int atom = ATOM_MAJOR ? atom_w : atm[m];
                 int q[ZMAX];
 #pragma unroll
                 for (int j = 0; j < ZMAX; ++j) q[j] = nb[m][j] + zz;
@@ -230,7 +238,7 @@ __global__ void __launch_bounds__(dopri5_max_block<SPT>()) k_dopri5(LatDev Lk, D
     auto store_stage = [&](const LdsState& dst, const double (&y)[SPT][3]) {
 #pragma unroll
         for (int m = 0; m < SPT; ++m)
-            if (act[m]) dst.store(tid + m * T, y[m]);
+            if (act[m]) dst.store(site(m), y[m]);
     };
 
     // One attempt of size h from (x, k1) at t: stages 2..7 (inputs alternate
@@ -478,7 +486,7 @@ __global__ void __launch_bounds__(dopri5_max_block<SPT>()) k_dopri5(LatDev Lk, D
         double* out = A.final_states + size_t(b) * L0.N * 3;
 #pragma unroll
         for (int m = 0; m < SPT; ++m) {
-            const int p = tid + m * T;
+            const int p = site(m);
             if (act[m]) {
                 const int cell = p - atm[m] * L0.npad;
                 const size_t i = size_t(cell) * L0.n_atoms + atm[m];
@@ -492,9 +500,9 @@ __global__ void __launch_bounds__(dopri5_max_block<SPT>()) k_dopri5(LatDev Lk, D
 
 namespace {
 
-template <int SPT, int ZMAX, bool UNI>
+template <int SPT, int ZMAX, bool UNI, bool ATOM_MAJOR = false>
 hipError_t launch_d(const LatDev& L, const Dopri5Args& A, int block, size_t lds, hipStream_t st) {
-    auto kern = k_dopri5<SPT, ZMAX, UNI>;
+    auto kern = k_dopri5<SPT, ZMAX, UNI, ATOM_MAJOR>;
     hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
                                        hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));
     if (e != hipSuccess) return e;
@@ -504,6 +512,8 @@ hipError_t launch_d(const LatDev& L, const Dopri5Args& A, int block, size_t lds,
 
 template <int SPT, int ZMAX>
 hipError_t launch_d_uni(const LatDev& L, const Dopri5Args& A, int block, size_t lds, hipStream_t st) {
+    if (L.uniform && L.npad % (kWave * SPT) == 0 && block == L.n_atoms * (L.npad / (kWave * SPT)) * kWave)
+        return launch_d<SPT, ZMAX, true, true>(L, A, block, lds, st);
     return L.uniform ? launch_d<SPT, ZMAX, true>(L, A, block, lds, st)
                      : launch_d<SPT, ZMAX, false>(L, A, block, lds, st);
 }
