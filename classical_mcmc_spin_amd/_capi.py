@@ -71,6 +71,7 @@ SIGNATURES = {
     "csl_set_launch_config": (_i, [_vp, _sz, _i, _i, _i]),
     "csl_set_batch_split": (_i, [_vp, _i]),
     "csl_last_grid": (_i, [_vp, _pi]),
+    "csl_split_schedule": (_i, [_sz, _sz, _sz, _sz, _pi, _psz]),
     "csl_set_replica_hamiltonians": (_i, [_vp, _sz, _pd, _pd]),
     "csl_quench": (_i, [_vp, _sz, _pd, _sz, _d, _d, _psz]),
     "csl_nonlinear_spectrum": (_i, [_sz, _sz, _sz, _pd, _pd, _pd, _sz, _pd, _psz, _psz]),

@@ -322,6 +322,44 @@ struct BatchOut {
     size_t n_steps = 0, n_rec = 0;
 };
 
+// Work split of B trajectories of n steps over G workgroups (B > G):
+// workgroup g takes the range [g*W, (g+1)*W) of the trajectory-major step
+// order, W = ceil(B*n/G) >= n, as pieces (trajectory, first step, end step,
+// slot to load, slot to save): first the head of the trajectory it shares
+// with g+1 (saved to slot g), then its whole trajectories, last the rest of
+// the trajectory begun by g-1 (loaded from slot g-1).  Rows [G][max_p][5],
+// unused pieces -1.
+static std::vector<int> split_schedule(size_t B, size_t G, size_t n, size_t& max_p) {
+    const size_t W = B * n, per = (W + G - 1) / G;
+    std::vector<std::vector<std::array<int, 5>>> pc(G);
+    max_p = 1;
+    for (size_t g = 0; g < G; ++g) {
+        const size_t a = g * per, e = std::min(W, (g + 1) * per);
+        std::vector<std::array<int, 5>> lay;
+        for (size_t u = a; u < e;) {
+            const size_t j = u / n, s0 = u - j * n, s1 = std::min(n, e - j * n);
+            lay.push_back({int(j), int(s0), int(s1), -1, -1});
+            u = j * n + s1;
+        }
+        auto& out = pc[g];
+        if (!lay.empty() && lay.back()[1] == 0 && size_t(lay.back()[2]) < n) {
+            lay.back()[4] = int(g);
+            out.push_back(lay.back());
+            lay.pop_back();
+        }
+        const bool tail = !lay.empty() && lay.front()[1] > 0;
+        if (tail) lay.front()[3] = int(g) - 1;
+        for (size_t i = tail ? 1 : 0; i < lay.size(); ++i) out.push_back(lay[i]);
+        if (tail) out.push_back(lay.front());
+        max_p = std::max(max_p, out.size());
+    }
+    std::vector<int> rows(G * max_p * 5, -1);
+    for (size_t g = 0; g < G; ++g)
+        for (size_t i = 0; i < pc[g].size(); ++i)
+            for (int k = 0; k < 5; ++k) rows[(g * max_p + i) * 5 + k] = pc[g][i][k];
+    return rows;
+}
+
 // Runs the persistent kernel; device outputs are left in h->rec_mags /
 // h->rec_energy / h->bt_final.
 void run_persistent(csl_lattice* h, size_t B, const double* d_init, size_t init_stride,
@@ -372,33 +410,8 @@ void run_persistent(csl_lattice* h, size_t B, const double* d_init, size_t init_
     A.sched = nullptr;
     h->last_grid = int(B);
     if (G > 0 && B > G && n_steps > 0 && G <= size_t(std::numeric_limits<int>::max())) {
-        const size_t n = n_steps, W = B * n, per = (W + G - 1) / G;
-        std::vector<std::vector<std::array<int, 5>>> pc(G);
         size_t max_p = 1;
-        for (size_t g = 0; g < G; ++g) {
-            const size_t a = g * per, e = std::min(W, (g + 1) * per);
-            std::vector<std::array<int, 5>> lay;
-            for (size_t u = a; u < e;) {
-                const size_t j = u / n, s0 = u - j * n, s1 = std::min(n, e - j * n);
-                lay.push_back({int(j), int(s0), int(s1), -1, -1});
-                u = j * n + s1;
-            }
-            auto& out = pc[g];
-            if (!lay.empty() && lay.back()[1] == 0 && size_t(lay.back()[2]) < n) {
-                lay.back()[4] = int(g);  // the first part of a trajectory that g+1 finishes
-                out.push_back(lay.back());
-                lay.pop_back();
-            }
-            const bool tail = !lay.empty() && lay.front()[1] > 0;
-            if (tail) lay.front()[3] = int(g) - 1;  // finished here, begun by g-1
-            for (size_t i = tail ? 1 : 0; i < lay.size(); ++i) out.push_back(lay[i]);
-            if (tail) out.push_back(lay.front());
-            max_p = std::max(max_p, out.size());
-        }
-        std::vector<int> sched(G * max_p * 5, -1);
-        for (size_t g = 0; g < G; ++g)
-            for (size_t i = 0; i < pc[g].size(); ++i)
-                for (int k = 0; k < 5; ++k) sched[(g * max_p + i) * 5 + k] = pc[g][i][k];
+        const std::vector<int> sched = split_schedule(B, G, n_steps, max_p);
         h->ps_sched.upload(sched.data(), sched.size(), s);
         h->ps_saved.ensure(G * 3 * size_t(h->L.NP));
         h->ps_flags.ensure(G);
@@ -1825,6 +1838,19 @@ int csl_set_launch_config(csl_lattice* h, size_t B, int block, int sites_per_thr
                 return;
             }
         fail(CSL_ERR_ARG, "launch shape does not fit this lattice");
+    });
+}
+
+int csl_split_schedule(size_t B, size_t G, size_t n_steps, size_t max_rows, int* rows, size_t* pieces_per_wg) {
+    return guarded([&] {
+        if (G == 0 || B <= G || n_steps == 0) fail(CSL_ERR_ARG, "a split needs B > G >= 1 and n_steps >= 1");
+        if (B > (size_t(1) << 31) / std::max<size_t>(n_steps, 1)) fail(CSL_ERR_ARG, "too many trajectory-steps");
+        size_t max_p = 1;
+        const auto r = split_schedule(B, G, n_steps, max_p);
+        if (pieces_per_wg) *pieces_per_wg = max_p;
+        if (G * max_p > max_rows) fail(CSL_ERR_CAPACITY, "max_rows is smaller than G * pieces_per_wg");
+        if (!rows) fail(CSL_ERR_NULL, "null rows");
+        std::memcpy(rows, r.data(), r.size() * sizeof(int));
     });
 }
 

@@ -328,6 +328,14 @@ int csl_set_batch_split(csl_lattice* h, int enable);
  * unless the work was split). */
 int csl_last_grid(csl_lattice* h, int* workgroups);
 
+/* The split used for B trajectories of n_steps steps on G workgroups
+ * (B > G): rows[(g*pieces_per_wg + i)*5 + {0..4}] = (trajectory, first step,
+ * end step, slot loaded (-1: initial state), slot saved (-1: none)) of piece
+ * i of workgroup g, -1 rows unused; *pieces_per_wg receives the row stride.
+ * Host-only. */
+int csl_split_schedule(size_t B, size_t G, size_t n_steps, size_t max_rows, int* rows,
+                       size_t* pieces_per_wg);
+
 /* Zero-temperature relaxation of B states in place (states [B*N*3], AoS):
  * Lattice::greedy_quench (lattice.h:3352-3368) — sweeps that set every spin
  * antiparallel to its local field, S = -H/|H| * spin_length (skipped when

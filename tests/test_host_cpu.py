@@ -235,3 +235,47 @@ def test_pulse_segments_product_equals_oracle():
         assert a.shape == b.shape and np.array_equal(a, b), (T0, T1, c, w, st)
     with pytest.raises(_capi.CslError):
         G.pulse_segments(0.0, float("nan"), [0.0], 1.0, 0.1)
+
+
+def test_split_schedule_covers_every_step_once():
+    """csl_split_schedule (the persistent kernel's work split): every step of
+    every trajectory in exactly one piece, pieces of a trajectory contiguous,
+    a cut trajectory begun first on workgroup g (saving slot g) and finished
+    last on g + 1 (loading slot g), and no workgroup with more than
+    ceil(B * n / G) steps (the launch's length)."""
+    L = _capi.lib()
+    rng = np.random.default_rng(5)
+    cases = [(513, 256, 8192), (300, 256, 300), (257, 256, 1), (1000, 7, 13)]
+    cases += [(int(g + rng.integers(1, 3 * g + 5)), int(g), int(rng.integers(1, 400)))
+              for g in rng.integers(1, 300, size=60)]
+    for B, G, n in cases:
+        cap = G * (2 + (B * n + G - 1) // G // n + 2)
+        rows = np.empty(cap * 5, dtype=np.int32)
+        pp = C.c_size_t()
+        _capi.check(L.csl_split_schedule(B, G, n, cap, rows.ctypes.data_as(C.POINTER(C.c_int)), C.byref(pp)))
+        P = pp.value
+        r = rows[:G * P * 5].reshape(G, P, 5)
+        cover = {}
+        heads = {}
+        work = []
+        for g in range(G):
+            used = [q for q in r[g] if q[0] >= 0]
+            work.append(sum(q[2] - q[1] for q in used))
+            for i, (j, s0, s1, si, so) in enumerate(used):
+                assert 0 <= s0 < s1 <= n
+                cover.setdefault(j, []).append((s0, s1))
+                if so >= 0:
+                    assert (i, so, s0) == (0, g, 0) and s1 < n
+                    heads[g] = (j, s1)
+                if si >= 0:
+                    assert i == len(used) - 1 and si == g - 1 and s0 > 0
+        for g in range(G):
+            for (j, s0, s1, si, so) in r[g]:
+                if j >= 0 and si >= 0:
+                    assert heads[si] == (j, s0)
+        assert sorted(cover) == list(range(B))
+        for j, iv in cover.items():
+            iv.sort()
+            assert iv[0][0] == 0 and iv[-1][1] == n and all(a[1] == b[0] for a, b in zip(iv, iv[1:]))
+        assert max(work) == -(-B * n // G)
+    assert L.csl_split_schedule(10, 10, 5, 100, None, None) == _capi.CSL_ERR_ARG
