@@ -72,6 +72,8 @@ SIGNATURES = {
     "csl_set_batch_split": (_i, [_vp, _i]),
     "csl_last_grid": (_i, [_vp, _pi]),
     "csl_split_schedule": (_i, [_sz, _sz, _sz, _sz, _pi, _psz]),
+    "csl_integrate_multi": (_i, [C.POINTER(_vp), _sz, _psz, _pd, _pd, _pd, _d, _d, _d, _sz, _i, _sz, _pd, _pd, _pd,
+                                 _psz]),
     "csl_set_replica_hamiltonians": (_i, [_vp, _sz, _pd, _pd]),
     "csl_quench": (_i, [_vp, _sz, _pd, _sz, _d, _d, _psz]),
     "csl_nonlinear_spectrum": (_i, [_sz, _sz, _sz, _pd, _pd, _pd, _sz, _pd, _psz, _psz]),

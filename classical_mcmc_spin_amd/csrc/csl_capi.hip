@@ -69,6 +69,9 @@ struct csl_lattice {
     DBuf<int> ps_sched;
     DBuf<double> ps_saved;
     DBuf<unsigned> ps_flags, ps_status;
+    DBuf<csl::LatDev> ps_lats;          // lattices of a multi-lattice batch
+    DBuf<int> ps_latof;
+    DBuf<size_t> ps_initoff;
     int split_batches = 1;
     int last_grid = 0;
     DBuf<long long> dp_counters;  // [B][4]
@@ -1054,6 +1057,159 @@ int csl_integrate_batched(csl_lattice* h, size_t B, const double* init, int init
         copy_back(mags, h->rec_mags.p, n_rec * B * 9, s);
         copy_back(energy, h->rec_energy.p, n_rec * B, s);
         hip_check(hipStreamSynchronize(s), "integrate_batched");
+        if (times) {
+            size_t r = 0;
+            for (size_t n = 0; n < n_steps; n += save_interval) times[r++] = grid[n];
+            times[r] = grid[n_steps];
+        }
+    });
+}
+
+int csl_integrate_multi(csl_lattice* const* hs, size_t n_lat, const size_t* B_of, const double* init,
+                        const double* t_pulse, const double* pulse_amp, double T0, double T1, double dt,
+                        size_t save_interval, int tgrid, size_t max_rec, double* times, double* mags,
+                        double* energy, size_t* n_rec_out) {
+    return guarded([&] {
+        if (!hs || !B_of || !init || !t_pulse) fail(CSL_ERR_NULL, "null handles / B_of / init / t_pulse");
+        if (n_lat == 0) fail(CSL_ERR_ARG, "need at least one lattice");
+        size_t B = 0, n_init = 0, maxNP = 0, rep = 0;
+        int zmax = 0;
+        for (size_t i = 0; i < n_lat; ++i) {
+            check_handle(hs[i]);
+            if (hs[i]->device != hs[0]->device) fail(CSL_ERR_ARG, "all lattices must live on one device");
+            if (hs[i]->rep_B) fail(CSL_ERR_UNSUPPORTED, "per-replica Hamiltonians are not combined with lattices");
+            if (hs[i]->L.uniform != hs[0]->L.uniform)
+                fail(CSL_ERR_UNSUPPORTED, "the lattices must all have translation-invariant tables or none");
+            if (B_of[i] == 0) fail(CSL_ERR_ARG, "every lattice needs at least one trajectory");
+            B += B_of[i];
+            n_init += B_of[i] * hs[i]->N * 3;
+            if (size_t(hs[i]->L.NP) > maxNP) {
+                maxNP = size_t(hs[i]->L.NP);
+                rep = i;
+            }
+            zmax = std::max(zmax, hs[i]->L.zmax);
+        }
+        if (B > 65535) fail(CSL_ERR_ARG, "at most 65535 trajectories per call");
+        if (save_interval == 0) fail(CSL_ERR_ARG, "save_interval must be >= 1");
+        csl_lattice* h = hs[0];
+        auto grid = make_grid(T0, T1, dt, tgrid);
+        const size_t n_steps = grid.size() - 1;
+        const size_t n_rec = records_for(n_steps, save_interval);
+        if (n_rec_out) *n_rec_out = n_rec;
+        if ((times || mags || energy) && max_rec < n_rec)
+            fail(CSL_ERR_CAPACITY, "max_rec is smaller than the number of records");
+        if (n_steps > size_t(std::numeric_limits<int>::max())) fail(CSL_ERR_ARG, "too many steps");
+        check_finite(init, n_init, "initial states");
+        check_finite(t_pulse, 2 * B, "t_pulse");
+        // launch shape: the fewest sites per thread whose block holds the
+        // largest lattice and whose LDS holds every lattice
+        csl::LatDev Lrep = hs[rep]->L;
+        Lrep.zmax = zmax;
+        const int flags = csl::kPersistDoubleBuffer | csl::kPersistUnrollStages | csl::kPersistMulti;
+        int block = 0, spt = 0;
+        size_t lds = 0;
+        for (int c : {1, 2, 3, 4, 6}) {
+            int bl = int((maxNP + size_t(c) - 1) / size_t(c));
+            bl = (bl + 63) / 64 * 64;
+            if (bl > csl::persistent_block_limit(c)) continue;
+            size_t l = 0;
+            for (size_t i = 0; i < n_lat; ++i) l = std::max(l, csl::persistent_lds_bytes(hs[i]->L, bl, true));
+            if (l > h->lds_limit) continue;
+            block = bl;
+            spt = c;
+            lds = l;
+            break;
+        }
+        if (!block) fail(CSL_ERR_UNSUPPORTED, "the largest lattice does not fit one workgroup");
+        (void)lds;
+        hipStream_t s = h->stream;
+        std::vector<csl::LatDev> lats(n_lat);
+        std::vector<int> lat_of(B);
+        std::vector<size_t> off(B);
+        std::vector<double> am(2 * B);
+        for (size_t i = 0, b = 0, o = 0; i < n_lat; ++i)
+            for (size_t k = 0; k < B_of[i]; ++k, ++b) {
+                lats[i] = hs[i]->L;
+                lat_of[b] = int(i);
+                off[b] = o;
+                o += hs[i]->N * 3;
+                am[2 * b] = am[2 * b + 1] = hs[i]->amp;
+            }
+        if (pulse_amp) {
+            check_finite(pulse_amp, 2 * B, "pulse_amp");
+            am.assign(pulse_amp, pulse_amp + 2 * B);
+        }
+        // every lattice's tables must be on the device before the launch on
+        // the first lattice's stream
+        for (size_t i = 1; i < n_lat; ++i) hip_check(hipStreamSynchronize(hs[i]->stream), "lattice tables");
+        h->bt_init.upload(init, n_init, s);
+        h->bt_tp.upload(t_pulse, 2 * B, s);
+        h->bt_amp.upload(am.data(), am.size(), s);
+        h->ps_lats.upload(lats.data(), n_lat, s);
+        h->ps_latof.upload(lat_of.data(), B, s);
+        h->ps_initoff.upload(off.data(), B, s);
+        h->tgrid.upload(grid.data(), grid.size(), s);
+        h->rec_mags.ensure(n_rec * B * 9);
+        if (energy) h->rec_energy.ensure(n_rec * B);
+        csl::PersistArgs A{};
+        A.B = int(B);
+        A.init = h->bt_init.p;
+        A.t_pulse = h->bt_tp.p;
+        A.amp = h->bt_amp.p;
+        A.tgrid = h->tgrid.p;
+        A.n_steps = int(n_steps);
+        A.save_interval = int(save_interval);
+        A.n_rec = int(n_rec);
+        A.dt = dt;
+        A.mags = h->rec_mags.p;
+        A.energy = energy ? h->rec_energy.p : nullptr;
+        A.lats = h->ps_lats.p;
+        A.lat_of = h->ps_latof.p;
+        A.init_off = h->ps_initoff.p;
+        const int occ = h->split_batches ? csl::persistent_occupancy(Lrep, block, spt, flags) : 0;
+        const size_t G = size_t(std::max(occ, 0)) * size_t(h->n_cu);
+        h->last_grid = int(B);
+        if (G > 0 && B > G && n_steps > 0) {
+            size_t max_p = 1;
+            const std::vector<int> sched = split_schedule(B, G, n_steps, max_p);
+            h->ps_sched.upload(sched.data(), sched.size(), s);
+            h->ps_saved.ensure(G * 3 * maxNP);
+            h->ps_flags.ensure(G);
+            h->ps_status.ensure(1);
+            hip_check(hipMemsetAsync(h->ps_flags.p, 0, G * sizeof(unsigned), s), "memset");
+            hip_check(hipMemsetAsync(h->ps_status.p, 0, sizeof(unsigned), s), "memset");
+            A.sched = h->ps_sched.p;
+            A.max_pieces = int(max_p);
+            A.grid = int(G);
+            A.saved = h->ps_saved.p;
+            A.flags = h->ps_flags.p;
+            A.status = h->ps_status.p;
+            h->last_grid = int(G);
+        }
+        hipEvent_t e0, e1;
+        hip_check(hipEventCreate(&e0), "event");
+        hip_check(hipEventCreate(&e1), "event");
+        hip_check(hipEventRecord(e0, s), "event record");
+        hip_check(csl::launch_persistent(Lrep, A, block, spt, flags, s), "k_persistent launch");
+        hip_check(hipEventRecord(e1, s), "event record");
+        hip_check(hipEventSynchronize(e1), "k_persistent");
+        float ms = 0.f;
+        hip_check(hipEventElapsedTime(&ms, e0, e1), "elapsed");
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        h->last_ms = ms;
+        h->last_block = block;
+        h->last_spt = spt;
+        h->last_dbuf = flags;
+        h->last_launches = 1;
+        if (A.sched) {
+            unsigned st = 0;
+            hip_check(hipMemcpy(&st, h->ps_status.p, sizeof(unsigned), hipMemcpyDeviceToHost), "D2H");
+            if (st) fail(CSL_ERR_HIP, "a workgroup of the split batch waited too long for a trajectory's state");
+        }
+        copy_back(mags, h->rec_mags.p, n_rec * B * 9, s);
+        copy_back(energy, h->rec_energy.p, n_rec * B, s);
+        hip_check(hipStreamSynchronize(s), "integrate_multi");
         if (times) {
             size_t r = 0;
             for (size_t n = 0; n < n_steps; n += save_interval) times[r++] = grid[n];

@@ -38,6 +38,11 @@ struct PersistArgs {
     double* saved;           // [slots][NP][3] device-order states at the seams
     unsigned* flags;         // [slots], zero before the launch
     unsigned* status;        // set to 1 if a wait for a slot gave up
+    // Lattices of different sizes in one launch (kPersistMulti): trajectory b
+    // runs on lats[lat_of[b]] and starts from init + init_off[b]
+    const LatDev* lats;
+    const int* lat_of;
+    const size_t* init_off;
 };
 
 // Discrete-adjoint gradient run (csl_adjoint.hip): one workgroup per trajectory.
@@ -150,6 +155,7 @@ constexpr int persistent_max_block() {
 // Launch-shape flags of the persistent kernel.
 constexpr int kPersistDoubleBuffer = 1;  // two LDS state buffers, one barrier per stage
 constexpr int kPersistUnrollStages = 2;  // the four RK stages specialised at compile time
+constexpr int kPersistMulti = 4;         // per-trajectory lattices (PersistArgs::lats)
 
 // Launches the persistent kernel, or with occupancy != null only stores its
 // co-resident workgroups per CU for this shape.

@@ -48,12 +48,15 @@ __device__ __forceinline__ double site_energy_atom(const LatDev& L, int atom, co
 // sublattice, site m of lane l of wave w at atom(w) * npad + (w_in_atom *
 // SPT + m) * 64 + l, so all owned sites share one atom and its bond matrices
 // are loaded once per bond for all of them (instead of once per site slot).
-template <int SPT, int ZMAX, bool UNI, bool DBUF, bool UNROLL, bool FIELD_ONLY, bool ATOM_MAJOR = false>
+// MULTI: every trajectory carries its own lattice (A.lats[A.lat_of[b]]),
+// so the LDS layout and the site ownership are set up per piece.
+template <int SPT, int ZMAX, bool UNI, bool DBUF, bool UNROLL, bool FIELD_ONLY, bool ATOM_MAJOR = false,
+          bool MULTI = false>
 __global__ void __launch_bounds__(persistent_max_block<SPT>()) k_persistent(LatDev Lk, PersistArgs A) {
     extern __shared__ double smem[];
-    const int NP = Lk.NP;
-    const LdsState bufA = lds_state(smem, NP);
-    const LdsState bufB = DBUF ? lds_state(smem + 3 * NP, NP) : bufA;
+    int NP = Lk.NP;
+    LdsState bufA = lds_state(smem, NP);
+    LdsState bufB = DBUF ? lds_state(smem + 3 * NP, NP) : bufA;
     double* drv = smem + (DBUF ? 6 : 3) * NP;  // [2 parity][3 stage times][2 pulses]
     double* red = drv + 12;                    // [nwaves][10]
 
@@ -74,17 +77,20 @@ __global__ void __launch_bounds__(persistent_max_block<SPT>()) k_persistent(LatD
     int nb[SPT][ZMAX];
     int cnt[SPT], atm[SPT];
     bool act[SPT];
+    auto own = [&](const LatDev& Lo) {
 #pragma unroll
-    for (int m = 0; m < SPT; ++m) {
-        const int p = site(m);
-        const int pw = ATOM_MAJOR ? atom_w * Lk.npad : wave * kWave + m * T;  // wave-uniform
-        atm[m] = ATOM_MAJOR ? atom_w : uniform(pw < NP ? pw / Lk.npad : 0);
-        act[m] = active_site(Lk, p);
-        const int c = UNI ? uniform(pw < NP ? as_const(Lk.cnt_atom)[atm[m]] : 0) : (p < NP ? Lk.cnt[p] : 0);
-        cnt[m] = c;
+        for (int m = 0; m < SPT; ++m) {
+            const int p = site(m);
+            const int pw = ATOM_MAJOR ? atom_w * Lo.npad : wave * kWave + m * T;  // wave-uniform
+            atm[m] = ATOM_MAJOR ? atom_w : uniform(pw < NP ? pw / Lo.npad : 0);
+            act[m] = active_site(Lo, p);
+            const int c = UNI ? uniform(pw < NP ? as_const(Lo.cnt_atom)[atm[m]] : 0) : (p < NP ? Lo.cnt[p] : 0);
+            cnt[m] = c;
 #pragma unroll
-        for (int k = 0; k < ZMAX; ++k) nb[m][k] = (act[m] && k < c) ? Lk.col[size_t(k) * NP + p] : 0;
-    }
+            for (int k = 0; k < ZMAX; ++k) nb[m][k] = (act[m] && k < c) ? Lo.col[size_t(k) * NP + p] : 0;
+        }
+    };
+    if (!MULTI) own(Lk);
 
     const int n_pieces = A.sched ? A.max_pieces : 1;
     for (int piece = 0; piece < n_pieces; ++piece) {
@@ -99,10 +105,21 @@ __global__ void __launch_bounds__(persistent_max_block<SPT>()) k_persistent(LatD
         if (b < 0) continue;  // uniform
     }
     // this trajectory's Hamiltonian tables
-    LatDev L0 = Lk;
-    L0.mats += size_t(b) * A.mats_stride;
-    L0.field += size_t(b) * A.field_stride;
+    LatDev L0 = MULTI ? A.lats[A.lat_of[b]] : Lk;
+    if (!MULTI) {
+        L0.mats += size_t(b) * A.mats_stride;
+        L0.field += size_t(b) * A.field_stride;
+    }
     const LatDev& L = L0;
+    if (MULTI) {
+        __syncthreads();  // the previous piece is done with the LDS buffers and red[]
+        NP = L.NP;
+        bufA = lds_state(smem, NP);
+        bufB = DBUF ? lds_state(smem + 3 * NP, NP) : bufA;
+        drv = smem + (DBUF ? 6 : 3) * NP;
+        red = drv + 12;
+        own(L);
+    }
 
     const double tp1 = A.t_pulse[2 * b], tp2 = A.t_pulse[2 * b + 1];
     const double am1 = A.amp[2 * b], am2 = A.amp[2 * b + 1];
@@ -112,7 +129,7 @@ __global__ void __launch_bounds__(persistent_max_block<SPT>()) k_persistent(LatD
     __syncthreads();  // the previous piece is done with the LDS buffers
     if (slot_in < 0) {
         // initial state -> LDS (device order)
-        const double* init = A.init + size_t(b) * A.init_stride;
+        const double* init = A.init + (MULTI ? A.init_off[b] : size_t(b) * A.init_stride);
         for (int p = tid; p < NP; p += T) {
             const int atom = p / L.npad, cell = p - atom * L.npad;
             double v[3] = {0.0, 0.0, 0.0};
@@ -139,7 +156,7 @@ __global__ void __launch_bounds__(persistent_max_block<SPT>()) k_persistent(LatD
         }
         __syncthreads();
         __atomic_thread_fence(__ATOMIC_ACQUIRE);
-        const double* sv = A.saved + size_t(slot_in) * 3 * NP;
+        const double* sv = A.saved + size_t(slot_in) * 3 * size_t(Lk.NP);
         for (int p = tid; p < NP; p += T) {
             const double v[3] = {sv[3 * p], sv[3 * p + 1], sv[3 * p + 2]};
             bufA.store(p, v);
@@ -301,7 +318,7 @@ __global__ void __launch_bounds__(persistent_max_block<SPT>()) k_persistent(LatD
 
     if (slot_out >= 0) {
         // hand the state at s_end to the piece that continues the trajectory
-        double* sv = A.saved + size_t(slot_out) * 3 * NP;
+        double* sv = A.saved + size_t(slot_out) * 3 * size_t(Lk.NP);
 #pragma unroll
         for (int m = 0; m < SPT; ++m) {
             const int p = site(m);
@@ -351,9 +368,10 @@ __global__ void __launch_bounds__(persistent_max_block<SPT>()) k_persistent(LatD
 
 namespace {
 
-template <int SPT, int ZMAX, bool UNI, bool DBUF, bool UNROLL, bool FIELD_ONLY = false, bool ATOM_MAJOR = false>
+template <int SPT, int ZMAX, bool UNI, bool DBUF, bool UNROLL, bool FIELD_ONLY = false, bool ATOM_MAJOR = false,
+          bool MULTI = false>
 hipError_t launch_p(const LatDev& L, const PersistArgs& A, int block, size_t lds, hipStream_t st, int* occ) {
-    auto kern = k_persistent<SPT, ZMAX, UNI, DBUF, UNROLL, FIELD_ONLY, ATOM_MAJOR>;
+    auto kern = k_persistent<SPT, ZMAX, UNI, DBUF, UNROLL, FIELD_ONLY, ATOM_MAJOR, MULTI>;
     hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
                                        hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));
     if (e != hipSuccess) return e;
@@ -366,6 +384,7 @@ template <int SPT, int ZMAX, bool UNI>
 hipError_t launch_p_db(const LatDev& L, const PersistArgs& A, int block, int flags, size_t lds,
                        hipStream_t st, int* occ) {
     const bool dbuf = flags & kPersistDoubleBuffer, unroll = flags & kPersistUnrollStages;
+    if (flags & kPersistMulti) return launch_p<SPT, ZMAX, UNI, true, true, false, false, true>(L, A, block, lds, st, occ);
     if (unroll && dbuf && UNI && L.site_uniform && !L.has_onsite && L.full_coord && L.zmax == ZMAX) {
         const bool atom_major = L.npad % (kWave * SPT) == 0 && block == L.n_atoms * (L.npad / (kWave * SPT)) * kWave;
         if (atom_major) return launch_p<SPT, ZMAX, UNI, true, true, true, true>(L, A, block, lds, st, occ);

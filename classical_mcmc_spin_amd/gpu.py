@@ -360,6 +360,34 @@ class GPULattice:
 # ---------------------------------------------------------------------------
 # Reference-style simulator entry points (CPU semantics, GPU arithmetic)
 
+def integrate_multi(lattices, inits, t_pulse, pulse_amp=None, T0=0.0, T1=1.0, dt=0.01, save_interval=1,
+                    tgrid="batched", want_energy=True):
+    """csl_integrate_multi: RK4 trajectories on lattices of different sizes
+    in one persistent launch.  lattices: GPULattice handles on one device;
+    inits[i]: [B_i, N_i, 3] states of lattice i; t_pulse, pulse_amp: [B, 2]
+    per trajectory in lattice order (pulse_amp None: each lattice's own
+    amplitude).  Returns dict(times, mags [n, B, 3, 3], energy [n, B])."""
+    if not lattices or len(lattices) != len(inits):
+        raise ValueError("need one [B_i, N_i, 3] array of initial states per lattice")
+    ins = [f64(x).reshape(-1, gl.N, 3) for gl, x in zip(lattices, inits)]
+    b_of = np.array([x.shape[0] for x in ins], dtype=np.uintp)
+    B = int(b_of.sum())
+    tp = f64(t_pulse).reshape(B, 2)
+    amp = None if pulse_amp is None else f64(pulse_amp).reshape(B, 2)
+    init = np.ascontiguousarray(np.concatenate([x.reshape(-1) for x in ins]))
+    _, nr = lattices[0].count_records(T0, T1, dt, save_interval, tgrid)
+    times = np.empty(nr)
+    mags = np.empty(nr * B * 9)
+    energy = np.empty(nr * B) if want_energy else None
+    hs = (C.c_void_p * len(lattices))(*[gl.h for gl in lattices])
+    n = C.c_size_t()
+    check(lib().csl_integrate_multi(hs, len(lattices), szptr(b_of), dptr(init), dptr(tp.reshape(-1)),
+                                    dptr(None if amp is None else amp.reshape(-1)), T0, T1, dt, save_interval,
+                                    _capi.TGRID[tgrid], nr, dptr(times), dptr(mags), dptr(energy), C.byref(n)))
+    return {"times": times, "mags": mags.reshape(nr, B, 3, 3),
+            "energy": None if energy is None else energy.reshape(nr, B)}
+
+
 def _drive_dirs(lattice: Lattice, field_in) -> np.ndarray:
     """Lattice::set_pulse frame rotation field_drive = R_atom * B (lattice_md.cpp:215-216)."""
     fi = np.asarray(field_in, dtype=float).reshape(lattice.N_atoms, lattice.spin_dim)

@@ -247,6 +247,22 @@ int csl_pump_probe_ex(csl_lattice* h, const double* ground, const double* field_
                       double* times, double* M0, double* M1, double* M01,
                       double* tau_values, int* used_w1, double* max_dsdt);
 
+/* Trajectories on lattices of different sizes in one persistent launch:
+ * lattice i (handle hs[i], its own Hamiltonian, drive directions, width and
+ * frequency) runs B_of[i] trajectories, taken in handle order, from the
+ * states init (per lattice B_of[i] states of N_i*3 doubles, reference
+ * order, concatenated); t_pulse / pulse_amp [B*2] per trajectory (pulse_amp
+ * NULL: each handle's amplitude); records as csl_integrate_batched for
+ * B = sum B_of[i]: times [max_rec], mags [max_rec*B*9], energy [max_rec*B]
+ * (E/N of the trajectory's lattice; may be NULL).  RK4 on the time grid of
+ * tgrid; the handles must share one device and either all or none have
+ * translation-invariant tables.  Timing and launch shape are reported on
+ * hs[0].  No counterpart in the reference (one lattice per integration). */
+int csl_integrate_multi(csl_lattice* const* hs, size_t n_lat, const size_t* B_of, const double* init,
+                        const double* t_pulse, const double* pulse_amp, double T0, double T1, double dt,
+                        size_t save_interval, int tgrid, size_t max_rec, double* times, double* mags,
+                        double* energy, size_t* n_rec);
+
 /* ---- Dormand-Prince 5(4) --------------------------------------------- */
 
 /* Integrates B trajectories with the Dormand-Prince 5(4) pair, one

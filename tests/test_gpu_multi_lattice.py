@@ -1,0 +1,86 @@
+"""Trajectories on lattices of different sizes in one persistent launch
+(csl_integrate_multi).  Each trajectory's records must match the same
+trajectory run alone on its lattice (csl_integrate_batched) to 1e-12 — the
+site ownership differs between the two launch shapes, so the sums behind the
+magnetisations are taken in another order — and the oracle to 1e-10.  The
+second case holds more trajectories than workgroups fit the GPU at once, so
+the work split (pieces continued from saved states) runs across lattices."""
+import numpy as np
+import pytest
+
+from classical_mcmc_spin_amd import GPULattice, seeding
+from classical_mcmc_spin_amd.gpu import integrate_multi
+from helpers import kitaev, unit_x_field
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def po():
+    import pyoracle
+    pyoracle.lib()
+    return pyoracle
+
+
+def _drive(gl, lat, amp, width, freq):
+    f = unit_x_field(lat)
+    fd = np.einsum("aij,aj->ai", lat.frames, f).reshape(-1)
+    gl.set_pulse(np.concatenate([fd, fd]), amp, width, freq, 0.0, 0.0)
+    return f
+
+
+def _run_case(po, sizes, counts, n, check_split=False):
+    dt = 0.01
+    T1 = (n + 0.5) * dt
+    lats = [kitaev(L, L, h=0.1 + 0.05 * i) for i, L in enumerate(sizes)]
+    B = sum(counts)
+    tp = np.zeros((B, 2))
+    tp[:, 1] = 0.013 * np.arange(B)
+    am = 0.3 + 0.1 * (np.arange(2 * B).reshape(B, 2) % 3)
+    inits = [np.stack([seeding.random_spins(lat.lattice_size, 100 * i + k) for k in range(c)])
+             for i, ((_, lat), c) in enumerate(zip(lats, counts))]
+    gls = [GPULattice(lat) for _, lat in lats]
+    try:
+        fields = [_drive(gl, lat, 0.5, 0.3 + 0.1 * i, 1.0 + 0.5 * i) for i, (gl, (_, lat)) in enumerate(zip(gls, lats))]
+        multi = integrate_multi(gls, inits, tp, am, 0.0, T1, dt, tgrid="boost")
+        grid = gls[0].last_grid()
+        print(f"lattices {sizes} x {counts}: {grid} workgroups", flush=True)
+        if check_split:
+            assert grid < B, f"{grid} workgroups hold all {B} trajectories: no split"
+        off = 0
+        for i, (gl, c) in enumerate(zip(gls, counts)):
+            alone = gl.integrate_batched(inits[i], tp[off:off + c], am[off:off + c], 0.0, T1, dt, tgrid="boost")
+            assert np.array_equal(multi["times"], alone["times"])
+            assert np.max(np.abs(multi["mags"][:, off:off + c] - alone["mags"])) <= 1e-12, i
+            assert np.max(np.abs(multi["energy"][:, off:off + c] - alone["energy"])) <= 1e-12, i
+            off += c
+    finally:
+        for gl in gls:
+            gl.close()
+    # the last trajectory of every lattice against the oracle
+    off = 0
+    for i, ((uc, lat), c, L) in enumerate(zip(lats, counts, sizes)):
+        b = off + c - 1
+        o = po.OracleLattice(uc, L, L, 1)
+        f = fields[i]
+        o.set_pulse(f * am[b, 0], tp[b, 0], f * am[b, 1], tp[b, 1], 1.0, 0.3 + 0.1 * i, 1.0 + 0.5 * i)
+        ref = o.integrate_md(inits[i][c - 1].reshape(-1), 0.0, T1, dt)
+        assert np.max(np.abs(multi["mags"][:, b] - ref["mags"])) <= 1e-10, (i, b)
+        off += c
+
+
+def test_three_sizes_one_launch(po):
+    _run_case(po, (8, 12, 6), (3, 2, 4), 200)
+
+
+def test_split_across_lattices(po):
+    """36x36 (130 KB of LDS state: one workgroup per CU) with 12x12: 300
+    trajectories over the workgroups that fit."""
+    _run_case(po, (36, 12), (200, 100), 120, check_split=True)
+
+
+def test_rejects_mismatched_inputs():
+    uc, lat = kitaev(6, 6, h=0.1)
+    with GPULattice(lat) as gl:
+        with pytest.raises(ValueError):
+            integrate_multi([gl], [], np.zeros((1, 2)))
