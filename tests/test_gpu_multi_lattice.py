@@ -84,3 +84,39 @@ def test_rejects_mismatched_inputs():
     with GPULattice(lat) as gl:
         with pytest.raises(ValueError):
             integrate_multi([gl], [], np.zeros((1, 2)))
+
+
+def test_mixed_coordination_one_launch(po):
+    """Kitaev 8x8 (3 bonds per site, driven) with pyrochlore 2x2x2 (6 bonds
+    per site, no drive directions set) in one launch: the bond loop runs at
+    the larger coordination, each site over its own bond count."""
+    from classical_mcmc_spin_amd import workloads
+    uc, kl = kitaev(8, 8, h=0.1)
+    pl = workloads.pyrochlore_lattice(2)
+    Bk, Bp = 3, 3
+    ki = np.stack([seeding.random_spins(kl.lattice_size, 40 + k) for k in range(Bk)])
+    pi, _ = workloads.pyrochlore_tilted_aiao(pl, Bp, seed=21)
+    B = Bk + Bp
+    tp = np.zeros((B, 2))
+    tp[:, 1] = 0.1 * np.arange(B)
+    am = np.full((B, 2), 0.4)
+    dt, n = 0.01, 200
+    T1 = (n + 0.5) * dt
+    with GPULattice(kl) as gk, GPULattice(pl) as gp:
+        f = _drive(gk, kl, 0.4, 0.3, 1.0)
+        multi = integrate_multi([gk, gp], [ki, pi], tp, am, 0.0, T1, dt, tgrid="boost")
+        ak = gk.integrate_batched(ki, tp[:Bk], am[:Bk], 0.0, T1, dt, tgrid="boost")
+        ap = gp.integrate_batched(pi, tp[Bk:], am[Bk:], 0.0, T1, dt, tgrid="boost")
+    assert np.max(np.abs(multi["mags"][:, :Bk] - ak["mags"])) <= 1e-12
+    assert np.max(np.abs(multi["mags"][:, Bk:] - ap["mags"])) <= 1e-12
+    assert np.max(np.abs(multi["energy"][:, Bk:] - ap["energy"])) <= 1e-12
+    ok = po.OracleLattice(uc, 8, 8, 1)
+    b = Bk - 1
+    ok.set_pulse(f * am[b, 0], tp[b, 0], f * am[b, 1], tp[b, 1], 1.0, 0.3, 1.0)
+    ref = ok.integrate_md(ki[b].reshape(-1), 0.0, T1, dt)
+    assert np.max(np.abs(multi["mags"][:, b] - ref["mags"])) <= 1e-10
+    op = po.OracleLattice(pl.unit_cell, 2, 2, 2)
+    for k in range(Bp):
+        ref = op.integrate_md(pi[k].reshape(-1), 0.0, T1, dt)
+        assert np.max(np.abs(multi["mags"][:, Bk + k] - ref["mags"])) <= 1e-10, k
+        assert np.max(np.abs(multi["energy"][:, Bk + k] - ref["energy"])) <= 1e-10, k
