@@ -196,6 +196,36 @@ __global__ void __launch_bounds__(persistent_max_block<SPT>()) k_persistent(LatD
     };
     // wave-reduce the observables of the owned sites and publish them in red[]
     auto reduce_obs = [&](const LatDev& Lr, const double* e) {
+        if constexpr (ATOM_MAJOR) {
+            // every owned site of the wave is in sublattice atom_w: the frame
+            // rotation and the staggered sign act once on the wave's sum
+            double sx[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+            for (int m = 0; m < SPT; ++m) {
+                if (act[m]) {
+#pragma unroll
+                    for (int d = 0; d < 3; ++d) sx[d] += x[m][d];
+                    sx[3] += e[m];
+                }
+            }
+            double v[4];
+#pragma unroll
+            for (int d = 0; d < 3; ++d) v[d] = wave_sum_lane63(sx[d]);
+            v[3] = want_e ? wave_sum_lane63(sx[3]) : 0.0;
+            if (lane == kWave - 1) {
+                cptr<double> R = as_const(Lr.frames) + atom_w * 9;
+                const double sg = as_const(Lr.afm)[atom_w];
+#pragma unroll
+                for (int d = 0; d < 3; ++d) {
+                    const double g = R[d * 3 + 0] * v[0] + R[d * 3 + 1] * v[1] + R[d * 3 + 2] * v[2];
+                    red[wave * 10 + d] = sg * g;
+                    red[wave * 10 + 3 + d] = v[d];
+                    red[wave * 10 + 6 + d] = g;
+                }
+                red[wave * 10 + 9] = v[3];
+            }
+            return;
+        }
         double part[10];
 #pragma unroll
         for (int j = 0; j < 10; ++j) part[j] = 0.0;
