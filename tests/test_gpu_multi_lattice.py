@@ -120,3 +120,47 @@ def test_mixed_coordination_one_launch(po):
         ref = op.integrate_md(pi[k].reshape(-1), 0.0, T1, dt)
         assert np.max(np.abs(multi["mags"][:, Bk + k] - ref["mags"])) <= 1e-10, k
         assert np.max(np.abs(multi["energy"][:, Bk + k] - ref["energy"])) <= 1e-10, k
+
+
+def _perturbed(L1, L2, seed):
+    """Kitaev lattice with per-bond distinct matrices and per-site fields (no
+    translation-invariant tables) and the oracle built from the same CSR."""
+    _, lat = kitaev(L1, L2, h=0.1)
+    rng = np.random.default_rng(seed)
+    J = lat.bi_J + 0.05 * rng.standard_normal(lat.bi_J.shape)
+    off = lat.bi_offset.astype(np.int64)
+    par = lat.bi_partner.astype(np.int64)
+    seen = {}
+    for i in range(lat.lattice_size):
+        for k in range(off[i], off[i + 1]):
+            seen.setdefault((min(i, par[k]), max(i, par[k])), []).append(k)
+    for ks in seen.values():
+        if len(ks) == 2:
+            J[ks[1]] = J[ks[0]].T
+    lat.bi_J = J
+    lat.field = lat.field + 0.01 * rng.standard_normal(lat.field.shape)
+    return lat, (lat.bi_offset, lat.bi_partner, J, lat.field, lat.onsite, lat.frames, lat.afm_signs)
+
+
+def test_general_tables_one_launch(po):
+    """Two lattices on the general (per-site table) path in one launch."""
+    shapes = ((5, 4), (4, 3))
+    lats = [_perturbed(L1, L2, 5 + i) for i, (L1, L2) in enumerate(shapes)]
+    inits = [np.stack([seeding.random_spins(lat.lattice_size, 60 + 10 * i + k) for k in range(2)])
+             for i, (lat, _) in enumerate(lats)]
+    B = 4
+    dt, n = 0.01, 200
+    T1 = (n + 0.5) * dt
+    tp = np.zeros((B, 2))
+    with GPULattice(lats[0][0]) as g0, GPULattice(lats[1][0]) as g1:
+        assert not g0.info()["uniform"] and not g1.info()["uniform"]
+        multi = integrate_multi([g0, g1], inits, tp, None, 0.0, T1, dt, tgrid="boost")
+        alone = [g.integrate_batched(x, tp[:2], None, 0.0, T1, dt, tgrid="boost") for g, x in zip((g0, g1), inits)]
+    for i in range(2):
+        assert np.max(np.abs(multi["mags"][:, 2 * i:2 * i + 2] - alone[i]["mags"])) <= 1e-12, i
+        assert np.max(np.abs(multi["energy"][:, 2 * i:2 * i + 2] - alone[i]["energy"])) <= 1e-12, i
+        (L1, L2), (lat, csr) = shapes[i], lats[i]
+        o = po.OracleLattice.from_csr(3, 2, (L1, L2, 1), *csr)
+        ref = o.integrate_md(inits[i][1].reshape(-1), 0.0, T1, dt)
+        assert np.max(np.abs(multi["mags"][:, 2 * i + 1] - ref["mags"])) <= 1e-10, i
+        assert np.max(np.abs(multi["energy"][:, 2 * i + 1] - ref["energy"])) <= 1e-10, i
