@@ -98,17 +98,26 @@ def build_h5(verbose: bool = False) -> str:
     return out
 
 
+CXX_TESTS = ["test_gpu_api_shim", "test_mixed_gpu_api_shim"]
+
+
 def build_cxx_test(verbose: bool = False) -> str:
-    """C++ program that drives the gpu:: drop-in API (tests/cxx), linked
-    against the in-tree library with an rpath to it."""
-    src = os.path.join(ROOT, "tests", "cxx", "test_gpu_api_shim.cpp")
-    exe = os.path.join(ROOT, "tests", "cxx", "test_gpu_api_shim")
-    if not os.path.exists(src):
-        return ""
-    cmd = ["g++", "-O2", "-std=c++17", "-I", os.path.join(ROOT, "include"), src, "-o", exe,
-           "-L", HERE, "-lcsl_hip", f"-Wl,-rpath,{HERE}", "-Wl,-rpath,$ORIGIN/../../classical_mcmc_spin_amd"]
-    _run(cmd, verbose)
-    return exe
+    """C++ programs that drive the gpu:: and mixed_gpu:: drop-in APIs
+    (tests/cxx), linked against the in-tree library with an rpath to it.
+    The forwarding include directory is on the path, so the reference's
+    include names (classical_spin/gpu/...) resolve to the drop-in headers."""
+    built = ""
+    for name in CXX_TESTS:
+        src = os.path.join(ROOT, "tests", "cxx", name + ".cpp")
+        exe = os.path.join(ROOT, "tests", "cxx", name)
+        if not os.path.exists(src):
+            continue
+        cmd = ["g++", "-O2", "-std=c++17", "-I", os.path.join(ROOT, "include"),
+               "-I", os.path.join(ROOT, "include", "classical_spin_amd_forward"), src, "-o", exe,
+               "-L", HERE, "-lcsl_hip", f"-Wl,-rpath,{HERE}", "-Wl,-rpath,$ORIGIN/../../classical_mcmc_spin_amd"]
+        _run(cmd, verbose)
+        built = exe
+    return built
 
 
 def _run(cmd, verbose):

@@ -24,6 +24,7 @@
 #include "../../include/csl_gpu.h"
 #include "csl_host.hpp"
 #include "csl_kernels.hpp"
+#include "csl_tableau.hpp"
 
 namespace {
 
@@ -63,6 +64,7 @@ struct csl_lattice {
     bool has_state = false;
     // workspaces
     DBuf<double> X, ACC, Y1, Y2, partials, fac, tgrid, rec_mags, rec_energy, rec_states, aux;
+    DBuf<double> KS;              // stage derivatives of the generic fixed-step engine [S][3][NP]
     DBuf<double> bt_init, bt_tp, bt_amp, bt_final;
     DBuf<double> dp_times;        // Dormand-Prince records: times [B][max_rec]
     // persistent-kernel work split (batches larger than the co-resident workgroups)
@@ -581,6 +583,100 @@ void run_stages(csl_lattice* h, size_t B, const double* d_tp, const double* d_am
     h->last_dbuf = 0;
 }
 
+// Reference drive factor f = A exp(-((t - t_c) / 2w)^2) cos(w (t - t_c))
+// (lattice_md.cpp:45-56), evaluated on the host for one stage time.
+double host_drive_factor(double amp, double t, double tc, double width, double freq) {
+    if (amp == 0.0) return 0.0;
+    const double d = t - tc;
+    const double a = d / (2.0 * width);
+    return amp * std::exp(-(a * a)) * std::cos(freq * d);
+}
+
+// Method name of the reference GPU stepper -> stage table; an unknown name is
+// reported on stderr and runs ssprk53, as gpu::ode::step does
+// (integrator.cuh:454-458).
+Tableau method_tableau(const char* method) {
+    bool known = true;
+    Tableau T = tableau_for(method, &known);
+    if (!known)
+        std::fprintf(stderr, "Warning: Unknown GPU integration method '%s', using ssprk53\n",
+                     method ? method : "");
+    return T;
+}
+
+// Generic fixed-step engine for the reference GPU stepper's methods other
+// than rk4 (integrator.cuh:91-458), one trajectory in h->X (SoA): per stage
+// one k_eval launch (the derivative of the stage input at t + c_s dt into
+// K[s], drive factors from the host) and one k_rk_combine launch (the next
+// stage input, or the result in place of x, from x and the K's of the same
+// element).  Records as run_stages: M, E and optionally the state at every
+// save_interval-th step start and after the last step.
+void run_tableau(csl_lattice* h, const Tableau& T, const std::vector<double>& grid, double dt, size_t si,
+                 bool want_energy, double* d_states_out) {
+    const size_t n_steps = grid.size() - 1;
+    const size_t n_rec = records_for(n_steps, si);
+    hipStream_t s = h->stream;
+    const csl::LatDev& L = h->L;
+    const size_t n = 3 * size_t(L.NP);
+    const int S = T.S;
+    h->KS.ensure(size_t(S) * n);
+    h->Y1.ensure(n);
+    // padded slots of K are never written by k_eval; keep them zero
+    hip_check(hipMemsetAsync(h->KS.p, 0, size_t(S) * n * sizeof(double), s), "memset");
+    hip_check(hipMemsetAsync(h->Y1.p, 0, n * sizeof(double), s), "memset");
+    const int nblk = csl::stage_blocks(L);
+    h->partials.ensure(size_t(nblk) * 10);
+    h->rec_mags.ensure(n_rec * 9);
+    if (want_energy) h->rec_energy.ensure(n_rec);
+    const bool drive_on = h->amp != 0.0;
+    const double inv_n = 1.0 / double(h->N);
+    // combination rows: stage inputs 1..S-1 and the result
+    std::vector<csl::RkCombine> rows(static_cast<size_t>(S));
+    for (int st = 0; st < S; ++st) {
+        const Lin& l = st + 1 < S ? T.in[st + 1] : T.out;
+        csl::RkCombine& c = rows[size_t(st)];
+        c.g = l.g;
+        c.m = lin_terms(l, st + 1, dt, c.idx, c.w);
+    }
+    auto observe = [&](size_t ri) {
+        hip_check(csl::launch_observe(L, h->X.p, h->partials.p, 1, s), "observe");
+        hip_check(csl::launch_reduce_obs(h->partials.p, nblk, 1, int(ri), inv_n, h->rec_mags.p,
+                                         want_energy ? h->rec_energy.p : nullptr, s),
+                  "reduce");
+        if (d_states_out)
+            hip_check(csl::launch_perm_to_aos(L, h->X.p, d_states_out + ri * h->N * 3, 1, s), "state record");
+    };
+    hipEvent_t e0, e1;
+    hip_check(hipEventCreate(&e0), "event");
+    hip_check(hipEventCreate(&e1), "event");
+    hip_check(hipEventRecord(e0, s), "event record");
+    for (size_t k = 0; k < n_steps; ++k) {
+        if (k % si == 0) observe(k / si);
+        const double t = grid[k];
+        for (int st = 0; st < S; ++st) {
+            const double ts = t + T.c[st] * dt;
+            const double f1 = drive_on ? host_drive_factor(h->amp, ts, h->t1, h->width, h->freq) : 0.0;
+            const double f2 = drive_on ? host_drive_factor(h->amp, ts, h->t2, h->width, h->freq) : 0.0;
+            const double* in = st == 0 ? h->X.p : h->Y1.p;
+            hip_check(csl::launch_eval(L, in, nullptr, h->KS.p + size_t(st) * n, f1, f2, drive_on ? 1 : 0, s),
+                      "k_eval");
+            double* out = st + 1 < S ? h->Y1.p : h->X.p;
+            hip_check(csl::launch_rk_combine(out, h->X.p, h->KS.p, n, rows[size_t(st)], s), "k_rk_combine");
+        }
+    }
+    hip_check(hipEventRecord(e1, s), "event record");
+    observe(n_rec - 1);
+    hip_check(hipStreamSynchronize(s), "stage integration");
+    float ms = 0.f;
+    hip_check(hipEventElapsedTime(&ms, e0, e1), "elapsed");
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    h->last_launches = n_steps * size_t(S) * 2;
+    h->last_ms = h->last_launches ? ms / double(h->last_launches) : 0.0;
+    h->last_block = -1;
+    h->last_spt = 0;
+    h->last_dbuf = 0;
+}
 
 }  // namespace
 
@@ -830,8 +926,10 @@ int csl_integrate(csl_lattice* h, double T0, double T1, double dt, size_t save_i
     return guarded([&] {
         check_handle(h);
         if (!h->has_state) fail(CSL_ERR_NOSTATE, "spins were never set");
-        if (!method_is_rk4(method)) fail(CSL_ERR_UNSUPPORTED, "only method \"rk4\" is implemented");
         if (save_interval == 0) fail(CSL_ERR_ARG, "save_interval must be >= 1");
+        const bool rk4 = method_is_rk4(method);
+        if (!rk4 && engine != 0 && engine != 1)
+            fail(CSL_ERR_UNSUPPORTED, "methods other than \"rk4\" run on the stage engine (engine 0 or 1)");
         auto grid = make_grid(T0, T1, dt, tgrid);
         const size_t n_steps = grid.size() - 1;
         const size_t n_rec = records_for(n_steps, save_interval);
@@ -847,10 +945,17 @@ int csl_integrate(csl_lattice* h, double T0, double T1, double dt, size_t save_i
         h->bt_amp.upload(am, 2, s);
         bool persistent = false;
         if (engine == 2) persistent = true;
-        else if (engine == 0) persistent = !states && !persistent_candidates(h).empty();
+        else if (engine == 0) persistent = rk4 && !states && !persistent_candidates(h).empty();
         else if (engine != 1 && engine != 3) fail(CSL_ERR_ARG, "engine must be 0, 1, 2 or 3");
         if (persistent && states) fail(CSL_ERR_ARG, "the persistent engine does not record full states");
-        if (persistent) {
+        if (!rk4) {
+            const Tableau T = method_tableau(method);
+            h->X.ensure(3 * size_t(L.NP));
+            hip_check(csl::launch_perm_to_soa(L, h->state.p, 0, h->X.p, 1, s), "perm");
+            if (states) h->rec_states.ensure(n_rec * h->N * 3);
+            run_tableau(h, T, grid, dt, save_interval, energy != nullptr, states ? h->rec_states.p : nullptr);
+            hip_check(csl::launch_perm_to_aos(L, h->X.p, h->state.p, 1, s), "perm");
+        } else if (persistent) {
             TuneVal tv = pick_persistent(h, 1, 0);
             run_persistent(h, 1, h->state.p, 0, h->bt_tp.p, h->bt_amp.p, grid, dt, save_interval,
                            energy != nullptr, true, tv, nullptr);
@@ -881,11 +986,19 @@ int csl_step(csl_lattice* h, double t, double dt, const char* method) {
     return guarded([&] {
         check_handle(h);
         if (!h->has_state) fail(CSL_ERR_NOSTATE, "spins were never set");
-        if (!method_is_rk4(method)) fail(CSL_ERR_UNSUPPORTED, "only method \"rk4\" is implemented");
         if (!(dt > 0.0) || !std::isfinite(t)) fail(CSL_ERR_ARG, "need finite t and dt > 0");
         hipStream_t s = h->stream;
         const csl::LatDev& L = h->L;
         std::vector<double> grid = {t, t + dt};
+        if (!method_is_rk4(method)) {
+            const Tableau T = method_tableau(method);
+            h->X.ensure(3 * size_t(L.NP));
+            hip_check(csl::launch_perm_to_soa(L, h->state.p, 0, h->X.p, 1, s), "perm");
+            run_tableau(h, T, grid, dt, 1, false, nullptr);
+            hip_check(csl::launch_perm_to_aos(L, h->X.p, h->state.p, 1, s), "perm");
+            hip_check(hipStreamSynchronize(s), "step");
+            return;
+        }
         const double tp[2] = {h->t1, h->t2};
         const double am[2] = {h->amp, h->amp};
         h->bt_tp.upload(tp, 2, s);

@@ -287,6 +287,19 @@ __global__ void k_drive_table(const double* __restrict__ tgrid, int n_steps, dou
     fac[id] = drive_factor(amp[2 * b + pl], ts, t_pulse[2 * b + pl], width, freq);
 }
 
+// stage input / result of the generic fixed-step engine, element-wise (the
+// reference's update_arrays / rk_stage_update / final-update kernels of one
+// stage folded into one pass, gpu_common_helpers.cu:9-120).  out may alias x:
+// each element is read before it is written by the same thread.
+__global__ void __launch_bounds__(256) k_rk_combine(double* out, const double* x, const double* __restrict__ K,
+                                                    size_t n, RkCombine c) {
+    const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double v = c.g * x[i];
+    for (int j = 0; j < c.m; ++j) v = fma(c.w[j], K[size_t(c.idx[j]) * n + i], v);
+    out[i] = v;
+}
+
 // ---------------------------------------------------------------------------
 // launchers
 
@@ -423,6 +436,14 @@ hipError_t launch_perm_to_aos(const LatDev& L, const double* soa, double* aos, i
 
 hipError_t launch_normalize(double* s, int N, double len, hipStream_t st) {
     hipLaunchKernelGGL(k_normalize, dim3((N + 255) / 256), dim3(256), 0, st, s, N, len);
+    return hipGetLastError();
+}
+
+hipError_t launch_rk_combine(double* out, const double* x, const double* K, size_t n, const RkCombine& c,
+                             hipStream_t st) {
+    if (n == 0) return hipSuccess;
+    if (c.m < 0 || c.m > 13) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_rk_combine, dim3(unsigned((n + 255) / 256)), dim3(256), 0, st, out, x, K, n, c);
     return hipGetLastError();
 }
 

@@ -193,4 +193,17 @@ hipError_t launch_drive_table(const double* tgrid, int n_steps, double dt, int B
                               const double* t_pulse, const double* amp, double width, double freq,
                               double* fac, hipStream_t st);
 
+// One element-wise combination g*x + sum_i w[i] * K[idx[i]] of the generic
+// fixed-step stage engine (the methods of the reference GPU stepper other
+// than rk4): K holds the stage derivatives, n doubles each.
+struct RkCombine {
+    int m;
+    int idx[13];
+    double g;
+    double w[13];
+};
+// out[i] = g*x[i] + sum_j w[j] * K[idx[j]*n + i] for i < n; out may be x.
+hipError_t launch_rk_combine(double* out, const double* x, const double* K, size_t n, const RkCombine& c,
+                             hipStream_t st);
+
 }  // namespace csl

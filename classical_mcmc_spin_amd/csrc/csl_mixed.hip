@@ -25,6 +25,8 @@
 #include "../../include/csl_mixed.h"
 #include "csl_host.hpp"
 #include "csl_internal.hpp"
+#include "csl_kernels.hpp"
+#include "csl_tableau.hpp"
 
 namespace csl {
 
@@ -1332,6 +1334,7 @@ struct csl_mixed {
     DBuf<double> state;                      // SoA [SZ]
     bool has_state = false;
     DBuf<double> X, ACC, Y1, Y2, partials, recs, tp, tp3, scale, init, flat, aux;
+    DBuf<double> KS;                         // stage derivatives of the generic fixed-step engine [S][SZ]
     DBuf<double> ZX, ZACC, ZY1, ZY2, tpartials, trecs, zaux, target, loss, grad;  // tangent-linear runs
     // Dormand-Prince runs: x / k1 pairs, k2..k6, stage inputs, controller state
     DBuf<double> dpX0, dpX1, dpK10, dpK11, dpK2, dpK3, dpK4, dpK5, dpK6, dpY0, dpY1, dp_errp, dp_recs, dp_times,
@@ -1533,6 +1536,105 @@ void run_mixed(csl_mixed* h, size_t B, const double* d_tp, const double* d_tp3, 
 
 void need_state(const csl_mixed* h) {
     if (!h->has_state) fail(CSL_ERR_NOSTATE, "no spin state set");
+}
+
+// Method name of the reference GPU stepper -> stage table (unknown names are
+// reported on stderr and run ssprk53, integrator.cuh:454-458).
+Tableau mixed_method_tableau(const char* method) {
+    bool known = true;
+    Tableau T = tableau_for(method, &known);
+    if (!known)
+        std::fprintf(stderr, "Warning: Unknown GPU integration method '%s', using ssprk53\n",
+                     method ? method : "");
+    return T;
+}
+
+// Generic fixed-step engine for the reference GPU stepper's methods other
+// than rk4 (integrator.cuh:91-458; the mixed lattice steps through the same
+// gpu::ode::step, mixed_lattice_gpu.cu:456-488), one trajectory in h->X:
+// per stage one k_mixed_eval launch (derivative of the stage input at
+// t + c_s dt with host-evaluated envelopes into K[s]) and one k_rk_combine
+// launch (next stage input, or the result in place of x).  Records at every
+// save_interval-th step start and after the last step, as run_mixed.
+void run_mixed_tableau(csl_mixed* h, const Tableau& T, const std::vector<double>& grid, double dt, size_t si,
+                       double* d_states) {
+    const size_t n_steps = grid.size() - 1;
+    const size_t n_rec = records_for(n_steps, si);
+    hipStream_t s = h->stream;
+    const size_t n = h->sz();
+    const int S = T.S;
+    h->KS.ensure(size_t(S) * n);
+    h->Y1.ensure(n);
+    h->aux.ensure(n);
+    hip_check(hipMemsetAsync(h->KS.p, 0, size_t(S) * n * sizeof(double), s), "memset");
+    hip_check(hipMemsetAsync(h->Y1.p, 0, n * sizeof(double), s), "memset");
+    const int nblk2 = blocks_of(h->M.NP2), nblk3 = blocks_of(h->M.NP3);
+    const int nblk = nblk2 + nblk3;
+    h->partials.ensure(size_t(nblk) * CSL_MIXED_RECORD);
+    h->recs.ensure(n_rec * CSL_MIXED_RECORD);
+    const csl::MixDev& M = h->M;
+    const double inv2 = 1.0 / double(h->N2), inv3 = 1.0 / double(h->N3), invE = 1.0 / double(h->N2 + h->N3);
+    std::vector<csl::RkCombine> rows(static_cast<size_t>(S));
+    for (int st = 0; st < S; ++st) {
+        const Lin& l = st + 1 < S ? T.in[st + 1] : T.out;
+        rows[size_t(st)].g = l.g;
+        rows[size_t(st)].m = lin_terms(l, st + 1, dt, rows[size_t(st)].idx, rows[size_t(st)].w);
+    }
+    auto env = [](double amp, double t, double c, double w, double fr) {
+        if (amp == 0.0) return 0.0;
+        const double d = t - c;
+        const double a = d / (2.0 * w);
+        return amp * std::exp(-(a * a)) * std::cos(fr * d);
+    };
+    // record of the state in X: a stage-0 evaluation whose outputs go to scratch
+    auto record = [&](size_t ri) {
+        if (d_states) to_flat(h, h->X.p, d_states + ri * h->nflat(), 1);
+        csl::MixStageArgs A{};
+        A.stage = 0;
+        A.nblk2 = nblk2;
+        A.x = h->X.p;
+        A.acc = h->aux.p;
+        A.ycur = h->X.p;
+        A.ynext = h->aux.p;
+        A.dt = dt;
+        A.rec = 1;
+        A.partials = h->partials.p;
+        hipLaunchKernelGGL(csl::k_mixed_stage<false>, dim3(unsigned(nblk), 1u, 1u), dim3(256), 0, s, M, A);
+        hip_check(hipGetLastError(), "k_mixed_stage");
+        hipLaunchKernelGGL(csl::k_mixed_reduce, dim3(1u), dim3(256), 0, s, h->partials.p, nblk, 1, int(ri), inv2,
+                           inv3, invE, h->recs.p);
+        hip_check(hipGetLastError(), "k_mixed_reduce");
+    };
+    hipEvent_t e0, e1;
+    hip_check(hipEventCreate(&e0), "event");
+    hip_check(hipEventCreate(&e1), "event");
+    hip_check(hipEventRecord(e0, s), "event record");
+    for (size_t k = 0; k < n_steps; ++k) {
+        if (k % si == 0) record(k / si);
+        const double t = grid[k];
+        for (int st = 0; st < S; ++st) {
+            const double ts = t + T.c[st] * dt;
+            const double f21 = env(M.amp2, ts, h->h_tp2[0], M.width2, M.freq2);
+            const double f22 = env(M.amp2, ts, h->h_tp2[1], M.width2, M.freq2);
+            const double f31 = env(M.amp3, ts, h->h_tp3[0], M.width3, M.freq3);
+            const double f32 = env(M.amp3, ts, h->h_tp3[1], M.width3, M.freq3);
+            const double* in = st == 0 ? h->X.p : h->Y1.p;
+            hipLaunchKernelGGL(csl::k_mixed_eval, dim3(unsigned(nblk)), dim3(256), 0, s, M, in, f21, f22, f31, f32,
+                               h->KS.p + size_t(st) * n);
+            hip_check(hipGetLastError(), "k_mixed_eval");
+            double* out = st + 1 < S ? h->Y1.p : h->X.p;
+            hip_check(csl::launch_rk_combine(out, h->X.p, h->KS.p, n, rows[size_t(st)], s), "k_rk_combine");
+        }
+    }
+    hip_check(hipEventRecord(e1, s), "event record");
+    record(n_rec - 1);
+    hip_check(hipStreamSynchronize(s), "mixed integration");
+    float ms = 0.f;
+    hip_check(hipEventElapsedTime(&ms, e0, e1), "elapsed");
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    h->last_launches = n_steps * size_t(S) * 2;
+    h->last_ms = h->last_launches ? ms / double(h->last_launches) : 0.0;
 }
 
 }  // namespace
@@ -1806,7 +1908,6 @@ int csl_mixed_integrate(csl_mixed* h, double T0, double T1, double dt, size_t sa
     return guarded([&] {
         check_mixed(h);
         need_state(h);
-        if (!method_is_rk4(method)) fail(CSL_ERR_UNSUPPORTED, "the mixed path implements \"rk4\"");
         if (save_interval == 0) fail(CSL_ERR_ARG, "save_interval must be >= 1");
         auto grid = make_grid(T0, T1, dt, tgrid);
         const size_t n_steps = grid.size() - 1;
@@ -1822,7 +1923,10 @@ int csl_mixed_integrate(csl_mixed* h, double T0, double T1, double dt, size_t sa
         h->tp3.upload(h->h_tp3, 2, s);
         DBuf<double> st;
         if (states) st.ensure(n_rec * h->nflat());
-        run_mixed(h, 1, h->tp.p, h->tp3.p, h->scale.p, grid, dt, save_interval, states ? st.p : nullptr);
+        if (method_is_rk4(method))
+            run_mixed(h, 1, h->tp.p, h->tp3.p, h->scale.p, grid, dt, save_interval, states ? st.p : nullptr);
+        else
+            run_mixed_tableau(h, mixed_method_tableau(method), grid, dt, save_interval, states ? st.p : nullptr);
         hip_check(hipMemcpyAsync(h->state.p, h->X.p, h->sz() * sizeof(double), hipMemcpyDeviceToDevice, s), "copy");
         copy_back(recs, h->recs.p, n_rec * CSL_MIXED_RECORD, s);
         if (states) copy_back(states, st.p, n_rec * h->nflat(), s);
@@ -1836,7 +1940,6 @@ int csl_mixed_step(csl_mixed* h, double t, double dt, const char* method) {
     return guarded([&] {
         check_mixed(h);
         need_state(h);
-        if (!method_is_rk4(method)) fail(CSL_ERR_UNSUPPORTED, "the mixed path implements \"rk4\"");
         if (!(dt > 0.0) || !std::isfinite(t)) fail(CSL_ERR_ARG, "step needs finite t and dt > 0");
         hipStream_t s = h->stream;
         h->X.ensure(h->sz());
@@ -1846,7 +1949,10 @@ int csl_mixed_step(csl_mixed* h, double t, double dt, const char* method) {
         h->tp.upload(h->h_tp2, 2, s);
         h->tp3.upload(h->h_tp3, 2, s);
         std::vector<double> grid{t, t + dt};
-        run_mixed(h, 1, h->tp.p, h->tp3.p, h->scale.p, grid, dt, 1, nullptr);
+        if (method_is_rk4(method))
+            run_mixed(h, 1, h->tp.p, h->tp3.p, h->scale.p, grid, dt, 1, nullptr);
+        else
+            run_mixed_tableau(h, mixed_method_tableau(method), grid, dt, 1, nullptr);
         hip_check(hipMemcpyAsync(h->state.p, h->X.p, h->sz() * sizeof(double), hipMemcpyDeviceToDevice, s), "copy");
         hip_check(hipStreamSynchronize(s), "step");
     });

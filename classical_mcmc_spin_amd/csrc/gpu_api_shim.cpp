@@ -3,6 +3,7 @@
 // include/classical_spin_amd/lattice_gpu_api.h for the behavioural notes.
 #include "classical_spin_amd/lattice_gpu_api.h"
 
+#include <iostream>
 #include <stdexcept>
 #include <string>
 
@@ -21,13 +22,6 @@ namespace {
 
 void chk(int rc, const char* where) {
     if (rc != CSL_OK) throw std::runtime_error(std::string(where) + ": " + csl_last_error());
-}
-
-int method_rk4(const std::string& method, const char* where) {
-    if (method != "rk4")
-        throw std::runtime_error(std::string(where) + ": method '" + method +
-                                 "' is not implemented on this device library (use \"rk4\")");
-    return 0;
 }
 
 }  // namespace
@@ -95,14 +89,16 @@ void integrate_gpu(GPULatticeDataHandle* handle, double T_start, double T_end, d
                    size_t save_interval, std::vector<std::pair<double, std::vector<double>>>& trajectory,
                    const std::string& method) {
     if (!handle) return;
-    method_rk4(method, "integrate_gpu");
+    // every method of gpu::ode::step (integrator.cuh:91-458) with its
+    // fixed-step loop (integrator.cuh:470-487); unknown names warn and run
+    // ssprk53 inside the library, as the reference does
     size_t n_steps = 0, n_rec = 0;
     chk(csl_count_records(T_start, T_end, dt, save_interval, CSL_TGRID_GPU, &n_steps, &n_rec),
         "integrate_gpu");
     const size_t nd = handle->N * handle->D;
     std::vector<double> times(n_rec), states(n_rec * nd);
     size_t got = 0;
-    chk(csl_integrate(handle->h, T_start, T_end, dt, save_interval, "rk4", CSL_TGRID_GPU, 1, n_rec,
+    chk(csl_integrate(handle->h, T_start, T_end, dt, save_interval, method.c_str(), CSL_TGRID_GPU, 1, n_rec,
                       times.data(), states.data(), nullptr, nullptr, &got),
         "integrate_gpu");
     for (size_t r = 0; r < got; ++r)
@@ -111,8 +107,7 @@ void integrate_gpu(GPULatticeDataHandle* handle, double T_start, double T_end, d
 
 void step_gpu(GPULatticeDataHandle* handle, double t, double dt, const std::string& method) {
     if (!handle) return;
-    method_rk4(method, "step_gpu");
-    chk(csl_step(handle->h, t, dt, "rk4"), "step_gpu");
+    chk(csl_step(handle->h, t, dt, method.c_str()), "step_gpu");
 }
 
 double compute_energy_gpu(GPULatticeDataHandle* handle) {
@@ -174,7 +169,11 @@ BatchedMagResult integrate_gpu_batched(GPULatticeDataHandle* handle,
                                        double T_end, double dt, size_t save_interval,
                                        const std::string& method) {
     if (!handle || tau2_values.empty()) return BatchedMagResult{};
-    method_rk4(method, "integrate_gpu_batched");
+    // batched mode is RK4 only; other methods warn and run rk4
+    // (lattice_gpu.cu:779-784)
+    if (method != "rk4")
+        std::cerr << "[integrate_gpu_batched] method='" << method
+                  << "' is not supported in batched mode; using rk4." << std::endl;
     // reference semantics: t_pulse_1 and the amplitude are the handle's, only
     // t_pulse_2 varies per replica (lattice_gpu.cu:336-397)
     const size_t B = tau2_values.size();

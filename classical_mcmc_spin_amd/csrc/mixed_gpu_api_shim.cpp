@@ -23,12 +23,6 @@ void chk(int rc, const char* where) {
     if (rc != CSL_OK) throw std::runtime_error(std::string(where) + ": " + csl_last_error());
 }
 
-void method_rk4(const std::string& method, const char* where) {
-    if (method != "rk4")
-        throw std::runtime_error(std::string(where) + ": method '" + method +
-                                 "' is not implemented on this device library (use \"rk4\")");
-}
-
 // padded per-site slots (SIZE_MAX partners beyond the count,
 // mixed_lattice.h:5490-5597) -> CSR
 void to_csr(size_t N, size_t maxb, size_t m, const std::vector<size_t>& counts, const std::vector<size_t>& partners,
@@ -123,13 +117,15 @@ void integrate_mixed_gpu(GPUMixedLatticeDataHandle* handle, double T_start, doub
                          size_t save_interval, std::vector<std::pair<double, std::vector<double>>>& trajectory,
                          const std::string& method) {
     if (!handle || !handle->has_state) return;
-    method_rk4(method, "integrate_mixed_gpu");
+    // every method of gpu::ode::step (mixed_lattice_gpu.cu:469-488 drives the
+    // same stepper); unknown names warn and run ssprk53 inside the library
     size_t n_steps = 0, n_rec = 0;
     chk(csl_count_records(T_start, T_end, dt, save_interval, CSL_TGRID_GPU, &n_steps, &n_rec), "integrate_mixed_gpu");
     const size_t nd = handle->N2 * 3 + handle->N3 * 8;
     std::vector<double> times(n_rec), states(n_rec * nd);
     size_t got = 0;
-    chk(csl_mixed_integrate(handle->h, T_start, T_end, dt, save_interval, "rk4", CSL_TGRID_GPU, n_rec, times.data(),
+    chk(csl_mixed_integrate(handle->h, T_start, T_end, dt, save_interval, method.c_str(), CSL_TGRID_GPU, n_rec,
+                            times.data(),
                             states.data(), nullptr, &got),
         "integrate_mixed_gpu");
     for (size_t r = 0; r < got; ++r)
@@ -138,8 +134,7 @@ void integrate_mixed_gpu(GPUMixedLatticeDataHandle* handle, double T_start, doub
 
 void step_mixed_gpu(GPUMixedLatticeDataHandle* handle, double t, double dt, const std::string& method) {
     if (!handle || !handle->has_state) return;
-    method_rk4(method, "step_mixed_gpu");
-    chk(csl_mixed_step(handle->h, t, dt, "rk4"), "step_mixed_gpu");
+    chk(csl_mixed_step(handle->h, t, dt, method.c_str()), "step_mixed_gpu");
 }
 
 // compute_magnetization_kernel semantics (gpu_common_helpers.cu:163-210):

@@ -5,10 +5,13 @@
 // Same namespace, types and signatures, so the reference's Lattice glue
 // (lattice.h:5419-5967) compiles and links against libcsl_hip.so unchanged.
 // Behavioural differences (documented, not silent):
-//   * CUDA-style errors and unsupported requests throw std::runtime_error with
-//     the library's message (the reference prints to stderr and falls back,
-//     integrator.cuh:454-458, lattice_gpu.cu:781-784); a null handle stays a
-//     no-op as in lattice_gpu.cu:660-710.
+//   * every method name of gpu::ode::step is accepted (integrator.cuh:91-458);
+//     as in the reference an unknown name warns on stderr and runs ssprk53
+//     (integrator.cuh:454-458) and integrate_gpu_batched warns and runs rk4
+//     for any other method (lattice_gpu.cu:779-784).
+//   * device errors throw std::runtime_error with the library's message (the
+//     reference checks only at the end of an integration); a null handle
+//     stays a no-op as in lattice_gpu.cu:660-710.
 //   * compute_energy_gpu returns the energy (the reference throws,
 //     lattice_gpu.cu:298-303).
 //   * integrate_gpu_batched additionally accepts the per-replica form through

@@ -5,9 +5,11 @@
 // Same namespace and signatures, so the reference's MixedLattice glue
 // (mixed_lattice.h:6041-6560) compiles and links against libcsl_hip.so.
 // Behavioural differences (documented, not silent):
-//   * errors and unsupported requests (methods other than "rk4") throw
-//     std::runtime_error with the library's message; a null handle stays a
-//     no-op as in the reference.
+//   * every method name of gpu::ode::step is accepted ("rk4" on the fused
+//     RK4 stage kernels, the others on the generic stage engine; unknown
+//     names warn on stderr and run ssprk53, as the reference does); errors
+//     throw std::runtime_error with the library's message; a null handle
+//     stays a no-op as in the reference.
 //   * the device RHS also carries the CPU path's SU(2) Gilbert and SU(3)
 //     Bloch damping; both are off unless csl_mixed_set_damping is called, so
 //     a reference caller sees the reference GPU's undamped dynamics.

@@ -108,7 +108,13 @@ int csl_get_spins(csl_lattice* h, double* spins, size_t n);
 
 /* ---- single trajectory ---------------------------------------------- */
 
-/* Replaces gpu::step_gpu (lattice_gpu_api.h:142-147).  method: "rk4". */
+/* Replaces gpu::step_gpu (lattice_gpu_api.h:142-147): one fixed step of
+ * the named method.  Methods are those of the reference GPU stepper
+ * gpu::ode::step (integrator.cuh:91-458): "euler", "rk2"/"midpoint", "rk4",
+ * "rk5"/"rkck54", "dopri5" (fixed step, no error control), "rk54"/"rkf54"
+ * (= dopri5), "rk78"/"rkf78", "bulirsch_stoer"/"bs", "ssprk53"; any other
+ * name prints the reference's warning on stderr and runs ssprk53.  NULL
+ * means "rk4". */
 int csl_step(csl_lattice* h, double t, double dt, const char* method);
 
 /* Replaces gpu::integrate_gpu (lattice_gpu_api.h:117-125).  Integrates the
@@ -120,7 +126,10 @@ int csl_step(csl_lattice* h, double t, double dt, const char* method);
  * 2 = persistent single-workgroup kernel (lattice resident in LDS),
  * 3 = the stage-kernel launch sequence captured once into a hipGraph and
  * replayed by later calls with the same shapes, lattice parameters and
- * device buffers (the inputs are refreshed in place before each replay). */
+ * device buffers (the inputs are refreshed in place before each replay).
+ * method: any name of csl_step.  Methods other than "rk4" run on the
+ * generic stage engine (one derivative launch and one combination launch
+ * per stage; engine 0 or 1, CSL_ERR_UNSUPPORTED for 2 and 3). */
 int csl_integrate(csl_lattice* h, double T0, double T1, double dt,
                   size_t save_interval, const char* method, int tgrid, int engine,
                   size_t max_rec, double* times, double* states, double* mags,

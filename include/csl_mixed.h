@@ -101,11 +101,14 @@ int csl_mixed_rhs(csl_mixed* h, double t, double* dsdt);
 /* normalize_spins_mixed_gpu (:196-200). */
 int csl_mixed_normalize(csl_mixed* h, double spin_length2, double spin_length3);
 
-/* step_mixed_gpu (:166-171), method "rk4". */
+/* step_mixed_gpu (:166-171): one fixed step of any method name of
+ * csl_step (mixed_lattice_gpu.cu:456-467 steps through the same
+ * gpu::ode::step); "rk4" runs the fused RK4 stage kernels, the others the
+ * generic stage engine. */
 int csl_mixed_step(csl_mixed* h, double t, double dt, const char* method);
 
-/* integrate_mixed_gpu (:150-157) from the handle's state, which is advanced.
- * tgrid: CSL_TGRID_*.  Outputs (any may be NULL): times [max_rec],
+/* integrate_mixed_gpu (:150-157) from the handle's state, which is advanced,
+ * with any method name of csl_step.  tgrid: CSL_TGRID_*.  Outputs (any may be NULL): times [max_rec],
  * states [max_rec*(N2*3+N3*8)], recs [max_rec*34]. */
 int csl_mixed_integrate(csl_mixed* h, double T0, double T1, double dt, size_t save_interval,
                         const char* method, int tgrid, size_t max_rec, double* times,

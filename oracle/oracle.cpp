@@ -49,6 +49,7 @@
 #include <cstring>
 #include <functional>
 #include <limits>
+#include <string>
 #include <vector>
 
 #ifdef _OPENMP
@@ -579,6 +580,213 @@ bool dp5_controls(F&& f, double* state, size_t n, double T0, double T1, double d
     return ok;
 }
 
+// ---- the reference GPU stepper gpu::ode::step (integrator.cuh:91-458) ----
+// Restated operation by operation on host arrays: every buffer update is one
+// of the reference's element-wise kernels (gpu_common_helpers.cu:9-120), in
+// the order the reference launches them.  f(in, out, t) is the RHS.
+
+struct GpuStepWork {
+    std::vector<double> k[11], tmp;
+    explicit GpuStepWork(size_t n) : tmp(n) {
+        for (auto& v : k) v.assign(n, 0.0);
+    }
+};
+
+// update_arrays_kernel: out = a1*in1 + a2*in2
+void upd2(double* out, double a1, const double* in1, double a2, const double* in2, size_t n) {
+    for (size_t i = 0; i < n; ++i) out[i] = a1 * in1[i] + a2 * in2[i];
+}
+// update_arrays_three_kernel: out = a1*in1 + a2*in2 + a3*in3
+void upd3(double* out, double a1, const double* in1, double a2, const double* in2, double a3, const double* in3,
+          size_t n) {
+    for (size_t i = 0; i < n; ++i) out[i] = a1 * in1[i] + a2 * in2[i] + a3 * in3[i];
+}
+// rk_stage_update_m_kernel: out = x + dt * (sum a_j k_j)
+void stage_upd(double* out, const double* x, double dt, std::initializer_list<std::pair<double, const double*>> t,
+               size_t n) {
+    for (size_t i = 0; i < n; ++i) {
+        double acc = 0.0;
+        for (const auto& p : t) acc += p.first * p.second[i];
+        out[i] = x[i] + dt * acc;
+    }
+}
+
+// returns false for an unknown method name (the caller falls back to
+// ssprk53 as integrator.cuh:454-458 does)
+template <class F>
+bool gpu_step(F&& f, double* x, size_t n, double t, double dt, const char* method, GpuStepWork& w) {
+    const std::string m = method ? method : "rk4";
+    double* k1 = w.k[0].data();
+    double* k2 = w.k[1].data();
+    double* k3 = w.k[2].data();
+    double* k4 = w.k[3].data();
+    double* k5 = w.k[4].data();
+    double* k6 = w.k[5].data();
+    double* tmp = w.tmp.data();
+    if (m == "euler") {  // integrator.cuh:91-99
+        f(x, k1, t);
+        upd2(x, 1.0, x, dt, k1, n);
+    } else if (m == "rk2" || m == "midpoint") {  // :100-114
+        f(x, k1, t);
+        upd2(tmp, 1.0, x, 0.5 * dt, k1, n);
+        f(tmp, k2, t + 0.5 * dt);
+        upd2(x, 1.0, x, dt, k2, n);
+    } else if (m == "rk4") {  // :115-139 with rk4_final_update_kernel
+        f(x, k1, t);
+        upd2(tmp, 1.0, x, 0.5 * dt, k1, n);
+        f(tmp, k2, t + 0.5 * dt);
+        upd2(tmp, 1.0, x, 0.5 * dt, k2, n);
+        f(tmp, k3, t + 0.5 * dt);
+        upd2(tmp, 1.0, x, dt, k3, n);
+        f(tmp, k4, t + dt);
+        const double d6 = dt / 6.0;
+        for (size_t i = 0; i < n; ++i) x[i] += d6 * (k1[i] + 2.0 * k2[i] + 2.0 * k3[i] + k4[i]);
+    } else if (m == "rk5" || m == "rkck54" || m == "dopri5" || m == "rk54" || m == "rkf54") {
+        // :141-185 (Cash-Karp) and :186-229 (Dormand-Prince), fixed step;
+        // rk54 / rkf54 reuse dopri5 (:450-452)
+        const bool ck = (m == "rk5" || m == "rkck54");
+        double a21, a31, a32, a41, a42, a43, a51, a52, a53, a54, a61, a62, a63, a64, a65;
+        double c2, c3, c4, c5, c6, b1, b3, b4, b5, b6;
+        if (ck) {
+            a21 = 1.0 / 5.0; a31 = 3.0 / 40.0; a32 = 9.0 / 40.0;
+            a41 = 3.0 / 10.0; a42 = -9.0 / 10.0; a43 = 6.0 / 5.0;
+            a51 = -11.0 / 54.0; a52 = 5.0 / 2.0; a53 = -70.0 / 27.0; a54 = 35.0 / 27.0;
+            a61 = 1631.0 / 55296.0; a62 = 175.0 / 512.0; a63 = 575.0 / 13824.0;
+            a64 = 44275.0 / 110592.0; a65 = 253.0 / 4096.0;
+            c2 = 1.0 / 5.0; c3 = 3.0 / 10.0; c4 = 3.0 / 5.0; c5 = 1.0; c6 = 7.0 / 8.0;
+            b1 = 37.0 / 378.0; b3 = 250.0 / 621.0; b4 = 125.0 / 594.0; b5 = 0.0; b6 = 512.0 / 1771.0;
+        } else {
+            a21 = 1.0 / 5.0; a31 = 3.0 / 40.0; a32 = 9.0 / 40.0;
+            a41 = 44.0 / 45.0; a42 = -56.0 / 15.0; a43 = 32.0 / 9.0;
+            a51 = 19372.0 / 6561.0; a52 = -25360.0 / 2187.0; a53 = 64448.0 / 6561.0; a54 = -212.0 / 729.0;
+            a61 = 9017.0 / 3168.0; a62 = -355.0 / 33.0; a63 = 46732.0 / 5247.0; a64 = 49.0 / 176.0;
+            a65 = -5103.0 / 18656.0;
+            c2 = 1.0 / 5.0; c3 = 3.0 / 10.0; c4 = 4.0 / 5.0; c5 = 8.0 / 9.0; c6 = 1.0;
+            b1 = 35.0 / 384.0; b3 = 500.0 / 1113.0; b4 = 125.0 / 192.0; b5 = -2187.0 / 6784.0; b6 = 11.0 / 84.0;
+        }
+        f(x, k1, t);
+        upd2(tmp, 1.0, x, a21 * dt, k1, n);
+        f(tmp, k2, t + c2 * dt);
+        stage_upd(tmp, x, dt, {{a31, k1}, {a32, k2}}, n);
+        f(tmp, k3, t + c3 * dt);
+        stage_upd(tmp, x, dt, {{a41, k1}, {a42, k2}, {a43, k3}}, n);
+        f(tmp, k4, t + c4 * dt);
+        stage_upd(tmp, x, dt, {{a51, k1}, {a52, k2}, {a53, k3}, {a54, k4}}, n);
+        f(tmp, k5, t + c5 * dt);
+        stage_upd(tmp, x, dt, {{a61, k1}, {a62, k2}, {a63, k3}, {a64, k4}, {a65, k5}}, n);
+        f(tmp, k6, t + c6 * dt);
+        // dopri5_final_update_kernel; Cash-Karp passes k6 in the k5 slot with b5 = 0
+        const double* k5s = ck ? k6 : k5;
+        for (size_t i = 0; i < n; ++i)
+            x[i] += dt * (b1 * k1[i] + b3 * k3[i] + b4 * k4[i] + b5 * k5s[i] + b6 * k6[i]);
+    } else if (m == "rk78" || m == "rkf78") {  // :230-370
+        double* K[11];
+        for (int j = 0; j < 11; ++j) K[j] = w.k[j].data();
+        // tmp = x + c*dt*K_a, then tmp += c*dt*K_b ... as the reference's
+        // chains of update_arrays_kernel calls
+        auto chain = [&](std::initializer_list<std::pair<int, double>> terms) {
+            bool first = true;
+            for (const auto& p : terms) {
+                upd2(tmp, 1.0, first ? x : tmp, p.second * dt, K[p.first], n);
+                first = false;
+            }
+        };
+        const double c[11] = {0.0, 2.0 / 27.0, 1.0 / 9.0, 1.0 / 6.0, 5.0 / 12.0, 1.0 / 2.0,
+                              5.0 / 6.0, 1.0 / 6.0, 2.0 / 3.0, 1.0 / 3.0, 1.0};
+        f(x, K[0], t);
+        chain({{0, 2.0 / 27.0}});
+        f(tmp, K[1], t + c[1] * dt);
+        chain({{0, 1.0 / 36.0}, {1, 1.0 / 12.0}});
+        f(tmp, K[2], t + c[2] * dt);
+        chain({{0, 1.0 / 24.0}, {2, 1.0 / 8.0}});
+        f(tmp, K[3], t + c[3] * dt);
+        chain({{0, 5.0 / 12.0}, {3, -25.0 / 16.0}, {2, 25.0 / 16.0}});
+        f(tmp, K[4], t + c[4] * dt);
+        chain({{0, 1.0 / 20.0}, {3, 1.0 / 4.0}, {4, 1.0 / 5.0}});
+        f(tmp, K[5], t + c[5] * dt);
+        chain({{0, -25.0 / 108.0}, {3, 125.0 / 108.0}, {4, -65.0 / 27.0}, {5, 125.0 / 54.0}});
+        f(tmp, K[6], t + c[6] * dt);
+        chain({{0, 31.0 / 300.0}, {4, 61.0 / 225.0}, {5, -2.0 / 9.0}, {6, 13.0 / 900.0}});
+        f(tmp, K[7], t + c[7] * dt);
+        chain({{0, 2.0}, {3, -53.0 / 6.0}, {4, 704.0 / 45.0}, {5, -107.0 / 9.0}, {6, 67.0 / 90.0}, {7, 3.0}});
+        f(tmp, K[8], t + c[8] * dt);
+        chain({{0, -91.0 / 108.0}, {3, 23.0 / 108.0}, {4, -976.0 / 135.0}, {5, 311.0 / 54.0},
+               {6, -19.0 / 60.0}, {7, 17.0 / 6.0}, {8, -1.0 / 12.0}});
+        f(tmp, K[9], t + c[9] * dt);
+        chain({{0, 2383.0 / 4100.0}, {3, -341.0 / 164.0}, {4, 4496.0 / 1025.0}, {5, -301.0 / 82.0},
+               {6, 2133.0 / 4100.0}, {7, 45.0 / 82.0}, {8, 45.0 / 164.0}, {9, 18.0 / 41.0}});
+        f(tmp, K[10], t + c[10] * dt);
+        const std::pair<int, double> fin[7] = {{0, 41.0 / 840.0}, {5, 34.0 / 105.0}, {6, 9.0 / 35.0},
+                                               {7, 9.0 / 35.0},   {8, 9.0 / 280.0},  {9, 9.0 / 280.0},
+                                               {10, 41.0 / 840.0}};
+        for (const auto& p : fin) upd2(x, 1.0, x, p.second * dt, K[p.first], n);
+    } else if (m == "bulirsch_stoer" || m == "bs") {  // :371-405
+        double* kk = k1;
+        double* y_prev = k2;
+        double* y_mid = k3;
+        double* y_next = k4;
+        const int n_sub = 4;
+        const double hs = dt / n_sub;
+        std::memcpy(y_prev, x, n * sizeof(double));
+        f(x, kk, t);
+        upd2(y_mid, 1.0, x, hs, kk, n);
+        for (int i = 1; i < n_sub; ++i) {
+            f(y_mid, kk, t + i * hs);
+            upd2(y_next, 1.0, y_prev, 2.0 * hs, kk, n);
+            std::memcpy(y_prev, y_mid, n * sizeof(double));
+            std::memcpy(y_mid, y_next, n * sizeof(double));
+        }
+        f(y_mid, kk, t + dt);
+        upd2(y_next, 0.5, y_mid, 0.5, y_prev, n);
+        upd2(x, 1.0, y_next, 0.5 * hs, kk, n);
+    } else if (m == "ssprk53") {  // :406-449
+        constexpr double a30 = 0.355909775063327, a32 = 0.644090224936674;
+        constexpr double a40 = 0.367933791638137, a43 = 0.632066208361863;
+        constexpr double a52 = 0.237593836598569, a54 = 0.762406163401431;
+        constexpr double b10 = 0.377268915331368, b21 = 0.377268915331368;
+        constexpr double b32 = 0.242995220537396, b43 = 0.238458932846290;
+        constexpr double b54 = 0.287632146308408;
+        constexpr double c1 = 0.377268915331368, c2 = 0.754537830662736;
+        constexpr double c3 = 0.728985661612188, c4 = 0.699226135931670;
+        double* kk = k1;
+        double* u = k2;
+        f(x, kk, t);
+        upd2(tmp, 1.0, x, b10 * dt, kk, n);
+        f(tmp, kk, t + c1 * dt);
+        upd2(u, 1.0, tmp, b21 * dt, kk, n);
+        f(u, kk, t + c2 * dt);
+        upd3(tmp, a30, x, a32, u, b32 * dt, kk, n);
+        f(tmp, kk, t + c3 * dt);
+        upd3(tmp, a40, x, a43, tmp, b43 * dt, kk, n);
+        f(tmp, kk, t + c4 * dt);
+        upd3(x, a52, u, a54, tmp, b54 * dt, kk, n);
+    } else {
+        return false;
+    }
+    return true;
+}
+
+// gpu::ode::integrate (integrator.cuh:470-487): t += dt while t < T_end, the
+// observer every save_interval steps and once after the loop.
+template <class F, class Obs>
+void gpu_integrate(F&& f, double* x, size_t n, double T0, double T1, double dt, size_t save_interval,
+                   const char* method, Obs&& obs) {
+    GpuStepWork w(n);
+    const char* m = method;
+    double t = T0;
+    size_t step = 0;
+    while (t < T1) {
+        if (step % save_interval == 0) obs(t);
+        if (!gpu_step(f, x, n, t, dt, m, w)) {
+            m = "ssprk53";
+            gpu_step(f, x, n, t, dt, m, w);
+        }
+        t += dt;
+        ++step;
+    }
+    obs(t);
+}
+
 }  // namespace
 
 extern "C" {
@@ -753,6 +961,33 @@ size_t orc_integrate_md(const OracleLattice* L, double* state, double T0, double
     auto stp = [&](double t, double h) { rk4_step_generic(f, state, n, t, h, w); };
     integrate_const(stp, T0, T1, dt, obs);
     return nrec;
+}
+
+int orc_integrate_gpu(const OracleLattice* L, double* state, double T0, double T1, double dt,
+                      size_t save_interval, const char* method, size_t max_rec, size_t* n_rec, double* times,
+                      double* states, double* out_mags, double* energy) {
+    const size_t n = L->N * L->D;
+    GpuStepWork probe(0);
+    double dummy = 0.0;
+    const bool known = gpu_step([](const double*, double*, double) {}, &dummy, 0, 0.0, 1.0, method, probe);
+    size_t nrec = 0;
+    std::vector<double> m(3 * L->D);
+    auto obs = [&](double t) {
+        if (nrec < max_rec) {
+            if (times) times[nrec] = t;
+            if (states) std::memcpy(states + nrec * n, state, n * sizeof(double));
+            if (out_mags) {
+                mags(*L, state, m.data());
+                std::memcpy(out_mags + nrec * 3 * L->D, m.data(), sizeof(double) * 3 * L->D);
+            }
+            if (energy) energy[nrec] = orc_energy(L, state) / double(L->N);
+        }
+        ++nrec;
+    };
+    auto f = [&](const double* a, double* b, double tt) { rhs(*L, a, b, tt); };
+    gpu_integrate(f, state, n, T0, T1, dt, save_interval, method, obs);
+    if (n_rec) *n_rec = nrec;
+    return known ? 1 : 0;
 }
 
 long orc_integrate_dopri5(const OracleLattice* L, double* state, double T0, double T1, double dt,
@@ -1456,6 +1691,28 @@ size_t orc_mixed_integrate_md(const MixedOracle* M, double* state, double T0, do
     auto stp = [&](double t, double h) { rk4_step_generic(F, state, n, t, h, w); };
     integrate_const(stp, T0, T1, dt, obs);
     return nrec;
+}
+
+int orc_mixed_integrate_gpu(const MixedOracle* M, double* state, double T0, double T1, double dt,
+                            size_t save_interval, const char* method, size_t max_rec, size_t* n_rec,
+                            double* times, double* states, double* recs) {
+    const size_t n = M->N2 * 3 + M->N3 * 8;
+    GpuStepWork probe(0);
+    double dummy = 0.0;
+    const bool known = gpu_step([](const double*, double*, double) {}, &dummy, 0, 0.0, 1.0, method, probe);
+    size_t nrec = 0;
+    auto obs = [&](double t) {
+        if (nrec < max_rec) {
+            if (times) times[nrec] = t;
+            if (states) std::memcpy(states + nrec * n, state, n * sizeof(double));
+            if (recs) mixed_record(*M, state, recs + nrec * 34);
+        }
+        ++nrec;
+    };
+    auto F = [M](const double* x, double* dx, double t) { mixed_rhs(*M, x, dx, t); };
+    gpu_integrate(F, state, n, T0, T1, dt, save_interval, method, obs);
+    if (n_rec) *n_rec = nrec;
+    return known ? 1 : 0;
 }
 
 // Dormand-Prince 5(4) over the flat mixed state (MixedLattice::

@@ -77,6 +77,16 @@ size_t orc_boost_num_steps(double t0, double t1, double dt);
 size_t orc_integrate_md(const OracleLattice* L, double* state, double T0, double T1,
                         double dt, size_t save_interval, size_t max_rec,
                         double* times, double* mags, double* energy_per_site);
+/* The reference GPU path's integrate_gpu (lattice_gpu.cu:688-702) with any
+ * method of gpu::ode::step (integrator.cuh:91-458), restated operation by
+ * operation: loop t += dt while t < T1 (integrator.cuh:470-487), observer
+ * every save_interval steps and after the loop; records times, full states
+ * [n_rec*N*D], mags [n_rec*3*D], E/N (any may be NULL).  An unknown method
+ * runs ssprk53 (integrator.cuh:454-458) and the call returns 0 (1 when the
+ * name is known).  *n_rec receives the number of observer calls. */
+int    orc_integrate_gpu(const OracleLattice* L, double* state, double T0, double T1, double dt,
+                         size_t save_interval, const char* method, size_t max_rec, size_t* n_rec,
+                         double* times, double* states, double* mags, double* energy_per_site);
 /* Dormand-Prince 5(4) (Boost.Odeint runge_kutta_dopri5, make_controlled(abs,
  * rel)) as integrate_ode_system uses it (lattice.h:4072-4085):
  *   control 0: fixed steps on the integrate_const grid (no error control,
@@ -181,6 +191,11 @@ void   orc_mixed_record(const MixedOracle* M, const double* x, double* out34);
 /* Boost RK4 + integrate_const over [T0, T1]; records every step. */
 size_t orc_mixed_integrate_md(const MixedOracle* M, double* state, double T0, double T1,
                               double dt, size_t max_rec, double* times, double* recs);
+/* integrate_mixed_gpu (mixed_lattice_gpu.cu:469-488, :679-693) with any
+ * method of gpu::ode::step, as orc_integrate_gpu; records of 34 doubles. */
+int    orc_mixed_integrate_gpu(const MixedOracle* M, double* state, double T0, double T1, double dt,
+                               size_t save_interval, const char* method, size_t max_rec, size_t* n_rec,
+                               double* times, double* states, double* recs);
 /* Dormand-Prince 5(4) over the flat mixed state with the controls of
  * orc_integrate_dopri5 (mixed_lattice.h:3960-4010); records of 34 doubles
  * every save_interval-th observer call.  Returns the number of records, or

@@ -59,6 +59,8 @@ _SIG = {
     "orc_mixed_energy": (_d, [_vp, _pd]),
     "orc_mixed_record": (None, [_vp, _pd, _pd]),
     "orc_mixed_integrate_md": (_sz, [_vp, _pd, _d, _d, _d, _sz, _pd, _pd]),
+    "orc_integrate_gpu": (_i, [_vp, _pd, _d, _d, _d, _sz, C.c_char_p, _sz, _psz, _pd, _pd, _pd, _pd]),
+    "orc_mixed_integrate_gpu": (_i, [_vp, _pd, _d, _d, _d, _sz, C.c_char_p, _sz, _psz, _pd, _pd, _pd]),
     "orc_mixed_integrate_dopri5": (C.c_long, [_vp, _pd, _d, _d, _d, _i, _d, _d, _sz, _pd, _sz, _sz, _pd, _pd,
                                               _psz, _psz]),
 }
@@ -209,6 +211,27 @@ class OracleLattice:
             mx = nr
         return {"times": t[:nr], "mags": m.reshape(mx, 3, self.D)[:nr], "energy": e[:nr], "final": s,
                 "accepted": na.value, "rejected": nr_.value}
+
+    def integrate_gpu(self, state, T0, T1, dt, method, save_interval=1, want_states=False):
+        """The reference GPU integrate_gpu loop (t += dt while t < T1) with any
+        gpu::ode::step method; records times, mags, E/N and optionally the
+        full states.  'known' is False when the name fell back to ssprk53."""
+        n_steps = gpu_num_steps(T0, T1, dt)
+        mx = n_steps // save_interval + 2
+        s = _f(state).copy()
+        t = np.zeros(mx)
+        m = np.zeros(mx * 3 * self.D)
+        e = np.zeros(mx)
+        st = np.zeros(mx * s.size) if want_states else None
+        nr = C.c_size_t()
+        known = lib().orc_integrate_gpu(self.h, _p(s), T0, T1, dt, save_interval, method.encode(), mx, C.byref(nr),
+                                        _p(t), _p(st), _p(m), _p(e))
+        k = nr.value
+        out = {"times": t[:k], "mags": m.reshape(mx, 3, self.D)[:k], "energy": e[:k], "final": s,
+               "known": bool(known)}
+        if want_states:
+            out["states"] = st.reshape(mx, -1)[:k]
+        return out
 
     def single_pulse_drive(self, init, field_in, t_B, amp, width, freq, T0, T1, step, chunking=False):
         n = lib().orc_boost_num_steps(T0, T1, step) + 4
@@ -364,6 +387,24 @@ class MixedOracle:
         nr = lib().orc_mixed_integrate_md(self.h, _p(s), T0, T1, dt, n, _p(t), _p(r))
         return {"times": t[:nr], "records": r.reshape(n, self.REC)[:nr], "final": s}
 
+    def integrate_gpu(self, state, T0, T1, dt, method, save_interval=1, want_states=False):
+        """integrate_mixed_gpu with any gpu::ode::step method (records of 34
+        values, optional full states)."""
+        n_steps = gpu_num_steps(T0, T1, dt)
+        mx = n_steps // save_interval + 2
+        s = _f(state).copy()
+        t = np.zeros(mx)
+        r = np.zeros(mx * self.REC)
+        st = np.zeros(mx * s.size) if want_states else None
+        nr = C.c_size_t()
+        known = lib().orc_mixed_integrate_gpu(self.h, _p(s), T0, T1, dt, save_interval, method.encode(), mx,
+                                              C.byref(nr), _p(t), _p(st), _p(r))
+        k = nr.value
+        out = {"times": t[:k], "records": r.reshape(mx, self.REC)[:k], "final": s, "known": bool(known)}
+        if want_states:
+            out["states"] = st.reshape(mx, -1)[:k]
+        return out
+
     def integrate_dopri5(self, state, T0, T1, dt, control=1, abs_tol=1e-6, rel_tol=1e-6, save_interval=1,
                          segments=None):
         """Dormand-Prince 5(4) over the flat state with the controls of
@@ -390,6 +431,15 @@ class MixedOracle:
 
 def boost_num_steps(T0, T1, dt):
     return lib().orc_boost_num_steps(T0, T1, dt)
+
+
+def gpu_num_steps(T0, T1, dt):
+    """Steps of the reference GPU loop t += dt while t < T1 (integrator.cuh:477-484)."""
+    t, n = T0, 0
+    while t < T1:
+        t += dt
+        n += 1
+    return n
 
 
 def pulse_segments(T0, T1, centers, window, T_step, free_dt_factor):

@@ -1,7 +1,8 @@
 // Drives the gpu:: drop-in API (include/classical_spin_amd/lattice_gpu_api.h)
 // the way the reference's Lattice glue does (lattice.h:5429-5522, 5527-5602,
 // 5617-5750): padded tables, pulse upload, spins, integrate_gpu with full
-// state records, integrate_gpu_batched with tau2 per replica.
+// state records (rk4, the glue's default dopri5 and the API default
+// ssprk53), step_gpu, integrate_gpu_batched with tau2 per replica.
 //
 // Inputs (binary, little-endian doubles / size_t) are written by
 // tests/test_cxx_shim.py; outputs are written back for comparison with the
@@ -72,21 +73,38 @@ int main(int argc, char** argv) {
         std::vector<double> ml, ms;
         gpu::compute_magnetization_gpu(h, ml, ms);
 
-        bool threw = false;
-        try {
-            gpu::step_gpu(h, 0.0, 0.01, "dopri5");
-        } catch (const std::runtime_error&) {
-            threw = true;
-        }
+        // one step_gpu with the Dormand-Prince tableau from the state the
+        // integration left on the device
+        gpu::step_gpu(h, 0.0, 0.01, "dopri5");
+        std::vector<double> after_step;
+        gpu::get_gpu_spins(h, after_step);
 
+        // molecular_dynamics_gpu with the glue's default method "dopri5"
+        // (spin_config.h md_integrator; lattice.h:5527-5602)
+        gpu::set_gpu_spins(h, spins);
+        std::vector<std::pair<double, std::vector<double>>> traj_dp;
+        gpu::integrate_gpu(h, tgrid[0], tgrid[1], tgrid[2], 10, traj_dp, "dopri5");
+        std::vector<double> t_dp, s_dp;
+        for (auto& r : traj_dp) {
+            t_dp.push_back(r.first);
+            s_dp.insert(s_dp.end(), r.second.begin(), r.second.end());
+        }
+        // the API default (ssprk53) with save_interval 25
+        gpu::set_gpu_spins(h, spins);
+        std::vector<std::pair<double, std::vector<double>>> traj_ss;
+        gpu::integrate_gpu(h, tgrid[0], tgrid[1], tgrid[2], 25, traj_ss);
+        std::vector<double> s_ss;
+        for (auto& r : traj_ss) s_ss.insert(s_ss.end(), r.second.begin(), r.second.end());
+
+        // batched mode with a non-rk4 method warns and runs rk4 (lattice_gpu.cu:779-784)
         gpu::BatchedMagResult bm = gpu::integrate_gpu_batched(h, spins, tau2, afm, frames, tgrid[0], tgrid[1],
-                                                              tgrid[2], 1, "rk4");
+                                                              tgrid[2], 1, "dopri5");
         gpu::destroy_gpu_lattice_data(h);
         gpu::destroy_gpu_lattice_data(nullptr);  // reference: null is a no-op
 
         FILE* out = std::fopen(argv[2], "wb");
         if (!out) throw std::runtime_error("cannot open output");
-        write_vec(out, std::vector<double>{E0, threw ? 1.0 : 0.0});
+        write_vec(out, std::vector<double>{E0, 0.0});
         write_vec(out, t_out);
         write_vec(out, s_out);
         write_vec(out, ml);
@@ -94,6 +112,10 @@ int main(int argc, char** argv) {
         write_vec(out, std::vector<double>{double(bm.B), double(bm.spin_dim), double(bm.n_time_points)});
         write_vec(out, bm.times);
         write_vec(out, bm.mag_data);
+        write_vec(out, after_step);
+        write_vec(out, t_dp);
+        write_vec(out, s_dp);
+        write_vec(out, s_ss);
         std::fclose(out);
     } catch (const std::exception& e) {
         std::fprintf(stderr, "test_gpu_api_shim: %s\n", e.what());

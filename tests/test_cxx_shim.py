@@ -59,10 +59,8 @@ def test_shim_against_oracle(tmp_path):
     print(f"running {EXE}", flush=True)
     subprocess.run(["timeout", "-k", "5", "60", EXE, str(inp), str(out)], check=True)
     with open(out, "rb") as fh:
-        meta, t_out, s_out, ml, ms, bmeta, btimes, bmag = (_r(fh) for _ in range(8))
-    # energy and the unsupported-method error path
+        meta, t_out, s_out, ml, ms, bmeta, btimes, bmag, after_step, t_dp, s_dp, s_ss = (_r(fh) for _ in range(12))
     assert meta[0] == pytest.approx(o.energy(s.reshape(-1)), rel=1e-12)
-    assert meta[1] == 1.0
     # integrate_gpu: GPU grid t += dt while t < T1, records every 10 steps + final
     o.set_pulse(f, t1, f, t2, amp, width, freq)
     grid = []
@@ -103,3 +101,115 @@ def test_shim_against_oracle(tmp_path):
             t += dt
             k += 1
         assert np.max(np.abs(o.mags(x) - bmag[k, b])) <= 1e-10
+    # step_gpu("dopri5") from the integrated state, at t = 0 (pulses at t1, t2)
+    o.set_pulse(f, t1, f, t2, amp, width, freq)
+    one = o.integrate_gpu(ref_s[-1], 0.0, 0.005, 0.01, "dopri5")
+    assert np.max(np.abs(after_step - one["final"])) <= 1e-13
+    # integrate_gpu with the glue's default "dopri5" and the API default ssprk53
+    dp = o.integrate_gpu(s.reshape(-1), T0, T1, dt, "dopri5", save_interval=10, want_states=True)
+    assert np.array_equal(t_dp, dp["times"])
+    assert np.max(np.abs(s_dp - dp["states"].reshape(-1))) <= 1e-10
+    ss = o.integrate_gpu(s.reshape(-1), T0, T1, dt, "ssprk53", save_interval=25, want_states=True)
+    assert np.max(np.abs(s_ss - ss["states"].reshape(-1))) <= 1e-10
+
+
+MIXED_EXE = os.path.join(ROOT, "tests", "cxx", "test_mixed_gpu_api_shim")
+
+
+def _padded_mixed(offset, partner, J, n_sites, pad_partner):
+    """Per-site padded mixed tables as the MixedLattice glue flattens them
+    (mixed_lattice.h:6193-6233): partners SIZE_MAX and zero matrices beyond
+    a site's count."""
+    off = offset.astype(np.int64)
+    cnt = np.diff(off)
+    mx = int(cnt.max()) if n_sites else 0
+    m = J.shape[1] * J.shape[2]
+    mats = np.zeros((n_sites, mx, m))
+    par = np.full((n_sites, mx), np.iinfo(np.uint64).max, dtype=np.uint64)
+    src = np.full((n_sites, mx), np.iinfo(np.uint64).max, dtype=np.uint64)
+    for i in range(n_sites):
+        for n in range(cnt[i]):
+            mats[i, n] = J[off[i] + n].reshape(-1)
+            par[i, n] = partner[off[i] + n]
+            src[i, n] = i
+    return mx, mats, par, src, cnt.astype(np.uint64)
+
+
+def test_mixed_shim_binary_built():
+    assert os.path.exists(MIXED_EXE), "tests/cxx/test_mixed_gpu_api_shim is built by __graft_entry__.build()"
+
+
+@pytest.mark.gpu
+def test_mixed_shim_against_oracle(tmp_path):
+    """mixed_gpu:: through the reference include name
+    classical_spin/gpu/mixed_lattice_gpu_api.h, driven like MixedLattice's
+    GPU glue, against the mixed oracle."""
+    import pyoracle
+    from classical_mcmc_spin_amd import mixed_model as mm
+    from test_mixed_oracle import random_flat, rich_cell
+    muc = rich_cell()
+    ml = mm.MixedLattice(muc, 2, 2, 2)
+    o = pyoracle.MixedOracle(muc, 2, 2, 2)
+    s2, s3 = ml.su2, ml.su3
+    x = random_flat(ml, 91)
+    rng = np.random.default_rng(12)
+    f2a, f2b = rng.normal(size=(s2.N_atoms, 3)), rng.normal(size=(s2.N_atoms, 3))
+    f3a, f3b = rng.normal(size=(s3.N_atoms, 8)), rng.normal(size=(s3.N_atoms, 8))
+    dd2 = np.stack([np.einsum("aji,aj->ai", s2.frames, f) for f in (f2a, f2b)])
+    dd3 = np.stack([np.einsum("aji,aj->ai", s3.frames, f) for f in (f3a, f3b)])
+    p2 = [0.5, 0.3, 2.0, 0.0, 0.2]
+    p3 = [0.4, 0.25, 1.5, 0.1, 0.2]
+    T0, T1, dt = -0.2, 0.3, 0.005
+    b2, q2, c2 = s2.padded_tables()
+    b3, q3, c3 = s3.padded_tables()
+    m23, mats23, par23, src23, cnt23 = _padded_mixed(ml.m2_offset, ml.m2_partner, ml.m2_J, ml.N2, True)
+    m32, mats32, par32, _, cnt32 = _padded_mixed(ml.m3_offset, ml.m3_partner, ml.m3_J, ml.N3, True)
+    inp, out = tmp_path / "in.bin", tmp_path / "out.bin"
+    with open(inp, "wb") as fh:
+        _w(fh, [ml.N2, s2.N_atoms, ml.N3, s3.N_atoms, s2.num_bi, s3.num_bi, m23, m32], "u")
+        _w(fh, s2.field, "d")
+        _w(fh, s2.onsite, "d")
+        _w(fh, b2, "d")
+        _w(fh, q2, "u")
+        _w(fh, c2, "u")
+        _w(fh, s3.field, "d")
+        _w(fh, s3.onsite, "d")
+        _w(fh, b3, "d")
+        _w(fh, q3, "u")
+        _w(fh, c3, "u")
+        _w(fh, mats23, "d")
+        _w(fh, src23, "u")
+        _w(fh, par23, "u")
+        _w(fh, cnt23, "u")
+        _w(fh, mats32, "d")
+        _w(fh, par32, "u")
+        _w(fh, cnt32, "u")
+        _w(fh, dd2, "d")
+        _w(fh, p2, "d")
+        _w(fh, dd3, "d")
+        _w(fh, p3, "d")
+        _w(fh, x, "d")
+        _w(fh, [T0, T1, dt], "d")
+    print(f"running {MIXED_EXE}", flush=True)
+    subprocess.run(["timeout", "-k", "5", "60", MIXED_EXE, str(inp), str(out)], check=True)
+    with open(out, "rb") as fh:
+        back, t_dp, s_dp, t_rk, s_rk, stepped, m2, st2, m3, st3, normed, dims = (_r(fh) for _ in range(12))
+    assert np.array_equal(back, x)
+    o.set_pulse(2, f2a, p2[3], f2b, p2[4], p2[0], p2[1], p2[2])
+    o.set_pulse(3, f3a, p3[3], f3b, p3[4], p3[0], p3[1], p3[2])
+    for method, tt, ss in (("dopri5", t_dp, s_dp), ("rk4", t_rk, s_rk)):
+        ref = o.integrate_gpu(x, T0, T1, dt, method, save_interval=5, want_states=True)
+        assert np.array_equal(tt, ref["times"]), method
+        assert np.max(np.abs(ss - ref["states"].reshape(-1))) <= 1e-10, method
+    one = o.integrate_gpu(x, T0, T0 + 0.5 * dt, dt, "ssprk53")["final"]
+    assert np.max(np.abs(stepped - one)) <= 1e-13
+    # compute_magnetization_mixed_gpu: sums / N with sign 1 - 2((i % N_atoms) & 1)
+    y2, y3 = ml.split_state(one)
+    for y, A, loc, stg in ((y2, s2.N_atoms, m2, st2), (y3, s3.N_atoms, m3, st3)):
+        sg = np.where((np.arange(len(y)) % A) & 1, -1.0, 1.0)
+        assert np.allclose(loc, y.mean(axis=0), atol=1e-12)
+        assert np.allclose(stg, (y * sg[:, None]).mean(axis=0), atol=1e-12)
+    n2, n3 = ml.split_state(normed)
+    assert np.allclose(n2, y2 / np.linalg.norm(y2, axis=1, keepdims=True), atol=1e-13)
+    assert np.allclose(n3, y3 / np.linalg.norm(y3, axis=1, keepdims=True), atol=1e-13)
+    assert tuple(dims) == (ml.N2, 3, s2.N_atoms, ml.N3, 8, s3.N_atoms)
