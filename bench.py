@@ -1,19 +1,37 @@
 #!/usr/bin/env python3
-"""Benchmark of the batched FP64 2DCS workload on one MI355X.
+"""Benchmark of the FP64 Landau-Lifshitz RK4 path on one MI355X.
 
-Workload (one "step" = one pass of the hot path over one batch):
-  Kitaev honeycomb 36x36x1 (2592 SU(2) spins), K=-1, Gamma=0.25, Gamma'=-0.02,
-  static h = 0.1 along [111] (unit direction), Gaussian pulses A=0.5, w=0.5
-  along global x, omega = 0; pulse 1 at t=0, delays tau = 0, 0.25, ..., 63.75
-  (256 points).  M0 + 256 M1(tau) + 256 M01(tau) = 513 trajectories of 8192
-  RK4 steps (dt = 0.01, t in [-10, 71.92]) from one seeded relaxed state tilted
-  by N(0, 0.01) noise and renormalised; M(t) recorded every step.
+Default run: the batched 2DCS workload is the headline (one "step" = one
+pass of the hot path over its batch); the four other workloads run after it
+and are reported as sub-results of the same JSON line ("configs"), each with
+its value, its error against the oracle, the roofline of its dominant kernel
+and a CPU baseline at one thread and at every available thread.
 
-Prints one JSON line (rank 0).  `value` = trajectories/s of the batched
-persistent kernel with the inputs and the M(t) records resident in HBM (the
-timed steps do not copy the 302 MB of records to the host); the rate with that
-copy (pcie_inclusive_traj_per_s), the plain one-trajectory path, the oracle CPU
-baseline and the error against the oracle are reported beside it.
+  2dcs        Kitaev honeycomb 36x36x1 (2592 SU(2) spins), K=-1, Gamma=0.25,
+              Gamma'=-0.02, static h = 0.1 along [111] (unit direction),
+              Gaussian pulses A = 0.5 along global x, omega = 0, width w = 0.5
+              in the reference's exp(-((t-t_c)/(2w))^2) (sigma = sqrt(2) w =
+              0.707 in exp(-(t-t_c)^2/(2 sigma^2))); pulse 1 at t = 0, delays
+              tau on the 256-point grid linspace(0, 64); M0 + 256 M1(tau) + 256
+              M01(tau) = 513 trajectories of 8192 RK4 steps (dt = 0.01,
+              t in [-10, 71.92]) from one seeded relaxed state tilted by
+              N(0, 0.01) noise and renormalised; M(t) recorded every step.
+  md          plain LL dynamics: Kitaev 24x24x1 (1152 sites), no field,
+              seeded random start, 10^4 RK4 steps (dt = 0.01), M(t) and E(t)
+              every step, one trajectory per launch.
+  pyrochlore  64 samples of the 16^3-cell conventional pyrochlore lattice
+              (65 536 sites), 4096 RK4 steps (dt = 0.02).
+  adjoint     32 parameter starts of the discrete-adjoint gradient on the
+              24x24 Kitaev lattice, 2000 RK4 steps, Gilbert alpha = 0.02.
+  mixed       257-trajectory SU(2)+SU(3) 2DCS batch on the 8x8x8 TmFeO3-like
+              lattice, 8192 RK4 steps (dt = 0.005), and the gradient w.r.t.
+              the Fe-Tm coupling scale.
+
+`value` = trajectories/s of the 2DCS batch with the inputs and the M(t)
+records resident in HBM (the timed steps do not copy the 302 MB of records
+to the host); the rate with that copy is pcie_inclusive_traj_per_s.
+--workload NAME runs one workload alone (its own JSON line; used for
+profiling).
 """
 from __future__ import annotations
 
@@ -30,150 +48,253 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 METRIC = "LL RK4 site-steps/s + 2DCS trajectories/s, batched vs plain, at fixed FP64 error"
-HBM_PEAK = 8.0e12          # MI355X HBM3E, B/s
-BYTES_PER_SITE_STEP = 192  # 4 RK stages x (read + write) x 3 doubles x 8 B
-FLOPS_PER_SITE_STEP = 306  # z = 3 SU(2) site, RK4
+HBM_PEAK = 8.0e12           # MI355X HBM3E, B/s
+FP64_PEAK = 78.6e12         # MI355X FP64 (vector and matrix), FLOP/s
+BYTES_PER_SITE_STEP = 192   # 4 RK stages x (read + write) x 3 doubles x 8 B
+# Algorithmic FP64 flops per SU(2) site-step (FMA = 2 flops), z = 3 bonds:
+# per RK stage 54 (three 3x3 bond mat-vecs) + 3 (field) + 9 (H x S) + 12
+# (accumulator and next stage input), i.e. 78, the last stage 72; records of
+# M and E every step 13  ->  4 * 78 - 6 + 13 = 319; a pulse drive adds 12 per
+# stage (f1 d1 + f2 d2), 48 per step.
+FLOPS_Z3 = 319
+FLOPS_DRIVE = 48
 
-L1 = L2 = 36
 DT = 0.01
+L1 = L2 = 36
 T0 = -10.0
 N_STEPS = 8192
 T1 = T0 + (N_STEPS + 0.5) * DT     # Boost grid: exactly N_STEPS steps
-TAU0, TAU_STEP, N_TAU = 0.0, 0.25, 256
+N_TAU = 256
+TAUS = np.linspace(0.0, 64.0, N_TAU)
 AMP, WIDTH, FREQ = 0.5, 0.5, 0.0
 
 
-def build_workload():
-    from classical_mcmc_spin_amd import workloads
-    return workloads.kitaev_2dcs(L1, L2)
+def cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for ln in f:
+                if ln.startswith("model name"):
+                    return ln.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor() or platform.machine()
 
 
-def replica_tables(n_tau):
-    from classical_mcmc_spin_amd import workloads
-    taus = TAU0 + TAU_STEP * np.arange(n_tau)
-    tp, am = workloads.pump_probe_replicas(taus, AMP)
-    return tp, am, taus
-
-
-def cpu_baseline(uc, init, field_in, n_tau_sample):
-    """Oracle (FP64 C++/OpenMP restatement, outer OpenMP over tau with serial
-    inner RHS as the reference's W2 path) on a bounded sample of the workload."""
+def _oracle():
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle
-    o = pyoracle.OracleLattice(uc, L1, L2, 1)
-    threads = pyoracle.max_threads()
-    t = time.perf_counter()
-    o.pump_probe(init.reshape(-1), field_in, AMP, WIDTH, FREQ, TAU0, TAU0 + TAU_STEP * (n_tau_sample - 1),
-                 TAU_STEP, T0, T1, DT, reuse_m0=False, n_outer=threads)
-    el = time.perf_counter() - t
-    n_traj = 1 + 2 * n_tau_sample
-    return {"value": n_traj / el, "unit": "trajectories/s", "cores": threads, "kind": "port",
-            "sample": f"{n_traj} trajectories (M0 + {n_tau_sample} M1 + {n_tau_sample} M01) x {N_STEPS} "
-                      f"RK4 steps of the same 36x36 lattice, {el:.1f} s on {threads} OpenMP threads "
-                      f"({platform.processor() or platform.machine()})",
-            "site_steps_per_s": n_traj * N_STEPS * L1 * L2 * 2 / el}
+    pyoracle.lib()
+    return pyoracle
 
 
-def parity_check(uc, lat, gl, init, field_in, n_steps=500):
-    """Short-horizon error of the batched kernel against the oracle (3
-    trajectories: M0, M1(tau=0.25), M01(tau=0.25))."""
-    sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    import pyoracle
-    o = pyoracle.OracleLattice(uc, L1, L2, 1)
-    Tend = T0 + (n_steps + 0.5) * DT
-    tp = np.array([[0.0, 0.0], [0.25, 0.0], [0.0, 0.25]])
-    am = np.array([[AMP, 0.0], [AMP, 0.0], [AMP, AMP]])
-    res = gl.integrate_batched(init, tp, am, T0, Tend, DT, tgrid="boost", want_final=True)
-    err_m = err_s = 0.0
-    f = field_in
-    for b in range(3):
-        o.set_pulse(f * (am[b, 0] != 0), tp[b, 0], f * (am[b, 1] != 0), tp[b, 1], AMP, WIDTH, FREQ)
-        ref = o.integrate_md(init.reshape(-1), T0, Tend, DT)
-        err_m = max(err_m, float(np.max(np.abs(res["mags"][:, b] - ref["mags"]))))
-        err_s = max(err_s, float(np.max(np.abs(res["final"][b] - ref["final"].reshape(-1, 3)))))
-    return err_m, err_s
+def cpu_baseline(unit, per_run, runs_nproc, runs_one, describe):
+    """Times the oracle (the FP64 C++/OpenMP restatement of the reference CPU
+    path, "port") at every available OpenMP thread and at one thread.
+    per_run(n) runs a sample of size n and returns the work it stands for in
+    `unit` (e.g. trajectories); runs_* are the sample sizes."""
+    po = _oracle()
+    full = po.max_threads()
+    out = {}
+    for threads, n in ((full, runs_nproc), (1, runs_one)):
+        po.set_threads(threads)
+        t = time.perf_counter()
+        work = per_run(n)
+        el = time.perf_counter() - t
+        out[threads] = (work / el, el, n)
+    po.set_threads(full)
+    v, el, n = out[full]
+    v1, el1, n1 = out[1]
+    return {"value": v, "unit": unit, "cores": full, "kind": "port", "cpu_model": cpu_model(),
+            "sample": describe(n, el, full),
+            "single_thread": {"value": v1, "unit": unit, "cores": 1, "sample": describe(n1, el1, 1)}}
 
 
-PYRO_L, PYRO_B, PYRO_STEPS, PYRO_DT = 16, 64, 4096, 0.02
+def fp64_roofline(kernel, flops_launch, kernel_ms, **extra):
+    a = flops_launch / (kernel_ms * 1e-3)
+    r = {"bound": "fp64", "achieved": a / 1e12, "peak": FP64_PEAK / 1e12, "unit": "TFLOP/s",
+         "frac": a / FP64_PEAK, "traffic": None, "kernel": kernel, "kernel_ms": kernel_ms,
+         "algorithmic_flops_per_launch": flops_launch}
+    r.update(extra)
+    return r
 
 
-def bench_pyrochlore(args):
-    """64 samples x 4096 RK4 steps (dt = 0.02) of the 16^3-cell conventional
-    pyrochlore lattice (65 536 sites, z = 6), all-in-all-out + tilt starts,
-    M(t) and E(t) every step.  The lattice is too large for one workgroup's
-    LDS, so the grid-wide stage kernels run (4 launches per step)."""
+def hbm_roofline(kernel, bytes_launch, kernel_ms, **extra):
+    a = bytes_launch / (kernel_ms * 1e-3)
+    r = {"bound": "hbm", "achieved": a / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": a / HBM_PEAK,
+         "traffic": None, "kernel": kernel, "kernel_ms": kernel_ms, "algorithmic_bytes_per_launch": bytes_launch}
+    r.update(extra)
+    return r
+
+
+def profile_number(name, key):
+    """A `key: value` line of a committed rocprofv3 summary in profiles/."""
+    path = os.path.join(ROOT, "profiles", name)
+    if os.path.exists(path):
+        for ln in open(path):
+            if ln.startswith(key + ":"):
+                return float(ln.split(":", 1)[1].split()[0])
+    return None
+
+
+# --------------------------------------------------------------------- 2DCS
+
+def bench_2dcs(args):
+    """The batched 2DCS workload (module docstring).  Parity: the timed
+    launch shape at the full batch (513 trajectories, work split active)
+    over 500 steps, M0, M1 and M01 trajectories and the trajectories whose
+    state crosses a workgroup seam, against the oracle."""
     from classical_mcmc_spin_amd import GPULattice, workloads
-    lat = workloads.pyrochlore_lattice(PYRO_L)
+    from classical_mcmc_spin_amd import gpu as G
+    uc, lat, init, field_in = workloads.kitaev_2dcs(L1, L2)
+    tp, am = workloads.pump_probe_replicas(TAUS, AMP)
+    B = tp.shape[0]
     N = lat.lattice_size
-    init, _ = workloads.pyrochlore_tilted_aiao(lat, PYRO_B, seed=1234)
-    T1p = (PYRO_STEPS + 0.5) * PYRO_DT
-    zero = np.zeros((PYRO_B, 2))
     gl = GPULattice(lat)
+    fd = np.einsum("aij,aj->ai", lat.frames, field_in).reshape(-1)
+    gl.set_pulse(np.concatenate([fd, fd]), AMP, WIDTH, FREQ, 0.0, 0.0)
+    if args.launch:
+        blk, spt, fl = (int(v) for v in args.launch.split(","))
+        gl.set_launch_config(B, blk, spt, fl)
+    elif not args.profile:
+        blk, ms = gl.autotune(B, 64)
+        print(f"[bench] 2dcs: autotuned block {blk} ({ms:.2f} ms / 64 steps)", file=sys.stderr, flush=True)
 
-    def one_step():
-        return gl.integrate_batched(init, zero, zero, 0.0, T1p, PYRO_DT, tgrid="boost")
+    def one_step(records_to_host):
+        return gl.integrate_batched(init, tp, am, T0, T1, DT, tgrid="boost", want_energy=False,
+                                    want_mags=records_to_host)
 
-    for _ in range(args.warmup):
-        one_step()
+    for _ in range(max(args.warmup, 1)):
+        res = one_step(True)
+    assert res["mags"].shape[0] == N_STEPS + 1 and np.all(np.isfinite(res["mags"]))
     kernel_ms = []
     t = time.perf_counter()
     for _ in range(args.steps):
-        res = one_step()
+        one_step(False)
         kernel_ms.append(gl.last_kernel_ms()[0])
     el = time.perf_counter() - t
-    assert res["mags"].shape[0] == PYRO_STEPS + 1 and np.all(np.isfinite(res["mags"]))
-    gl.close()
+    ms_step = 1e3 * el / args.steps
+    traj_s = B * args.steps / el
+    launch = gl.last_launch_config()
+    launch["workgroups"] = gl.last_grid()
     kms = float(np.mean(kernel_ms))
-    site_steps = PYRO_B * N * PYRO_STEPS
-    samples_s = PYRO_B * args.steps / el
-    bytes_launch = PYRO_B * N * BYTES_PER_SITE_STEP / 4       # one RK stage of every site
+    if args.profile:
+        gl.close()
+        return {"profile": True, "workload": "2dcs", "kernel_ms": kms, "ms_per_step": ms_step, "launch": launch,
+                "traj_per_s": traj_s}
+    t = time.perf_counter()
+    one_step(True)
+    pcie_traj_s = B / (time.perf_counter() - t)
+
+    # plain one-trajectory-per-launch paths (M01 at tau = 0.25)
+    gl.set_spins(init)
+    gl.set_pulse(np.concatenate([fd, fd]), AMP, WIDTH, FREQ, 0.0, 0.25)
+    plain = {}
+    for name, eng in (("stage_kernels", 1), ("persistent_workgroup", 2)):
+        gl.set_spins(init)
+        t = time.perf_counter()
+        gl.integrate(T0, T1, DT, 1, tgrid="boost", engine=eng, want_energy=False)
+        plain[name] = 1.0 / (time.perf_counter() - t)
+    gl.set_pulse(np.concatenate([fd, fd]), AMP, WIDTH, FREQ, 0.0, 0.0)
+
+    # parity on the timed launch shape: the full batch over 500 steps
+    n_par = 500
+    Tp = T0 + (n_par + 0.5) * DT
+    gl.set_launch_config(B, launch["block"], launch["sites_per_thread"], launch["flags"])
+    pr = gl.integrate_batched(init, tp, am, T0, Tp, DT, tgrid="boost", want_final=True)
+    lp = gl.last_launch_config()
+    same_shape = all(lp[k] == launch[k] for k in ("block", "sites_per_thread", "flags", "field_only", "atom_major"))
+    Gp = gl.last_grid()
+    seams = G.seam_trajectories(B, Gp, n_par) if Gp < B else []
+    check = sorted({0, 1, N_TAU, N_TAU + 1, B - 1, *seams[:3]})
+    po = _oracle()
+    o = po.OracleLattice(uc, L1, L2, 1)
+    err_m = err_s = 0.0
+    for b in check:
+        o.set_pulse(field_in * (am[b, 0] != 0), tp[b, 0], field_in * (am[b, 1] != 0), tp[b, 1], AMP, WIDTH, FREQ)
+        ref = o.integrate_md(init.reshape(-1), T0, Tp, DT)
+        err_m = max(err_m, float(np.max(np.abs(pr["mags"][:, b] - ref["mags"]))))
+        err_s = max(err_s, float(np.max(np.abs(pr["final"][b] - ref["final"].reshape(-1, 3)))))
+    gl.close()
+
+    site_steps = B * N * N_STEPS
+    flops_launch = site_steps * (FLOPS_Z3 + FLOPS_DRIVE)
+    roof = fp64_roofline("k_persistent", flops_launch, kms,
+                         algorithmic_flops_per_site_step=FLOPS_Z3 + FLOPS_DRIVE,
+                         hbm_algorithmic_frac=site_steps * BYTES_PER_SITE_STEP / (kms * 1e-3) / HBM_PEAK)
+    roof["traffic"] = profile_number("traffic_k_persistent_2dcs_36x36.txt", "traffic_bytes_per_launch")
+    valu = profile_number("counters_k_persistent_2dcs_36x36.txt", "valu_issue_frac")
+    if valu is not None:
+        roof["valu_issue_frac"] = valu
     line = {
-        "metric": METRIC, "value": samples_s, "unit": "trajectories/s", "n_gpus": 1,
-        "steps": args.steps, "warmup": args.warmup, "ms_per_step": 1e3 * el / args.steps,
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
-        "data": "synthetic",
-        "config": {"workload": f"pyrochlore_16x16x16_conventional_64samples_{PYRO_STEPS}steps_rk4_dt0.02",
-                   "sites": N, "trajectories": PYRO_B, "rk4_steps": PYRO_STEPS, "record_every": 1},
-        "site_steps_per_s": samples_s * N * PYRO_STEPS,
-        "roofline": {"bound": "hbm", "achieved": bytes_launch / (kms * 1e-3) / 1e9,
-                     "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                     "frac": bytes_launch / (kms * 1e-3) / HBM_PEAK, "traffic": None,
-                     "kernel": "k_stage", "kernel_ms": kms, "algorithmic_bytes_per_launch": bytes_launch},
+        "metric": METRIC, "value": traj_s, "unit": "trajectories/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": ms_step, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "2dcs_kitaev_honeycomb_36x36x1_513traj_8192steps_rk4_dt0.01",
+                   "lattice": [L1, L2, 1], "sites": N, "trajectories": B, "rk4_steps": N_STEPS, "record_every": 1,
+                   "tau_grid": "linspace(0, 64, 256)",
+                   "pulse": {"amp": AMP, "w": WIDTH, "convention": "exp(-((t-tc)/(2w))^2)",
+                             "sigma_equivalent": float(np.sqrt(2.0) * WIDTH), "freq": FREQ},
+                   "launch": launch},
+        "site_steps_per_s": traj_s * N * N_STEPS,
+        "pcie_inclusive_traj_per_s": pcie_traj_s,
+        "plain_traj_per_s": plain,
+        "batched_over_plain": traj_s / max(plain.values()),
+        "batched_over_plain_stage_kernels": traj_s / plain["stage_kernels"],
+        "max_abs_err_vs_oracle_500_steps": {"M": err_m, "S": err_s, "trajectories": check,
+                                            "seam_trajectories": seams[:3], "workgroups": Gp,
+                                            "timed_launch_shape": same_shape},
+        "roofline": roof,
     }
     if not args.no_cpu:
-        sys.path.insert(0, os.path.join(ROOT, "oracle"))
-        import pyoracle
-        o = pyoracle.OracleLattice(lat.unit_cell, PYRO_L, PYRO_L, PYRO_L)
-        n = 60
-        t = time.perf_counter()
-        o.integrate_md(init[0].reshape(-1), 0.0, (n + 0.5) * PYRO_DT, PYRO_DT)
-        el = time.perf_counter() - t
-        line["cpu_baseline"] = {
-            "value": n / el / PYRO_STEPS, "unit": "trajectories/s", "cores": pyoracle.max_threads(),
-            "kind": "port",
-            "sample": f"1 sample x {n} RK4 steps of the same lattice ({el:.1f} s, OpenMP over sites "
-                      f"inside each RHS as the reference's MD path)",
-            "site_steps_per_s": n * N / el}
-    print(json.dumps(line), flush=True)
+        o = po.OracleLattice(uc, L1, L2, 1)
+        tau_s = TAUS[:8]
 
+        def run(n_tau):   # outer OpenMP over tau, serial RHS (the reference's W2 path)
+            o.pump_probe(init.reshape(-1), field_in, AMP, WIDTH, FREQ, float(tau_s[0]), float(tau_s[n_tau - 1]),
+                         float(tau_s[1] - tau_s[0]), T0, T1, DT, reuse_m0=False, n_outer=po.max_threads())
+            return 1 + 2 * n_tau
+
+        def run1(n_steps):  # one M01 trajectory, part of its steps
+            o.set_pulse(field_in, 0.0, field_in, 0.25, AMP, WIDTH, FREQ)
+            o.integrate_md(init.reshape(-1), T0, T0 + (n_steps + 0.5) * DT, DT)
+            return n_steps / N_STEPS
+
+        full = po.max_threads()
+        t = time.perf_counter()
+        w = run(6)
+        el_n = time.perf_counter() - t
+        po.set_threads(1)
+        t = time.perf_counter()
+        w1 = run1(1500)
+        el1 = time.perf_counter() - t
+        po.set_threads(full)
+        line["cpu_baseline"] = {
+            "value": w / el_n, "unit": "trajectories/s", "cores": full, "kind": "port", "cpu_model": cpu_model(),
+            "sample": f"13 trajectories (M0 + 6 M1 + 6 M01) x 8192 RK4 steps of the same 36x36 lattice, "
+                      f"{el_n:.1f} s, outer OpenMP over tau on {full} threads",
+            "site_steps_per_s": w / el_n * N * N_STEPS,
+            "single_thread": {"value": w1 / el1, "unit": "trajectories/s", "cores": 1,
+                              "sample": f"1500 of the 8192 RK4 steps of one M01 trajectory, {el1:.1f} s"}}
+    return line
+
+
+# ----------------------------------------------------------------- plain MD
 
 MD_L, MD_STEPS, MD_DT = 24, 10000, 0.01
 
 
 def bench_md(args):
-    """Plain one-trajectory MD: Kitaev 24x24x1 (1152 sites), K=-1, G=0.25,
-    G'=-0.02, h = 0, seeded random start, 10^4 RK4 steps of dt = 0.01, M(t)
-    and E(t) every step.  value = trajectories/s of the fastest device engine;
-    the three engines (grid-wide stage kernels launched eagerly, the same
-    launch sequence replayed from a hipGraph, one persistent workgroup) are
-    reported beside it."""
-    from classical_mcmc_spin_amd import GPULattice, model, seeding
-    uc = model.build_kitaev_honeycomb(K=-1.0, Gamma=0.25, Gammap=-0.02)
-    lat = model.Lattice(uc, MD_L, MD_L, 1)
-    s0 = seeding.random_spins(lat.lattice_size, 1234)
+    """Plain one-trajectory MD (module docstring).  value = trajectories/s
+    of the fastest engine; every engine is reported.  Error: 500 steps of
+    the fastest engine against the oracle, and the drift of E and |S| over
+    the whole run against the oracle's drift on the same run."""
+    from classical_mcmc_spin_amd import GPULattice, workloads
+    uc, lat, s0 = workloads.kitaev_md(MD_L, MD_L, seed=1234)
+    N = lat.lattice_size
     T1m = (MD_STEPS + 0.5) * MD_DT
     gl = GPULattice(lat)
     eng = {}
+    last = {}
     for name, e in (("stage_kernels", 1), ("stage_kernels_hipgraph", 3), ("persistent_workgroup", 2)):
         for _ in range(max(args.warmup, 1)):
             gl.set_spins(s0)
@@ -184,56 +305,69 @@ def bench_md(args):
             r = gl.integrate(0.0, T1m, MD_DT, 1, tgrid="boost", engine=e)
         el = (time.perf_counter() - t) / args.steps
         assert r["mags"].shape[0] == MD_STEPS + 1 and np.all(np.isfinite(r["mags"]))
-        eng[name] = {"traj_per_s": 1.0 / el, "ms_per_traj": 1e3 * el,
-                     "site_steps_per_s": MD_STEPS * lat.lattice_size / el, "kernel_ms": gl.last_kernel_ms()[0]}
-    gl.close()
+        eng[name] = {"traj_per_s": 1.0 / el, "ms_per_traj": 1e3 * el, "site_steps_per_s": MD_STEPS * N / el,
+                     "kernel_ms": gl.last_kernel_ms()[0], "launches": gl.last_kernel_ms()[1]}
+        last[name] = (r, gl.get_spins(), e)
     best = max(eng, key=lambda k: eng[k]["traj_per_s"])
+    r_full, fin_full, e_best = last[best]
+    gl.set_spins(s0)
+    r500 = gl.integrate(0.0, (500 + 0.5) * MD_DT, MD_DT, 1, tgrid="boost", engine=e_best)
+    fin500 = gl.get_spins()
+    gl.close()
+    po = _oracle()
+    o = po.OracleLattice(uc, MD_L, MD_L, 1)
+    ref500 = o.integrate_md(s0.reshape(-1), 0.0, (500 + 0.5) * MD_DT, MD_DT)
+    t = time.perf_counter()
+    ref = o.integrate_md(s0.reshape(-1), 0.0, T1m, MD_DT)
+    el_full = time.perf_counter() - t
+    drift = lambda e, f: (float(np.max(np.abs(e - e[0]))), float(np.max(np.abs(np.linalg.norm(f, axis=-1) - 1))))
+    dE, dS = drift(r_full["energy"], fin_full)
+    dEo, dSo = drift(ref["energy"], ref["final"].reshape(-1, 3))
     v = eng[best]["traj_per_s"]
-    line = {
-        "metric": METRIC, "value": v, "unit": "trajectories/s", "n_gpus": 1, "steps": args.steps,
-        "warmup": args.warmup, "ms_per_step": 1e3 / v, "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
-        "config": {"workload": "md_kitaev_honeycomb_24x24x1_1traj_10000steps_rk4_dt0.01", "sites": lat.lattice_size,
+    kms = eng[best]["kernel_ms"] * eng[best]["launches"]
+    out = {
+        "value": v, "unit": "trajectories/s", "ms_per_step": 1e3 / v,
+        "config": {"workload": "md_kitaev_honeycomb_24x24x1_1traj_10000steps_rk4_dt0.01", "sites": N,
                    "rk4_steps": MD_STEPS, "record_every": 1, "engine": best},
         "site_steps_per_s": eng[best]["site_steps_per_s"], "engines": eng,
-        "roofline": {"bound": "hbm", "achieved": v * MD_STEPS * lat.lattice_size * BYTES_PER_SITE_STEP / 1e9,
-                     "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                     "frac": v * MD_STEPS * lat.lattice_size * BYTES_PER_SITE_STEP / HBM_PEAK, "traffic": None,
-                     "kernel": best},
+        "max_abs_err_vs_oracle_500_steps": {
+            "M": float(np.max(np.abs(r500["mags"] - ref500["mags"]))),
+            "E": float(np.max(np.abs(r500["energy"] - ref500["energy"]))),
+            "S": float(np.max(np.abs(fin500 - ref500["final"].reshape(-1, 3))))},
+        "drift_10k_steps": {"E_per_site": dE, "spin_length": dS, "oracle_E_per_site": dEo,
+                            "oracle_spin_length": dSo},
+        "roofline": fp64_roofline(best, N * MD_STEPS * FLOPS_Z3, kms, algorithmic_flops_per_site_step=FLOPS_Z3,
+                                  note="one trajectory: one workgroup (persistent) or a few workgroups per "
+                                       "launch (stage kernels) of the 256 CUs"),
     }
     if not args.no_cpu:
-        sys.path.insert(0, os.path.join(ROOT, "oracle"))
-        import pyoracle
-        o = pyoracle.OracleLattice(uc, MD_L, MD_L, 1)
-        n = 2000
+        base = {"value": 1.0 / el_full, "unit": "trajectories/s", "cores": po.max_threads(), "kind": "port",
+                "cpu_model": cpu_model(),
+                "sample": f"the whole 10^4-step trajectory ({el_full:.1f} s, OpenMP over sites inside each RHS "
+                          f"as the reference's MD path)",
+                "site_steps_per_s": MD_STEPS * N / el_full}
+        po.set_threads(1)
         t = time.perf_counter()
-        o.integrate_md(s0.reshape(-1), 0.0, (n + 0.5) * MD_DT, MD_DT)
-        el = time.perf_counter() - t
-        line["cpu_baseline"] = {
-            "value": n / el / MD_STEPS, "unit": "trajectories/s", "cores": pyoracle.max_threads(), "kind": "port",
-            "sample": f"{n} of the {MD_STEPS} RK4 steps ({el:.1f} s, OpenMP over sites inside each RHS as the "
-                      f"reference's MD path)",
-            "site_steps_per_s": n * lat.lattice_size / el}
-    print(json.dumps(line), flush=True)
-
-
-DP_T1, DP_TOL, DP_B = 100.0, 1e-6, 256
+        o.integrate_md(s0.reshape(-1), 0.0, (1500 + 0.5) * MD_DT, MD_DT)
+        el1 = time.perf_counter() - t
+        po.set_threads(base["cores"])
+        base["single_thread"] = {"value": 1500 / el1 / MD_STEPS, "unit": "trajectories/s", "cores": 1,
+                                 "sample": f"1500 of the 10^4 steps, {el1:.1f} s"}
+        out["cpu_baseline"] = base
+    return out
 
 
 def bench_md_dopri5(args):
     """MD with the reference's default integrator (md_integrator "dopri5",
     md_abs_tol = md_rel_tol = 1e-6, spin_config.h:93,111-112):
-    integrate_adaptive of the Kitaev 24x24x1 lattice (1152 sites) from t = 0
-    to 100 (the horizon of the RK4 MD workload), M(t) and E(t) at every
-    accepted step.  value = trajectories/s of one trajectory on the
-    Dormand-Prince kernel; a batch of 256 trajectories (one workgroup per CU,
-    seeded starts) is reported beside it."""
-    from classical_mcmc_spin_amd import GPULattice, model, seeding
-    uc = model.build_kitaev_honeycomb(K=-1.0, Gamma=0.25, Gammap=-0.02)
-    lat = model.Lattice(uc, MD_L, MD_L, 1)
-    s0 = seeding.random_spins(lat.lattice_size, 1234)
+    integrate_adaptive of the Kitaev 24x24x1 lattice from t = 0 to 100, M(t)
+    and E(t) at every accepted step; a batch of 256 trajectories (one
+    workgroup per CU, seeded starts) is reported beside it."""
+    from classical_mcmc_spin_amd import GPULattice, workloads, seeding
+    uc, lat, s0 = workloads.kitaev_md(MD_L, MD_L, seed=1234)
+    T1d, tol, Bd = 100.0, 1e-6, 256
     gl = GPULattice(lat)
-    run = lambda init: gl.integrate_dopri5(init, 0.0, DP_T1, MD_DT, control=1, abs_tol=DP_TOL, rel_tol=DP_TOL)
+    run = lambda init: gl.integrate_dopri5(init, 0.0, T1d, MD_DT, control=1, abs_tol=tol, rel_tol=tol)
     for _ in range(max(args.warmup, 1)):
         r = run(s0)
     t = time.perf_counter()
@@ -241,50 +375,99 @@ def bench_md_dopri5(args):
         r = run(s0)
     el = (time.perf_counter() - t) / args.steps
     kms = gl.last_kernel_ms()[0]
-    acc, rej = r["accepted"][0], r["rejected"][0]
-    assert np.all(np.isfinite(r["mags"][0]))
-    inits = np.stack([seeding.random_spins(lat.lattice_size, 5000 + b) for b in range(DP_B)])
-    rb = run(inits)
+    acc, rej = int(r["accepted"][0]), int(r["rejected"][0])
+    inits = np.stack([seeding.random_spins(lat.lattice_size, 5000 + b) for b in range(Bd)])
+    run(inits)
     t = time.perf_counter()
     rb = run(inits)
     elb = time.perf_counter() - t
     kmsb = gl.last_kernel_ms()[0]
     gl.close()
-    v = 1.0 / el
-    line = {
-        "metric": METRIC, "value": v, "unit": "trajectories/s", "n_gpus": 1, "steps": args.steps,
-        "warmup": args.warmup, "ms_per_step": 1e3 * el, "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
-        "config": {"workload": "md_kitaev_honeycomb_24x24x1_1traj_t100_dopri5_adaptive_tol1e-6",
-                   "sites": lat.lattice_size, "T": DP_T1, "abs_tol": DP_TOL, "rel_tol": DP_TOL,
-                   "record_every": "accepted step"},
-        "accepted_steps": acc, "rejected_steps": rej, "kernel_ms": kms,
-        "site_steps_per_s": acc * lat.lattice_size / el,
-        "batched": {"trajectories": DP_B, "traj_per_s": DP_B / elb, "kernel_ms": kmsb,
-                    "accepted_steps_mean": float(np.mean(rb["accepted"])),
-                    "rejected_steps_mean": float(np.mean(rb["rejected"]))},
+    return {"value": 1.0 / el, "unit": "trajectories/s", "ms_per_step": 1e3 * el,
+            "config": {"workload": "md_kitaev_honeycomb_24x24x1_1traj_t100_dopri5_adaptive_tol1e-6",
+                       "sites": lat.lattice_size, "T": T1d, "abs_tol": tol, "rel_tol": tol},
+            "accepted_steps": acc, "rejected_steps": rej, "kernel_ms": kms,
+            "batched": {"trajectories": Bd, "traj_per_s": Bd / elb, "kernel_ms": kmsb}}
+
+
+# --------------------------------------------------------------- pyrochlore
+
+PYRO_L, PYRO_B, PYRO_STEPS, PYRO_DT = 16, 64, 4096, 0.02
+
+
+def bench_pyrochlore(args):
+    """64 samples x 4096 RK4 steps of the 16^3-cell conventional pyrochlore
+    lattice (65 536 sites, z = 6), all-in-all-out + tilt starts, M(t) and
+    E(t) every step.  The lattice is too large for one workgroup's LDS, so
+    the grid-wide stage kernels run (4 launches per step).  Error: two
+    samples over 200 steps against the oracle."""
+    from classical_mcmc_spin_amd import GPULattice, workloads
+    lat = workloads.pyrochlore_lattice(PYRO_L)
+    N = lat.lattice_size
+    init, _ = workloads.pyrochlore_tilted_aiao(lat, PYRO_B, seed=1234)
+    T1p = (PYRO_STEPS + 0.5) * PYRO_DT
+    zero = np.zeros((PYRO_B, 2))
+    gl = GPULattice(lat)
+    one_step = lambda: gl.integrate_batched(init, zero, zero, 0.0, T1p, PYRO_DT, tgrid="boost")
+    for _ in range(args.warmup):
+        one_step()
+    kernel_ms = []
+    t = time.perf_counter()
+    for _ in range(args.steps):
+        res = one_step()
+        kernel_ms.append(gl.last_kernel_ms()[0])
+    el = time.perf_counter() - t
+    assert res["mags"].shape[0] == PYRO_STEPS + 1 and np.all(np.isfinite(res["mags"]))
+    kms = float(np.mean(kernel_ms))
+    if args.profile:
+        gl.close()
+        return {"profile": True, "workload": "pyrochlore", "kernel_ms": kms, "ms_per_step": 1e3 * el / args.steps}
+    n_par = 200
+    pr = gl.integrate_batched(init[[0, PYRO_B - 1]], np.zeros((2, 2)), np.zeros((2, 2)), 0.0,
+                              (n_par + 0.5) * PYRO_DT, PYRO_DT, tgrid="boost", want_final=True)
+    gl.close()
+    po = _oracle()
+    o = po.OracleLattice(lat.unit_cell, PYRO_L, PYRO_L, PYRO_L)
+    err = {"M": 0.0, "E": 0.0, "S": 0.0}
+    for j, b in enumerate((0, PYRO_B - 1)):
+        ref = o.integrate_md(init[b].reshape(-1), 0.0, (n_par + 0.5) * PYRO_DT, PYRO_DT)
+        err["M"] = max(err["M"], float(np.max(np.abs(pr["mags"][:, j] - ref["mags"]))))
+        err["E"] = max(err["E"], float(np.max(np.abs(pr["energy"][:, j] - ref["energy"]))))
+        err["S"] = max(err["S"], float(np.max(np.abs(pr["final"][j] - ref["final"].reshape(-1, 3)))))
+    samples_s = PYRO_B * args.steps / el
+    bytes_launch = PYRO_B * N * BYTES_PER_SITE_STEP / 4       # one RK stage of every site
+    roof = hbm_roofline("k_stage", bytes_launch, kms)
+    roof["traffic"] = profile_number("traffic_l2_k_stage_pyrochlore_16x16x16.txt", "traffic_bytes_per_launch")
+    out = {
+        "value": samples_s, "unit": "trajectories/s", "ms_per_step": 1e3 * el / args.steps,
+        "config": {"workload": f"pyrochlore_16x16x16_conventional_64samples_{PYRO_STEPS}steps_rk4_dt0.02",
+                   "sites": N, "trajectories": PYRO_B, "rk4_steps": PYRO_STEPS, "record_every": 1},
+        "site_steps_per_s": samples_s * N * PYRO_STEPS,
+        "max_abs_err_vs_oracle_200_steps": err,
+        "roofline": roof,
     }
     if not args.no_cpu:
-        sys.path.insert(0, os.path.join(ROOT, "oracle"))
-        import pyoracle
-        o = pyoracle.OracleLattice(uc, MD_L, MD_L, 1)
-        runs, t = 0, time.perf_counter()
-        while True:  # whole trajectories, repeated for about 3 s
-            ro = o.integrate_dopri5(s0.reshape(-1), 0.0, DP_T1, MD_DT, control=1, abs_tol=DP_TOL, rel_tol=DP_TOL,
-                                    max_rec=4096)
-            runs += 1
-            elc = time.perf_counter() - t
-            if elc >= 3.0:
-                break
-        line["cpu_baseline"] = {
-            "value": runs / elc, "unit": "trajectories/s", "cores": pyoracle.max_threads(), "kind": "port",
-            "sample": f"{runs} whole trajectories t = 0..{DP_T1:g} ({ro['accepted']} accepted steps each, {elc:.1f} s, "
-                      f"OpenMP over sites inside each RHS)",
-            "site_steps_per_s": runs * ro["accepted"] * lat.lattice_size / elc}
-    print(json.dumps(line), flush=True)
+        def per_run(n):
+            o.integrate_md(init[0].reshape(-1), 0.0, (n + 0.5) * PYRO_DT, PYRO_DT)
+            return n / PYRO_STEPS
 
+        desc = lambda n, el_, th: (f"1 sample x {n} of the {PYRO_STEPS} RK4 steps, {el_:.1f} s on {th} thread(s) "
+                                   f"(OpenMP over sites inside each RHS as the reference's MD path)")
+        out["cpu_baseline"] = cpu_baseline("trajectories/s", per_run, 40, 6, desc)
+    return out
+
+
+# ------------------------------------------------------------------ adjoint
 
 ADJ_L, ADJ_STEPS, ADJ_B, ADJ_B_FULL = 24, 2000, 32, 256
+# Algorithmic FP64 flops per site-step of one gradient (z = 3, Gilbert
+# damping, pulse drive, 5 parameters): forward RK4 with records 465,
+# re-integration of the checkpoint segment 465, transposed step 2470 (per
+# stage: field at the stage input 69, H x S and the Gilbert factor 16, the
+# cotangent map w, u 24, the direct term 46, the transposed bond gather 54,
+# five parameter derivatives u . (sum dJ S - dB) 310, combinations 10; plus
+# the three forward stages that rebuild the stage inputs 351).
+ADJ_FLOPS_PER_SITE_STEP = 465 + 465 + 2470
 
 
 def bench_adjoint(args):
@@ -292,9 +475,8 @@ def bench_adjoint(args):
     w = 0.5 at t = 1), Gilbert alpha = 0.02, loss on M_global over 2000 RK4
     steps (dt = 0.01) against a target run at (K, G, G', J, h) =
     (-1, 0.25, -0.02, 0, 0.05); 32 parameter starts within +-10 %.
-    value = gradients (starts) per second.  One workgroup per start and one
-    per CU (the LDS state of 1152 sites takes 83 KB), so 32 starts use 32 of
-    the 256 CUs; a batch of 256 starts is reported beside it."""
+    value = gradients (starts) per second; a batch of 256 starts beside it.
+    Error: two starts against the oracle's tangent-linear gradient."""
     from classical_mcmc_spin_amd import adjoint as AD
     from classical_mcmc_spin_amd import GPULattice, model, seeding
     lat = model.Lattice(model.build_kitaev_honeycomb(), ADJ_L, ADJ_L, 1)
@@ -303,10 +485,10 @@ def bench_adjoint(args):
     T1a = (ADJ_STEPS + 0.5) * dt
     star = np.array([-1.0, 0.25, -0.02, 0.0, 0.05])
     tl = AD.kitaev_lattice(star, ADJ_L, ADJ_L)
+    fdir = np.zeros((2, 3))
+    fdir[:, 0] = 1.0
     with GPULattice(tl) as g0:
         g0.set_damping(0.02)
-        fdir = np.zeros((2, 3))
-        fdir[:, 0] = 1.0
         d = np.einsum("aij,aj->ai", tl.frames, fdir).reshape(-1)
         g0.set_pulse(np.concatenate([d, d]), 0.5, 0.5, 0.0, 1.0, 0.0)
         tr = g0.integrate_batched(init, np.array([[1.0, 0.0]]), np.array([[0.5, 0.0]]), 0.0, T1a, dt,
@@ -324,6 +506,9 @@ def bench_adjoint(args):
     el = (time.perf_counter() - t) / args.steps
     kms = fit.gl.last_kernel_ms()[0]
     assert np.all(np.isfinite(grad))
+    if args.profile:
+        fit.close()
+        return {"profile": True, "workload": "adjoint", "kernel_ms": kms, "ms_per_step": 1e3 * el}
     fit(thetas_full)
     t = time.perf_counter()
     _, grad_full = fit(thetas_full)
@@ -331,62 +516,72 @@ def bench_adjoint(args):
     kms_full = fit.gl.last_kernel_ms()[0]
     fit.close()
     assert np.all(np.isfinite(grad_full))
-    v = ADJ_B / el
-    # forward-equivalent site-steps: forward + recomputation + transposed steps
-    line = {
-        "metric": METRIC, "value": v, "unit": "gradients/s", "n_gpus": 1, "steps": args.steps,
-        "warmup": args.warmup, "ms_per_step": 1e3 * el, "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
-        "config": {"workload": f"adjoint_kitaev_honeycomb_{ADJ_L}x{ADJ_L}x1_{ADJ_B}starts_{ADJ_STEPS}steps_rk4_dt0.01",
-                   "sites": lat.lattice_size, "starts": ADJ_B, "rk4_steps": ADJ_STEPS, "parameters": 5},
-        "site_steps_per_s": v * ADJ_STEPS * lat.lattice_size, "kernel_ms": kms,
-        "roofline": {"bound": "hbm",
-                     "achieved": v * ADJ_STEPS * lat.lattice_size * BYTES_PER_SITE_STEP / 1e9,
-                     "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                     "frac": v * ADJ_STEPS * lat.lattice_size * BYTES_PER_SITE_STEP / HBM_PEAK, "traffic": None,
-                     "kernel": "k_adjoint"},
-        "starts_256": {"gradients_per_s": ADJ_B_FULL / el_full, "kernel_ms": kms_full},
-    }
-    if not args.no_cpu:
-        sys.path.insert(0, os.path.join(ROOT, "oracle"))
-        import pyoracle
-        uc = model.build_kitaev_honeycomb(K=thetas[0, 0], Gamma=thetas[0, 1], Gammap=thetas[0, 2], J=thetas[0, 3],
-                                          field_strength=thetas[0, 4], field_direction=(1.0, 1.0, 1.0))
-        o = pyoracle.OracleLattice(uc, ADJ_L, ADJ_L, 1)
+    po = _oracle()
+    N = lat.lattice_size
+
+    # per unit-cell bond dJ and per-atom dB of each of the 5 parameters
+    unit = []
+    for p in range(5):
+        e = np.zeros(5)
+        e[p] = 1.0
+        ue = model.build_kitaev_honeycomb(K=e[0], Gamma=e[1], Gammap=e[2], J=e[3], field_strength=e[4],
+                                          field_direction=(1.0, 1.0, 1.0))
+        unit.append((np.array([bd.J for bd in ue.bilinear]).reshape(-1), ue.field.reshape(-1)))
+
+    def oracle_grad(th, n_steps):
+        """Tangent-linear gradient of the loss on M_global: one forward-mode
+        sweep of the oracle per parameter."""
+        uc = model.build_kitaev_honeycomb(K=th[0], Gamma=th[1], Gammap=th[2], J=th[3], field_strength=th[4],
+                                          field_direction=(1.0, 1.0, 1.0))
+        o = po.OracleLattice(uc, ADJ_L, ADJ_L, 1)
         o.set_alpha(0.02)
         o.set_pulse(fdir, 1.0, fdir * 0.0, 0.0, 0.5, 0.5, 0.0)
-        eK = model.build_kitaev_honeycomb(K=1.0, Gamma=0.0, Gammap=0.0)
-        dJ = np.array([b.J for b in eK.bilinear]).reshape(-1)
-        n = 400
-        t = time.perf_counter()
-        o.tangent_md(init.reshape(-1), np.zeros(init.size), dJ, np.zeros(6), 0.0, (n + 0.5) * dt, dt)
-        el = time.perf_counter() - t
-        # a 5-parameter gradient by tangent-linear sweeps costs 5 such runs
-        line["cpu_baseline"] = {
-            "value": n / (5 * el * ADJ_STEPS), "unit": "gradients/s", "cores": pyoracle.max_threads(),
-            "kind": "port",
-            "sample": f"one tangent-linear sweep over {n} of the {ADJ_STEPS} steps ({el:.1f} s); a 5-parameter "
-                      f"gradient needs 5 sweeps",
-            "site_steps_per_s": n * lat.lattice_size / el}
-    print(json.dumps(line), flush=True)
+        Tn = (n_steps + 0.5) * dt
+        g = np.zeros(5)
+        lo = 0.0
+        for p, (dJ, dB) in enumerate(unit):
+            m, dm = o.tangent_md(init.reshape(-1), np.zeros(init.size), dJ, dB, 0.0, Tn, dt)
+            diff = (m.reshape(-1, 9) - target[:n_steps + 1])[:, 6:9]
+            lo = float(np.sum(diff ** 2))
+            g[p] = float(np.sum(2.0 * diff * dm.reshape(-1, 9)[:, 6:9]))
+        return lo, g
 
+    err = 0.0
+    for b in (0, ADJ_B - 1):
+        lo, go = oracle_grad(thetas[b], ADJ_STEPS)
+        err = max(err, float(np.max(np.abs(grad[b] - go)) / np.max(np.abs(go))))
+    v = ADJ_B / el
+    out = {
+        "value": v, "unit": "gradients/s", "ms_per_step": 1e3 * el,
+        "config": {"workload": f"adjoint_kitaev_honeycomb_{ADJ_L}x{ADJ_L}x1_{ADJ_B}starts_{ADJ_STEPS}steps_rk4_dt0.01",
+                   "sites": N, "starts": ADJ_B, "rk4_steps": ADJ_STEPS, "parameters": 5, "alpha": 0.02},
+        "kernel_ms": kms,
+        "max_rel_err_vs_oracle_tangent_linear": err,
+        "roofline": fp64_roofline("k_adjoint", ADJ_B * N * ADJ_STEPS * ADJ_FLOPS_PER_SITE_STEP, kms,
+                                  algorithmic_flops_per_site_step=ADJ_FLOPS_PER_SITE_STEP,
+                                  note=f"one workgroup per start: {ADJ_B} of the 256 CUs busy"),
+        "starts_256": {"gradients_per_s": ADJ_B_FULL / el_full, "kernel_ms": kms_full,
+                       "fp64_frac": ADJ_B_FULL * N * ADJ_STEPS * ADJ_FLOPS_PER_SITE_STEP / (kms_full * 1e-3)
+                       / FP64_PEAK},
+    }
+    if not args.no_cpu:
+        def per_run(n):
+            oracle_grad(thetas[0], n)
+            return n / ADJ_STEPS
+
+        desc = lambda n, el_, th: (f"tangent-linear gradient (5 forward-mode sweeps) over {n} of the {ADJ_STEPS} "
+                                   f"steps, {el_:.1f} s on {th} thread(s)")
+        out["cpu_baseline"] = cpu_baseline("gradients/s", per_run, 200, 40, desc)
+    return out
+
+
+# -------------------------------------------------------------------- mixed
 
 MIX_L, MIX_NTAU, MIX_STEPS, MIX_DT = 8, 128, 8192, 0.005
-
-
 MIXDP_INTERVALS, MIXDP_TOL = 2048, 1e-8
 
 
-def bench_mixed_dopri5(args):
-    """The mixed 2DCS batch of bench_mixed (257 trajectories, 8x8x8 cells)
-    with the reference's default pulse-drive integrator: Dormand-Prince 5(4)
-    through integrate_const with step control at T_step = 0.005, tolerance
-    1e-8 (kDefaultPumpProbeAbsTol / RelTol), W3 pulse-window segments per
-    trajectory (windows of 9 widths around its pulse centres), over 2048
-    observation intervals from t = -1.  Every trajectory runs its own step
-    sequence; value = trajectories/s."""
-    from classical_mcmc_spin_amd import gpu as G
-    from classical_mcmc_spin_amd import gpu_mixed as GM
+def _mixed_setup():
     from classical_mcmc_spin_amd import mixed_model as mm
     from classical_mcmc_spin_amd import seeding
     muc = mm.build_tmfeo3_mixed()
@@ -401,62 +596,11 @@ def bench_mixed_dopri5(args):
     sc[1:1 + MIX_NTAU, 0] = 1.0
     tp[1 + MIX_NTAU:, 1] = taus
     sc[1 + MIX_NTAU:, :] = 1.0
-    centers = [[0.0]] + [[t] for t in taus] + [[0.0, t] for t in taus]
     f2 = np.zeros((4, 3))
     f2[:, 0] = 1.0
     f3 = np.zeros((4, 8))
     f3[:, 1] = 1.0
-    T0 = -1.0
-    T1 = T0 + (MIXDP_INTERVALS + 0.5) * MIX_DT
-    segs = [G.pulse_segments(T0, T1, c, 9.0 * WIDTH, MIX_DT) for c in centers]
-    gm = GM.GPUMixedLattice(ml)
-    GM._set_pulses(gm, f2, f2, f3, f3, 0.0, 0.0, AMP, WIDTH, 0.0, AMP, WIDTH, 0.0)
-
-    def one_step():
-        return gm.integrate_dopri5(g, tp, sc, T0, T1, MIX_DT, control=2, abs_tol=MIXDP_TOL, rel_tol=MIXDP_TOL,
-                                   segments=segs)
-
-    for _ in range(args.warmup):
-        one_step()
-    t = time.perf_counter()
-    for _ in range(args.steps):
-        r = one_step()
-    el = (time.perf_counter() - t) / args.steps
-    kms, launches = gm.last_kernel_ms()
-    gm.close()
-    acc = np.array(r["accepted"])
-    assert all(np.all(np.isfinite(x)) for x in r["records"])
-    n_sites = ml.N2 + ml.N3
-    line = {
-        "metric": METRIC, "value": B / el, "unit": "trajectories/s", "n_gpus": 1, "steps": args.steps,
-        "warmup": args.warmup, "ms_per_step": 1e3 * el, "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
-        "config": {"workload": f"2dcs_tmfeo3_mixed_8x8x8_257traj_{MIXDP_INTERVALS}x0.005_dopri5_tol1e-8_w3",
-                   "sites_su2": ml.N2, "sites_su3": ml.N3, "trajectories": B, "intervals": MIXDP_INTERVALS,
-                   "abs_tol": MIXDP_TOL, "rel_tol": MIXDP_TOL, "segments_max": max(len(q) for q in segs)},
-        "accepted_steps_mean": float(acc.mean()), "rejected_steps_mean": float(np.mean(r["rejected"])),
-        "launch_ms_mean": kms, "launches": launches,
-        "site_steps_per_s": float(acc.sum()) * n_sites / el,
-    }
-    if not args.no_cpu:
-        sys.path.insert(0, os.path.join(ROOT, "oracle"))
-        import pyoracle
-        o = pyoracle.MixedOracle(muc, MIX_L, MIX_L, MIX_L)
-        b = 1 + MIX_NTAU + 20  # an M01 trajectory
-        o.set_pulse(2, f2, tp[b, 0], f2, tp[b, 1], AMP, WIDTH, 0.0)
-        o.set_pulse(3, f3, tp[b, 0], f3, tp[b, 1], AMP, WIDTH, 0.0)
-        n = 200
-        Tc = T0 + (n + 0.5) * MIX_DT
-        seg_c = G.pulse_segments(T0, Tc, centers[b], 9.0 * WIDTH, MIX_DT)
-        t = time.perf_counter()
-        ro = o.integrate_dopri5(g, T0, Tc, MIX_DT, control=2, abs_tol=MIXDP_TOL, rel_tol=MIXDP_TOL, segments=seg_c)
-        elc = time.perf_counter() - t
-        line["cpu_baseline"] = {
-            "value": n / elc / MIXDP_INTERVALS, "unit": "trajectories/s", "cores": pyoracle.max_threads(),
-            "kind": "port",
-            "sample": f"1 trajectory x {n} of the {MIXDP_INTERVALS} intervals ({ro['accepted']} accepted steps, "
-                      f"{elc:.1f} s, OpenMP over sites inside each RHS)"}
-    print(json.dumps(line), flush=True)
+    return muc, ml, g, taus, B, tp, sc, f2, f3
 
 
 def bench_mixed(args):
@@ -464,35 +608,15 @@ def bench_mixed(args):
     + 2048 Tm SU(3) sites), pulses A = 0.5, w = 0.5 along x (Fe) and
     lambda_2 (Tm) at 0 and tau = 0, 0.05, ..., 6.35 (128 delays):
     M0 + 128 M1 + 128 M01 = 257 trajectories x 8192 RK4 steps (dt = 0.005),
-    records every step.  Grid-wide stage kernels (the 180 KB state exceeds
-    one workgroup's LDS)."""
+    records every step; then d loss / d (Fe-Tm scale) for every trajectory.
+    Error: two trajectories over 200 steps against the mixed oracle."""
     from classical_mcmc_spin_amd import gpu_mixed as GM
-    from classical_mcmc_spin_amd import mixed_model as mm
-    from classical_mcmc_spin_amd import seeding
-    muc = mm.build_tmfeo3_mixed()
-    ml = mm.MixedLattice(muc, MIX_L, MIX_L, MIX_L)
-    g = ml.flat_state(seeding.random_spins(ml.N2, 1234), mm.random_su3_states(ml.N3, 4321))
-    taus = 0.05 * np.arange(MIX_NTAU)
-    B = 1 + 2 * MIX_NTAU
-    tp = np.zeros((B, 2))
-    sc = np.zeros((B, 2))
-    sc[0, 0] = 1.0
-    tp[1:1 + MIX_NTAU, 0] = taus
-    sc[1:1 + MIX_NTAU, 0] = 1.0
-    tp[1 + MIX_NTAU:, 1] = taus
-    sc[1 + MIX_NTAU:, :] = 1.0
-    f2 = np.zeros((4, 3))
-    f2[:, 0] = 1.0
-    f3 = np.zeros((4, 8))
-    f3[:, 1] = 1.0
-    T0 = -1.0
-    T1m = T0 + (MIX_STEPS + 0.5) * MIX_DT
+    muc, ml, g, taus, B, tp, sc, f2, f3 = _mixed_setup()
+    T0m = -1.0
+    T1m = T0m + (MIX_STEPS + 0.5) * MIX_DT
     gm = GM.GPUMixedLattice(ml)
     GM._set_pulses(gm, f2, f2, f3, f3, 0.0, 0.0, AMP, WIDTH, 0.0, AMP, WIDTH, 0.0)
-
-    def one_step():
-        return gm.integrate_batched(g, tp, sc, T0, T1m, MIX_DT)
-
+    one_step = lambda: gm.integrate_batched(g, tp, sc, T0m, T1m, MIX_DT)
     for _ in range(args.warmup):
         one_step()
     kernel_ms = []
@@ -503,59 +627,110 @@ def bench_mixed(args):
     el = time.perf_counter() - t
     assert res["records"].shape[0] == MIX_STEPS + 1 and np.all(np.isfinite(res["records"]))
     kms = float(np.mean(kernel_ms))
-    # d loss / d (Fe-Tm coupling scale) for every trajectory of the batch:
-    # forward and tangent-linear run in the same stage launches; loss on the
-    # 33 magnetisation entries of the records against the M0 records x 1.01
+    if args.profile:
+        gm.close()
+        return {"profile": True, "workload": "mixed", "kernel_ms": kms, "ms_per_step": 1e3 * el / args.steps}
     target = res["records"][:, 0] * 1.01
     w = np.r_[np.ones(33), 0.0]
     t = time.perf_counter()
-    gr = gm.scale_gradient(g, tp, target, w, sc, T0, T1m, MIX_DT)
+    gr = gm.scale_gradient(g, tp, target, w, sc, T0m, T1m, MIX_DT)
     el_g = time.perf_counter() - t
     kms_g = gm.last_kernel_ms()[0]
     assert np.all(np.isfinite(gr["grad"]))
+    n_par = 200
+    Tp = T0m + (n_par + 0.5) * MIX_DT
+    pick = [0, B - 1]
+    pr = gm.integrate_batched(g, tp[pick], sc[pick], T0m, Tp, MIX_DT, want_final=True)
     gm.close()
+    po = _oracle()
+    err = {"records": 0.0, "state": 0.0}
+    for j, b in enumerate(pick):
+        o = po.MixedOracle(muc, MIX_L, MIX_L, MIX_L)
+        o.set_pulse(2, f2 * sc[b, 0], tp[b, 0], f2 * sc[b, 1], tp[b, 1], AMP, WIDTH, 0.0)
+        o.set_pulse(3, f3 * sc[b, 0], tp[b, 0], f3 * sc[b, 1], tp[b, 1], AMP, WIDTH, 0.0)
+        ref = o.integrate_md(g, T0m, Tp, MIX_DT)
+        err["records"] = max(err["records"], float(np.max(np.abs(pr["records"][:, j] - ref["records"]))))
+        err["state"] = max(err["state"], float(np.max(np.abs(pr["final"][j] - ref["final"]))))
     traj_s = B * args.steps / el
     n_sites = ml.N2 + ml.N3
-    # one stage: SU(2) sites read+write 3 doubles, SU(3) sites 8 doubles
-    bytes_launch = B * (ml.N2 * 3 + ml.N3 * 8) * 2 * 8
-    line = {
-        "metric": METRIC, "value": traj_s, "unit": "trajectories/s", "n_gpus": 1,
-        "steps": args.steps, "warmup": args.warmup, "ms_per_step": 1e3 * el / args.steps,
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
-        "data": "synthetic",
+    bytes_launch = B * (ml.N2 * 3 + ml.N3 * 8) * 2 * 8   # one stage: read + write of every component
+    roof = hbm_roofline("k_mixed_stage", bytes_launch, kms)
+    roof["traffic"] = profile_number("traffic_l2_k_mixed_stage_tmfeo3_8x8x8.txt", "traffic_bytes_per_launch")
+    out = {
+        "value": traj_s, "unit": "trajectories/s", "ms_per_step": 1e3 * el / args.steps,
         "config": {"workload": f"2dcs_tmfeo3_mixed_8x8x8_257traj_{MIX_STEPS}steps_rk4_dt0.005",
                    "sites_su2": ml.N2, "sites_su3": ml.N3, "trajectories": B, "rk4_steps": MIX_STEPS,
                    "record_every": 1},
         "site_steps_per_s": traj_s * n_sites * MIX_STEPS,
-        "roofline": {"bound": "hbm", "achieved": bytes_launch / (kms * 1e-3) / 1e9,
-                     "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                     "frac": bytes_launch / (kms * 1e-3) / HBM_PEAK, "traffic": None,
-                     "kernel": "k_mixed_stage", "kernel_ms": kms, "algorithmic_bytes_per_launch": bytes_launch},
-        "scale_gradient": {"gradients_per_s": B / el_g, "ms_per_batch": 1e3 * el_g,
-                           "kernel_ms_per_launch": kms_g, "forward_plus_tangent_over_forward": el_g / (el / args.steps),
-                           "grad_first_last": [float(gr["grad"][0]), float(gr["grad"][-1])]},
+        "max_abs_err_vs_oracle_200_steps": err,
+        "roofline": roof,
+        "scale_gradient": {"gradients_per_s": B / el_g, "ms_per_batch": 1e3 * el_g, "kernel_ms_per_launch": kms_g,
+                           "forward_plus_tangent_over_forward": el_g / (el / args.steps)},
     }
     if not args.no_cpu:
-        sys.path.insert(0, os.path.join(ROOT, "oracle"))
-        import pyoracle
-        o = pyoracle.MixedOracle(muc, MIX_L, MIX_L, MIX_L)
+        o = po.MixedOracle(muc, MIX_L, MIX_L, MIX_L)
         o.set_pulse(2, f2, 0.0, f2 * 0.0, 0.0, AMP, WIDTH, 0.0)
         o.set_pulse(3, f3, 0.0, f3 * 0.0, 0.0, AMP, WIDTH, 0.0)
-        n = 400
-        t = time.perf_counter()
-        o.integrate_md(g, T0, T0 + (n + 0.5) * MIX_DT, MIX_DT)
-        el = time.perf_counter() - t
-        line["cpu_baseline"] = {
-            "value": n / el / MIX_STEPS, "unit": "trajectories/s", "cores": pyoracle.max_threads(),
-            "kind": "port",
-            "sample": f"1 trajectory x {n} RK4 steps of the same lattice ({el:.1f} s, OpenMP over sites "
-                      f"inside each RHS as the reference's mixed MD path)",
-            "site_steps_per_s": n * n_sites / el}
-        # the CPU gradient by central differences: two forward runs per trajectory
-        line["scale_gradient"]["cpu_baseline"] = {
-            "value": n / el / MIX_STEPS / 2.0, "unit": "gradients/s", "cores": pyoracle.max_threads(),
-            "kind": "port", "sample": "central difference = 2 forward runs, timed on the sample above"}
-    print(json.dumps(line), flush=True)
+
+        def per_run(n):
+            o.integrate_md(g, T0m, T0m + (n + 0.5) * MIX_DT, MIX_DT)
+            return n / MIX_STEPS
+
+        desc = lambda n, el_, th: (f"1 trajectory x {n} of the {MIX_STEPS} RK4 steps, {el_:.1f} s on {th} "
+                                   f"thread(s) (OpenMP over sites inside each RHS as the reference's mixed MD path)")
+        out["cpu_baseline"] = cpu_baseline("trajectories/s", per_run, 300, 60, desc)
+        out["scale_gradient"]["cpu_baseline_gradients_per_s"] = out["cpu_baseline"]["value"] / 2.0
+    return out
+
+
+def bench_mixed_dopri5(args):
+    """The mixed 2DCS batch with the reference's default pulse-drive
+    integrator: Dormand-Prince 5(4) through integrate_const with step
+    control at T_step = 0.005, tolerance 1e-8, W3 pulse-window segments per
+    trajectory, over 2048 observation intervals from t = -1."""
+    from classical_mcmc_spin_amd import gpu as G
+    from classical_mcmc_spin_amd import gpu_mixed as GM
+    muc, ml, g, taus, B, tp, sc, f2, f3 = _mixed_setup()
+    centers = [[0.0]] + [[t] for t in taus] + [[0.0, t] for t in taus]
+    T0m = -1.0
+    T1m = T0m + (MIXDP_INTERVALS + 0.5) * MIX_DT
+    segs = [G.pulse_segments(T0m, T1m, c, 9.0 * WIDTH, MIX_DT) for c in centers]
+    gm = GM.GPUMixedLattice(ml)
+    GM._set_pulses(gm, f2, f2, f3, f3, 0.0, 0.0, AMP, WIDTH, 0.0, AMP, WIDTH, 0.0)
+    one_step = lambda: gm.integrate_dopri5(g, tp, sc, T0m, T1m, MIX_DT, control=2, abs_tol=MIXDP_TOL,
+                                           rel_tol=MIXDP_TOL, segments=segs)
+    for _ in range(args.warmup):
+        one_step()
+    t = time.perf_counter()
+    for _ in range(args.steps):
+        r = one_step()
+    el = (time.perf_counter() - t) / args.steps
+    kms, launches = gm.last_kernel_ms()
+    gm.close()
+    acc = np.array(r["accepted"])
+    return {"value": B / el, "unit": "trajectories/s", "ms_per_step": 1e3 * el,
+            "config": {"workload": f"2dcs_tmfeo3_mixed_8x8x8_257traj_{MIXDP_INTERVALS}x0.005_dopri5_tol1e-8_w3",
+                       "trajectories": B, "intervals": MIXDP_INTERVALS},
+            "accepted_steps_mean": float(acc.mean()), "rejected_steps_mean": float(np.mean(r["rejected"])),
+            "launch_ms_mean": kms, "launches": launches}
+
+
+# --------------------------------------------------------------------- main
+
+SUBS = (("plain_md_kitaev_24x24", bench_md), ("pyrochlore_16x16x16_dssf", bench_pyrochlore),
+        ("adjoint_kitaev_24x24", bench_adjoint), ("mixed_tmfeo3_8x8x8", bench_mixed))
+WORKLOADS = {"2dcs": bench_2dcs, "md": bench_md, "md_dopri5": bench_md_dopri5, "pyrochlore": bench_pyrochlore,
+             "adjoint": bench_adjoint, "mixed": bench_mixed, "mixed_dopri5": bench_mixed_dopri5}
+
+
+def _standalone(name, args, r):
+    """A workload run alone: its result as a full bench line."""
+    if r.get("profile"):
+        return r
+    line = {"metric": METRIC, "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic"}
+    line.update(r)
+    return line
 
 
 def main():
@@ -563,150 +738,39 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--cpu-tau", type=int, default=6, help="tau points in the CPU baseline sample")
-    ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--block", type=int, default=0)
-    ap.add_argument("--launch", default="", help="fixed persistent launch shape 'block,spt,dbuf' "
-                                                   "instead of the autotuner's choice")
-    ap.add_argument("--workload", default="2dcs",
-                    choices=["2dcs", "pyrochlore", "mixed", "mixed_dopri5", "md", "md_dopri5", "adjoint"],
-                    help="2dcs: the batched 2DCS workload (default); pyrochlore: 64 samples of the "
-                         "65 536-site pyrochlore lattice; mixed: 257-trajectory SU(2)+SU(3) 2DCS batch "
-                         "(both on the grid-wide stage kernels)")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baselines")
+    ap.add_argument("--launch", default="", help="fixed persistent launch shape 'block,spt,flags' for the 2DCS "
+                                                   "batch instead of the autotuner's choice")
+    ap.add_argument("--workload", default="all", choices=["all"] + sorted(WORKLOADS),
+                    help="all (default): the 2DCS headline plus every other workload as a sub-result")
+    ap.add_argument("--sub-steps", type=int, default=2, help="timed steps of each sub-result workload")
     ap.add_argument("--rk-steps", type=int, default=0,
-                    help="RK4 steps of the pyrochlore / mixed integrations (default 4096 / 8192); "
-                         "shorter runs keep counter-collection passes within a few hundred dispatches")
+                    help="RK4 steps of the pyrochlore / mixed integrations (default 4096 / 8192); shorter runs "
+                         "keep counter-collection passes within a few hundred dispatches")
     ap.add_argument("--profile", action="store_true",
-                    help="only the timed batched launches (for rocprofv3 runs): no autotune sweep, "
-                         "plain paths, parity check or CPU baseline")
+                    help="only the timed launches of the chosen workload (for rocprofv3 runs)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}): the batched 2DCS "
-              "trajectories shard by batch across GPUs with no exchange; multi-GPU runs are not "
-              "part of this build.", file=sys.stderr)
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}): the batched trajectories shard by "
+              "batch across GPUs with no exchange; multi-GPU runs are not part of this build.", file=sys.stderr)
         sys.exit(2)
-
     global PYRO_STEPS, MIX_STEPS
     if args.rk_steps > 0:
         PYRO_STEPS = MIX_STEPS = args.rk_steps
-    if args.workload == "pyrochlore":
-        return bench_pyrochlore(args)
-    if args.workload == "mixed":
-        return bench_mixed(args)
-    if args.workload == "mixed_dopri5":
-        return bench_mixed_dopri5(args)
-    if args.workload == "md":
-        return bench_md(args)
-    if args.workload == "md_dopri5":
-        return bench_md_dopri5(args)
-    if args.workload == "adjoint":
-        return bench_adjoint(args)
-    from classical_mcmc_spin_amd import GPULattice
-
-    uc, lat, init, field_in = build_workload()
-    tp, am, taus = replica_tables(N_TAU)
-    B = tp.shape[0]
-    N = lat.lattice_size
-    gl = GPULattice(lat)
-    fd = np.einsum("aij,aj->ai", lat.frames, field_in).reshape(-1)
-    gl.set_pulse(np.concatenate([fd, fd]), AMP, WIDTH, FREQ, 0.0, 0.0)
-
-    if args.launch:
-        blk, spt, db = (int(v) for v in args.launch.split(","))
-        gl.set_launch_config(B, blk, spt, db)
-        tuned_block, tuned_ms = blk, 0.0
-    elif args.profile:
-        tuned_block, tuned_ms = args.block, 0.0
-    else:
-        tuned_block, tuned_ms = gl.autotune(B, 64)
-        print(f"[bench] autotuned block {tuned_block} ({tuned_ms:.2f} ms / 64 steps)", file=sys.stderr,
-              flush=True)
-
-    def one_step(records_to_host):
-        return gl.integrate_batched(init, tp, am, T0, T1, DT, tgrid="boost", block=args.block,
-                                    want_energy=False, want_mags=records_to_host)
-
-    # warm-up steps bring the records back and check them; the timed steps
-    # leave M(t) in HBM (inputs and outputs device-resident), and one more
-    # step with the 302 MB record copy gives the PCIe-inclusive rate
-    for _ in range(max(args.warmup, 1)):
-        res = one_step(True)
-    n_rec = res["mags"].shape[0]
-    assert n_rec == N_STEPS + 1 and np.all(np.isfinite(res["mags"]))
-    kernel_ms = []
-    t = time.perf_counter()
-    for _ in range(args.steps):
-        one_step(False)
-        kernel_ms.append(gl.last_kernel_ms()[0])
-    el = time.perf_counter() - t
-    ms_step = 1e3 * el / args.steps
-    traj_s = B * args.steps / el
-    t = time.perf_counter()
-    one_step(True)
-    pcie_traj_s = B / (time.perf_counter() - t)
-
-    # roofline of the dominant kernel (k_persistent, one launch per step)
-    kms = float(np.mean(kernel_ms))
-    site_steps_launch = B * N * N_STEPS
-    achieved = site_steps_launch * BYTES_PER_SITE_STEP / (kms * 1e-3)
-    launch_cfg = gl.last_launch_config()
-    launch_cfg["workgroups"] = gl.last_grid()  # < B when the batch is split over the co-resident workgroups
-    if args.profile:
-        print(json.dumps({"profile": True, "kernel_ms": kms, "ms_per_step": ms_step, "launch": launch_cfg,
-                          "traj_per_s": traj_s, "pcie_inclusive_traj_per_s": pcie_traj_s}), flush=True)
-        gl.close()
+    if args.workload != "all":
+        print(json.dumps(_standalone(args.workload, args, WORKLOADS[args.workload](args))), flush=True)
         return
-
-    # plain one-trajectory-per-launch paths (M01 at tau = 0.25), one trajectory
-    gl.set_spins(init)
-    gl.set_pulse(np.concatenate([fd, fd]), AMP, WIDTH, FREQ, 0.0, 0.25)
-    t = time.perf_counter()
-    gl.integrate(T0, T1, DT, 1, tgrid="boost", engine=1, want_energy=False)
-    plain_stage_s = time.perf_counter() - t
-    gl.set_spins(init)
-    t = time.perf_counter()
-    gl.integrate(T0, T1, DT, 1, tgrid="boost", engine=2, want_energy=False)
-    plain_persist_s = time.perf_counter() - t
-    gl.set_pulse(np.concatenate([fd, fd]), AMP, WIDTH, FREQ, 0.0, 0.0)
-
-    err_m, err_s = parity_check(uc, lat, gl, init, field_in)
-
-    line = {
-        "metric": METRIC,
-        "value": traj_s,
-        "unit": "trajectories/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": ms_step,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f64",
-        "data": "synthetic",
-        "config": {"workload": "2dcs_kitaev_honeycomb_36x36x1_513traj_8192steps_rk4_dt0.01",
-                   "lattice": [L1, L2, 1], "sites": N, "trajectories": B, "rk4_steps": N_STEPS,
-                   "record_every": 1, "launch": launch_cfg},
-        "site_steps_per_s": traj_s * N * N_STEPS,
-        "pcie_inclusive_traj_per_s": pcie_traj_s,
-        "plain_stage_kernels_traj_per_s": 1.0 / plain_stage_s,
-        "plain_persistent_traj_per_s": 1.0 / plain_persist_s,
-        "batched_over_plain": traj_s * plain_stage_s,
-        "max_abs_err_vs_oracle_500_steps": {"M": err_m, "S": err_s},
-        "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK, "traffic": None,
-                     "kernel": "k_persistent", "kernel_ms": kms,
-                     "algorithmic_bytes_per_launch": site_steps_launch * BYTES_PER_SITE_STEP,
-                     "fp64_tflops": site_steps_launch * FLOPS_PER_SITE_STEP / (kms * 1e-3) / 1e12},
-    }
-    traffic_file = os.path.join(ROOT, "profiles", "traffic_k_persistent_2dcs_36x36.txt")
-    if os.path.exists(traffic_file):
-        for ln in open(traffic_file):
-            if ln.startswith("traffic_bytes_per_launch:"):
-                line["roofline"]["traffic"] = float(ln.split(":", 1)[1])
-    if not args.no_cpu:
-        line["cpu_baseline"] = cpu_baseline(uc, init, field_in, args.cpu_tau)
-    gl.close()
+    line = bench_2dcs(args)
+    sub_args = argparse.Namespace(**vars(args))
+    sub_args.steps = max(1, min(args.sub_steps, args.steps))
+    sub_args.warmup = min(args.warmup, 1)
+    line["configs"] = {}
+    for name, fn in SUBS:
+        t = time.perf_counter()
+        print(f"[bench] {name}", file=sys.stderr, flush=True)
+        r = fn(sub_args)
+        r["wall_s"] = time.perf_counter() - t
+        line["configs"][name] = r
     print(json.dumps(line), flush=True)
 
 

@@ -70,6 +70,8 @@ SIGNATURES = {
     "csl_last_launch_config": (_i, [_vp, _pi, _pi, _pi]),
     "csl_set_launch_config": (_i, [_vp, _sz, _i, _i, _i]),
     "csl_set_batch_split": (_i, [_vp, _i]),
+    "csl_persistent_candidates": (_i, [_vp, _sz, _pi, _pi, _pi, _pi, _psz]),
+    "csl_last_launch_variant": (_i, [_vp, _pi]),
     "csl_last_grid": (_i, [_vp, _pi]),
     "csl_split_schedule": (_i, [_sz, _sz, _sz, _sz, _pi, _psz]),
     "csl_integrate_multi": (_i, [C.POINTER(_vp), _sz, _psz, _pd, _pd, _pd, _d, _d, _d, _sz, _i, _sz, _pd, _pd, _pd,

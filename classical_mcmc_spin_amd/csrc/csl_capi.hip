@@ -31,18 +31,14 @@ namespace {
 using namespace cslh;
 
 
-struct TuneKey {
-    int NP, zmax, uniform, Bclass;
-    bool operator<(const TuneKey& o) const {
-        return std::tie(NP, zmax, uniform, Bclass) < std::tie(o.NP, o.zmax, o.uniform, o.Bclass);
-    }
-};
 struct TuneVal {
     int block, spt;
     int flags;  // csl::kPersistDoubleBuffer | csl::kPersistUnrollStages
     bool dbuf() const { return flags & csl::kPersistDoubleBuffer; }
 };
-std::map<TuneKey, TuneVal> g_tune;
+
+// batch-size class of the launch-shape cache: 1 trajectory, a few, >= 64
+int batch_class(size_t B) { return B >= 64 ? 2 : (B > 1 ? 1 : 0); }
 
 }  // namespace
 
@@ -84,8 +80,11 @@ struct csl_lattice {
     // per-trajectory Hamiltonians (csl_set_replica_hamiltonians)
     DBuf<double> rep_mats, rep_field;
     size_t rep_B = 0, rep_mats_stride = 0, rep_field_stride = 0;
+    // launch shapes chosen by the autotuner / csl_set_launch_config for this
+    // handle's lattice, per batch-size class (batch_class)
+    std::map<int, TuneVal> tune;
     double last_ms = 0.0;
-    int last_block = 0, last_spt = 0, last_dbuf = 0;
+    int last_block = 0, last_spt = 0, last_dbuf = 0, last_variant = 0;
     size_t last_launches = 0;
     // stage-kernel launch sequence captured as a hipGraph (engine 3)
     hipGraphExec_t graph_exec = nullptr;
@@ -311,9 +310,8 @@ TuneVal pick_persistent(const csl_lattice* h, size_t B, int block_req) {
             if (c.block == block_req) return c;
         fail(CSL_ERR_ARG, "requested block size is not a valid persistent launch shape");
     }
-    const TuneKey key{h->L.NP, h->L.zmax, h->L.uniform, B >= 64 ? 2 : (B > 1 ? 1 : 0)};
-    auto it = g_tune.find(key);
-    if (it != g_tune.end()) return it->second;
+    auto it = h->tune.find(batch_class(B));
+    if (it != h->tune.end()) return it->second;
     // heuristic: two LDS buffers (one barrier per stage) and compile-time
     // stages with the fewest sites per thread that fit
     for (int want : {csl::kPersistDoubleBuffer | csl::kPersistUnrollStages, csl::kPersistDoubleBuffer})
@@ -451,6 +449,7 @@ void run_persistent(csl_lattice* h, size_t B, const double* d_init, size_t init_
     h->last_block = tv.block;
     h->last_spt = tv.spt;
     h->last_dbuf = tv.flags;
+    h->last_variant = csl::persistent_variant(L, tv.block, tv.spt, tv.flags);
     h->last_launches = 1;
     if (ms_out) *ms_out = ms;
 }
@@ -1739,8 +1738,7 @@ int csl_autotune_batched(csl_lattice* h, size_t B, size_t n_probe_steps, int* be
                 bv = c;
             }
         }
-        const TuneKey key{h->L.NP, h->L.zmax, h->L.uniform, B >= 64 ? 2 : (B > 1 ? 1 : 0)};
-        g_tune[key] = bv;
+        h->tune[batch_class(B)] = bv;
         if (best_block) *best_block = bv.block;
         if (best_ms) *best_ms = best;
     });
@@ -2103,10 +2101,33 @@ int csl_set_launch_config(csl_lattice* h, size_t B, int block, int sites_per_thr
         check_handle(h);
         for (const auto& c : persistent_candidates(h))
             if (c.block == block && c.spt == sites_per_thread && c.flags == double_buffer) {
-                g_tune[TuneKey{h->L.NP, h->L.zmax, h->L.uniform, B >= 64 ? 2 : (B > 1 ? 1 : 0)}] = c;
+                h->tune[batch_class(B)] = c;
                 return;
             }
         fail(CSL_ERR_ARG, "launch shape does not fit this lattice");
+    });
+}
+
+int csl_persistent_candidates(csl_lattice* h, size_t max_out, int* block, int* sites_per_thread, int* flags,
+                              int* variant, size_t* n_out) {
+    return guarded([&] {
+        check_handle(h);
+        const auto c = persistent_candidates(h);
+        if (n_out) *n_out = c.size();
+        if (c.size() > max_out) fail(CSL_ERR_CAPACITY, "max_out is smaller than the number of candidates");
+        for (size_t i = 0; i < c.size(); ++i) {
+            if (block) block[i] = c[i].block;
+            if (sites_per_thread) sites_per_thread[i] = c[i].spt;
+            if (flags) flags[i] = c[i].flags;
+            if (variant) variant[i] = csl::persistent_variant(h->L, c[i].block, c[i].spt, c[i].flags);
+        }
+    });
+}
+
+int csl_last_launch_variant(csl_lattice* h, int* variant) {
+    return guarded([&] {
+        if (!h) fail(CSL_ERR_NULL, "null lattice handle");
+        if (variant) *variant = h->last_variant;
     });
 }
 

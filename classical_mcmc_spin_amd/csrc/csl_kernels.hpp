@@ -157,6 +157,26 @@ constexpr int kPersistDoubleBuffer = 1;  // two LDS state buffers, one barrier p
 constexpr int kPersistUnrollStages = 2;  // the four RK stages specialised at compile time
 constexpr int kPersistMulti = 4;         // per-trajectory lattices (PersistArgs::lats)
 
+// Compile-time specialisation the launcher picks for a persistent launch
+// shape (reported by csl_persistent_candidates; the launcher applies the same
+// rule): FIELD_ONLY for double-buffered, stage-unrolled shapes of lattices
+// with translation-invariant tables, per-sublattice fields, no onsite matrix
+// and full coordination; ATOM_MAJOR in addition when the block tiles the
+// sublattices (npad = waves_per_atom * 64 * SPT, block = n_atoms *
+// waves_per_atom * 64).
+constexpr int kVariantFieldOnly = 1;
+constexpr int kVariantAtomMajor = 2;
+inline int zmax_bucket(int zmax) { return zmax <= 3 ? 3 : (zmax <= 6 ? 6 : 12); }
+inline int persistent_variant(const LatDev& L, int block, int spt, int flags) {
+    if (flags & kPersistMulti) return 0;
+    const bool dbuf = flags & kPersistDoubleBuffer, unroll = flags & kPersistUnrollStages;
+    if (!(unroll && dbuf && L.uniform && L.site_uniform && !L.has_onsite && L.full_coord &&
+          L.zmax == zmax_bucket(L.zmax)))
+        return 0;
+    const bool am = L.npad % (kWave * spt) == 0 && block == L.n_atoms * (L.npad / (kWave * spt)) * kWave;
+    return kVariantFieldOnly | (am ? kVariantAtomMajor : 0);
+}
+
 // Launches the persistent kernel, or with occupancy != null only stores its
 // co-resident workgroups per CU for this shape.
 template <int SPT>

@@ -415,9 +415,9 @@ hipError_t launch_p_db(const LatDev& L, const PersistArgs& A, int block, int fla
                        hipStream_t st, int* occ) {
     const bool dbuf = flags & kPersistDoubleBuffer, unroll = flags & kPersistUnrollStages;
     if (flags & kPersistMulti) return launch_p<SPT, ZMAX, UNI, true, true, false, false, true>(L, A, block, lds, st, occ);
-    if (unroll && dbuf && UNI && L.site_uniform && !L.has_onsite && L.full_coord && L.zmax == ZMAX) {
-        const bool atom_major = L.npad % (kWave * SPT) == 0 && block == L.n_atoms * (L.npad / (kWave * SPT)) * kWave;
-        if (atom_major) return launch_p<SPT, ZMAX, UNI, true, true, true, true>(L, A, block, lds, st, occ);
+    const int variant = persistent_variant(L, block, SPT, flags);
+    if (UNI && (variant & kVariantFieldOnly)) {
+        if (variant & kVariantAtomMajor) return launch_p<SPT, ZMAX, UNI, true, true, true, true>(L, A, block, lds, st, occ);
         return launch_p<SPT, ZMAX, UNI, true, true, true>(L, A, block, lds, st, occ);
     }
     if (unroll)

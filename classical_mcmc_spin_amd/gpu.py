@@ -250,10 +250,24 @@ class GPULattice:
         return g.value
 
     def last_launch_config(self):
-        b, s, d = C.c_int(), C.c_int(), C.c_int()
+        b, s, d, v = C.c_int(), C.c_int(), C.c_int(), C.c_int()
         check(lib().csl_last_launch_config(self.h, C.byref(b), C.byref(s), C.byref(d)))
+        check(lib().csl_last_launch_variant(self.h, C.byref(v)))
         return {"block": b.value, "sites_per_thread": s.value, "double_buffer": bool(d.value & 1),
-                "unrolled_stages": bool(d.value & 2)}
+                "unrolled_stages": bool(d.value & 2), "flags": d.value,
+                "field_only": bool(v.value & 1), "atom_major": bool(v.value & 2)}
+
+    def launch_candidates(self):
+        """csl_persistent_candidates: every persistent launch shape that fits
+        this lattice, as dicts (block, sites_per_thread, flags, field_only,
+        atom_major)."""
+        n = C.c_size_t()
+        m = 64
+        blk, spt, fl, var = (np.zeros(m, dtype=np.int32) for _ in range(4))
+        ip = lambda a: a.ctypes.data_as(C.POINTER(C.c_int))
+        check(lib().csl_persistent_candidates(self.h, m, ip(blk), ip(spt), ip(fl), ip(var), C.byref(n)))
+        return [{"block": int(blk[i]), "sites_per_thread": int(spt[i]), "flags": int(fl[i]),
+                 "field_only": bool(var[i] & 1), "atom_major": bool(var[i] & 2)} for i in range(n.value)]
 
     def last_kernel_ms(self):
         ms, n = C.c_double(), C.c_size_t()
@@ -406,6 +420,29 @@ def _const_calls(t0, t1, dt):
 # factor of the reference's W3 chunking (pulse_chunking.h:173,178)
 PULSE_WINDOW_WIDTHS = 9.0
 FREE_DT_FACTOR = 20.0
+
+
+def split_schedule(B: int, G: int, n_steps: int) -> np.ndarray:
+    """csl_split_schedule: the persistent kernel's even work split of B
+    trajectories of n_steps over G workgroups, rows [G, pieces, 5] =
+    (trajectory, first step, end step, slot loaded, slot saved), -1 unused."""
+    cap = G * 8
+    while True:
+        rows = np.full(cap * 5, -1, dtype=np.int32)
+        pp = C.c_size_t()
+        rc = lib().csl_split_schedule(B, G, n_steps, cap, rows.ctypes.data_as(C.POINTER(C.c_int)), C.byref(pp))
+        if rc == _capi.CSL_ERR_CAPACITY:
+            cap = G * pp.value
+            continue
+        check(rc)
+        return rows[:G * pp.value * 5].reshape(G, pp.value, 5)
+
+
+def seam_trajectories(B: int, G: int, n_steps: int) -> list:
+    """Trajectories the work split cuts between two workgroups (their state
+    crosses a seam through global memory)."""
+    r = split_schedule(B, G, n_steps)
+    return sorted({int(t) for t, _, _, load, _ in r.reshape(-1, 5) if t >= 0 and load >= 0})
 
 
 def pulse_segments(T_start, T_end, centers, window, T_step, free_dt_factor=FREE_DT_FACTOR):

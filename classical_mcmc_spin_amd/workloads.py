@@ -1,6 +1,9 @@
 """Seeded synthetic inputs of the standard workloads (lattices, couplings and
 initial states), shared by the benchmark, the tests and user scripts.
 
+  kitaev_md             Kitaev honeycomb 24x24x1, K=-1, G=0.25, G'=-0.02, no
+                        field, spins uniform on the sphere from a seed (plain
+                        LL dynamics, RK4 dt = 0.01, 10^4 undamped steps)
   kitaev_2dcs           Kitaev honeycomb 36x36x1, K=-1, G=0.25, G'=-0.02,
                         h = 0.1 along (1,1,1)/sqrt3, relaxed + tilted start,
                         2 pulses along x (A = 0.5, w = 0.5, omega = 0),
@@ -17,6 +20,14 @@ import numpy as np
 from . import model, seeding
 
 PYRO_COUPLINGS = dict(Jzz=1.0, Jpm=0.05, Jpmpm=0.02, Jzpm=-0.01)
+
+
+def kitaev_md(L1: int = 24, L2: int = 24, seed: int = 1234):
+    """(unit cell, lattice, initial state [N, 3]) of the plain MD workload:
+    zero field, spins uniform on the unit sphere from the seed."""
+    uc = model.build_kitaev_honeycomb(K=-1.0, Gamma=0.25, Gammap=-0.02, field_strength=0.0)
+    lat = model.Lattice(uc, L1, L2, 1)
+    return uc, lat, seeding.random_spins(lat.lattice_size, seed)
 
 
 def kitaev_2dcs(L1: int = 36, L2: int = 36, seed: int = 1234, tilt_seed: int = 4321, sweeps: int = 200):

@@ -336,9 +336,24 @@ int csl_last_kernel_ms(csl_lattice* h, double* ms, size_t* launches);
  * thread and flags 0) after the grid-wide stage kernels. */
 int csl_last_launch_config(csl_lattice* h, int* block, int* sites_per_thread, int* flags);
 
-/* Fix the persistent launch shape used for this lattice shape and batch size
- * class (as the autotuner would); rejects shapes that do not fit. */
+/* Fix the persistent launch shape used by this handle for batches of B's
+ * size class (1, 2..63, >= 64 trajectories), as the autotuner would;
+ * rejects shapes that do not fit.  The choice is per handle. */
 int csl_set_launch_config(csl_lattice* h, size_t B, int block, int sites_per_thread, int flags);
+
+/* The persistent launch shapes that fit this handle's lattice (the
+ * autotuner's candidate set): workgroup size, sites per thread, flags (as
+ * csl_last_launch_config) and the compile-time variant the launcher uses
+ * for the shape (bit 0: per-sublattice field with full coordination, bit 1:
+ * atom-major site mapping).  *n_out receives the count; CSL_ERR_CAPACITY
+ * when it exceeds max_out.  No counterpart in the reference (one launch
+ * shape per kernel). */
+int csl_persistent_candidates(csl_lattice* h, size_t max_out, int* block, int* sites_per_thread,
+                              int* flags, int* variant, size_t* n_out);
+
+/* Variant bits (as csl_persistent_candidates) of the last persistent launch
+ * on this handle. */
+int csl_last_launch_variant(csl_lattice* h, int* variant);
 
 /* Work split of the one-workgroup-per-trajectory kernel (on by default):
  * when a batch holds more trajectories than workgroups fit the GPU at once,

@@ -1203,6 +1203,14 @@ int orc_max_threads(void) {
 #endif
 }
 
+void orc_set_threads(int n) {
+#ifdef _OPENMP
+    if (n > 0) omp_set_num_threads(n);
+#else
+    (void)n;
+#endif
+}
+
 /* ---------------- tangent-linear model (forward mode) ---------------- */
 
 size_t orc_tangent_md(OracleLattice* L, double* state, double* dstate, const double* dJ,

@@ -142,6 +142,8 @@ size_t orc_build_pulse_segments(double T0, double T1, const double* centers, siz
                                 size_t max_seg, double* out);
 /* OpenMP thread count actually available to the oracle. */
 int    orc_max_threads(void);
+/* Threads of the oracle's OpenMP regions from now on (n >= 1). */
+void   orc_set_threads(int n);
 
 /* Tangent-linear (forward-mode) RK4: propagates (state, dstate) together for
  * a directional derivative of the flow w.r.t. the initial state and a

@@ -46,6 +46,7 @@ _SIG = {
                             _psz, _pd, _pd, _pd, _pd, _pd, _pi]),
     "orc_build_pulse_segments": (_sz, [_d, _d, _pd, _sz, _d, _d, _d, _sz, _pd]),
     "orc_max_threads": (_i, []),
+    "orc_set_threads": (None, [_i]),
     "orc_tangent_md": (_sz, [_vp, _pd, _pd, _pd, _pd, _d, _d, _d, _sz, _pd, _pd]),
     "orc_mixed_new": (_vp, [_i, _i, _i, _i, _pd, _pd, _pd, _pd, _i, _pi, _pi, _pi, _pd,
                             _i, _pd, _pd, _pd, _i, _pi, _pi, _pi, _pd, _i, _pi, _pi, _pi, _pd]),
@@ -451,6 +452,10 @@ def pulse_segments(T0, T1, centers, window, T_step, free_dt_factor):
 
 def max_threads():
     return lib().orc_max_threads()
+
+
+def set_threads(n):
+    lib().orc_set_threads(int(n))
 
 
 def dopri5_scalar(lam, omega, y0, T0, T1, dt, control=1, abs_tol=1e-6, rel_tol=1e-6):
