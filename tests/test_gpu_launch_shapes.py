@@ -59,7 +59,9 @@ def _reference(oracle, name):
         T1 = (N_STEPS + 0.5) * DT
         refs = []
         for b in range(len(TP)):
-            o.set_pulse(f * (AM[b, 0] != 0), TP[b, 0], f * (AM[b, 1] != 0), TP[b, 1], 0.5, 0.5, 0.0)
+            # the reference's pulse has one amplitude (0.5): per-pulse
+            # amplitudes of the batch scale the pulse directions instead
+            o.set_pulse(f * (AM[b, 0] / 0.5), TP[b, 0], f * (AM[b, 1] / 0.5), TP[b, 1], 0.5, 0.5, 0.0)
             refs.append(o.integrate_md(s.reshape(-1), 0.0, T1, DT))
         _REF[name] = (uc, lat, s, f, refs)
     return _REF[name]
