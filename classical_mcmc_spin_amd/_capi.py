@@ -70,6 +70,8 @@ SIGNATURES = {
     "csl_last_launch_config": (_i, [_vp, _pi, _pi, _pi]),
     "csl_set_launch_config": (_i, [_vp, _sz, _i, _i, _i]),
     "csl_set_batch_split": (_i, [_vp, _i]),
+    "csl_set_stage_graph": (_i, [_vp, _i]),
+    "csl_mixed_set_stage_graph": (_i, [_vp, _i]),
     "csl_persistent_candidates": (_i, [_vp, _sz, _pi, _pi, _pi, _pi, _psz]),
     "csl_last_launch_variant": (_i, [_vp, _pi]),
     "csl_last_grid": (_i, [_vp, _pi]),

@@ -71,6 +71,7 @@ struct csl_lattice {
     DBuf<int> ps_latof;
     DBuf<size_t> ps_initoff;
     int split_batches = 1;
+    int stage_graph = 0;          // batched stage-kernel path replayed from a hipGraph
     int last_grid = 0;
     DBuf<long long> dp_counters;  // [B][4]
     DBuf<double> dp_segs;         // control-2 segments [B][max_seg][2]
@@ -569,6 +570,7 @@ void run_stages(csl_lattice* h, size_t B, const double* d_tp, const double* d_am
         hip_check(hipEventRecord(e0, s), "event record");
         hip_check(hipGraphLaunch(h->graph_exec, s), "graph launch");
         hip_check(hipEventRecord(e1, s), "event record");
+        launches = 4 * n_steps;  // stage launches of the replayed sequence
     }
     hip_check(hipStreamSynchronize(s), "stage integration");
     float ms = 0.f;
@@ -1159,7 +1161,7 @@ int csl_integrate_batched(csl_lattice* h, size_t B, const double* init, int init
             hip_check(csl::launch_perm_to_soa(L, h->bt_init.p, init_per_replica ? nd : 0, h->X.p, int(B), s),
                       "perm");
             run_stages(h, B, h->bt_tp.p, h->bt_amp.p, drive_on, grid, dt, save_interval, energy != nullptr,
-                       nullptr);
+                       nullptr, {}, h->stage_graph != 0);
             if (final_states) {
                 h->bt_final.ensure(B * nd);
                 hip_check(csl::launch_perm_to_aos(L, h->X.p, h->bt_final.p, int(B), s), "perm");
@@ -2141,6 +2143,13 @@ int csl_split_schedule(size_t B, size_t G, size_t n_steps, size_t max_rows, int*
         if (G * max_p > max_rows) fail(CSL_ERR_CAPACITY, "max_rows is smaller than G * pieces_per_wg");
         if (!rows) fail(CSL_ERR_NULL, "null rows");
         std::memcpy(rows, r.data(), r.size() * sizeof(int));
+    });
+}
+
+int csl_set_stage_graph(csl_lattice* h, int enable) {
+    return guarded([&] {
+        check_handle(h);
+        h->stage_graph = enable ? 1 : 0;
     });
 }
 

@@ -1344,7 +1344,12 @@ struct csl_mixed {
     double h_tp2[2] = {0.0, 0.0}, h_tp3[2] = {0.0, 0.0};  // single-trajectory pulse centres
     double last_ms = 0.0;
     size_t last_launches = 0;
+    // batched stage-kernel launch sequence replayed from a hipGraph
+    int stage_graph = 0;
+    hipGraphExec_t graph_exec = nullptr;
+    std::vector<unsigned char> graph_key;
     ~csl_mixed() {
+        if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
         if (stream) (void)hipStreamDestroy(stream);
     }
     size_t sz() const { return size_t(3) * M.NP2 + size_t(8) * M.NP3; }
@@ -1418,7 +1423,8 @@ void to_flat(csl_mixed* h, const double* d_soa, double* d_flat, size_t B) {
 // With tl the tangent z = d state / d theta (h->ZX, zero at the start) is
 // advanced in the same launches and its records go to h->trecs.
 void run_mixed(csl_mixed* h, size_t B, const double* d_tp, const double* d_tp3, const double* d_scale,
-               const std::vector<double>& grid, double dt, size_t si, double* d_states, bool tl = false) {
+               const std::vector<double>& grid, double dt, size_t si, double* d_states, bool tl = false,
+               bool use_graph = false) {
     const size_t n_steps = grid.size() - 1;
     const size_t n_rec = records_for(n_steps, si);
     hipStream_t s = h->stream;
@@ -1439,10 +1445,11 @@ void run_mixed(csl_mixed* h, size_t B, const double* d_tp, const double* d_tp3, 
     }
     const bool drive = d_tp && (h->M.amp2 != 0.0 || h->M.amp3 != 0.0);
     const double inv2 = 1.0 / double(h->N2), inv3 = 1.0 / double(h->N3), invE = 1.0 / double(h->N2 + h->N3);
+    h->aux.ensure(plane);
+    if (tl) h->zaux.ensure(plane);
     hipEvent_t e0, e1;
     hip_check(hipEventCreate(&e0), "event");
     hip_check(hipEventCreate(&e1), "event");
-    hip_check(hipEventRecord(e0, s), "event record");
     size_t launches = 0;
     const dim3 grid_dim{unsigned(nblk), unsigned(B), 1u};
     auto stage = [&](int st, double t, int rec) {
@@ -1483,6 +1490,12 @@ void run_mixed(csl_mixed* h, size_t B, const double* d_tp, const double* d_tp3, 
             hip_check(hipGetLastError(), "k_mixed_reduce");
         }
     };
+    // The whole launch sequence; with use_graph it is captured once into a
+    // hipGraph and replayed while the key (shapes, time grid, device
+    // buffers, lattice parameters) is unchanged.
+    auto issue = [&](bool timed) {
+    if (timed) hip_check(hipEventRecord(e0, s), "event record");
+    launches = 0;
     for (size_t n = 0; n < n_steps; ++n) {
         const bool rec = (n % si) == 0;
         const size_t ri = n / si;
@@ -1495,10 +1508,9 @@ void run_mixed(csl_mixed* h, size_t B, const double* d_tp, const double* d_tp3, 
         stage(3, t + 1.0 * dt, 0);
         if (rec) reduce(ri);
     }
-    hip_check(hipEventRecord(e1, s), "event record");
+    if (timed) hip_check(hipEventRecord(e1, s), "event record");
     // final record: a stage-0 evaluation whose outputs go to scratch
     {
-        h->aux.ensure(plane);
         csl::MixStageArgs A{};
         A.stage = 0;
         A.nblk2 = nblk2;
@@ -1511,7 +1523,6 @@ void run_mixed(csl_mixed* h, size_t B, const double* d_tp, const double* d_tp3, 
         A.rec = 1;
         A.partials = h->partials.p;
         if (tl) {
-            h->zaux.ensure(plane);
             A.zx = h->ZX.p;
             A.zacc = h->zaux.p;
             A.zcur = h->ZX.p;
@@ -1525,6 +1536,48 @@ void run_mixed(csl_mixed* h, size_t B, const double* d_tp, const double* d_tp3, 
         reduce(n_rec - 1);
     }
     if (d_states) to_flat(h, h->X.p, d_states + (n_rec - 1) * h->nflat(), 1);
+    };
+    if (!use_graph) {
+        issue(true);
+    } else {
+        std::vector<unsigned char> key;
+        auto put = [&](const void* p, size_t nb) {
+            const auto* c = static_cast<const unsigned char*>(p);
+            key.insert(key.end(), c, c + nb);
+        };
+        put(&h->M, sizeof(csl::MixDev));
+        put(grid.data(), grid.size() * sizeof(double));
+        const size_t shape[4] = {B, si, size_t(tl ? 1 : 0), grid.size()};
+        put(shape, sizeof(shape));
+        put(&dt, sizeof(dt));
+        const void* ptrs[] = {h->X.p, h->ACC.p, h->Y1.p, h->Y2.p, h->partials.p, h->recs.p, h->aux.p, d_tp, d_tp3,
+                              d_scale, d_states, tl ? h->ZX.p : nullptr, tl ? h->ZACC.p : nullptr,
+                              tl ? h->ZY1.p : nullptr, tl ? h->ZY2.p : nullptr, tl ? h->tpartials.p : nullptr,
+                              tl ? h->trecs.p : nullptr, tl ? h->zaux.p : nullptr};
+        put(ptrs, sizeof(ptrs));
+        if (!h->graph_exec || key != h->graph_key) {
+            if (h->graph_exec) (void)hipGraphExecDestroy(h->graph_exec);
+            h->graph_exec = nullptr;
+            hipGraph_t g = nullptr;
+            hip_check(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), "begin capture");
+            try {
+                issue(false);
+            } catch (...) {
+                (void)hipStreamEndCapture(s, &g);
+                if (g) (void)hipGraphDestroy(g);
+                throw;
+            }
+            hip_check(hipStreamEndCapture(s, &g), "end capture");
+            const hipError_t ie = hipGraphInstantiate(&h->graph_exec, g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            hip_check(ie, "graph instantiate");
+            h->graph_key = key;
+        }
+        hip_check(hipEventRecord(e0, s), "event record");
+        hip_check(hipGraphLaunch(h->graph_exec, s), "graph launch");
+        hip_check(hipEventRecord(e1, s), "event record");
+        launches = 4 * n_steps;  // stage launches of the replayed sequence
+    }
     hip_check(hipStreamSynchronize(s), "mixed integration");
     float ms = 0.f;
     hip_check(hipEventElapsedTime(&ms, e0, e1), "elapsed");
@@ -1990,7 +2043,7 @@ int csl_mixed_integrate_batched(csl_mixed* h, size_t B, const double* init, int 
         h->X.ensure(B * h->sz());
         hip_check(hipMemsetAsync(h->X.p, 0, B * h->sz() * sizeof(double), s), "memset");
         to_soa(h, h->init.p, init_per_replica ? nf : 0, h->X.p, B);
-        run_mixed(h, B, h->tp.p, h->tp.p, h->scale.p, grid, dt, save_interval, nullptr);
+        run_mixed(h, B, h->tp.p, h->tp.p, h->scale.p, grid, dt, save_interval, nullptr, false, h->stage_graph != 0);
         copy_back(recs, h->recs.p, n_rec * B * CSL_MIXED_RECORD, s);
         if (final_states) {
             h->flat.ensure(B * nf);
@@ -2269,6 +2322,13 @@ int csl_mixed_scale_gradient(csl_mixed* h, size_t B, const double* init, int ini
         copy_back(recs, h->recs.p, n_rec * B * CSL_MIXED_RECORD, s);
         copy_back(trecs, h->trecs.p, n_rec * B * CSL_MIXED_RECORD, s);
         hip_check(hipStreamSynchronize(s), "scale_gradient");
+    });
+}
+
+int csl_mixed_set_stage_graph(csl_mixed* h, int enable) {
+    return guarded([&] {
+        check_mixed(h);
+        h->stage_graph = enable ? 1 : 0;
     });
 }
 

@@ -239,6 +239,11 @@ class GPULattice:
         """flags: bit 0 two LDS state buffers, bit 1 compile-time RK stages."""
         check(lib().csl_set_launch_config(self.h, B, block, sites_per_thread, int(flags)))
 
+    def set_stage_graph(self, enable: bool):
+        """csl_set_stage_graph: replay the batched stage-kernel launch
+        sequence from a hipGraph (lattices larger than one workgroup)."""
+        check(lib().csl_set_stage_graph(self.h, int(bool(enable))))
+
     def set_batch_split(self, enable: bool):
         """csl_set_batch_split: share the trajectory-steps of a batch larger
         than the co-resident workgroups evenly over them (default on)."""

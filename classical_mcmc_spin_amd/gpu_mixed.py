@@ -126,6 +126,11 @@ class GPUMixedLattice:
         check(lib().csl_mixed_rhs(self.h, t, dptr(out)))
         return out
 
+    def set_stage_graph(self, enable: bool):
+        """csl_mixed_set_stage_graph: replay the batched launch sequence from
+        a hipGraph."""
+        check(lib().csl_mixed_set_stage_graph(self.h, int(bool(enable))))
+
     def normalize(self, len2=1.0, len3=1.0):
         check(lib().csl_mixed_normalize(self.h, len2, len3))
 

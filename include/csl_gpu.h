@@ -351,6 +351,15 @@ int csl_set_launch_config(csl_lattice* h, size_t B, int block, int sites_per_thr
 int csl_persistent_candidates(csl_lattice* h, size_t max_out, int* block, int* sites_per_thread,
                               int* flags, int* variant, size_t* n_out);
 
+/* With enable != 0, csl_integrate_batched calls that run on the grid-wide
+ * stage kernels (lattices larger than one workgroup, or block = -1) issue
+ * their launch sequence (drive table, 4 stage launches and one record
+ * reduction per step) from a hipGraph captured on the first call and
+ * replayed by later calls with the same shapes, device buffers and lattice
+ * parameters, as engine 3 of csl_integrate does.  Default off.  No
+ * counterpart in the reference. */
+int csl_set_stage_graph(csl_lattice* h, int enable);
+
 /* Variant bits (as csl_persistent_candidates) of the last persistent launch
  * on this handle. */
 int csl_last_launch_variant(csl_lattice* h, int* variant);

@@ -167,6 +167,13 @@ int csl_mixed_scale_gradient(csl_mixed* h, size_t B, const double* init, int ini
  * (HIP events on the handle's stream) and the number of stage launches. */
 int csl_mixed_last_kernel_ms(csl_mixed* h, double* ms, size_t* launches);
 
+/* With enable != 0, csl_mixed_integrate_batched issues its launch sequence
+ * (4 stage launches and one record reduction per step) from a hipGraph
+ * captured on the first call and replayed by later calls with the same
+ * shapes, time grid, device buffers and lattice parameters.  Default off.
+ * No counterpart in the reference. */
+int csl_mixed_set_stage_graph(csl_mixed* h, int enable);
+
 #ifdef __cplusplus
 }
 #endif

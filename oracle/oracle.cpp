@@ -1195,6 +1195,32 @@ size_t orc_build_pulse_segments(double T0, double T1, const double* centers, siz
     return segs.size();
 }
 
+size_t orc_greedy_quench(const OracleLattice* L, double* state, const size_t* order, double spin_length,
+                         double rel_tol, size_t max_sweeps) {
+    const size_t N = L->N;
+    const int D = L->D;
+    std::vector<double> H(D);
+    double E_prev = orc_energy(L, state);  // greedy_quench, lattice.h:3355
+    size_t sweep = 0;
+    while (sweep < max_sweeps) {
+        // deterministic_sweep(1), lattice.h:3334-3349, sites in the given order
+        for (size_t k = 0; k < N; ++k) {
+            const size_t i = order[k];
+            local_field(*L, state, i, H.data());
+            double n2 = 0.0;
+            for (int d = 0; d < D; ++d) n2 += H[d] * H[d];
+            const double norm = std::sqrt(n2);
+            if (norm < 1e-15) continue;
+            for (int d = 0; d < D; ++d) state[i * D + d] = -H[d] / norm * spin_length;
+        }
+        ++sweep;
+        const double E_curr = orc_energy(L, state);  // lattice.h:3360-3364
+        if (std::abs(E_curr - E_prev) <= rel_tol * (std::abs(E_prev) + 1e-18)) break;
+        E_prev = E_curr;
+    }
+    return sweep;
+}
+
 int orc_max_threads(void) {
 #ifdef _OPENMP
     return omp_get_max_threads();

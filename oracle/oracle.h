@@ -140,6 +140,14 @@ int    orc_pump_probe(const OracleLattice* L, const double* ground, const double
 size_t orc_build_pulse_segments(double T0, double T1, const double* centers, size_t n_centers,
                                 double window, double T_step, double free_dt_factor,
                                 size_t max_seg, double* out);
+/* Lattice::greedy_quench (lattice.h:3352-3368): deterministic sweeps
+ * (deterministic_sweep, lattice.h:3334-3349) setting S_i = -H_i/|H_i| *
+ * spin_length (skipped when |H_i| < 1e-15) until |E - E_prev| <= rel_tol *
+ * (|E_prev| + 1e-18) or max_sweeps.  The reference draws the sites of a
+ * sweep from its Lehmer generator; here every sweep visits the N sites in
+ * the given order (order[N], a permutation).  Returns the sweeps done. */
+size_t orc_greedy_quench(const OracleLattice* L, double* state, const size_t* order, double spin_length,
+                         double rel_tol, size_t max_sweeps);
 /* OpenMP thread count actually available to the oracle. */
 int    orc_max_threads(void);
 /* Threads of the oracle's OpenMP regions from now on (n >= 1). */

@@ -47,6 +47,7 @@ _SIG = {
     "orc_build_pulse_segments": (_sz, [_d, _d, _pd, _sz, _d, _d, _d, _sz, _pd]),
     "orc_max_threads": (_i, []),
     "orc_set_threads": (None, [_i]),
+    "orc_greedy_quench": (_sz, [_vp, _pd, _psz, _d, _d, _sz]),
     "orc_tangent_md": (_sz, [_vp, _pd, _pd, _pd, _pd, _d, _d, _d, _sz, _pd, _pd]),
     "orc_mixed_new": (_vp, [_i, _i, _i, _i, _pd, _pd, _pd, _pd, _i, _pi, _pi, _pi, _pd,
                             _i, _pd, _pd, _pd, _i, _pi, _pi, _pi, _pd, _i, _pi, _pi, _pi, _pd]),
@@ -249,6 +250,14 @@ class OracleLattice:
         nr = lib().orc_double_pulse_drive(self.h, _p(_f(init)), _p(_f(f1)), t1, _p(_f(f2)), t2, amp, width, freq,
                                           T0, T1, step, int(chunking), n, _p(t), _p(m))
         return t[:nr], m.reshape(n, 3, self.D)[:nr]
+
+    def greedy_quench(self, state, order, spin_length=1.0, rel_tol=1e-12, max_sweeps=10000):
+        """Lattice::greedy_quench with every sweep visiting the sites in
+        `order`; returns (relaxed state, sweeps done)."""
+        s = _f(state).copy()
+        o = np.ascontiguousarray(order, dtype=np.uintp)
+        n = lib().orc_greedy_quench(self.h, _p(s), o.ctypes.data_as(_psz), spin_length, rel_tol, max_sweeps)
+        return s, int(n)
 
     def max_dsdt_no_drive(self, state):
         return lib().orc_max_dsdt_no_drive(self.h, _p(_f(state)))
