@@ -35,6 +35,27 @@
 namespace csl {
 
 namespace {
+// ll_su2 for a site that may be a padding slot with a zero spin: the Gilbert
+// factor 1/|S| is taken as 0 there instead of inf (same bits for |S| > 0)
+__device__ __forceinline__ void ll_su2_guarded(const double H[3], const double S[3], double alpha, double k[3]) {
+    double dx = H[1] * S[2] - H[2] * S[1];
+    double dy = H[2] * S[0] - H[0] * S[2];
+    double dz = H[0] * S[1] - H[1] * S[0];
+    if (alpha > 0.0) {
+        const double a = S[1] * dz - S[2] * dy;
+        const double b = S[2] * dx - S[0] * dz;
+        const double c = S[0] * dy - S[1] * dx;
+        const double n2 = S[0] * S[0] + S[1] * S[1] + S[2] * S[2];
+        const double inv_S = n2 > 0.0 ? 1.0 / sqrt(n2) : 0.0;
+        dx -= alpha * inv_S * a;
+        dy -= alpha * inv_S * b;
+        dz -= alpha * inv_S * c;
+    }
+    k[0] = dx;
+    k[1] = dy;
+    k[2] = dz;
+}
+
 // e_i = 1/2 S.(H - B) with the field of the site's sublattice
 __device__ __forceinline__ double site_energy_atom(const LatDev& L, int atom, const double S[3],
                                                    const double H[3]) {
@@ -59,6 +80,33 @@ __global__ void __launch_bounds__(persistent_max_block<SPT>()) k_persistent(LatD
     LdsState bufB = DBUF ? lds_state(smem + 3 * NP, NP) : bufA;
     double* drv = smem + (DBUF ? 6 : 3) * NP;  // [2 parity][3 stage times][2 pulses]
     double* red = drv + 12;                    // [nwaves][10]
+    // PAIR (the FIELD_ONLY + ATOM_MAJOR shapes, always double-buffered): the
+    // two buffers of a site share one 48-byte record [x y | x' y' | z z'],
+    // so every access to site q of either buffer is one address (48 q)
+    // plus an immediate offset: one address register per gathered
+    // neighbour instead of four.  Other shapes keep the planar LdsState
+    // buffers.
+    constexpr bool PAIR = FIELD_ONLY && ATOM_MAJOR && DBUF;
+    auto lds_ld = [&](int which, int q, double S[3]) {
+        if constexpr (PAIR) {
+            const double* r = smem + 6 * q;
+            const double2 v = *reinterpret_cast<const double2*>(r + 2 * which);
+            S[0] = v.x;
+            S[1] = v.y;
+            S[2] = r[4 + which];
+        } else {
+            (which ? bufB : bufA).load(q, S);
+        }
+    };
+    auto lds_st = [&](int which, int q, const double S[3]) {
+        if constexpr (PAIR) {
+            double* r = smem + 6 * q;
+            *reinterpret_cast<double2*>(r + 2 * which) = make_double2(S[0], S[1]);
+            r[4 + which] = S[2];
+        } else {
+            (which ? bufB : bufA).store(q, S);
+        }
+    };
 
     const int T = blockDim.x;
     const int tid = threadIdx.x;
@@ -139,7 +187,7 @@ __global__ void __launch_bounds__(persistent_max_block<SPT>()) k_persistent(LatD
                 v[1] = init[3 * i + 1];
                 v[2] = init[3 * i + 2];
             }
-            bufA.store(p, v);
+            lds_st(0, p, v);
         }
     } else {
         // the state at step s_begin, saved by the workgroup that ran the
@@ -159,7 +207,7 @@ __global__ void __launch_bounds__(persistent_max_block<SPT>()) k_persistent(LatD
         const double* sv = A.saved + size_t(slot_in) * 3 * size_t(Lk.NP);
         for (int p = tid; p < NP; p += T) {
             const double v[3] = {sv[3 * p], sv[3 * p + 1], sv[3 * p + 2]};
-            bufA.store(p, v);
+            lds_st(0, p, v);
         }
     }
 #pragma unroll
@@ -169,7 +217,7 @@ __global__ void __launch_bounds__(persistent_max_block<SPT>()) k_persistent(LatD
     for (int m = 0; m < SPT; ++m) {
         const int p = site(m);
         if (act[m]) {
-            bufA.load(p, x[m]);
+            lds_ld(0, p, x[m]);
         } else {
             x[m][0] = x[m][1] = x[m][2] = 0.0;
         }
@@ -272,7 +320,89 @@ __global__ void __launch_bounds__(persistent_max_block<SPT>()) k_persistent(LatD
     // stages are specialised and the selects disappear).
     auto stage = [&](auto s_, int par, bool rec, int ri) {
         const int s = static_cast<int>(s_);
-        {
+        if constexpr (PAIR) {
+            // Every owned site of the wave is in sublattice atom_w and has
+            // ZMAX bonds with the same matrices: the sites are advanced
+            // together, bond by bond, so each bond matrix is read once per
+            // wave and stage (scalar loads shared by the SPT sites) and the
+            // gathers of the SPT sites are in flight at once.  There is no
+            // per-site branch: padding slots (act = false) hold zero spins,
+            // gather site 0 and store their (zero) stage values into padding
+            // slots that no real site reads; only the records and the state
+            // outputs skip them.  The arithmetic per site is the general
+            // path's: H = -B + sum_k J_k S_k (one FMA chain per component),
+            // the energy from the undriven field, then H -= f1 d1 + f2 d2.
+            const LatDev L = refresh(L0);
+            const int cur = s & 1, nxt = 1 - (s & 1);
+            const int st = s == 0 ? 0 : (s == 3 ? 2 : 1);
+            const double f1 = drive_on ? drv[par * 6 + st * 2 + 0] : 0.0;
+            const double f2 = drive_on ? drv[par * 6 + st * 2 + 1] : 0.0;
+            const double cdt = (s == 2) ? 1.0 * dt : 0.5 * dt;
+            const double bdt = (s == 0 || s == 3) ? b16 : b13;
+            const bool obs = (s == 0) && rec;
+            cptr<double> Bu = as_const(L.field) + atom_w * 3;
+            // drive vector of the wave's sublattice: the same sum per
+            // component as sub_drive, formed once per wave
+            const bool drive_now = drive_on && (f1 != 0.0 || f2 != 0.0);
+            double D[3] = {0.0, 0.0, 0.0};
+            if (drive_now) {
+                cptr<double> d0 = as_const(L.fdrive) + atom_w * 3;
+                cptr<double> d1 = as_const(L.fdrive) + (L.n_atoms + atom_w) * 3;
+#pragma unroll
+                for (int d = 0; d < 3; ++d) D[d] = d0[d] * f1 + d1[d] * f2;
+            }
+            // the wave's bond matrices, read once per stage before any
+            // gather (scalar loads: one wait, then only LDS traffic)
+            double Jr[ZMAX][9];
+#pragma unroll
+            for (int k = 0; k < ZMAX; ++k) {
+                cptr<double> J = as_const(L.mats) + (atom_w * L.zmax + k) * 9;
+#pragma unroll
+                for (int q = 0; q < 9; ++q) Jr[k][q] = J[q];
+            }
+            double e_obs[SPT];
+#pragma unroll
+            for (int m = 0; m < SPT; ++m) {
+                double S[3], H[3];
+                // stage 0 reads x itself (the LDS copy holds the same bits)
+                if (s == 0) {
+#pragma unroll
+                    for (int d = 0; d < 3; ++d) S[d] = x[m][d];
+                } else {
+                    lds_ld(cur, site(m), S);
+                }
+#pragma unroll
+                for (int d = 0; d < 3; ++d) H[d] = -Bu[d];
+#pragma unroll
+                for (int k = 0; k < ZMAX; ++k) {
+                    double Sp[3];
+                    lds_ld(cur, nb[m][k], Sp);
+#pragma unroll
+                    for (int d = 0; d < 3; ++d)
+#pragma unroll
+                        for (int e = 0; e < 3; ++e) H[d] = fma(Jr[k][d * 3 + e], Sp[e], H[d]);
+                }
+                e_obs[m] = (obs && want_e) ? site_energy_atom(L, atom_w, S, H) : 0.0;
+                if (drive_now) {
+#pragma unroll
+                    for (int d = 0; d < 3; ++d) H[d] -= D[d];
+                }
+                double k[3];
+                ll_su2_guarded(H, S, alpha, k);
+                double y[3];
+#pragma unroll
+                for (int d = 0; d < 3; ++d) {
+                    if (s == 0) acc[m][d] = x[m][d] + bdt * k[d];
+                    else if (s < 3) acc[m][d] = acc[m][d] + bdt * k[d];
+                    else x[m][d] = acc[m][d] + bdt * k[d];
+                    y[d] = (s < 3) ? x[m][d] + cdt * k[d] : x[m][d];
+                }
+                lds_st(nxt, site(m), y);
+            }
+            if (obs) reduce_obs(L, e_obs);
+            __syncthreads();
+            if (obs) publish(ri);
+        } else {
             const LatDev L = refresh(L0);
             const LdsState cur = (DBUF && (s & 1)) ? bufB : bufA;
             const LdsState nxt = DBUF ? ((s & 1) ? bufA : bufB) : bufA;
@@ -371,7 +501,19 @@ __global__ void __launch_bounds__(persistent_max_block<SPT>()) k_persistent(LatD
             e[m] = 0.0;
             if (act[m]) {
                 double H[3];
-                local_field_lds<ZMAX, UNI, FIELD_ONLY>(L, bufA, p, atm[m], nb[m], cnt[m], x[m], H);
+                if constexpr (PAIR) {
+                    cptr<double> Bu = as_const(L.field) + atom_w * 3;
+#pragma unroll
+                    for (int d = 0; d < 3; ++d) H[d] = -Bu[d];
+#pragma unroll
+                    for (int k = 0; k < ZMAX; ++k) {
+                        double Sp[3];
+                        lds_ld(0, nb[m][k], Sp);
+                        add_bond_fused(as_const(L.mats) + (atom_w * L.zmax + k) * 9, Sp, H);
+                    }
+                } else {
+                    local_field_lds<ZMAX, UNI, FIELD_ONLY>(L, bufA, p, atm[m], nb[m], cnt[m], x[m], H);
+                }
                 e[m] = FIELD_ONLY ? site_energy_atom(L, atm[m], x[m], H) : site_energy(L, atm[m], p, x[m], H);
             }
         }
