@@ -132,6 +132,17 @@ def lib():
         _load_error = (f"{LIB_PATH} is missing: build it with "
                        "`python -m classical_mcmc_spin_amd.build` (needs hipcc)")
         raise CslError(CSL_ERR_HIP, _load_error)
+    # torch ships its own HIP runtime (torch/lib/libamdhip64.so, soname
+    # libamdhip64.so.7) and loads it by path.  Were this library loaded first,
+    # it would bind the system runtime, torch would then load a second one,
+    # and the second runtime finds no GPU ("No HIP GPUs are available").
+    # Importing torch first makes this library bind the runtime torch already
+    # holds (same soname), so device memory from torch and these entry points
+    # share one runtime.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     try:
         L = C.CDLL(LIB_PATH)
     except OSError as e:  # missing HIP runtime etc.
