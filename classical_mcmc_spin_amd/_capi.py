@@ -72,6 +72,7 @@ SIGNATURES = {
     "csl_set_batch_split": (_i, [_vp, _i]),
     "csl_set_stage_graph": (_i, [_vp, _i]),
     "csl_mixed_set_stage_graph": (_i, [_vp, _i]),
+    "csl_integrate_batched_noalloc": (_i, [_vp, _sz, _vp, _i, _vp, _vp, _d, _d, _d, _sz, _i, _vp, _vp, _vp, _psz]),
     "csl_persistent_candidates": (_i, [_vp, _sz, _pi, _pi, _pi, _pi, _psz]),
     "csl_last_launch_variant": (_i, [_vp, _pi]),
     "csl_last_grid": (_i, [_vp, _pi]),

@@ -14,6 +14,7 @@
 #include <functional>
 #include <limits>
 #include <map>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <tuple>
@@ -26,9 +27,31 @@
 #include "csl_kernels.hpp"
 #include "csl_tableau.hpp"
 
+// Scratch of csl_integrate_batched_noalloc: one region per GPU, in the code
+// object's device memory, so that the entry point allocates nothing.  Host
+// code uses the address hipGetSymbolAddress returns for the current device
+// (scratch_ptr), never the array's name.
+static __device__ __align__(256) unsigned char scratch[CSL_NOALLOC_SCRATCH_BYTES];
+
 namespace {
 
 using namespace cslh;
+
+// Address of `scratch` on the current device, looked up once per device.
+unsigned char* scratch_ptr() {
+    static std::mutex mu;
+    static std::map<int, unsigned char*> addr;
+    int dev = 0;
+    hip_check(hipGetDevice(&dev), "hipGetDevice");
+    std::lock_guard<std::mutex> lock(mu);
+    auto it = addr.find(dev);
+    if (it != addr.end()) return it->second;
+    void* p = nullptr;
+    hip_check(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)");
+    if (!p) fail(CSL_ERR_HIP, "no device address for the scratch region");
+    addr[dev] = static_cast<unsigned char*>(p);
+    return addr[dev];
+}
 
 
 struct TuneVal {
@@ -364,34 +387,71 @@ static std::vector<int> split_schedule(size_t B, size_t G, size_t n, size_t& max
     return rows;
 }
 
+// Device memory of one persistent launch supplied by the caller instead of
+// the handle's buffers (csl_integrate_batched_noalloc): outputs, and scratch
+// carved out of a preallocated region for the time grid, the work-split
+// schedule, the seam states and the flags.
+struct ExtBufs {
+    double* mags;              // [n_rec][B][9] or null
+    double* energy;            // [n_rec][B] or null
+    double* final_states;      // [B][N*3] or null
+    unsigned char* scratch;    // preallocated device region
+    size_t scratch_bytes;
+};
+
+// Byte layout of the scratch region of one launch: time grid, status word,
+// flags, schedule, seam states; every part 256-byte aligned.
+struct ScratchLayout {
+    size_t tgrid, status, flags, sched, saved, total;
+};
+
+ScratchLayout scratch_layout(size_t n_grid, size_t G, size_t sched_ints, size_t NP) {
+    auto up = [](size_t v) { return (v + 255) / 256 * 256; };
+    ScratchLayout l{};
+    l.tgrid = 0;
+    l.status = up(l.tgrid + n_grid * sizeof(double));
+    l.flags = up(l.status + sizeof(unsigned));
+    l.sched = up(l.flags + G * sizeof(unsigned));
+    l.saved = up(l.sched + sched_ints * sizeof(int));
+    l.total = up(l.saved + G * 3 * NP * sizeof(double));
+    return l;
+}
+
 // Runs the persistent kernel; device outputs are left in h->rec_mags /
-// h->rec_energy / h->bt_final.
+// h->rec_energy / h->bt_final, or in ext's buffers (then no device memory is
+// allocated and the launch is not timed).
 void run_persistent(csl_lattice* h, size_t B, const double* d_init, size_t init_stride,
                     const double* d_tp, const double* d_amp, const std::vector<double>& grid,
                     double dt, size_t si, bool want_energy, bool want_final, const TuneVal& tv,
-                    double* ms_out) {
+                    double* ms_out, const ExtBufs* ext = nullptr) {
     const size_t n_steps = grid.size() - 1;
     const size_t n_rec = records_for(n_steps, si);
     if (n_steps > size_t(std::numeric_limits<int>::max())) fail(CSL_ERR_ARG, "too many steps");
     hipStream_t s = h->stream;
-    h->tgrid.upload(grid.data(), grid.size(), s);
-    h->rec_mags.ensure(n_rec * B * 9);
-    if (want_energy) h->rec_energy.ensure(n_rec * B);
-    if (want_final) h->bt_final.ensure(B * h->N * 3);
     csl::PersistArgs A{};
+    if (ext) {
+        A.mags = ext->mags;
+        A.energy = want_energy ? ext->energy : nullptr;
+        A.final_states = want_final ? ext->final_states : nullptr;
+    } else {
+        h->tgrid.upload(grid.data(), grid.size(), s);
+        h->rec_mags.ensure(n_rec * B * 9);
+        if (want_energy) h->rec_energy.ensure(n_rec * B);
+        if (want_final) h->bt_final.ensure(B * h->N * 3);
+        A.tgrid = h->tgrid.p;
+        A.mags = h->rec_mags.p;
+        A.energy = want_energy ? h->rec_energy.p : nullptr;
+        A.final_states = want_final ? h->bt_final.p : nullptr;
+    }
     A.B = int(B);
     A.init = d_init;
     A.init_stride = init_stride;
     A.t_pulse = d_tp;
     A.amp = d_amp;
-    A.tgrid = h->tgrid.p;
     A.n_steps = int(n_steps);
     A.save_interval = int(si);
     A.n_rec = int(n_rec);
     A.dt = dt;
-    A.mags = h->rec_mags.p;
-    A.energy = want_energy ? h->rec_energy.p : nullptr;
-    A.final_states = want_final ? h->bt_final.p : nullptr;
     csl::LatDev L = h->L;
     if (h->rep_B) {
         if (B != h->rep_B) fail(CSL_ERR_ARG, "batch size differs from the per-replica Hamiltonians set");
@@ -413,9 +473,32 @@ void run_persistent(csl_lattice* h, size_t B, const double* d_init, size_t init_
     const size_t G = size_t(std::max(occ, 0)) * size_t(h->n_cu);
     A.sched = nullptr;
     h->last_grid = int(B);
-    if (G > 0 && B > G && n_steps > 0 && G <= size_t(std::numeric_limits<int>::max())) {
-        size_t max_p = 1;
-        const std::vector<int> sched = split_schedule(B, G, n_steps, max_p);
+    const bool split = G > 0 && B > G && n_steps > 0 && G <= size_t(std::numeric_limits<int>::max());
+    size_t max_p = 1;
+    const std::vector<int> sched = split ? split_schedule(B, G, n_steps, max_p) : std::vector<int>();
+    unsigned* status = nullptr;
+    if (ext) {
+        // everything in the caller's preallocated scratch; cleared by a fill
+        // kernel (no memset nodes for callers that capture the launch)
+        const ScratchLayout l = scratch_layout(grid.size(), split ? G : 0, sched.size(), size_t(h->L.NP));
+        if (l.total > ext->scratch_bytes)
+            fail(CSL_ERR_CAPACITY, "the launch needs " + std::to_string(l.total) + " bytes of scratch, more than the " +
+                                       std::to_string(ext->scratch_bytes) + " available");
+        unsigned char* base = ext->scratch;
+        hip_check(hipMemcpyAsync(base + l.tgrid, grid.data(), grid.size() * sizeof(double), hipMemcpyHostToDevice, s),
+                  "H2D");
+        A.tgrid = reinterpret_cast<const double*>(base + l.tgrid);
+        status = reinterpret_cast<unsigned*>(base + l.status);
+        if (split) {
+            hip_check(hipMemcpyAsync(base + l.sched, sched.data(), sched.size() * sizeof(int), hipMemcpyHostToDevice, s),
+                      "H2D");
+            A.sched = reinterpret_cast<const int*>(base + l.sched);
+            A.saved = reinterpret_cast<double*>(base + l.saved);
+            A.flags = reinterpret_cast<unsigned*>(base + l.flags);
+        }
+        // status and flags are contiguous words: one fill clears both
+        hip_check(csl::launch_fill_u32(status, 0u, (l.sched - l.status) / sizeof(unsigned), s), "fill");
+    } else if (split) {
         h->ps_sched.upload(sched.data(), sched.size(), s);
         h->ps_saved.ensure(G * 3 * size_t(h->L.NP));
         h->ps_flags.ensure(G);
@@ -423,12 +506,31 @@ void run_persistent(csl_lattice* h, size_t B, const double* d_init, size_t init_
         hip_check(hipMemsetAsync(h->ps_flags.p, 0, G * sizeof(unsigned), s), "memset");
         hip_check(hipMemsetAsync(h->ps_status.p, 0, sizeof(unsigned), s), "memset");
         A.sched = h->ps_sched.p;
-        A.max_pieces = int(max_p);
-        A.grid = int(G);
         A.saved = h->ps_saved.p;
         A.flags = h->ps_flags.p;
-        A.status = h->ps_status.p;
+        status = h->ps_status.p;
+    }
+    if (split) {
+        A.max_pieces = int(max_p);
+        A.grid = int(G);
+        A.status = status;
         h->last_grid = int(G);
+    }
+    if (ext) {
+        hip_check(csl::launch_persistent(L, A, tv.block, tv.spt, tv.flags, s), "k_persistent launch");
+        hip_check(hipStreamSynchronize(s), "k_persistent");
+        if (A.sched) {
+            unsigned st = 0;
+            hip_check(hipMemcpy(&st, status, sizeof(unsigned), hipMemcpyDeviceToHost), "D2H");
+            if (st) fail(CSL_ERR_HIP, "a workgroup of the split batch waited too long for a trajectory's state");
+        }
+        h->last_ms = 0.0;
+        h->last_block = tv.block;
+        h->last_spt = tv.spt;
+        h->last_dbuf = tv.flags;
+        h->last_variant = csl::persistent_variant(L, tv.block, tv.spt, tv.flags);
+        h->last_launches = 1;
+        return;
     }
     hipEvent_t e0, e1;
     hip_check(hipEventCreate(&e0), "event");
@@ -1176,6 +1278,29 @@ int csl_integrate_batched(csl_lattice* h, size_t B, const double* init, int init
             for (size_t n = 0; n < n_steps; n += save_interval) times[r++] = grid[n];
             times[r] = grid[n_steps];
         }
+    });
+}
+
+int csl_integrate_batched_noalloc(csl_lattice* h, size_t B, const double* d_init, int init_per_replica,
+                                  const double* d_t_pulse, const double* d_pulse_amp, double T0, double T1,
+                                  double dt, size_t save_interval, int tgrid, double* d_mags, double* d_energy,
+                                  double* d_final_states, size_t* n_rec_out) {
+    return guarded([&] {
+        check_handle(h);
+        if (B == 0) fail(CSL_ERR_ARG, "batch size must be >= 1");
+        if (B > 65535) fail(CSL_ERR_ARG, "at most 65535 trajectories per call");
+        if (!d_init || !d_t_pulse || !d_pulse_amp) fail(CSL_ERR_NULL, "null init / t_pulse / pulse_amp");
+        if (save_interval == 0) fail(CSL_ERR_ARG, "save_interval must be >= 1");
+        if (persistent_candidates(h).empty())
+            fail(CSL_ERR_UNSUPPORTED, "the lattice does not fit one workgroup (no allocation-free path)");
+        auto grid = make_grid(T0, T1, dt, tgrid);
+        const size_t n_steps = grid.size() - 1;
+        if (n_rec_out) *n_rec_out = records_for(n_steps, save_interval);
+        unsigned char* sp = scratch_ptr();
+        const TuneVal tv = pick_persistent(h, B, 0);
+        const ExtBufs ext{d_mags, d_energy, d_final_states, sp, size_t(CSL_NOALLOC_SCRATCH_BYTES)};
+        run_persistent(h, B, d_init, init_per_replica ? h->N * 3 : 0, d_t_pulse, d_pulse_amp, grid, dt,
+                       save_interval, d_energy != nullptr, d_final_states != nullptr, tv, nullptr, &ext);
     });
 }
 

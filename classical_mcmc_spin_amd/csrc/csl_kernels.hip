@@ -300,6 +300,11 @@ __global__ void __launch_bounds__(256) k_rk_combine(double* out, const double* x
     out[i] = v;
 }
 
+__global__ void __launch_bounds__(256) k_fill_u32(unsigned* __restrict__ p, unsigned v, size_t n) {
+    const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
 // ---------------------------------------------------------------------------
 // launchers
 
@@ -436,6 +441,12 @@ hipError_t launch_perm_to_aos(const LatDev& L, const double* soa, double* aos, i
 
 hipError_t launch_normalize(double* s, int N, double len, hipStream_t st) {
     hipLaunchKernelGGL(k_normalize, dim3((N + 255) / 256), dim3(256), 0, st, s, N, len);
+    return hipGetLastError();
+}
+
+hipError_t launch_fill_u32(unsigned* p, unsigned v, size_t n, hipStream_t st) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_fill_u32, dim3(unsigned((n + 255) / 256)), dim3(256), 0, st, p, v, n);
     return hipGetLastError();
 }
 

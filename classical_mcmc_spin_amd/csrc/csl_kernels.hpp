@@ -222,6 +222,8 @@ struct RkCombine {
     double g;
     double w[13];
 };
+// p[i] = v for i < n (clears flags and counters without a memset node).
+hipError_t launch_fill_u32(unsigned* p, unsigned v, size_t n, hipStream_t st);
 // out[i] = g*x[i] + sum_j w[j] * K[idx[j]*n + i] for i < n; out may be x.
 hipError_t launch_rk_combine(double* out, const double* x, const double* K, size_t n, const RkCombine& c,
                              hipStream_t st);

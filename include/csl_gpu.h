@@ -272,6 +272,29 @@ int csl_integrate_multi(csl_lattice* const* hs, size_t n_lat, const size_t* B_of
                         size_t save_interval, int tgrid, size_t max_rec, double* times, double* mags,
                         double* energy, size_t* n_rec);
 
+/* Bytes of the per-GPU scratch region of csl_integrate_batched_noalloc. */
+#define CSL_NOALLOC_SCRATCH_BYTES (24u << 20)
+
+/* csl_integrate_batched on the persistent kernel without allocating device
+ * memory.  No counterpart in the reference: for callers that cannot
+ * allocate (code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc).  Inputs and outputs are DEVICE pointers on the
+ * handle's device: d_init [N*3] (AoS, reference order) or [B*N*3] with
+ * init_per_replica, d_t_pulse and d_pulse_amp [B*2]; outputs d_mags
+ * [n_rec*B*9], d_energy [n_rec*B] (E/N, may be NULL), d_final_states
+ * [B*N*3] (may be NULL); *n_rec receives the record count.  The time grid,
+ * the work-split schedule, the seam states and the flags live in a
+ * file-scope device array of CSL_NOALLOC_SCRATCH_BYTES per GPU (held until
+ * the process ends); a launch that needs more returns CSL_ERR_CAPACITY.
+ * Every call on one GPU shares that array: calls that overlap on one GPU
+ * race.  Lattices that do not fit one workgroup return
+ * CSL_ERR_UNSUPPORTED.  Synchronous on the handle's stream; the launch
+ * shape is the handle's (autotuned or set) choice for B. */
+int csl_integrate_batched_noalloc(csl_lattice* h, size_t B, const double* d_init, int init_per_replica,
+                                  const double* d_t_pulse, const double* d_pulse_amp, double T0, double T1,
+                                  double dt, size_t save_interval, int tgrid, double* d_mags,
+                                  double* d_energy, double* d_final_states, size_t* n_rec);
+
 /* ---- Dormand-Prince 5(4) --------------------------------------------- */
 
 /* Integrates B trajectories with the Dormand-Prince 5(4) pair, one
