@@ -222,9 +222,10 @@ def bench_2dcs(args):
                          algorithmic_flops_per_site_step=FLOPS_Z3 + FLOPS_DRIVE,
                          hbm_algorithmic_frac=site_steps * BYTES_PER_SITE_STEP / (kms * 1e-3) / HBM_PEAK)
     roof["traffic"] = profile_number("traffic_k_persistent_2dcs_36x36.txt", "traffic_bytes_per_launch")
-    valu = profile_number("counters_k_persistent_2dcs_36x36.txt", "valu_issue_frac")
-    if valu is not None:
-        roof["valu_issue_frac"] = valu
+    for key in ("valu_issue_frac", "lds_busy_frac"):
+        v = profile_number("counters_k_persistent_2dcs_36x36.txt", key)
+        if v is not None:
+            roof[key] = v
     line = {
         "metric": METRIC, "value": traj_s, "unit": "trajectories/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": ms_step, "higher_is_better": True, "scaling": "weak",

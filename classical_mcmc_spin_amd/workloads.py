@@ -7,7 +7,7 @@ initial states), shared by the benchmark, the tests and user scripts.
   kitaev_2dcs           Kitaev honeycomb 36x36x1, K=-1, G=0.25, G'=-0.02,
                         h = 0.1 along (1,1,1)/sqrt3, relaxed + tilted start,
                         2 pulses along x (A = 0.5, w = 0.5, omega = 0),
-                        tau = 0 .. 63.75 step 0.25 -> 513 trajectories
+                        256 delays tau = linspace(0, 64, 256) -> 513 trajectories
   pyrochlore_dynamics   pyrochlore 16^3 conventional cells (65 536 sites),
                         Jzz=1, J+-=0.05, J+-+-=0.02, Jz+-=-0.01, 64 samples of
                         all-in-all-out + N(0, sigma^2) tilt, sigma ~ U[0.05, 0.5],

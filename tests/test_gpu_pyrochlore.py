@@ -92,6 +92,30 @@ def test_pyrochlore_full_workload_properties():
     assert np.max(np.abs(e[-1] - e[0]) / np.abs(e[0])) <= 1e-3
 
 
+def test_pyrochlore_drift_vs_oracle():
+    """Sample 37 of the full workload over its first 1024 steps (the oracle
+    needs about 20 s for them): spin-length and energy drift no more than
+    ten times the oracle's on the same trajectory, and records within 1e-10
+    of it over the first 200 steps."""
+    import pyoracle as po
+    lat = workloads.pyrochlore_lattice(16)
+    st, _ = workloads.pyrochlore_tilted_aiao(lat, 64, seed=1234)
+    T0, dt, n = 0.0, 0.02, 1024
+    with GPULattice(lat) as gl:
+        res = gl.integrate_batched(st[37], np.zeros((1, 2)), np.zeros((1, 2)), T0, _t1(T0, n, dt), dt,
+                                   tgrid="boost", want_final=True)
+    print(f"[pyro-drift] oracle, sample 37, {n} steps", flush=True)
+    ref = po.OracleLattice(lat.unit_cell, 16, 16, 16).integrate_md(st[37].reshape(-1), T0, _t1(T0, n, dt), dt)
+    assert np.max(np.abs(res["mags"][:201, 0] - ref["mags"][:201])) <= TOL
+    assert np.max(np.abs(res["energy"][:201, 0] - ref["energy"][:201])) <= TOL
+    d_gpu = np.max(np.abs(np.linalg.norm(res["final"][0], axis=1) - 1.0))
+    d_ref = np.max(np.abs(np.linalg.norm(ref["final"].reshape(-1, 3), axis=1) - 1.0))
+    assert d_gpu <= 10.0 * d_ref + 1e-13, (d_gpu, d_ref)
+    e_gpu = np.max(np.abs(res["energy"][:, 0] - res["energy"][0, 0]))
+    e_ref = np.max(np.abs(ref["energy"] - ref["energy"][0]))
+    assert e_gpu <= 10.0 * e_ref + 1e-13, (e_gpu, e_ref)
+
+
 def _sqw_numpy(snapshots, L, pos, N):
     """S(q, w) of snapshots [B, n_t, N, 3] with numpy FFTs (site index
     ((i*L + j)*L + k)*n_atoms + atom)."""
