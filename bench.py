@@ -57,6 +57,8 @@ BYTES_PER_SITE_STEP = 192   # 4 RK stages x (read + write) x 3 doubles x 8 B
 # M and E every step 13  ->  4 * 78 - 6 + 13 = 319; a pulse drive adds 12 per
 # stage (f1 d1 + f2 d2), 48 per step.
 FLOPS_Z3 = 319
+# the same count with z = 6 bonds: 4 stages x (18 z + 21) + 19
+FLOPS_Z6 = 535
 FLOPS_DRIVE = 48
 
 DT = 0.01
@@ -405,9 +407,11 @@ PYRO_L, PYRO_B, PYRO_STEPS, PYRO_DT = 16, 64, 4096, 0.02
 def bench_pyrochlore(args):
     """64 samples x 4096 RK4 steps of the 16^3-cell conventional pyrochlore
     lattice (65 536 sites, z = 6), all-in-all-out + tilt starts, M(t) and
-    E(t) every step.  The lattice is too large for one workgroup's LDS, so
-    the grid-wide stage kernels run (4 launches per step).  Error: two
-    samples over 200 steps against the oracle."""
+    E(t) every step.  The lattice is too large for one workgroup's LDS: the
+    default path is the partitioned persistent kernel (each sample spread
+    over co-resident workgroups exchanging halo values every stage, one
+    launch); the grid-wide stage kernels (4 launches per step) are timed
+    beside it.  Error: two samples over 200 steps against the oracle."""
     from classical_mcmc_spin_amd import GPULattice, workloads
     lat = workloads.pyrochlore_lattice(PYRO_L)
     N = lat.lattice_size
@@ -416,19 +420,31 @@ def bench_pyrochlore(args):
     zero = np.zeros((PYRO_B, 2))
     gl = GPULattice(lat)
     one_step = lambda: gl.integrate_batched(init, zero, zero, 0.0, T1p, PYRO_DT, tgrid="boost")
+
+    def timed(steps):
+        kernel_ms = []
+        t = time.perf_counter()
+        for _ in range(steps):
+            res = one_step()
+            kernel_ms.append(gl.last_kernel_ms())
+        el = time.perf_counter() - t
+        assert res["mags"].shape[0] == PYRO_STEPS + 1 and np.all(np.isfinite(res["mags"]))
+        return el, float(np.mean([k[0] for k in kernel_ms])), kernel_ms[-1][1]
+
     for _ in range(args.warmup):
         one_step()
-    kernel_ms = []
-    t = time.perf_counter()
-    for _ in range(args.steps):
-        res = one_step()
-        kernel_ms.append(gl.last_kernel_ms()[0])
-    el = time.perf_counter() - t
-    assert res["mags"].shape[0] == PYRO_STEPS + 1 and np.all(np.isfinite(res["mags"]))
-    kms = float(np.mean(kernel_ms))
+    el, kms, _ = timed(args.steps)
+    parts, groups = gl.last_partition()
+    launch = gl.last_launch_config()
     if args.profile:
         gl.close()
-        return {"profile": True, "workload": "pyrochlore", "kernel_ms": kms, "ms_per_step": 1e3 * el / args.steps}
+        return {"profile": True, "workload": "pyrochlore", "kernel_ms": kms, "ms_per_step": 1e3 * el / args.steps,
+                "parts": parts, "groups": groups}
+    # the grid-wide stage kernels on the same batch
+    gl.set_partition(-1)
+    one_step()
+    el_st, kms_st, launches_st = timed(args.steps)
+    gl.set_partition(0)
     n_par = 200
     pr = gl.integrate_batched(init[[0, PYRO_B - 1]], np.zeros((2, 2)), np.zeros((2, 2)), 0.0,
                               (n_par + 0.5) * PYRO_DT, PYRO_DT, tgrid="boost", want_final=True)
@@ -442,16 +458,25 @@ def bench_pyrochlore(args):
         err["E"] = max(err["E"], float(np.max(np.abs(pr["energy"][:, j] - ref["energy"]))))
         err["S"] = max(err["S"], float(np.max(np.abs(pr["final"][j] - ref["final"].reshape(-1, 3)))))
     samples_s = PYRO_B * args.steps / el
-    bytes_launch = PYRO_B * N * BYTES_PER_SITE_STEP / 4       # one RK stage of every site
-    roof = hbm_roofline("k_stage", bytes_launch, kms)
-    roof["traffic"] = profile_number("traffic_l2_k_stage_pyrochlore_16x16x16.txt", "traffic_bytes_per_launch")
+    site_steps = PYRO_B * N * PYRO_STEPS
+    roof = fp64_roofline("k_partitioned", site_steps * FLOPS_Z6, kms, algorithmic_flops_per_site_step=FLOPS_Z6,
+                         hbm_algorithmic_bytes_per_launch=site_steps * BYTES_PER_SITE_STEP)
+    roof["traffic"] = profile_number("traffic_k_partitioned_pyrochlore_16x16x16.txt", "traffic_bytes_per_launch")
+    stage_roof = hbm_roofline("k_stage", PYRO_B * N * BYTES_PER_SITE_STEP / 4, kms_st)  # one RK stage of every site
+    stage_roof["traffic"] = profile_number("traffic_l2_k_stage_pyrochlore_16x16x16.txt", "traffic_bytes_per_launch")
     out = {
         "value": samples_s, "unit": "trajectories/s", "ms_per_step": 1e3 * el / args.steps,
         "config": {"workload": f"pyrochlore_16x16x16_conventional_64samples_{PYRO_STEPS}steps_rk4_dt0.02",
-                   "sites": N, "trajectories": PYRO_B, "rk4_steps": PYRO_STEPS, "record_every": 1},
+                   "sites": N, "trajectories": PYRO_B, "rk4_steps": PYRO_STEPS, "record_every": 1,
+                   "path": "partitioned persistent kernel", "parts_per_sample": parts, "sample_groups": groups,
+                   "launch": launch},
         "site_steps_per_s": samples_s * N * PYRO_STEPS,
         "max_abs_err_vs_oracle_200_steps": err,
         "roofline": roof,
+        "stage_kernels": {"value": PYRO_B * args.steps / el_st, "unit": "trajectories/s",
+                          "ms_per_step": 1e3 * el_st / args.steps, "kernel_ms_per_launch": kms_st,
+                          "launches_per_run": launches_st, "roofline": stage_roof},
+        "speedup_over_stage_kernels": el_st / el,
     }
     if not args.no_cpu:
         def per_run(n):

@@ -70,6 +70,8 @@ SIGNATURES = {
     "csl_last_launch_config": (_i, [_vp, _pi, _pi, _pi]),
     "csl_set_launch_config": (_i, [_vp, _sz, _i, _i, _i]),
     "csl_set_batch_split": (_i, [_vp, _i]),
+    "csl_set_partition": (_i, [_vp, _i]),
+    "csl_last_partition": (_i, [_vp, _pi, _pi]),
     "csl_set_stage_graph": (_i, [_vp, _i]),
     "csl_mixed_set_stage_graph": (_i, [_vp, _i]),
     "csl_integrate_batched_noalloc": (_i, [_vp, _sz, _vp, _i, _vp, _vp, _d, _d, _d, _sz, _i, _vp, _vp, _vp, _psz]),

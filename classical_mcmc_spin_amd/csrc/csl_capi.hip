@@ -65,6 +65,15 @@ int batch_class(size_t B) { return B >= 64 ? 2 : (B > 1 ? 1 : 0); }
 
 }  // namespace
 
+// Partition of a lattice over the parts of the partitioned kernel
+// (csl_partition.hip; the field meanings are those of csl::PartArgs).
+struct PartPlan {
+    int P = 0;  // 0: no plan built
+    int c_per = 0, cpad = 0, n_loc = 0, n_halo_max = 0, n_nbr_max = 0, block = 0, spt = 0, dbuf = 0;
+    DBuf<unsigned> col, exported;
+    DBuf<int> halo_src, n_halo, nbr;
+};
+
 struct csl_lattice {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -94,6 +103,13 @@ struct csl_lattice {
     DBuf<int> ps_latof;
     DBuf<size_t> ps_initoff;
     int split_batches = 1;
+    // partitioned kernel: 0 = for lattices that do not fit one workgroup,
+    // P > 0 = always with P parts, -1 = never (csl_set_partition)
+    int part_req = 0;
+    PartPlan part;
+    DBuf<double> part_exch, part_obs;
+    DBuf<unsigned> part_ver;
+    int last_parts = 0, last_groups = 0;
     int stage_graph = 0;          // batched stage-kernel path replayed from a hipGraph
     int last_grid = 0;
     DBuf<long long> dp_counters;  // [B][4]
@@ -557,6 +573,254 @@ void run_persistent(csl_lattice* h, size_t B, const double* d_init, size_t init_
     if (ms_out) *ms_out = ms;
 }
 
+// Host-side partition of the lattice into P parts (csl_partition.hip):
+// part w owns the cells [w*c_per, (w+1)*c_per) of every sublattice, LDS
+// slot atom*cpad + (cell - w*c_per); partners owned by another part get a
+// halo slot n_loc + h.  False when P parts do not fit (an empty part, LDS,
+// 16-bit slots, more than 64 parts exchanged with, no thread shape).  Two
+// LDS buffers of the own stage values when they fit.
+struct PartHost {
+    int P = 0, c_per = 0, cpad = 0, n_loc = 0, n_halo_max = 0, n_nbr_max = 0, block = 0, spt = 0, dbuf = 0;
+    std::vector<unsigned> col, exported;
+    std::vector<int> halo_src, n_halo, nbr;
+};
+
+bool plan_partition(const csl_lattice* h, int P, PartHost& o) {
+    const csl::LatDev& L = h->L;
+    const int na = int(h->n_atoms), nc = L.n_cells, npad = L.npad, NP = L.NP, Z = L.zmax;
+    if (P < 1 || P > nc || Z < 1) return false;
+    const int c_per = (nc + P - 1) / P;
+    if ((P - 1) * c_per >= nc) return false;  // an empty part
+    const int cpad = (c_per + csl::kWave - 1) / csl::kWave * csl::kWave;
+    const long n_loc = long(na) * cpad;
+    if (n_loc > 65535) return false;
+    // thread shape: block = 64*w threads, n_loc = block * spt
+    const int waves = int(n_loc / csl::kWave);
+    int block = 0, spt = 0;
+    for (int w = 16; w >= 1 && !block; --w) {
+        if (waves % w) continue;
+        const int sp = waves / w;
+        if (csl::partitioned_spt_supported(sp) && w * csl::kWave <= csl::partitioned_block_limit(sp)) {
+            block = w * csl::kWave;
+            spt = sp;
+        }
+    }
+    if (!block) return false;
+    // device-order partner table from the CSR mirror (the slot order of build_tables)
+    const size_t N = h->N;
+    auto dev = [&](size_t i) { return int(i % h->n_atoms) * npad + int(i / h->n_atoms); };
+    std::vector<int> col(size_t(Z) * NP, -1), cnt(NP, 0);
+    for (size_t i = 0; i < N; ++i) {
+        const int p = dev(i);
+        cnt[p] = int(h->h_off[i + 1] - h->h_off[i]);
+        for (size_t k = h->h_off[i]; k < h->h_off[i + 1]; ++k)
+            col[h->h_slot_of[k] * size_t(NP) + p] = dev(h->h_par[k]);
+    }
+    // words per site of the packed partner table: the kernel is compiled
+    // for the bucketed bond count (3, 6 or 12)
+    const int Zb = csl::zmax_bucket(Z);
+    const int nw = (Zb + 1) / 2;
+    o = PartHost{};
+    o.col.assign(size_t(P) * nw * n_loc, 0u);
+    o.exported.assign(size_t(P) * n_loc, 0u);
+    std::vector<std::vector<int>> halo(P);
+    std::vector<std::vector<char>> link(P, std::vector<char>(P, 0));
+    std::vector<int> stamp(NP, -1), hidx(NP, 0);
+    for (int w = 0; w < P; ++w) {
+        for (int a = 0; a < na; ++a)
+            for (int c = 0; c < cpad; ++c) {
+                const int q = a * cpad + c, cell = w * c_per + c;
+                const bool real = c < c_per && cell < nc;
+                const int p = a * npad + cell;
+                for (int k = 0; k < Zb; ++k) {
+                    int slot = q;  // padding slots and unused bond slots read the site itself
+                    if (real && k < cnt[p]) {
+                        const int pd = col[size_t(k) * NP + p];
+                        const int ap = pd / npad, cp = pd - ap * npad, wp = cp / c_per;
+                        if (wp == w) {
+                            slot = ap * cpad + (cp - w * c_per);
+                        } else {
+                            if (stamp[pd] != w) {
+                                stamp[pd] = w;
+                                hidx[pd] = int(halo[w].size());
+                                halo[w].push_back(pd);
+                            }
+                            slot = int(n_loc) + hidx[pd];
+                            link[w][wp] = link[wp][w] = 1;
+                            o.exported[size_t(wp) * n_loc + ap * cpad + (cp - wp * c_per)] = 1u;
+                        }
+                    }
+                    if (slot > 65535) return false;
+                    o.col[(size_t(w) * nw + k / 2) * n_loc + q] |= unsigned(slot) << (16 * (k & 1));
+                }
+            }
+        if (n_loc + long(halo[w].size()) > 65535) return false;
+    }
+    int hmax = 0, nmax = 0;
+    for (int w = 0; w < P; ++w) {
+        hmax = std::max(hmax, int(halo[w].size()));
+        int c = 0;
+        for (int u = 0; u < P; ++u) c += link[w][u];
+        nmax = std::max(nmax, c);
+    }
+    if (nmax > 64) return false;
+    if (csl::partitioned_lds_bytes(int(n_loc), hmax, block, false) > h->lds_limit) return false;
+    o.dbuf = csl::partitioned_lds_bytes(int(n_loc), hmax, block, true) <= h->lds_limit ? 1 : 0;
+    o.P = P;
+    o.c_per = c_per;
+    o.cpad = cpad;
+    o.n_loc = int(n_loc);
+    o.n_halo_max = hmax;
+    o.n_nbr_max = nmax;
+    o.block = block;
+    o.spt = spt;
+    o.halo_src.assign(size_t(P) * std::max(hmax, 1), 0);
+    o.n_halo.assign(P, 0);
+    o.nbr.assign(size_t(P) * std::max(nmax, 1), -1);
+    for (int w = 0; w < P; ++w) {
+        o.n_halo[w] = int(halo[w].size());
+        std::copy(halo[w].begin(), halo[w].end(), o.halo_src.begin() + size_t(w) * std::max(hmax, 1));
+        int c = 0;
+        for (int u = 0; u < P; ++u)
+            if (link[w][u]) o.nbr[size_t(w) * std::max(nmax, 1) + c++] = u;
+    }
+    return true;
+}
+
+// Tables only the partitioned kernel can run: translation-invariant bond
+// matrices and per-sublattice fields / onsite matrices.
+bool partition_eligible(const csl_lattice* h) { return h->L.uniform && h->L.site_uniform; }
+
+// Builds (or keeps) the device plan for P parts; P = 0 picks the fewest
+// parts that fit.  False when none fits.
+bool ensure_partition(csl_lattice* h, int P) {
+    if (!partition_eligible(h)) return false;
+    if (h->part.P && (P == 0 || P == h->part.P)) return true;
+    PartHost o;
+    bool ok = false;
+    if (P > 0) {
+        ok = plan_partition(h, P, o);
+    } else {
+        const size_t state = 3 * sizeof(double) * size_t(h->L.NP);
+        const int p0 = int(std::max<size_t>(1, (state + h->lds_limit - 1) / h->lds_limit));
+        for (int p = p0; p <= std::min(h->n_cu, h->L.n_cells) && !ok; ++p) ok = plan_partition(h, p, o);
+    }
+    if (!ok) return false;
+    hipStream_t s = h->stream;
+    PartPlan& d = h->part;
+    d.P = o.P;
+    d.c_per = o.c_per;
+    d.cpad = o.cpad;
+    d.n_loc = o.n_loc;
+    d.n_halo_max = o.n_halo_max;
+    d.n_nbr_max = o.n_nbr_max;
+    d.block = o.block;
+    d.spt = o.spt;
+    d.dbuf = o.dbuf;
+    d.col.upload(o.col.data(), o.col.size(), s);
+    d.exported.upload(o.exported.data(), o.exported.size(), s);
+    d.halo_src.upload(o.halo_src.data(), o.halo_src.size(), s);
+    d.n_halo.upload(o.n_halo.data(), o.n_halo.size(), s);
+    d.nbr.upload(o.nbr.data(), o.nbr.size(), s);
+    hip_check(hipStreamSynchronize(s), "partition upload");
+    return true;
+}
+
+// Runs the partitioned kernel over B trajectories (plan built); device
+// outputs are left in h->rec_mags / h->rec_energy / h->bt_final.
+void run_partitioned(csl_lattice* h, size_t B, const double* d_init, size_t init_stride, const double* d_tp,
+                     const double* d_amp, const std::vector<double>& grid, double dt, size_t si, bool want_energy,
+                     bool want_final, double* ms_out) {
+    const PartPlan& pl = h->part;
+    const size_t n_steps = grid.size() - 1;
+    const size_t n_rec = records_for(n_steps, si);
+    if (n_steps > size_t(std::numeric_limits<int>::max()) || n_rec > size_t(std::numeric_limits<int>::max()))
+        fail(CSL_ERR_ARG, "too many steps");
+    hipStream_t s = h->stream;
+    csl::LatDev L = h->L;
+    csl::PartArgs A{};
+    A.B = int(B);
+    A.init = d_init;
+    A.init_stride = init_stride;
+    A.t_pulse = d_tp;
+    A.amp = d_amp;
+    A.n_steps = int(n_steps);
+    A.save_interval = int(si);
+    A.n_rec = int(n_rec);
+    A.dt = dt;
+    if (h->rep_B) {
+        if (B != h->rep_B) fail(CSL_ERR_ARG, "batch size differs from the per-replica Hamiltonians set");
+        L.mats = h->rep_mats.p;
+        L.field = h->rep_field.p;
+        A.mats_stride = h->rep_mats_stride;
+        A.field_stride = h->rep_field_stride;
+    }
+    A.P = pl.P;
+    A.c_per = pl.c_per;
+    A.cpad = pl.cpad;
+    A.n_loc = pl.n_loc;
+    A.n_halo_max = pl.n_halo_max;
+    A.n_nbr_max = pl.n_nbr_max;
+    A.dbuf = pl.dbuf;
+    A.col = pl.col.p;
+    A.halo_src = pl.halo_src.p;
+    A.n_halo = pl.n_halo.p;
+    A.exported = pl.exported.p;
+    A.nbr = pl.nbr.p;
+    int occ = 0;
+    hip_check(csl::launch_partitioned(L, A, pl.block, pl.spt, s, &occ), "partitioned occupancy");
+    const size_t co = size_t(std::max(occ, 0)) * size_t(h->n_cu);
+    if (co < size_t(pl.P)) fail(CSL_ERR_UNSUPPORTED, "the parts of one trajectory do not fit the GPU at once");
+    const int groups = int(std::min(B, co / size_t(pl.P)));
+    const int grid_wg = groups * pl.P;
+    A.groups = groups;
+    A.xcd_map = (grid_wg % 8 == 0 && (grid_wg / 8) % pl.P == 0) ? 1 : 0;
+    h->tgrid.upload(grid.data(), grid.size(), s);
+    A.tgrid = h->tgrid.p;
+    h->part_exch.ensure(size_t(groups) * 6 * size_t(L.NP));
+    h->part_ver.ensure(size_t(grid_wg));
+    h->ps_status.ensure(1);
+    h->part_obs.ensure(B * n_rec * size_t(pl.P) * 10);
+    h->rec_mags.ensure(n_rec * B * 9);
+    if (want_energy) h->rec_energy.ensure(n_rec * B);
+    if (want_final) h->bt_final.ensure(B * h->N * 3);
+    A.exch = h->part_exch.p;
+    A.ver = h->part_ver.p;
+    A.status = h->ps_status.p;
+    A.part_obs = h->part_obs.p;
+    A.final_states = want_final ? h->bt_final.p : nullptr;
+    A.want_energy = want_energy ? 1 : 0;
+    hip_check(csl::launch_fill_u32(h->part_ver.p, 0u, size_t(grid_wg), s), "fill");
+    hip_check(csl::launch_fill_u32(h->ps_status.p, 0u, 1, s), "fill");
+    hipEvent_t e0, e1;
+    hip_check(hipEventCreate(&e0), "event");
+    hip_check(hipEventCreate(&e1), "event");
+    hip_check(hipEventRecord(e0, s), "event record");
+    hip_check(csl::launch_partitioned(L, A, pl.block, pl.spt, s, nullptr), "k_partitioned launch");
+    hip_check(hipEventRecord(e1, s), "event record");
+    hip_check(csl::launch_part_reduce(h->part_obs.p, pl.P, int(B), int(n_rec), 1.0 / double(h->N), h->rec_mags.p,
+                                      want_energy ? h->rec_energy.p : nullptr, s),
+              "k_part_reduce");
+    hip_check(hipStreamSynchronize(s), "k_partitioned");
+    unsigned st = 0;
+    hip_check(hipMemcpy(&st, h->ps_status.p, sizeof(unsigned), hipMemcpyDeviceToHost), "D2H");
+    float ms = 0.f;
+    hip_check(hipEventElapsedTime(&ms, e0, e1), "elapsed");
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (st) fail(CSL_ERR_HIP, "a part of the partitioned launch waited too long for its neighbours' stage values");
+    h->last_ms = ms;
+    h->last_block = pl.block;
+    h->last_spt = pl.spt;
+    h->last_dbuf = pl.dbuf ? csl::kPersistDoubleBuffer | csl::kPersistUnrollStages : csl::kPersistUnrollStages;
+    h->last_variant = csl::kVariantPartitioned;
+    h->last_launches = 1;
+    h->last_grid = grid_wg;
+    h->last_parts = pl.P;
+    h->last_groups = groups;
+    if (ms_out) *ms_out = ms;
+}
+
 // Plain path: grid-wide stage kernels on SoA buffers, B trajectories in
 // grid.y; X must already hold the SoA initial state(s).
 void run_stages(csl_lattice* h, size_t B, const double* d_tp, const double* d_amp, bool drive_on,
@@ -564,6 +828,7 @@ void run_stages(csl_lattice* h, size_t B, const double* d_tp, const double* d_am
                 double* d_states_out /* [n_rec][B][N*3] or null */,
                 const std::function<void(size_t)>& at_step = {} /* called with X = x_n */,
                 bool use_graph = false) {
+    h->last_variant = 0;
     const size_t n_steps = grid.size() - 1;
     const size_t n_rec = records_for(n_steps, si);
     hipStream_t s = h->stream;
@@ -1251,7 +1516,13 @@ int csl_integrate_batched(csl_lattice* h, size_t B, const double* init, int init
         for (double a : am) drive_on |= (a != 0.0);
         if (block < -1) fail(CSL_ERR_ARG, "block must be >= -1");
         auto cands = persistent_candidates(h);
-        if (!cands.empty() && block >= 0) {
+        const bool part = block == 0 && (h->part_req > 0 || (h->part_req == 0 && cands.empty())) &&
+                          ensure_partition(h, std::max(h->part_req, 0));
+        if (part) {
+            run_partitioned(h, B, h->bt_init.p, init_per_replica ? nd : 0, h->bt_tp.p, h->bt_amp.p, grid, dt,
+                            save_interval, energy != nullptr, final_states != nullptr, nullptr);
+            copy_back(final_states, h->bt_final.p, B * nd, s);
+        } else if (!cands.empty() && block >= 0) {
             TuneVal tv = pick_persistent(h, B, block);
             run_persistent(h, B, h->bt_init.p, init_per_replica ? nd : 0, h->bt_tp.p, h->bt_amp.p, grid, dt,
                            save_interval, energy != nullptr, final_states != nullptr, tv, nullptr);
@@ -2282,6 +2553,31 @@ int csl_set_batch_split(csl_lattice* h, int enable) {
     return guarded([&] {
         check_handle(h);
         h->split_batches = enable ? 1 : 0;
+    });
+}
+
+int csl_set_partition(csl_lattice* h, int parts) {
+    return guarded([&] {
+        check_handle(h);
+        if (parts < -1) fail(CSL_ERR_ARG, "parts must be >= -1");
+        if (parts != h->part_req) h->part.P = 0;  // rebuild the plan for the new request
+        if (parts > 0) {
+            if (!partition_eligible(h))
+                fail(CSL_ERR_UNSUPPORTED, "the partitioned kernel needs translation-invariant tables");
+            if (!ensure_partition(h, parts))
+                fail(CSL_ERR_UNSUPPORTED, "the lattice does not split into " + std::to_string(parts) +
+                                              " parts that fit one workgroup each");
+        }
+        h->part_req = parts;
+    });
+}
+
+int csl_last_partition(csl_lattice* h, int* parts, int* groups) {
+    return guarded([&] {
+        check_handle(h);
+        const bool was = (h->last_variant & csl::kVariantPartitioned) != 0;
+        if (parts) *parts = was ? h->last_parts : 0;
+        if (groups) *groups = was ? h->last_groups : 0;
     });
 }
 

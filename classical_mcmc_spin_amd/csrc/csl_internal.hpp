@@ -158,6 +158,27 @@ __device__ __forceinline__ void ll_su2(const double H[3], const double S[3], dou
     k[2] = dz;
 }
 
+// ll_su2 for a site that may be a padding slot with a zero spin: the Gilbert
+// factor 1/|S| is taken as 0 there instead of inf (same bits for |S| > 0)
+__device__ __forceinline__ void ll_su2_guarded(const double H[3], const double S[3], double alpha, double k[3]) {
+    double dx = H[1] * S[2] - H[2] * S[1];
+    double dy = H[2] * S[0] - H[0] * S[2];
+    double dz = H[0] * S[1] - H[1] * S[0];
+    if (alpha > 0.0) {
+        const double a = S[1] * dz - S[2] * dy;
+        const double b = S[2] * dx - S[0] * dz;
+        const double c = S[0] * dy - S[1] * dx;
+        const double n2 = S[0] * S[0] + S[1] * S[1] + S[2] * S[2];
+        const double inv_S = n2 > 0.0 ? 1.0 / sqrt(n2) : 0.0;
+        dx -= alpha * inv_S * a;
+        dy -= alpha * inv_S * b;
+        dz -= alpha * inv_S * c;
+    }
+    k[0] = dx;
+    k[1] = dy;
+    k[2] = dz;
+}
+
 // Sum of a double over the 64 lanes of a wavefront (result valid in every lane).
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll

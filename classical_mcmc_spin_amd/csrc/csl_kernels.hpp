@@ -45,6 +45,66 @@ struct PersistArgs {
     const size_t* init_off;
 };
 
+// Partitioned persistent batched integration (csl_partition.hip), for
+// lattices whose state does not fit one workgroup's LDS: each trajectory is
+// spread over P workgroups ("parts", a contiguous range of c_per cells each,
+// in the reference's cell order); a part keeps the RK4 state of its own
+// sites in VGPRs and the stage values of its own sites plus a halo of the
+// remote neighbours in LDS.  Every stage a part writes the new values of
+// the sites other parts read into a global exchange buffer (two parities,
+// device site order) and publishes a version number; before a stage it
+// waits until every part it exchanges with has published that version.
+// The launch holds `groups` groups of P co-resident parts; group g runs
+// trajectories g, g + groups, ...
+struct PartArgs {
+    int B;
+    const double* init;        // AoS reference order; trajectory b starts at init + b*init_stride
+    size_t init_stride;
+    const double* t_pulse;     // [B*2]
+    const double* amp;         // [B*2]
+    const double* tgrid;       // [n_steps+1]
+    int n_steps;
+    int save_interval;
+    int n_rec;
+    double dt;
+    size_t mats_stride;        // per-trajectory Hamiltonians (0 = shared tables)
+    size_t field_stride;
+    int P;                     // parts per trajectory
+    int groups;                // trajectory groups (grid = groups * P)
+    int xcd_map;               // 1: the parts of a group are placed on one XCD
+    int c_per;                 // cells per part
+    int cpad;                  // cells per sublattice block of a part, multiple of 64
+    int n_loc;                 // n_atoms * cpad local slots (own sites)
+    int n_halo_max;            // halo slots per part (LDS slots n_loc ..)
+    int n_nbr_max;             // parts a part exchanges with, at most 64
+    int dbuf;                  // 1: two LDS buffers of the own stage values (no barrier between
+                               // a stage's reads and writes)
+    const unsigned* col;       // [P][(zmax+1)/2][n_loc] LDS slots of the bond partners, two 16-bit
+                               // slots per word (slot 2j in the low half)
+    const int* halo_src;       // [P][n_halo_max] device index of each halo site
+    const int* n_halo;         // [P]
+    const unsigned* exported;  // [P][n_loc] 1 where another part reads the site
+    const int* nbr;            // [P][n_nbr_max] parts exchanged with, -1 unused
+    double* exch;              // [groups][2][NP][3]
+    unsigned* ver;             // [groups*P] versions published, zero before the launch
+    unsigned* status;          // set to 1 when a wait gave up
+    double* part_obs;          // [B][n_rec][P][10] observable sums of each part
+    double* final_states;      // [B][N*3] AoS or null
+    int want_energy;
+};
+// Sites per thread values of the partitioned kernel and the largest block of each.
+bool partitioned_spt_supported(int spt);
+int partitioned_block_limit(int spt);
+size_t partitioned_lds_bytes(int n_loc, int n_halo_max, int block, bool dbuf);
+// Launches the partitioned kernel, or with occupancy != null only stores its
+// co-resident workgroups per CU.
+hipError_t launch_partitioned(const LatDev& L, const PartArgs& A, int block, int spt, hipStream_t st,
+                              int* occupancy);
+// mags[(ri*B + b)*9 + j] = inv_n * sum_p part_obs[((b*n_rec + ri)*P + p)*10 + j] (j < 9), energy
+// likewise from j = 9 (either output may be null).
+hipError_t launch_part_reduce(const double* part_obs, int P, int B, int n_rec, double inv_n, double* mags,
+                              double* energy, hipStream_t st);
+
 // Discrete-adjoint gradient run (csl_adjoint.hip): one workgroup per trajectory.
 struct AdjArgs {
     int B;
@@ -166,6 +226,7 @@ constexpr int kPersistMulti = 4;         // per-trajectory lattices (PersistArgs
 // waves_per_atom * 64).
 constexpr int kVariantFieldOnly = 1;
 constexpr int kVariantAtomMajor = 2;
+constexpr int kVariantPartitioned = 4;  // reported for launches of the partitioned kernel
 inline int zmax_bucket(int zmax) { return zmax <= 3 ? 3 : (zmax <= 6 ? 6 : 12); }
 inline int persistent_variant(const LatDev& L, int block, int spt, int flags) {
     if (flags & kPersistMulti) return 0;

@@ -244,6 +244,19 @@ class GPULattice:
         sequence from a hipGraph (lattices larger than one workgroup)."""
         check(lib().csl_set_stage_graph(self.h, int(bool(enable))))
 
+    def set_partition(self, parts: int):
+        """csl_set_partition: 0 = the partitioned kernel for lattices larger
+        than one workgroup's LDS (default), P > 0 = always with P parts,
+        -1 = never (grid-wide stage kernels)."""
+        check(lib().csl_set_partition(self.h, int(parts)))
+
+    def last_partition(self):
+        """(parts per trajectory, trajectory groups) of the last batched
+        launch; (0, 0) when it did not use the partitioned kernel."""
+        p, g = C.c_int(), C.c_int()
+        check(lib().csl_last_partition(self.h, C.byref(p), C.byref(g)))
+        return p.value, g.value
+
     def set_batch_split(self, enable: bool):
         """csl_set_batch_split: share the trajectory-steps of a batch larger
         than the co-resident workgroups evenly over them (default on)."""
@@ -260,7 +273,8 @@ class GPULattice:
         check(lib().csl_last_launch_variant(self.h, C.byref(v)))
         return {"block": b.value, "sites_per_thread": s.value, "double_buffer": bool(d.value & 1),
                 "unrolled_stages": bool(d.value & 2), "flags": d.value,
-                "field_only": bool(v.value & 1), "atom_major": bool(v.value & 2)}
+                "field_only": bool(v.value & 1), "atom_major": bool(v.value & 2),
+                "partitioned": bool(v.value & 4)}
 
     def launch_candidates(self):
         """csl_persistent_candidates: every persistent launch shape that fits

@@ -387,6 +387,21 @@ int csl_set_stage_graph(csl_lattice* h, int enable);
  * on this handle. */
 int csl_last_launch_variant(csl_lattice* h, int* variant);
 
+/* Partitioned persistent kernel for batched RK4 (csl_integrate_batched with
+ * block = 0): each trajectory is spread over P co-resident workgroups, each
+ * holding a contiguous range of cells in LDS, with the halo values exchanged
+ * through global memory once per RK stage.  No reference counterpart (the
+ * reference launches one kernel per stage, lattice_gpu.cu:502-565).
+ * parts = 0 (default): used for lattices whose state does not fit one
+ * workgroup, with the fewest parts that fit; parts > 0: always, with that
+ * many parts (CSL_ERR_UNSUPPORTED when they do not fit); parts = -1: never
+ * (grid-wide stage kernels).  Needs translation-invariant bond tables and
+ * per-sublattice fields; other lattices take the stage kernels. */
+int csl_set_partition(csl_lattice* h, int parts);
+/* Parts per trajectory and trajectory groups of the last batched launch
+ * (0, 0 when it did not use the partitioned kernel). */
+int csl_last_partition(csl_lattice* h, int* parts, int* groups);
+
 /* Work split of the one-workgroup-per-trajectory kernel (on by default):
  * when a batch holds more trajectories than workgroups fit the GPU at once,
  * that many workgroups share the trajectory-steps evenly, a trajectory cut

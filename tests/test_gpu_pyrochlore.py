@@ -1,7 +1,8 @@
 """GPU parity of the pyrochlore dynamics (6 neighbours per site, 16-site
-conventional cell) against the CPU oracle, on both device paths: one
-workgroup per trajectory with the lattice in LDS (block = 0) and the
-grid-wide stage kernels (block = -1; the path the 65 536-site lattice takes).
+conventional cell) against the CPU oracle, on the device paths: one
+workgroup per trajectory with the lattice in LDS (block = 0), the grid-wide
+stage kernels (block = -1) and, for the 65 536-site lattice that does not
+fit one workgroup, the partitioned kernel (its default) and the stage kernels.
 
 Bar: max|dM|, |dE|, |dS| <= 1e-10 over <= 500 RK4 steps.  At the full
 65 536-site / 64-sample / 4096-step size: two samples against the oracle over
@@ -50,16 +51,27 @@ def test_pyrochlore_small_both_paths(po, block):
         assert np.max(np.abs(res["final"][b] - ref["final"].reshape(-1, 3))) <= TOL
 
 
-def test_pyrochlore_full_size_vs_oracle(po):
+@pytest.mark.parametrize("path", ["partitioned", "stages"])
+def test_pyrochlore_full_size_vs_oracle(po, path):
+    """The 65 536-site lattice does not fit one workgroup's LDS: by default it
+    runs on the partitioned kernel; parts = -1 selects the stage kernels."""
     lat = workloads.pyrochlore_lattice(16)
     assert lat.lattice_size == 65536
     B = 64
     st, _ = workloads.pyrochlore_tilted_aiao(lat, B, seed=1234)
     T0, dt, n = 0.0, 0.02, 300
     with GPULattice(lat) as gl:
+        if path == "stages":
+            gl.set_partition(-1)
+        print(f"[pyro-full] {path}, B {B}, {n} steps", flush=True)
         res = gl.integrate_batched(st, np.zeros((B, 2)), np.zeros((B, 2)), T0, _t1(T0, n, dt), dt,
                                    tgrid="boost", want_final=True)
-        assert gl.last_launch_config()["block"] == -1      # too large for one workgroup's LDS
+        if path == "stages":
+            assert gl.last_launch_config()["block"] == -1
+            assert gl.last_partition() == (0, 0)
+        else:
+            assert gl.last_launch_config()["partitioned"]
+            assert gl.last_partition()[0] > 1
     o = po.OracleLattice(lat.unit_cell, 16, 16, 16)
     for b in (0, B - 1):
         ref = o.integrate_md(st[b].reshape(-1), T0, _t1(T0, n, dt), dt)
