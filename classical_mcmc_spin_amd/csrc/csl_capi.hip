@@ -110,6 +110,9 @@ struct csl_lattice {
     DBuf<double> part_exch, part_obs;
     DBuf<unsigned> part_ver;
     int last_parts = 0, last_groups = 0;
+    // adjoint work buffers (stage inputs, field cotangents, checkpoints, ...)
+    DBuf<double> adj_Y, adj_U, adj_ckpt, adj_lam, adj_lossn, adj_part;
+    hipStream_t stream2 = nullptr;  // second stream of the adjoint (parameter terms)
     int stage_graph = 0;          // batched stage-kernel path replayed from a hipGraph
     int last_grid = 0;
     DBuf<long long> dp_counters;  // [B][4]
@@ -131,6 +134,7 @@ struct csl_lattice {
     std::vector<unsigned char> graph_key;
     ~csl_lattice() {
         if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
+        if (stream2) (void)hipStreamDestroy(stream2);
         if (stream) (void)hipStreamDestroy(stream);
     }
 };
@@ -2259,7 +2263,7 @@ int csl_adjoint_gradient(csl_lattice* h, size_t B, const double* init, int init_
             }
         }
         hipStream_t s = h->stream;
-        DBuf<double> d_dm, d_df, d_tgt, d_ckpt, d_seg, d_rec, d_loss, d_grad, d_gi;
+        DBuf<double> d_dm, d_df, d_tgt, d_rec, d_loss, d_grad, d_gi;
         d_dm.upload(dm.data(), dm.size(), s);
         d_df.upload(dfl.data(), dfl.size(), s);
         d_tgt.upload(target, n_rec * 9, s);
@@ -2280,20 +2284,35 @@ int csl_adjoint_gradient(csl_lattice* h, size_t B, const double* init, int init_
                                               L.freq, h->fac.p, s),
                       "drive table");
         }
-        const size_t C = ckpt_interval ? ckpt_interval : size_t(std::ceil(std::sqrt(double(n_steps))));
-        const size_t n_ckpt = (n_steps + C - 1) / C;
+        // Stage inputs of every step in device memory when they fit (and no
+        // checkpoint spacing is asked for); otherwise checkpoints every C
+        // steps and one segment of stage inputs, re-integrated per segment.
         const size_t SZ = size_t(3) * L.NP;
-        d_ckpt.ensure(B * n_ckpt * SZ);
-        d_seg.ensure(B * C * SZ);
+        const size_t per_seg_step = 2 * B * 4 * SZ * sizeof(double);  // Y and U
+        size_t free_b = 0, total_b = 0;
+        hip_check(hipMemGetInfo(&free_b, &total_b), "hipMemGetInfo");
+        const size_t budget = std::min<size_t>(free_b / 2, size_t(48) << 30);
+        const bool full = ckpt_interval == 0 && per_seg_step * n_steps <= budget;
+        const size_t C = full ? n_steps
+                              : (ckpt_interval ? std::min(ckpt_interval, n_steps)
+                                               : size_t(std::ceil(std::sqrt(double(n_steps)))));
+        const size_t n_seg = (n_steps + C - 1) / C;
+        // kept on the handle between calls (a fit calls this every iteration)
+        DBuf<double>&d_Y = h->adj_Y, &d_U = h->adj_U, &d_lam = h->adj_lam, &d_lossn = h->adj_lossn,
+                     &d_part = h->adj_part, &d_ckpt = h->adj_ckpt;
+        d_Y.ensure(B * C * 4 * SZ);
+        d_U.ensure(B * C * 4 * SZ);
+        if (!full) d_ckpt.ensure(B * n_seg * SZ);
+        d_lam.ensure(B * SZ);
         d_rec.ensure(n_rec * B * 9);
+        d_lossn.ensure(B * n_rec);
+        d_part.ensure(B * n_steps * 4 * std::max<size_t>(n_params, 1));
         d_loss.ensure(B);
         d_grad.ensure(B * std::max<size_t>(n_params, 1));
         if (grad_init) d_gi.ensure(B * nd);
         csl::AdjArgs Aa{};
         Aa.B = int(B);
         Aa.n_steps = int(n_steps);
-        Aa.ckpt_interval = int(C);
-        Aa.n_ckpt = int(n_ckpt);
         Aa.P = int(n_params);
         Aa.dt = dt;
         Aa.init = h->bt_init.p;
@@ -2303,12 +2322,18 @@ int csl_adjoint_gradient(csl_lattice* h, size_t B, const double* init, int init_
         for (int j = 0; j < 9; ++j) Aa.w[j] = weights[j];
         Aa.dmats = d_dm.p;
         Aa.dfield = d_df.p;
+        Aa.seg_len = int(C);
+        Aa.n_ckpt = int(n_seg);
         Aa.ckpt = d_ckpt.p;
-        Aa.seg = d_seg.p;
+        Aa.Y = d_Y.p;
+        Aa.U = d_U.p;
+        Aa.lam = d_lam.p;
         Aa.rec = d_rec.p;
+        Aa.loss_n = d_lossn.p;
+        Aa.part = d_part.p;
+        Aa.grad_init = grad_init ? d_gi.p : nullptr;
         Aa.loss = d_loss.p;
         Aa.grad = d_grad.p;
-        Aa.grad_init = grad_init ? d_gi.p : nullptr;
         if (h->rep_B) {
             L.mats = h->rep_mats.p;
             L.field = h->rep_field.p;
@@ -2319,8 +2344,75 @@ int csl_adjoint_gradient(csl_lattice* h, size_t B, const double* init, int init_
         hip_check(hipEventCreate(&e0), "event");
         hip_check(hipEventCreate(&e1), "event");
         hip_check(hipEventRecord(e0, s), "event record");
-        hip_check(csl::launch_adjoint(L, Aa, s), "k_adjoint launch");
-        hip_check(hipEventRecord(e1, s), "event record");
+        size_t launches = 0;
+        {   // forward: records, and the stage inputs (full) or the checkpoints
+            csl::AdjArgs F = Aa;
+            F.n0 = 0;
+            F.n1 = int(n_steps);
+            F.ckpt_src = -1;
+            F.record = 1;
+            F.ckpt_interval = full ? 0 : int(C);
+            F.store_stages = full ? 1 : 0;
+            F.y0 = 0;
+            hip_check(csl::launch_adjoint_forward(L, F, s), "k_adj_fwd launch");
+            ++launches;
+        }
+        if (full) {
+            // the backward sweep in segments; the parameter terms of a
+            // finished segment run on the second stream (on the CUs the
+            // sweep leaves idle) while the sweep continues
+            if (!h->stream2) hip_check(hipStreamCreateWithFlags(&h->stream2, hipStreamNonBlocking), "stream");
+            const size_t n_part = std::min<size_t>(16, n_steps);
+            const size_t len = (n_steps + n_part - 1) / n_part;
+            std::vector<hipEvent_t> ev;
+            for (size_t j = n_part; j-- > 0;) {
+                csl::AdjArgs G = Aa;
+                G.n0 = int(std::min(n_steps, j * len));
+                G.n1 = int(std::min(n_steps, (j + 1) * len));
+                if (G.n0 >= G.n1) continue;
+                G.y0 = 0;
+                G.first = G.n1 == int(n_steps) ? 1 : 0;
+                hip_check(csl::launch_adjoint_backward(L, G, s), "k_adj_bwd launch");
+                hipEvent_t e;
+                hip_check(hipEventCreateWithFlags(&e, hipEventDisableTiming), "event");
+                ev.push_back(e);
+                hip_check(hipEventRecord(e, s), "event record");
+                hip_check(hipStreamWaitEvent(h->stream2, e, 0), "stream wait");
+                // few workgroups (one per idle CU for the whole batch), so
+                // that the next segment's sweep still finds room on the GPU
+                const int bpt = std::max(1, int((h->n_cu > int(B) ? h->n_cu - int(B) : 1) / int(B)));
+                hip_check(csl::launch_adjoint_params(L, G, bpt, h->stream2), "k_adj_par launch");
+                launches += 2;
+            }
+            hipEvent_t done;
+            hip_check(hipEventCreateWithFlags(&done, hipEventDisableTiming), "event");
+            ev.push_back(done);
+            hip_check(hipEventRecord(done, h->stream2), "event record");
+            hip_check(hipStreamWaitEvent(s, done, 0), "stream wait");
+            hip_check(csl::launch_adjoint_final(Aa, s), "k_adj_final launch");
+            hip_check(hipEventRecord(e1, s), "event record");
+            hip_check(hipEventSynchronize(e1), "adjoint");
+            for (hipEvent_t e : ev) (void)hipEventDestroy(e);
+        } else {
+            for (size_t j = n_seg; j-- > 0;) {
+                csl::AdjArgs G = Aa;
+                G.n0 = int(j * C);
+                G.n1 = int(std::min(n_steps, (j + 1) * C));
+                G.y0 = G.n0;
+                G.ckpt_src = int(j);
+                G.record = 0;
+                G.ckpt_interval = 0;
+                G.store_stages = 1;
+                hip_check(csl::launch_adjoint_forward(L, G, s), "k_adj_fwd launch");
+                G.first = j == n_seg - 1 ? 1 : 0;
+                hip_check(csl::launch_adjoint_backward(L, G, s), "k_adj_bwd launch");
+                hip_check(csl::launch_adjoint_params(L, G, 2048, s), "k_adj_par launch");
+                launches += 3;
+            }
+            hip_check(csl::launch_adjoint_final(Aa, s), "k_adj_final launch");
+            hip_check(hipEventRecord(e1, s), "event record");
+        }
+        ++launches;
         copy_back(loss, d_loss.p, B, s);
         copy_back(grad, d_grad.p, B * n_params, s);
         copy_back(records, d_rec.p, n_rec * B * 9, s);
@@ -2331,7 +2423,7 @@ int csl_adjoint_gradient(csl_lattice* h, size_t B, const double* init, int init_
         (void)hipEventDestroy(e0);
         (void)hipEventDestroy(e1);
         h->last_ms = ms;
-        h->last_launches = 1;
+        h->last_launches = launches;
     });
 }
 

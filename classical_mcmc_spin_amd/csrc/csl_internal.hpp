@@ -40,6 +40,16 @@ __device__ __forceinline__ cptr<T> as_const(const T* p) {
 // Force a value the compiler cannot prove wave-uniform into an SGPR.
 __device__ __forceinline__ int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
 
+// A per-lane index the compiler must treat as changed at this point: the
+// LDS and global addresses derived from it are recomputed where they are
+// used (a shift and an add) instead of being hoisted out of the time loop
+// and held in VGPRs (64-bit global addresses of every stage slot) for the
+// whole kernel, which spilled them to scratch.
+__device__ __forceinline__ int fresh(int v) {
+    asm volatile("" : "+v"(v));
+    return v;
+}
+
 struct LatDev {
     int N;          // real sites
     int n_atoms;

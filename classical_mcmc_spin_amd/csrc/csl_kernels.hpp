@@ -105,12 +105,16 @@ hipError_t launch_partitioned(const LatDev& L, const PartArgs& A, int block, int
 hipError_t launch_part_reduce(const double* part_obs, int P, int B, int n_rec, double inv_n, double* mags,
                               double* energy, hipStream_t st);
 
-// Discrete-adjoint gradient run (csl_adjoint.hip): one workgroup per trajectory.
+// Discrete-adjoint gradient run (csl_adjoint.hip).  Four kernels: the
+// forward integration (one workgroup per trajectory: records, checkpoints
+// and/or the stage inputs of every step), the backward sweep over a segment
+// of steps (one workgroup per trajectory: the transposed RK4 stages, storing
+// the cotangent u of every site's field at every stage), the parameter
+// gradient terms (all CUs: sum_i u_i . dH_i/dtheta at every stage of the
+// segment) and the final sums in a fixed order.
 struct AdjArgs {
     int B;
     int n_steps;
-    int ckpt_interval;         // steps per checkpoint segment
-    int n_ckpt;                // ceil(n_steps / ckpt_interval)
     int P;                     // parameters (<= 8)
     double dt;
     const double* init;        // AoS reference order; trajectory b at init + b*init_stride
@@ -120,15 +124,39 @@ struct AdjArgs {
     double w[9];               // loss weights of the 9 record components
     const double* dmats;       // [P][n_atoms][zmax][9] dJ/dtheta per (sublattice, slot)
     const double* dfield;      // [P][n_atoms][3]      dB/dtheta per sublattice
-    double* ckpt;              // [B][n_ckpt][3*NP] checkpoint states (SoA)
-    double* seg;               // [B][ckpt_interval][3*NP] states of one segment
-    double* rec;               // [n_steps+1][B][9] forward records
-    double* loss;              // [B]
-    double* grad;              // [B][P]
-    double* grad_init;         // [B][N*3] dL/dx_0 (AoS) or null
     size_t mats_stride;        // per-trajectory Hamiltonians (0 = shared)
     size_t field_stride;
+    int n0, n1;                // the launch covers steps [n0, n1)
+    int y0;                    // first step held in Y and U
+    int seg_len;               // steps held per trajectory in Y and U
+    // forward launches
+    int ckpt_src;              // >= 0: start from checkpoint ckpt_src (state of step n0); -1: from init
+    int record;                // write the records of steps n0 .. n1
+    int ckpt_interval;         // store x_n every ckpt_interval steps (0: none)
+    int n_ckpt;
+    double* ckpt;              // [B][n_ckpt][3*NP] checkpoint states (SoA)
+    int store_stages;          // store the stage inputs of every step into Y
+    double* Y;                 // [B][seg_len][4][3*NP] stage inputs y_1..y_4 of steps n0.. (SoA)
+    double* U;                 // [B][seg_len][4][3*NP] cotangents u of the fields at those stages
+    // backward launches
+    double* lam;               // [B][3*NP] adjoint state handed between backward launches (SoA)
+    int first;                 // the launch holds the last step: lambda starts from dl/dx_N
+    double* rec;               // [n_steps+1][B][9] forward records
+    double* loss_n;            // [B][n_steps+1] loss term of every record
+    double* part;              // [B][n_steps][4][P] parameter-gradient term of every stage
+    double* grad_init;         // [B][N*3] dL/dx_0 (AoS), written by the launch with n0 == 0, or null
+    double* loss;              // [B] (final kernel)
+    double* grad;              // [B][P] (final kernel)
 };
+
+size_t adjoint_lds_bytes(const LatDev& L, int block);
+int adjoint_sites_per_thread(const LatDev& L);
+hipError_t launch_adjoint_forward(const LatDev& L, const AdjArgs& A, hipStream_t st);
+hipError_t launch_adjoint_backward(const LatDev& L, const AdjArgs& A, hipStream_t st);
+// blocks_per_traj: workgroups per trajectory (each loops over its share of
+// the segment's stage slots)
+hipError_t launch_adjoint_params(const LatDev& L, const AdjArgs& A, int blocks_per_traj, hipStream_t st);
+hipError_t launch_adjoint_final(const AdjArgs& A, hipStream_t st);
 
 // Dormand-Prince 5(4) integration (csl_dopri5.hip): one workgroup per trajectory.
 struct Dopri5Args {
@@ -182,9 +210,6 @@ hipError_t launch_sq_combine(const double2* F, int B, int n_atoms, int L1, int L
                              double2* sqt, hipStream_t st);
 hipError_t launch_sq_power(const double2* sw, int T, int B, size_t Q, double scale, double* out, hipStream_t st);
 
-size_t adjoint_lds_bytes(const LatDev& L, int block);
-int adjoint_sites_per_thread(const LatDev& L);
-hipError_t launch_adjoint(const LatDev& L, const AdjArgs& A, hipStream_t st);
 
 // One RK4 stage over all sites of B trajectories (SoA [B][3][NP] buffers).
 struct StageArgs {
