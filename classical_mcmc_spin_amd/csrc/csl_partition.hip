@@ -69,7 +69,7 @@ __device__ __forceinline__ void part_field(const LatDev& L, const LdsState& Y, i
 #pragma unroll
     for (int k = 0; k < ZMAX; ++k) {
         if (k < cnt_a) {
-            const int q = int((nbp[k >> 1] >> (16 * (k & 1))) & 0xffffu);
+            const int q = int((unsigned(fresh(int(nbp[k >> 1]))) >> (16 * (k & 1))) & 0xffffu);
             double Sp[3];
             Y.load(slot(q), Sp);
             add_bond_fused(as_const(L.mats) + (atom * L.zmax + k) * 9, Sp, H);
@@ -186,10 +186,10 @@ __global__ void __launch_bounds__(part_max_block<SPT>()) k_partitioned(LatDev L0
     // where another part reads them; then publish v+1
     // (two LDS buffers: the values were stored by put() during the stage)
     auto put = [&](int m, int buf, const double y[3]) {
-        Y.store(buf * A.n_loc + tid + m * T, y);
+        Y.store(fresh(buf * A.n_loc + tid + m * T), y);
         if ((expm >> m) & 1u) {
             double* dst = exch + size_t((v + 1u) & 1u) * 3 * size_t(NP);
-            const size_t d = size_t(atm[m]) * L0.npad + cell_of(m);
+            const size_t d = size_t(fresh(atm[m] * L0.npad + cell_of(m)));
             dst[3 * d] = y[0];
             dst[3 * d + 1] = y[1];
             dst[3 * d + 2] = y[2];
@@ -318,7 +318,7 @@ __global__ void __launch_bounds__(part_max_block<SPT>()) k_partitioned(LatDev L0
                     S[1] = x[m][1];
                     S[2] = x[m][2];
                 } else {
-                    Y.load(cur * n_loc + tid + m * T, S);
+                    Y.load(fresh(cur * n_loc + tid + m * T), S);
                 }
                 part_field<ZMAX>(L, Y, atm[m], cnt[m], nbp[m], slot, S, H);
                 e[m] = (obs && want_e) ? site_energy(L, atm[m], 0, S, H) : 0.0;
