@@ -109,8 +109,13 @@ class KitaevFit:
         """(loss [B], grad [B, 5]) at the parameter rows of theta."""
         theta = np.atleast_2d(f64(theta))
         B = theta.shape[0]
-        lats = [kitaev_lattice(t, self.L1, self.L2, self.fd) for t in theta]
-        self.gl.set_replica_hamiltonians(lats)
+        # the Hamiltonian is linear in theta: the per-start bond matrices and
+        # fields are theta . (dJ, dB) -- the same numbers the builder gives
+        # (every entry is a sum of parameters times 0, +-1 or a fixed factor)
+        # without building a lattice per start
+        J = np.tensordot(theta, self.dJ, 1)
+        field = np.tensordot(theta, self.dB, 1)
+        self.gl.set_replica_hamiltonians(J=J, field=field)
         try:
             r = adjoint_gradient(self.gl, self.init, np.tile([self.tc, 0.0], (B, 1)),
                                  np.tile([self.amp, 0.0], (B, 1)), self.T0, self.T1, self.dt, self.target,

@@ -396,7 +396,11 @@ int csl_last_launch_variant(csl_lattice* h, int* variant);
  * workgroup, with the fewest parts that fit; parts > 0: always, with that
  * many parts (CSL_ERR_UNSUPPORTED when they do not fit); parts = -1: never
  * (grid-wide stage kernels).  Needs translation-invariant bond tables and
- * per-sublattice fields; other lattices take the stage kernels. */
+ * per-sublattice fields; other lattices take the stage kernels.  The parts
+ * of a trajectory wait for each other every stage, so the launch assumes its
+ * workgroups are co-resident (it is sized to fit the GPU at once); other
+ * work running on the device at the same time can delay a part beyond the
+ * bounded wait, and the call then fails with CSL_ERR_HIP. */
 int csl_set_partition(csl_lattice* h, int parts);
 /* Parts per trajectory and trajectory groups of the last batched launch
  * (0, 0 when it did not use the partitioned kernel). */

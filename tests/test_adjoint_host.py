@@ -28,6 +28,18 @@ def test_kitaev_tangents_are_linear_derivatives():
     assert np.array_equal(lat.bi_partner, lat2.bi_partner)
 
 
+def test_fit_tables_equal_the_builders():
+    """KitaevFit forms the per-start tables as theta . (dJ, dB); for the
+    Kitaev parametrisation that is bit for bit what the builder gives."""
+    dJ, dB = AD.kitaev_tangents(6, 5)
+    rng = np.random.default_rng(12)
+    star = np.array([-1.0, 0.25, -0.02, 0.0, 0.05])
+    for theta in star * (1.0 + 0.2 * rng.uniform(-1, 1, size=(40, 5))):
+        lat = AD.kitaev_lattice(theta, 6, 5)
+        assert np.array_equal(np.tensordot(theta, dJ, 1), lat.bi_J)
+        assert np.array_equal(np.tensordot(theta, dB, 1), lat.field)
+
+
 def test_oracle_tangent_matches_finite_differences(po):
     """dm/dK of the damped, driven trajectory from the oracle tangent-linear
     model vs central differences of the oracle trajectory (step 1e-6)."""
