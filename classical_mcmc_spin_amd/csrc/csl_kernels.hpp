@@ -85,7 +85,7 @@ struct PartArgs {
     const int* n_halo;         // [P]
     const unsigned* exported;  // [P][n_loc] 1 where another part reads the site
     const int* nbr;            // [P][n_nbr_max] parts exchanged with, -1 unused
-    double* exch;              // [groups][2][NP][3]
+    double* exch;              // [groups][2][3][NP]
     unsigned* ver;             // [groups*P] versions published, zero before the launch
     unsigned* status;          // set to 1 when a wait gave up
     double* part_obs;          // [B][n_rec][P][10] observable sums of each part
