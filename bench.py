@@ -500,6 +500,10 @@ ADJ_L, ADJ_STEPS, ADJ_B, ADJ_B_FULL = 24, 2000, 32, 256
 # gather 54, five parameter derivatives u . (sum dJ S - dB) 310,
 # combinations 10; plus the loss term 3).
 ADJ_FLOPS_PER_SITE_STEP = 465 + 2119
+# mixed TmFeO3 cell, per (SU(2) + SU(3)) site pair and RK4 step: SU(2) site
+# 2A S 18, 6 SU(2) bonds 108, 8 SU(3) bonds 384, drive 12, H x S 9, update 12
+# = 543; SU(3) site 8 SU(2) bonds 384, drive 32, f_abc 108, update 32 = 556
+MIX_FLOPS_PAIR_STEP = 4 * (543 + 556)
 
 
 def bench_adjoint(args):
@@ -641,8 +645,11 @@ def bench_mixed(args):
     + 2048 Tm SU(3) sites), pulses A = 0.5, w = 0.5 along x (Fe) and
     lambda_2 (Tm) at 0 and tau = 0, 0.05, ..., 6.35 (128 delays):
     M0 + 128 M1 + 128 M01 = 257 trajectories x 8192 RK4 steps (dt = 0.005),
-    records every step; then d loss / d (Fe-Tm scale) for every trajectory.
-    Error: two trajectories over 200 steps against the mixed oracle."""
+    records every step.  Default path: the persistent kernel, one workgroup
+    per trajectory on the co-resident workgroups, the one trajectory beyond
+    them on the stage kernels; the grid-wide stage kernels alone are timed
+    beside it.  Then d loss / d (Fe-Tm scale) for every trajectory.  Error:
+    two trajectories over 200 steps against the mixed oracle."""
     from classical_mcmc_spin_amd import gpu_mixed as GM
     muc, ml, g, taus, B, tp, sc, f2, f3 = _mixed_setup()
     T0m = -1.0
@@ -650,19 +657,34 @@ def bench_mixed(args):
     gm = GM.GPUMixedLattice(ml)
     GM._set_pulses(gm, f2, f2, f3, f3, 0.0, 0.0, AMP, WIDTH, 0.0, AMP, WIDTH, 0.0)
     one_step = lambda: gm.integrate_batched(g, tp, sc, T0m, T1m, MIX_DT)
+
+    def timed(steps):
+        paths, kernel_ms = [], []
+        t = time.perf_counter()
+        for _ in range(steps):
+            res = one_step()
+            paths.append(gm.last_path())
+            kernel_ms.append(gm.last_kernel_ms())
+        el = time.perf_counter() - t
+        assert res["records"].shape[0] == MIX_STEPS + 1 and np.all(np.isfinite(res["records"]))
+        return el, res, paths, kernel_ms
+
     for _ in range(args.warmup):
         one_step()
-    kernel_ms = []
-    t = time.perf_counter()
-    for _ in range(args.steps):
-        res = one_step()
-        kernel_ms.append(gm.last_kernel_ms()[0])
-    el = time.perf_counter() - t
-    assert res["records"].shape[0] == MIX_STEPS + 1 and np.all(np.isfinite(res["records"]))
-    kms = float(np.mean(kernel_ms))
+    el, res, paths, kernel_ms = timed(args.steps)
+    path = paths[-1]
+    kms = float(np.mean([p["kernel_ms"] for p in paths]))   # the persistent launch alone
+    kms_all = float(np.mean([k[0] for k in kernel_ms]))      # with the tail on the stage kernels
     if args.profile:
         gm.close()
-        return {"profile": True, "workload": "mixed", "kernel_ms": kms, "ms_per_step": 1e3 * el / args.steps}
+        return {"profile": True, "workload": "mixed", "kernel_ms": kms, "integration_ms": kms_all,
+                "ms_per_step": 1e3 * el / args.steps, "path": path}
+    # the grid-wide stage kernels on the same batch
+    gm.set_persistent(-1)
+    one_step()
+    el_st, _, _, kernel_ms_st = timed(args.steps)
+    kms_st = float(np.mean([k[0] for k in kernel_ms_st]))
+    gm.set_persistent(0)
     target = res["records"][:, 0] * 1.01
     w = np.r_[np.ones(33), 0.0]
     t = time.perf_counter()
@@ -686,17 +708,27 @@ def bench_mixed(args):
         err["state"] = max(err["state"], float(np.max(np.abs(pr["final"][j] - ref["final"]))))
     traj_s = B * args.steps / el
     n_sites = ml.N2 + ml.N3
-    bytes_launch = B * (ml.N2 * 3 + ml.N3 * 8) * 2 * 8   # one stage: read + write of every component
-    roof = hbm_roofline("k_mixed_stage", bytes_launch, kms)
-    roof["traffic"] = profile_number("traffic_l2_k_mixed_stage_tmfeo3_8x8x8.txt", "traffic_bytes_per_launch")
+    n_main = B - path["tail"]
+    roof = fp64_roofline("k_mixed_persist", MIX_FLOPS_PAIR_STEP * ml.N2 * n_main * MIX_STEPS, kms,
+                         algorithmic_flops_per_pair_step=MIX_FLOPS_PAIR_STEP, trajectories_in_launch=n_main)
+    roof["traffic"] = profile_number("traffic_k_mixed_persist_tmfeo3_8x8x8.txt", "traffic_bytes_per_launch")
+    bytes_stage = B * (ml.N2 * 3 + ml.N3 * 8) * 2 * 8   # one stage: read + write of every component
+    stage_roof = hbm_roofline("k_mixed_stage", bytes_stage, kms_st)
+    stage_roof["traffic"] = profile_number("traffic_l2_k_mixed_stage_tmfeo3_8x8x8.txt", "traffic_bytes_per_launch")
     out = {
         "value": traj_s, "unit": "trajectories/s", "ms_per_step": 1e3 * el / args.steps,
         "config": {"workload": f"2dcs_tmfeo3_mixed_8x8x8_257traj_{MIX_STEPS}steps_rk4_dt0.005",
                    "sites_su2": ml.N2, "sites_su3": ml.N3, "trajectories": B, "rk4_steps": MIX_STEPS,
-                   "record_every": 1},
+                   "record_every": 1, "path": "persistent kernel (one workgroup per trajectory)",
+                   "workgroups": path["workgroups"], "tail_on_stage_kernels": path["tail"]},
         "site_steps_per_s": traj_s * n_sites * MIX_STEPS,
         "max_abs_err_vs_oracle_200_steps": err,
         "roofline": roof,
+        "integration_ms": kms_all,
+        "stage_kernels": {"value": B * args.steps / el_st, "unit": "trajectories/s",
+                          "ms_per_step": 1e3 * el_st / args.steps, "kernel_ms_per_launch": kms_st,
+                          "roofline": stage_roof},
+        "speedup_over_stage_kernels": el_st / el,
         "scale_gradient": {"gradients_per_s": B / el_g, "ms_per_batch": 1e3 * el_g, "kernel_ms_per_launch": kms_g,
                            "forward_plus_tangent_over_forward": el_g / (el / args.steps)},
     }

@@ -111,6 +111,8 @@ SIGNATURES = {
     "csl_mixed_scale_gradient": (_i, [_vp, _sz, _pd, _i, _pd, _pd, _d, _d, _d, _sz, _i, _pd, _i, _pd, _pd, _pd,
                                       _sz, _pd, _pd, _psz]),
     "csl_mixed_last_kernel_ms": (_i, [_vp, _pd, _psz]),
+    "csl_mixed_set_persistent": (_i, [_vp, _i]),
+    "csl_mixed_last_path": (_i, [_vp, _pi, _pi, _psz, _pd]),
 }
 
 

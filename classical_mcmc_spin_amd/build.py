@@ -22,7 +22,7 @@ LIB_PATH = os.path.join(HERE, LIB_NAME)
 # (source, object suffix, extra flags)
 HIP_UNITS = [("csl_kernels.hip", "", []), ("csl_capi.hip", "", []), ("csl_mixed.hip", "", []), ("csl_adjoint.hip", "", []),
              ("csl_dssf.hip", "", []), ("csl_spectrum.hip", "", []), ("csl_quench.hip", "", []),
-             ("csl_dopri5.hip", "", []), ("csl_partition.hip", "", [])] + [
+             ("csl_dopri5.hip", "", []), ("csl_partition.hip", "", []), ("csl_mixed_persist.hip", "", [])] + [
     ("csl_persistent.hip", f"_spt{n}", [f"-DCSL_SPT={n}"]) for n in (1, 2, 3, 4, 6)]
 CXX_UNITS = ["gpu_api_shim.cpp", "mixed_gpu_api_shim.cpp"]
 

@@ -26,38 +26,10 @@
 #include "csl_host.hpp"
 #include "csl_internal.hpp"
 #include "csl_kernels.hpp"
+#include "csl_mixed_device.hpp"
 #include "csl_tableau.hpp"
 
 namespace csl {
-
-// One bond kind on the device (source species gathers partner species).
-struct Ell {
-    const int* __restrict__ col;        // [zmax][NP_src] partner device index
-    const int* __restrict__ cnt;        // [NP_src]
-    const int* __restrict__ cnt_atom;   // [A_src]
-    const double* __restrict__ mats;    // uniform: [A][zmax][m]; else [zmax][NP_src][m]
-    int zmax, uniform;
-};
-
-struct MixDev {
-    int N2, N3, A2, A3, n_cells, npad, NP2, NP3;
-    int site_uniform2, site_uniform3, has_onsite2, has_onsite3;
-    const double* __restrict__ field2;   // [A2][3] or [NP2][3]
-    const double* __restrict__ onsite2;  // 2A: [A2][9] or [NP2][9]
-    const double* __restrict__ field3;   // [A3][8] or [NP3][8]
-    const double* __restrict__ onsite3;  // 2A: [A3][64] or [NP3][64]
-    Ell e22, e23, e33, e32;
-    const double* __restrict__ frames2;  // [A2][9]
-    const double* __restrict__ afm2;     // [A2]
-    const double* __restrict__ frames3;  // [A3][64]
-    const double* __restrict__ fdrive2;  // [2][A2][3]
-    const double* __restrict__ fdrive3;  // [2][A3][8]
-    const double* __restrict__ eq3;      // [8][NP3] or null
-    double amp2, width2, freq2, amp3, width3, freq3;
-    double alpha;
-    double gamma[8];
-    int has_bloch;
-};
 
 struct MixStageArgs {
     int stage;                 // 0..3
@@ -80,8 +52,6 @@ struct MixStageArgs {
     double* znext;             // tangent next stage input
     double* tpartials;         // [B][nblk][34] record partials of z (entry 33 stays 0)
 };
-
-constexpr int kRec = CSL_MIXED_RECORD;
 
 __device__ __forceinline__ size_t mix_stride(const MixDev& M) {
     return size_t(3) * M.NP2 + size_t(8) * M.NP3;
@@ -189,35 +159,6 @@ __device__ __forceinline__ void field_su3(const MixDev& M, const double* __restr
     field_onsite<8>(M.field3, M.onsite3, M.site_uniform3, M.has_onsite3, atom, p, S, H);
     gather_bonds<8, 8>(M.e33, y + size_t(3) * M.NP2, M.NP3, M.NP3, atom, p, H);
     gather_bonds<8, 3>(M.e32, y, M.NP3, M.NP2, atom, p, H);
-}
-
-// d_a = f_abc H_b S_c over the 9 non-zero (a<b<c) structure constants
-// (cross_prod_SU3_flat, simple_linear_alg.h:78-169)
-__device__ __forceinline__ void su3_cross(const double* H, const double* S, double* d) {
-    constexpr double h = 0.5;
-    constexpr double r = 0.8660254037844386;  // sqrt(3)/2
-#pragma unroll
-    for (int i = 0; i < 8; ++i) d[i] = 0.0;
-    auto tri = [&](int a, int b, int c, double v) {
-        d[a] += v * (H[b] * S[c] - H[c] * S[b]);
-        d[b] += v * (H[c] * S[a] - H[a] * S[c]);
-        d[c] += v * (H[a] * S[b] - H[b] * S[a]);
-    };
-    tri(0, 1, 2, 1.0);
-    tri(0, 3, 6, h);
-    tri(0, 4, 5, -h);
-    tri(1, 3, 5, h);
-    tri(1, 4, 6, h);
-    tri(2, 3, 4, h);
-    tri(2, 5, 6, -h);
-    tri(3, 4, 7, r);
-    tri(5, 6, 7, r);
-}
-
-__device__ __forceinline__ double envelope(double amp, double t, double tc, double width, double freq) {
-    const double d = t - tc;
-    const double a = d / (2.0 * width);
-    return amp * exp(-(a * a)) * cos(freq * d);
 }
 
 // Workgroup -> (block, trajectory) with the XCD-contiguous mapping of k_stage.
@@ -585,7 +526,7 @@ __global__ void __launch_bounds__(256) k_mixed_loss_grad(const double* __restric
 // One workgroup per trajectory: partials -> record ri
 __global__ void __launch_bounds__(256) k_mixed_reduce(const double* __restrict__ partials, int nblk, int B,
                                                       int ri, double inv2, double inv3, double invE,
-                                                      double* __restrict__ recs) {
+                                                      double* __restrict__ recs, int rec_stride) {
     __shared__ double red[kRec][4];
     const int b = blockIdx.x;
     const int t = threadIdx.x;
@@ -606,7 +547,7 @@ __global__ void __launch_bounds__(256) k_mixed_reduce(const double* __restrict__
     if (t < kRec) {
         const double tot = (red[t][0] + red[t][1]) + (red[t][2] + red[t][3]);
         const double sc = t < 9 ? inv2 : (t < 33 ? inv3 : invE);
-        recs[(size_t(ri) * B + b) * kRec + t] = tot * sc;
+        recs[(size_t(ri) * rec_stride + b) * kRec + t] = tot * sc;
     }
 }
 
@@ -1344,6 +1285,15 @@ struct csl_mixed {
     double h_tp2[2] = {0.0, 0.0}, h_tp3[2] = {0.0, 0.0};  // single-trajectory pulse centres
     double last_ms = 0.0;
     size_t last_launches = 0;
+    // persistent batched kernel: 0 when the lattice qualifies, -1 stage
+    // kernels, n > 0 at most n co-resident workgroups
+    int persist_mode = 0;
+    int last_persist = 0, last_persist_grid = 0;
+    size_t last_tail = 0;
+    double last_persist_ms = 0.0;  // the persistent launch alone
+    DBuf<double> pG, ptgrid;
+    DBuf<csl::MixDev> pM;  // device copy of M for the persistent kernel
+    int zmax[4] = {0, 0, 0, 0};  // bonds per site of kinds 22, 23, 33, 32 (0: none)
     // batched stage-kernel launch sequence replayed from a hipGraph
     int stage_graph = 0;
     hipGraphExec_t graph_exec = nullptr;
@@ -1422,12 +1372,18 @@ void to_flat(csl_mixed* h, const double* d_soa, double* d_flat, size_t B) {
 // ([n_rec][B][34]); optional state snapshots (flat, B = 1) into d_states.
 // With tl the tangent z = d state / d theta (h->ZX, zero at the start) is
 // advanced in the same launches and its records go to h->trecs.
+// X0 / recs_out / rec_stride: a sub-batch whose states start at X0 and whose
+// records go to recs_out with row stride rec_stride (defaults: h->X, h->recs, B).
 void run_mixed(csl_mixed* h, size_t B, const double* d_tp, const double* d_tp3, const double* d_scale,
                const std::vector<double>& grid, double dt, size_t si, double* d_states, bool tl = false,
-               bool use_graph = false) {
+               bool use_graph = false, double* X0 = nullptr, double* recs_out = nullptr, size_t rec_stride = 0) {
     const size_t n_steps = grid.size() - 1;
     const size_t n_rec = records_for(n_steps, si);
     hipStream_t s = h->stream;
+    if (!recs_out) h->recs.ensure(n_rec * B * CSL_MIXED_RECORD);
+    double* const Xp = X0 ? X0 : h->X.p;
+    double* const Rp = recs_out ? recs_out : h->recs.p;
+    const int RS = int(rec_stride ? rec_stride : B);
     const size_t plane = B * h->sz();
     h->ACC.ensure(plane);
     h->Y1.ensure(plane);
@@ -1435,7 +1391,6 @@ void run_mixed(csl_mixed* h, size_t B, const double* d_tp, const double* d_tp3, 
     const int nblk2 = blocks_of(h->M.NP2), nblk3 = blocks_of(h->M.NP3);
     const int nblk = nblk2 + nblk3;
     h->partials.ensure(B * size_t(nblk) * CSL_MIXED_RECORD);
-    h->recs.ensure(n_rec * B * CSL_MIXED_RECORD);
     if (tl) {
         h->ZACC.ensure(plane);
         h->ZY1.ensure(plane);
@@ -1456,9 +1411,9 @@ void run_mixed(csl_mixed* h, size_t B, const double* d_tp, const double* d_tp3, 
         csl::MixStageArgs A{};
         A.stage = st;
         A.nblk2 = nblk2;
-        A.x = h->X.p;
+        A.x = Xp;
         A.acc = h->ACC.p;
-        A.ycur = st == 0 ? h->X.p : (st == 2 ? h->Y2.p : h->Y1.p);
+        A.ycur = st == 0 ? Xp : (st == 2 ? h->Y2.p : h->Y1.p);
         A.ynext = st == 0 ? h->Y1.p : (st == 1 ? h->Y2.p : (st == 2 ? h->Y1.p : h->Y2.p));
         A.t = t;
         A.dt = dt;
@@ -1482,11 +1437,11 @@ void run_mixed(csl_mixed* h, size_t B, const double* d_tp, const double* d_tp3, 
     };
     auto reduce = [&](size_t ri) {
         hipLaunchKernelGGL(csl::k_mixed_reduce, dim3(unsigned(B)), dim3(256), 0, s, h->partials.p, nblk, int(B),
-                           int(ri), inv2, inv3, invE, h->recs.p);
+                           int(ri), inv2, inv3, invE, Rp, RS);
         hip_check(hipGetLastError(), "k_mixed_reduce");
         if (tl) {
             hipLaunchKernelGGL(csl::k_mixed_reduce, dim3(unsigned(B)), dim3(256), 0, s, h->tpartials.p, nblk,
-                               int(B), int(ri), inv2, inv3, invE, h->trecs.p);
+                               int(B), int(ri), inv2, inv3, invE, h->trecs.p, int(B));
             hip_check(hipGetLastError(), "k_mixed_reduce");
         }
     };
@@ -1499,7 +1454,7 @@ void run_mixed(csl_mixed* h, size_t B, const double* d_tp, const double* d_tp3, 
     for (size_t n = 0; n < n_steps; ++n) {
         const bool rec = (n % si) == 0;
         const size_t ri = n / si;
-        if (rec && d_states) to_flat(h, h->X.p, d_states + ri * h->nflat(), 1);
+        if (rec && d_states) to_flat(h, Xp, d_states + ri * h->nflat(), 1);
         const double t = grid[n];
         // Boost runge_kutta4 stage times t, t + dt/2, t + dt/2, t + dt
         stage(0, t, rec ? 1 : 0);
@@ -1514,9 +1469,9 @@ void run_mixed(csl_mixed* h, size_t B, const double* d_tp, const double* d_tp3, 
         csl::MixStageArgs A{};
         A.stage = 0;
         A.nblk2 = nblk2;
-        A.x = h->X.p;
+        A.x = Xp;
         A.acc = h->aux.p;
-        A.ycur = h->X.p;
+        A.ycur = Xp;
         A.ynext = h->aux.p;
         A.t = grid[n_steps];
         A.dt = dt;
@@ -1535,7 +1490,7 @@ void run_mixed(csl_mixed* h, size_t B, const double* d_tp, const double* d_tp3, 
         hip_check(hipGetLastError(), "k_mixed_stage");
         reduce(n_rec - 1);
     }
-    if (d_states) to_flat(h, h->X.p, d_states + (n_rec - 1) * h->nflat(), 1);
+    if (d_states) to_flat(h, Xp, d_states + (n_rec - 1) * h->nflat(), 1);
     };
     if (!use_graph) {
         issue(true);
@@ -1547,10 +1502,10 @@ void run_mixed(csl_mixed* h, size_t B, const double* d_tp, const double* d_tp3, 
         };
         put(&h->M, sizeof(csl::MixDev));
         put(grid.data(), grid.size() * sizeof(double));
-        const size_t shape[4] = {B, si, size_t(tl ? 1 : 0), grid.size()};
+        const size_t shape[5] = {B, si, size_t(tl ? 1 : 0), grid.size(), size_t(RS)};
         put(shape, sizeof(shape));
         put(&dt, sizeof(dt));
-        const void* ptrs[] = {h->X.p, h->ACC.p, h->Y1.p, h->Y2.p, h->partials.p, h->recs.p, h->aux.p, d_tp, d_tp3,
+        const void* ptrs[] = {Xp, h->ACC.p, h->Y1.p, h->Y2.p, h->partials.p, Rp, h->aux.p, d_tp, d_tp3,
                               d_scale, d_states, tl ? h->ZX.p : nullptr, tl ? h->ZACC.p : nullptr,
                               tl ? h->ZY1.p : nullptr, tl ? h->ZY2.p : nullptr, tl ? h->tpartials.p : nullptr,
                               tl ? h->trecs.p : nullptr, tl ? h->zaux.p : nullptr};
@@ -1585,6 +1540,105 @@ void run_mixed(csl_mixed* h, size_t B, const double* d_tp, const double* d_tp3, 
     (void)hipEventDestroy(e1);
     h->last_ms = launches ? ms / double(launches) : 0.0;
     h->last_launches = launches;
+}
+
+// Launch shape of the persistent kernel for this lattice (block 0 when it
+// does not qualify: it needs translation-invariant bond tables of all four
+// kinds, per-sublattice fields and onsite matrices, at most 8 bonds per site
+// of kinds 22, 23, 32 and 4 of kind 33, and at most 2048 sites of each
+// species, so that the stage values fit LDS plus two global components).
+csl::MixPersistShape mixed_persist_shape(const csl_mixed* h) {
+    const csl::MixDev& M = h->M;
+    if (!(M.e22.uniform && M.e23.uniform && M.e33.uniform && M.e32.uniform && M.site_uniform2 && M.site_uniform3))
+        return {0, 0, false};
+    if (h->zmax[0] > csl::kMixZ22 || h->zmax[1] > csl::kMixZ23 || h->zmax[2] > csl::kMixZ33 ||
+        h->zmax[3] > csl::kMixZ32)
+        return {0, 0, false};
+    return csl::mixed_persist_shape(M.NP2, M.NP3, h->zmax[2] > 0);
+}
+
+// The batch on the persistent kernel (states in h->X, records into h->recs):
+// one workgroup per trajectory, as many as are co-resident, each running
+// trajectories w, w + grid, ...  When the last round would hold only a few
+// trajectories (at most 1/16 of the grid), they run on the stage kernels
+// after the persistent launch instead of costing a whole round.  False (and
+// nothing launched) when the kernel cannot be resident.
+bool run_mixed_persist(csl_mixed* h, size_t B, const double* d_tp, const double* d_tp3, const double* d_scale,
+                       const std::vector<double>& grid, double dt, size_t si) {
+    const csl::MixPersistShape shape = mixed_persist_shape(h);
+    if (!shape.block) return false;
+    const size_t n_steps = grid.size() - 1;
+    const size_t n_rec = records_for(n_steps, si);
+    hipStream_t s = h->stream;
+    csl::MixPersistArgs A{};
+    int occ = 0;
+    h->pM.upload(&h->M, 1, s);
+    if (csl::launch_mixed_persist(h->M, h->pM.p, A, shape, 0, s, &occ) != hipSuccess || occ < 1) {
+        (void)hipGetLastError();
+        return false;
+    }
+    int n_cu = 0;
+    hip_check(hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, h->device), "CU count");
+    size_t co = size_t(occ) * size_t(n_cu);
+    if (h->persist_mode > 0) co = std::min(co, size_t(h->persist_mode));
+    const size_t grid_wg = std::min(B, co);
+    size_t n_main = B, tail = 0;
+    if (B > grid_wg) {
+        const size_t r = B % grid_wg;
+        if (r > 0 && r * 16 <= grid_wg) {
+            n_main = B - r;
+            tail = r;
+        }
+    }
+    h->recs.ensure(n_rec * B * CSL_MIXED_RECORD);
+    h->pG.ensure(grid_wg * 4 * size_t(h->M.NP3));
+    h->ptgrid.upload(grid.data(), grid.size(), s);
+    A.b0 = 0;
+    A.B = int(n_main);
+    A.B_rec = int(B);
+    A.n_steps = int(n_steps);
+    A.save_interval = int(si);
+    A.n_rec = int(n_rec);
+    A.dt = dt;
+    A.tgrid = h->ptgrid.p;
+    A.X = h->X.p;
+    A.G = h->pG.p;
+    const bool drive = d_tp && (h->M.amp2 != 0.0 || h->M.amp3 != 0.0);
+    A.tp = drive ? d_tp : nullptr;
+    A.tp3 = drive ? d_tp3 : nullptr;
+    A.scale = drive ? d_scale : nullptr;
+    A.recs = h->recs.p;
+    A.inv2 = 1.0 / double(h->N2);
+    A.inv3 = 1.0 / double(h->N3);
+    A.invE = 1.0 / double(h->N2 + h->N3);
+    hipEvent_t e0, em, e1;
+    hip_check(hipEventCreate(&e0), "event");
+    hip_check(hipEventCreate(&em), "event");
+    hip_check(hipEventCreate(&e1), "event");
+    hip_check(hipEventRecord(e0, s), "event record");
+    hip_check(csl::launch_mixed_persist(h->M, h->pM.p, A, shape, int(grid_wg), s, nullptr), "k_mixed_persist");
+    hip_check(hipEventRecord(em, s), "event record");
+    if (tail) {
+        const size_t o = 2 * n_main;
+        run_mixed(h, tail, d_tp ? d_tp + o : nullptr, d_tp3 ? d_tp3 + o : nullptr, d_scale ? d_scale + o : nullptr,
+                  grid, dt, si, nullptr, false, false, h->X.p + n_main * h->sz(), h->recs.p + n_main * CSL_MIXED_RECORD,
+                  B);
+    }
+    hip_check(hipEventRecord(e1, s), "event record");
+    hip_check(hipStreamSynchronize(s), "mixed persistent integration");
+    float ms = 0.f, ms_main = 0.f;
+    hip_check(hipEventElapsedTime(&ms, e0, e1), "elapsed");
+    hip_check(hipEventElapsedTime(&ms_main, e0, em), "elapsed");
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(em);
+    (void)hipEventDestroy(e1);
+    h->last_ms = ms;
+    h->last_persist_ms = ms_main;
+    h->last_launches = 1;
+    h->last_persist = 1;
+    h->last_persist_grid = int(grid_wg);
+    h->last_tail = tail;
+    return true;
 }
 
 void need_state(const csl_mixed* h) {
@@ -1655,7 +1709,7 @@ void run_mixed_tableau(csl_mixed* h, const Tableau& T, const std::vector<double>
         hipLaunchKernelGGL(csl::k_mixed_stage<false>, dim3(unsigned(nblk), 1u, 1u), dim3(256), 0, s, M, A);
         hip_check(hipGetLastError(), "k_mixed_stage");
         hipLaunchKernelGGL(csl::k_mixed_reduce, dim3(1u), dim3(256), 0, s, h->partials.p, nblk, 1, int(ri), inv2,
-                           inv3, invE, h->recs.p);
+                           inv3, invE, h->recs.p, 1);
         hip_check(hipGetLastError(), "k_mixed_reduce");
     };
     hipEvent_t e0, e1;
@@ -1739,6 +1793,10 @@ int csl_mixed_create(size_t A2, size_t A3, size_t n_cells, const double* field2,
             h->e23.upload(e23, M.e23, s);
             h->e33.upload(e33, M.e33, s);
             h->e32.upload(e32, M.e32, s);
+            h->zmax[0] = e22.zmax;
+            h->zmax[1] = e23.zmax;
+            h->zmax[2] = e33.zmax;
+            h->zmax[3] = e32.zmax;
             site_tables<3>(N2, A2, npad, field2, onsite2, h->field2, h->onsite2, M.site_uniform2, M.has_onsite2, s);
             site_tables<8>(N3, A3, npad, field3, onsite3, h->field3, h->onsite3, M.site_uniform3, M.has_onsite3, s);
             M.field2 = h->field2.p;
@@ -2043,7 +2101,14 @@ int csl_mixed_integrate_batched(csl_mixed* h, size_t B, const double* init, int 
         h->X.ensure(B * h->sz());
         hip_check(hipMemsetAsync(h->X.p, 0, B * h->sz() * sizeof(double), s), "memset");
         to_soa(h, h->init.p, init_per_replica ? nf : 0, h->X.p, B);
-        run_mixed(h, B, h->tp.p, h->tp.p, h->scale.p, grid, dt, save_interval, nullptr, false, h->stage_graph != 0);
+        h->last_persist = 0;
+        h->last_persist_grid = 0;
+        h->last_tail = 0;
+        h->last_persist_ms = 0.0;
+        const bool persistent = h->persist_mode >= 0 && !h->stage_graph &&
+                                run_mixed_persist(h, B, h->tp.p, h->tp.p, h->scale.p, grid, dt, save_interval);
+        if (!persistent)
+            run_mixed(h, B, h->tp.p, h->tp.p, h->scale.p, grid, dt, save_interval, nullptr, false, h->stage_graph != 0);
         copy_back(recs, h->recs.p, n_rec * B * CSL_MIXED_RECORD, s);
         if (final_states) {
             h->flat.ensure(B * nf);
@@ -2329,6 +2394,24 @@ int csl_mixed_set_stage_graph(csl_mixed* h, int enable) {
     return guarded([&] {
         check_mixed(h);
         h->stage_graph = enable ? 1 : 0;
+    });
+}
+
+int csl_mixed_set_persistent(csl_mixed* h, int mode) {
+    return guarded([&] {
+        check_mixed(h);
+        if (mode < -1) fail(CSL_ERR_ARG, "mode must be -1, 0 or a positive workgroup cap");
+        h->persist_mode = mode;
+    });
+}
+
+int csl_mixed_last_path(csl_mixed* h, int* persistent, int* workgroups, size_t* tail, double* kernel_ms) {
+    return guarded([&] {
+        check_mixed(h);
+        if (persistent) *persistent = h->last_persist;
+        if (workgroups) *workgroups = h->last_persist_grid;
+        if (tail) *tail = h->last_tail;
+        if (kernel_ms) *kernel_ms = h->last_persist ? h->last_persist_ms : 0.0;
     });
 }
 

@@ -6,7 +6,9 @@ of the device forward run at theta = 1 +- h (h = 1e-5: truncation ~h^2, far
 below the 1e-6 relative bar), the loss gradient against central differences
 of the oracle's forward runs with the scaled couplings, and at the full
 8x8x8-cell size against device central differences.  The forward records of
-the gradient run equal those of csl_mixed_integrate_batched bit for bit.
+the gradient run equal those of csl_mixed_integrate_batched on the stage
+kernels bit for bit (same kernels), and those of its persistent kernel to
+1e-12 (records summed in another order).
 """
 import numpy as np
 import pytest
@@ -66,8 +68,12 @@ def test_tangent_records_vs_central_differences():
         _setup(gm, ml)
         g = gm.scale_gradient(inits, tp, np.zeros((nr, 34)), np.r_[np.ones(33), 0.0], sc, T0, T1, dt,
                               want_records=True)
+        fwd_persistent = gm.integrate_batched(inits, tp, sc, T0, T1, dt)["records"]
+        assert gm.last_path()["persistent"]
+        gm.set_persistent(-1)
         fwd = gm.integrate_batched(inits, tp, sc, T0, T1, dt)["records"]
     assert np.array_equal(g["records"], fwd)
+    assert np.max(np.abs(fwd_persistent - fwd)) <= 1e-12
     rp = _records(mm.scale_fe_tm(muc, 1.0 + H), dims, inits, tp, sc, T0, T1, dt)
     rm = _records(mm.scale_fe_tm(muc, 1.0 - H), dims, inits, tp, sc, T0, T1, dt)
     fd = (rp - rm) / (2 * H)
