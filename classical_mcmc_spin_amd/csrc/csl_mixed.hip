@@ -57,9 +57,23 @@ __device__ __forceinline__ size_t mix_stride(const MixDev& M) {
     return size_t(3) * M.NP2 + size_t(8) * M.NP3;
 }
 
+// Loads and stores of values other workgroups exchange within one launch
+// (the cooperative kernel): COH = agent scope, performed at the device's
+// coherence point (the writers' L2 may sit on another XCD); otherwise plain.
+template <bool COH>
+__device__ __forceinline__ double ld_x(const double* p) {
+    if constexpr (COH) return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    else return *p;
+}
+template <bool COH>
+__device__ __forceinline__ void st_x(double* p, double v) {
+    if constexpr (COH) __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    else *p = v;
+}
+
 // y_p += sum_k J_k x_partner(k), J m = DA x DB, accumulated straight into y
 // (one FMA chain per component)
-template <int DA, int DB>
+template <int DA, int DB, bool COH = false>
 __device__ __forceinline__ void gather_bonds(const Ell& E, const double* __restrict__ src, int NPsrc,
                                              int NPpart, int atom, int p, double* H) {
     const int c = E.uniform ? E.cnt_atom[atom] : E.cnt[p];
@@ -67,7 +81,7 @@ __device__ __forceinline__ void gather_bonds(const Ell& E, const double* __restr
         const int q = E.col[size_t(k) * NPsrc + p];
         double s[DB];
 #pragma unroll
-        for (int e = 0; e < DB; ++e) s[e] = src[size_t(e) * NPpart + q];
+        for (int e = 0; e < DB; ++e) s[e] = ld_x<COH>(src + size_t(e) * NPpart + q);
         if (E.uniform) {
             const cptr<double> J = as_const(E.mats) + (size_t(atom) * E.zmax + k) * (DA * DB);
 #pragma unroll
@@ -147,18 +161,20 @@ __device__ __forceinline__ void onsite_lin(const double* __restrict__ onsite, in
 }
 
 // SU(2) local field without drive at device site p of one trajectory y
+template <bool COH = false>
 __device__ __forceinline__ void field_su2(const MixDev& M, const double* __restrict__ y, int atom, int p,
                                           const double* S, double* H) {
     field_onsite<3>(M.field2, M.onsite2, M.site_uniform2, M.has_onsite2, atom, p, S, H);
-    gather_bonds<3, 3>(M.e22, y, M.NP2, M.NP2, atom, p, H);
-    gather_bonds<3, 8>(M.e23, y + size_t(3) * M.NP2, M.NP2, M.NP3, atom, p, H);
+    gather_bonds<3, 3, COH>(M.e22, y, M.NP2, M.NP2, atom, p, H);
+    gather_bonds<3, 8, COH>(M.e23, y + size_t(3) * M.NP2, M.NP2, M.NP3, atom, p, H);
 }
 
+template <bool COH = false>
 __device__ __forceinline__ void field_su3(const MixDev& M, const double* __restrict__ y, int atom, int p,
                                           const double* S, double* H) {
     field_onsite<8>(M.field3, M.onsite3, M.site_uniform3, M.has_onsite3, atom, p, S, H);
-    gather_bonds<8, 8>(M.e33, y + size_t(3) * M.NP2, M.NP3, M.NP3, atom, p, H);
-    gather_bonds<8, 3>(M.e32, y, M.NP3, M.NP2, atom, p, H);
+    gather_bonds<8, 8, COH>(M.e33, y + size_t(3) * M.NP2, M.NP3, M.NP3, atom, p, H);
+    gather_bonds<8, 3, COH>(M.e32, y, M.NP3, M.NP2, atom, p, H);
 }
 
 // Workgroup -> (block, trajectory) with the XCD-contiguous mapping of k_stage.
@@ -199,12 +215,14 @@ __device__ __forceinline__ void rk_update(int s, size_t q, double Sd, double kd,
 //          dk = f(dH, l) + f(H, dl) - Gamma dl
 // and the same RK4 coefficients, i.e. the exact derivative of the discrete
 // scheme (what a discrete adjoint computes for one parameter).
-template <bool TL>
-__global__ void __launch_bounds__(256) k_mixed_stage(MixDev M, MixStageArgs A) {
-    __shared__ double fac[4];
-    __shared__ double red[4][kRec];
-    int bx, b;
-    mix_block(bx, b);
+//
+// mixed_stage_wg is the work of one workgroup (block bx of trajectory b);
+// with COH the values other workgroups read (next stage values, the state
+// written in stage 3, record partials) are stored and gathered at agent
+// scope, for the cooperative kernel that runs all stages in one launch.
+template <bool TL, bool COH>
+__device__ __forceinline__ void mixed_stage_wg(const MixDev& M, const MixStageArgs& A, int bx, int b, double* fac,
+                                               double (*red)[kRec]) {
     const int tid = threadIdx.x;
     const int lane = tid & (kWave - 1);
     const int wave = uniform(tid >> 6);
@@ -244,7 +262,7 @@ __global__ void __launch_bounds__(256) k_mixed_stage(MixDev M, MixStageArgs A) {
             const int NP = M.NP2;
             const double S[3] = {cur[p], cur[NP + p], cur[2 * NP + p]};
             double H[3];
-            field_su2(M, cur, atom, p, S, H);
+            field_su2<COH>(M, cur, atom, p, S, H);
             if (A.rec) {
                 const cptr<double> R = as_const(M.frames2) + size_t(atom) * 9;
                 const double sg = M.afm2[atom];
@@ -323,7 +341,7 @@ __global__ void __launch_bounds__(256) k_mixed_stage(MixDev M, MixStageArgs A) {
                 for (int d = 0; d < 3; ++d) {
                     const size_t q = size_t(d) * NP + p;
                     acc[q] = S[d] + bdt * k[d];
-                    nxt[q] = S[d] + cdt * k[d];
+                    st_x<COH>(nxt + q, S[d] + cdt * k[d]);
                 }
             } else if (s < 3) {
 #pragma unroll
@@ -331,13 +349,13 @@ __global__ void __launch_bounds__(256) k_mixed_stage(MixDev M, MixStageArgs A) {
                     const size_t q = size_t(d) * NP + p;
                     const double xd = x[q];
                     acc[q] = acc[q] + bdt * k[d];
-                    nxt[q] = xd + cdt * k[d];
+                    st_x<COH>(nxt + q, xd + cdt * k[d]);
                 }
             } else {
 #pragma unroll
                 for (int d = 0; d < 3; ++d) {
                     const size_t q = size_t(d) * NP + p;
-                    x[q] = acc[q] + bdt * k[d];
+                    st_x<COH>(x + q, acc[q] + bdt * k[d]);
                 }
             }
         }
@@ -353,7 +371,7 @@ __global__ void __launch_bounds__(256) k_mixed_stage(MixDev M, MixStageArgs A) {
 #pragma unroll
             for (int a = 0; a < 8; ++a) S[a] = cur[o3 + size_t(a) * NP + p];
             double H[8];
-            field_su3(M, cur, atom, p, S, H);
+            field_su3<COH>(M, cur, atom, p, S, H);
             if (A.rec) {
                 const cptr<double> R = as_const(M.frames3) + size_t(atom) * 64;
                 const double sg = ((cell * M.A3 + atom) & 1) ? -1.0 : 1.0;
@@ -430,7 +448,7 @@ __global__ void __launch_bounds__(256) k_mixed_stage(MixDev M, MixStageArgs A) {
                 for (int d = 0; d < 8; ++d) {
                     const size_t q = o3 + size_t(d) * NP + p;
                     acc[q] = S[d] + bdt * k[d];
-                    nxt[q] = S[d] + cdt * k[d];
+                    st_x<COH>(nxt + q, S[d] + cdt * k[d]);
                 }
             } else if (s < 3) {
 #pragma unroll
@@ -438,13 +456,13 @@ __global__ void __launch_bounds__(256) k_mixed_stage(MixDev M, MixStageArgs A) {
                     const size_t q = o3 + size_t(d) * NP + p;
                     const double xd = x[q];
                     acc[q] = acc[q] + bdt * k[d];
-                    nxt[q] = xd + cdt * k[d];
+                    st_x<COH>(nxt + q, xd + cdt * k[d]);
                 }
             } else {
 #pragma unroll
                 for (int d = 0; d < 8; ++d) {
                     const size_t q = o3 + size_t(d) * NP + p;
-                    x[q] = acc[q] + bdt * k[d];
+                    st_x<COH>(x + q, acc[q] + bdt * k[d]);
                 }
             }
         }
@@ -463,7 +481,7 @@ __global__ void __launch_bounds__(256) k_mixed_stage(MixDev M, MixStageArgs A) {
         if (tid < kRec) {
             const bool mine = (tid >= j0 && tid < j1) || tid == 33;
             const double v = mine ? (red[0][tid] + red[1][tid]) + (red[2][tid] + red[3][tid]) : 0.0;
-            A.partials[(size_t(b) * gridDim.x + bx) * kRec + tid] = v;
+            st_x<COH>(A.partials + (size_t(b) * gridDim.x + bx) * kRec + tid, v);
         }
         if (TL) {
             __syncthreads();
@@ -482,6 +500,112 @@ __global__ void __launch_bounds__(256) k_mixed_stage(MixDev M, MixStageArgs A) {
             }
         }
     }
+}
+
+
+template <bool TL>
+__global__ void __launch_bounds__(256) k_mixed_stage(MixDev M, MixStageArgs A) {
+    __shared__ double fac[4];
+    __shared__ double red[4][kRec];
+    int bx, b;
+    mix_block(bx, b);
+    mixed_stage_wg<TL, false>(M, A, bx, b, fac, red);
+}
+
+// Cooperative form of the stage kernels for a small batch (the trajectories
+// a persistent launch leaves over): the nblk x B workgroups of one stage
+// launch, all co-resident, run every stage of every step in ONE launch with
+// a grid-wide barrier between stages instead of one launch per stage.  The
+// values other workgroups gather are written and read at agent scope; a
+// barrier is one agent-scope counter (every wave's stores complete, then one
+// increment per workgroup, then a bounded wait for the stage's count).  The
+// first workgroup of a trajectory reduces its record partials after the
+// barrier that follows a recorded stage 0.
+struct MixCoopArgs {
+    MixStageArgs A;        // x, acc, y1 (A.ynext), y2 (A.ycur), pulses, partials, dt, nblk2
+    double* aux;           // scratch outputs of the final record's evaluation
+    const double* tgrid;   // [n_steps + 1]
+    int n_steps, save_interval, n_rec;
+    int rec_stride;        // row stride of recs (trajectories)
+    double* recs;          // [n_rec][rec_stride][34], this batch's first trajectory at column 0
+    double inv2, inv3, invE;
+    unsigned* ctr;         // barrier counter, zero at launch
+    unsigned* status;      // set when a barrier wait gives up
+};
+
+constexpr int kCoopSpin = 1 << 22;
+
+__global__ void __launch_bounds__(256) k_mixed_coop(MixDev M, MixCoopArgs C) {
+    __shared__ double fac[4];
+    __shared__ double red[4][kRec];
+    __shared__ int ok;
+    int bx, b;
+    mix_block(bx, b);
+    const int tid = threadIdx.x;
+    const int nblk = int(gridDim.x);
+    const unsigned nwg = gridDim.x * gridDim.y;
+    unsigned epoch = 0;
+    double* const y1 = C.A.ynext;
+    double* const y2 = const_cast<double*>(C.A.ycur);
+    auto barrier = [&]() -> bool {
+        __builtin_amdgcn_s_waitcnt(0);  // this thread's agent-scope stores are performed
+        __syncthreads();
+        ++epoch;
+        if (tid == 0) {
+            __hip_atomic_fetch_add(C.ctr, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            const unsigned want = epoch * nwg;
+            int good = 0;
+            for (int it = 0; it < kCoopSpin; ++it) {
+                if (__hip_atomic_load(C.ctr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= want) {
+                    good = 1;
+                    break;
+                }
+                if ((it & 255) == 255 && __hip_atomic_load(C.status, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+                    break;
+                __builtin_amdgcn_s_sleep(1);
+            }
+            if (!good) __hip_atomic_store(C.status, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            ok = good;
+        }
+        __syncthreads();
+        return ok != 0;
+    };
+    auto reduce = [&](int ri) {
+        if (bx == 0 && tid < kRec) {
+            double tot = 0.0;
+            for (int k = 0; k < nblk; ++k) tot += ld_x<true>(C.A.partials + (size_t(b) * nblk + k) * kRec + tid);
+            const double sc = tid < 9 ? C.inv2 : (tid < 33 ? C.inv3 : C.invE);
+            C.recs[(size_t(ri) * C.rec_stride + b) * kRec + tid] = tot * sc;
+        }
+    };
+    const double dt = C.A.dt;
+    for (int n = 0; n < C.n_steps; ++n) {
+        const bool rec = (n % C.save_interval) == 0;
+        const double t = C.tgrid[n];
+        // stage times t, t + dt/2, t + dt/2, t + dt; inputs x, y1, y2, y1
+        for (int st = 0; st < 4; ++st) {
+            MixStageArgs A = C.A;
+            A.stage = st;
+            A.ycur = st == 0 ? A.x : (st == 2 ? y2 : y1);
+            A.ynext = st == 0 ? y1 : (st == 1 ? y2 : (st == 2 ? y1 : y2));
+            A.t = t + (st == 0 ? 0.0 : (st == 3 ? 1.0 * dt : 0.5 * dt));
+            A.rec = (st == 0 && rec) ? 1 : 0;
+            mixed_stage_wg<false, true>(M, A, bx, b, fac, red);
+            if (!barrier()) return;
+            if (A.rec) reduce(n / C.save_interval);
+        }
+    }
+    // final record: a stage-0 evaluation of the final state into scratch
+    MixStageArgs A = C.A;
+    A.stage = 0;
+    A.acc = C.aux;
+    A.ycur = A.x;
+    A.ynext = C.aux;
+    A.t = C.tgrid[C.n_steps];
+    A.rec = 1;
+    mixed_stage_wg<false, true>(M, A, bx, b, fac, red);
+    if (!barrier()) return;
+    reduce(C.n_rec - 1);
 }
 
 // Loss sum_t sum_j w_j (rec_j(t) - target_j(t))^2 of each trajectory and its
@@ -1293,6 +1417,7 @@ struct csl_mixed {
     double last_persist_ms = 0.0;  // the persistent launch alone
     DBuf<double> pG, ptgrid;
     DBuf<csl::MixDev> pM;  // device copy of M for the persistent kernel
+    DBuf<unsigned> coop_ctl;  // cooperative tail: barrier counter, status
     int zmax[4] = {0, 0, 0, 0};  // bonds per site of kinds 22, 23, 33, 32 (0: none)
     // batched stage-kernel launch sequence replayed from a hipGraph
     int stage_graph = 0;
@@ -1542,6 +1667,63 @@ void run_mixed(csl_mixed* h, size_t B, const double* d_tp, const double* d_tp3, 
     h->last_launches = launches;
 }
 
+// The trajectories a persistent launch leaves over (states at X0, records to
+// recs_out with row stride rec_stride) on the cooperative stage kernel: one
+// launch for the whole time grid.  False (nothing launched) when its
+// workgroups cannot all be resident; the caller checks h->coop_ctl[1] after
+// synchronising.
+bool run_mixed_coop(csl_mixed* h, size_t B, const double* d_tp, const double* d_tp3, const double* d_scale,
+                    const std::vector<double>& grid, double dt, size_t si, double* X0, double* recs_out,
+                    size_t rec_stride) {
+    const size_t n_steps = grid.size() - 1;
+    const size_t n_rec = records_for(n_steps, si);
+    hipStream_t s = h->stream;
+    const int nblk2 = blocks_of(h->M.NP2), nblk3 = blocks_of(h->M.NP3);
+    const int nblk = nblk2 + nblk3;
+    int occ = 0, n_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, csl::k_mixed_coop, 256, 0) != hipSuccess) {
+        (void)hipGetLastError();
+        return false;
+    }
+    hip_check(hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, h->device), "CU count");
+    if (size_t(nblk) * B > size_t(occ) * size_t(n_cu)) return false;
+    const size_t plane = B * h->sz();
+    h->ACC.ensure(plane);
+    h->Y1.ensure(plane);
+    h->Y2.ensure(plane);
+    h->aux.ensure(plane);
+    h->partials.ensure(B * size_t(nblk) * CSL_MIXED_RECORD);
+    h->coop_ctl.ensure(2);
+    hip_check(csl::launch_fill_u32(h->coop_ctl.p, 0u, 2, s), "fill");
+    const bool drive = d_tp && (h->M.amp2 != 0.0 || h->M.amp3 != 0.0);
+    csl::MixCoopArgs C{};
+    C.A.nblk2 = nblk2;
+    C.A.x = X0;
+    C.A.acc = h->ACC.p;
+    C.A.ycur = h->Y2.p;
+    C.A.ynext = h->Y1.p;
+    C.A.dt = dt;
+    C.A.tp = drive ? d_tp : nullptr;
+    C.A.tp3 = drive ? d_tp3 : nullptr;
+    C.A.scale = drive ? d_scale : nullptr;
+    C.A.partials = h->partials.p;
+    C.aux = h->aux.p;
+    C.tgrid = h->ptgrid.p;
+    C.n_steps = int(n_steps);
+    C.save_interval = int(si);
+    C.n_rec = int(n_rec);
+    C.rec_stride = int(rec_stride);
+    C.recs = recs_out;
+    C.inv2 = 1.0 / double(h->N2);
+    C.inv3 = 1.0 / double(h->N3);
+    C.invE = 1.0 / double(h->N2 + h->N3);
+    C.ctr = h->coop_ctl.p;
+    C.status = h->coop_ctl.p + 1;
+    hipLaunchKernelGGL(csl::k_mixed_coop, dim3(unsigned(nblk), unsigned(B), 1u), dim3(256), 0, s, h->M, C);
+    hip_check(hipGetLastError(), "k_mixed_coop");
+    return true;
+}
+
 // Launch shape of the persistent kernel for this lattice (block 0 when it
 // does not qualify: it needs translation-invariant bond tables of all four
 // kinds, per-sublattice fields and onsite matrices, at most 8 bonds per site
@@ -1618,14 +1800,24 @@ bool run_mixed_persist(csl_mixed* h, size_t B, const double* d_tp, const double*
     hip_check(hipEventRecord(e0, s), "event record");
     hip_check(csl::launch_mixed_persist(h->M, h->pM.p, A, shape, int(grid_wg), s, nullptr), "k_mixed_persist");
     hip_check(hipEventRecord(em, s), "event record");
+    bool coop = false;
     if (tail) {
         const size_t o = 2 * n_main;
-        run_mixed(h, tail, d_tp ? d_tp + o : nullptr, d_tp3 ? d_tp3 + o : nullptr, d_scale ? d_scale + o : nullptr,
-                  grid, dt, si, nullptr, false, false, h->X.p + n_main * h->sz(), h->recs.p + n_main * CSL_MIXED_RECORD,
-                  B);
+        const double* tp2 = d_tp ? d_tp + o : nullptr;
+        const double* tp3 = d_tp3 ? d_tp3 + o : nullptr;
+        const double* sc = d_scale ? d_scale + o : nullptr;
+        double* X0 = h->X.p + n_main * h->sz();
+        double* R0 = h->recs.p + n_main * CSL_MIXED_RECORD;
+        coop = run_mixed_coop(h, tail, tp2, tp3, sc, grid, dt, si, X0, R0, B);
+        if (!coop) run_mixed(h, tail, tp2, tp3, sc, grid, dt, si, nullptr, false, false, X0, R0, B);
     }
     hip_check(hipEventRecord(e1, s), "event record");
     hip_check(hipStreamSynchronize(s), "mixed persistent integration");
+    if (coop) {
+        unsigned st = 0;
+        hip_check(hipMemcpy(&st, h->coop_ctl.p + 1, sizeof(unsigned), hipMemcpyDeviceToHost), "status");
+        if (st) fail(CSL_ERR_HIP, "cooperative stage kernel: a grid barrier wait timed out");
+    }
     float ms = 0.f, ms_main = 0.f;
     hip_check(hipEventElapsedTime(&ms, e0, e1), "elapsed");
     hip_check(hipEventElapsedTime(&ms_main, e0, em), "elapsed");
