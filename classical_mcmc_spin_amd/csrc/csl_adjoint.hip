@@ -157,7 +157,7 @@ __global__ void __launch_bounds__(adjoint_max_block<SPT>()) k_adj_fwd(LatDev Lk,
                 add_bond_fused(as_const(L.mats) + (st.atm[m] * L.zmax + q) * 9, Sp, H);
             }
         if (drive) sub_drive(L, st.atm[m], f1, f2, H);
-        ll_su2(H, S, alpha, k);
+        ll_su2_rs(H, S, alpha, k);
     };
 
     double x[SPT][3];
@@ -417,7 +417,7 @@ __global__ void __launch_bounds__(adjoint_max_block<SPT>()) k_adj_bwd(LatDev Lk,
                 cross3(H, S, cc);
                 if (alpha > 0.0) {
                     const double S2 = S[0] * S[0] + S[1] * S[1] + S[2] * S[2];
-                    const double inv = 1.0 / sqrt(S2);
+                    const double inv = rsqrt_nr(S2);
                     const double g = alpha * inv;
                     double vxS[3], cxv[3], Sxc[3];
                     cross3(v, S, vxS);

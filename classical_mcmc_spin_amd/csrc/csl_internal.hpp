@@ -168,6 +168,41 @@ __device__ __forceinline__ void ll_su2(const double H[3], const double S[3], dou
     k[2] = dz;
 }
 
+// 1/sqrt(x) for x > 0: the hardware reciprocal square root refined by two
+// Newton-Raphson steps r <- r + r (1 - x r^2) / 2 (each step roughly doubles
+// the number of correct bits), within about an ulp of 1.0 / sqrt(x) at a
+// fraction of its cost (the correctly rounded FP64 square root and division
+// expand to some thirty instructions).
+__device__ __forceinline__ double rsqrt_nr(double x) {
+    double r = __builtin_amdgcn_rsq(x);
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+        const double e = fma(-x * r, r, 1.0);
+        r = fma(0.5 * r, e, r);
+    }
+    return r;
+}
+
+// ll_su2 with the Gilbert factor 1/|S| from rsqrt_nr (the adjoint kernels:
+// their forward map and its transpose use the same factor)
+__device__ __forceinline__ void ll_su2_rs(const double H[3], const double S[3], double alpha, double k[3]) {
+    double dx = H[1] * S[2] - H[2] * S[1];
+    double dy = H[2] * S[0] - H[0] * S[2];
+    double dz = H[0] * S[1] - H[1] * S[0];
+    if (alpha > 0.0) {
+        const double a = S[1] * dz - S[2] * dy;
+        const double b = S[2] * dx - S[0] * dz;
+        const double c = S[0] * dy - S[1] * dx;
+        const double inv_S = rsqrt_nr(S[0] * S[0] + S[1] * S[1] + S[2] * S[2]);
+        dx -= alpha * inv_S * a;
+        dy -= alpha * inv_S * b;
+        dz -= alpha * inv_S * c;
+    }
+    k[0] = dx;
+    k[1] = dy;
+    k[2] = dz;
+}
+
 // ll_su2 for a site that may be a padding slot with a zero spin: the Gilbert
 // factor 1/|S| is taken as 0 there instead of inf (same bits for |S| > 0)
 __device__ __forceinline__ void ll_su2_guarded(const double H[3], const double S[3], double alpha, double k[3]) {
