@@ -53,17 +53,20 @@ namespace csl {
 
 namespace {
 
+// One LDS buffer of 3 NP doubles: the (x, y) pairs of all sites, then the z
+// plane, so a site is one 16-byte and one 8-byte access (NP is a multiple
+// of 64, so every buffer of the kernels below starts 16-byte aligned).
 struct Lds3 {
-    double* a;  // [3][NP]
+    double* a;  // [(x,y)[NP] | z[NP]]
     int NP;
     __device__ __forceinline__ void load(int q, double S[3]) const {
-        S[0] = a[q];
-        S[1] = a[NP + q];
+        const double2 v = reinterpret_cast<const double2*>(a)[q];
+        S[0] = v.x;
+        S[1] = v.y;
         S[2] = a[2 * NP + q];
     }
     __device__ __forceinline__ void store(int q, const double S[3]) const {
-        a[q] = S[0];
-        a[NP + q] = S[1];
+        reinterpret_cast<double2*>(a)[q] = make_double2(S[0], S[1]);
         a[2 * NP + q] = S[2];
     }
 };

@@ -2379,8 +2379,10 @@ int csl_adjoint_gradient(csl_lattice* h, size_t B, const double* init, int init_
                 hip_check(hipEventRecord(e, s), "event record");
                 hip_check(hipStreamWaitEvent(h->stream2, e, 0), "stream wait");
                 // few workgroups (one per idle CU for the whole batch), so
-                // that the next segment's sweep still finds room on the GPU
-                const int bpt = std::max(1, int((h->n_cu > int(B) ? h->n_cu - int(B) : 1) / int(B)));
+                // that the next segment's sweep still finds room on the GPU;
+                // after the last segment the whole GPU is free
+                const int bpt = j == 0 ? std::max(1, 8 * h->n_cu / int(B))
+                                       : std::max(1, int((h->n_cu > int(B) ? h->n_cu - int(B) : 1) / int(B)));
                 hip_check(csl::launch_adjoint_params(L, G, bpt, h->stream2), "k_adj_par launch");
                 launches += 2;
             }
