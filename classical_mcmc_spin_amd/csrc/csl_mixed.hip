@@ -1370,6 +1370,46 @@ EllHost build_ell(size_t N, size_t A, int npad, const size_t* off, const size_t*
     return E;
 }
 
+// For translation-invariant SU(2)->SU(3) and SU(3)->SU(2) tables: the
+// offset in e23.mats of the matrix whose transpose is the SU(3)->SU(2)
+// matrix of each (SU(3) sublattice, slot) — the reverse bond of the same
+// pair, with the same slot for every cell and a bit-identical transposed
+// matrix — as [A3][8]; empty when any slot has no such twin or a slot
+// exceeds 8.
+std::vector<int> transpose_offsets(const EllHost& e23, const EllHost& e32, size_t A2, size_t A3, int n_cells,
+                                   int npad) {
+    const int z23 = std::max(e23.zmax, 1), z32 = std::max(e32.zmax, 1);
+    if (!e23.uniform || !e32.uniform || e32.zmax > 8 || e23.zmax == 0 || e32.zmax == 0) return {};
+    const size_t NP2 = A2 * size_t(npad), NP3 = A3 * size_t(npad);
+    std::vector<int> off(A3 * 8, 0);
+    for (size_t ta = 0; ta < A3; ++ta) {
+        const int c = e32.cnt_atom[ta];
+        for (int tk = 0; tk < c; ++tk) {
+            const double* J32 = &e32.mats[(ta * z32 + tk) * 24];
+            int found = -1;
+            for (int cell = 0; cell < n_cells; ++cell) {
+                const size_t p = ta * size_t(npad) + size_t(cell);
+                const int q = e32.col[size_t(tk) * NP3 + p];
+                const size_t fa = size_t(q) / size_t(npad);
+                if (fa >= A2) return {};
+                int here = -1;
+                for (int fk = 0; fk < e23.cnt_atom[fa] && here < 0; ++fk) {
+                    if (e23.col[size_t(fk) * NP2 + size_t(q)] != int(p)) continue;
+                    const double* J23 = &e23.mats[(fa * z23 + fk) * 24];
+                    bool same = true;
+                    for (int d = 0; d < 8 && same; ++d)
+                        for (int e = 0; e < 3 && same; ++e) same = J32[d * 3 + e] == J23[e * 8 + d];
+                    if (same) here = int((fa * z23 + fk) * 24);
+                }
+                if (here < 0 || (found >= 0 && here != found)) return {};
+                found = here;
+            }
+            off[ta * 8 + tk] = found;
+        }
+    }
+    return off;
+}
+
 struct EllDev {
     DBuf<int> col, cnt, cnt_atom;
     DBuf<double> mats;
@@ -1416,6 +1456,7 @@ struct csl_mixed {
     size_t last_tail = 0;
     double last_persist_ms = 0.0;  // the persistent launch alone
     DBuf<double> pG, ptgrid;
+    DBuf<int> j32off;  // see MixDev::j32off
     DBuf<csl::MixDev> pM;  // device copy of M for the persistent kernel
     DBuf<unsigned> coop_ctl;  // cooperative tail: barrier counter, status
     int zmax[4] = {0, 0, 0, 0};  // bonds per site of kinds 22, 23, 33, 32 (0: none)
@@ -1989,6 +2030,12 @@ int csl_mixed_create(size_t A2, size_t A3, size_t n_cells, const double* field2,
             h->zmax[1] = e23.zmax;
             h->zmax[2] = e33.zmax;
             h->zmax[3] = e32.zmax;
+            const std::vector<int> jo = transpose_offsets(e23, e32, A2, A3, int(n_cells), npad);
+            M.j32off = nullptr;
+            if (!jo.empty()) {
+                h->j32off.upload(jo.data(), jo.size(), s);
+                M.j32off = h->j32off.p;
+            }
             site_tables<3>(N2, A2, npad, field2, onsite2, h->field2, h->onsite2, M.site_uniform2, M.has_onsite2, s);
             site_tables<8>(N3, A3, npad, field3, onsite3, h->field3, h->onsite3, M.site_uniform3, M.has_onsite3, s);
             M.field2 = h->field2.p;

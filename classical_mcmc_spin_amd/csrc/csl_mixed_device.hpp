@@ -38,6 +38,11 @@ struct MixDev {
     double alpha;
     double gamma[8];
     int has_bloch;
+    // [A3][8]: offset (doubles) in e23.mats of the SU(2)->SU(3) matrix whose
+    // transpose is the SU(3)->SU(2) matrix of each (sublattice, slot), or
+    // null when the tables do not pair up that way (the persistent kernel
+    // then reads e32.mats)
+    const int* __restrict__ j32off;
 };
 
 constexpr int kRec = CSL_MIXED_RECORD;

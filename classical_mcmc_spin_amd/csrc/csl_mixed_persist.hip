@@ -176,6 +176,30 @@ __device__ __forceinline__ void tm_field(const MixDev* Mg, const MixStageView& V
             }
         }
     }
+    // the SU(3)->SU(2) matrices as transposes of the SU(2)->SU(3) ones when
+    // the tables pair up (half the bond-matrix footprint in the scalar
+    // cache: the SU(2) sites read the same matrices); same values, same order
+    const int* jo = M->j32off;
+    if (jo) {
+        int off[kMixZ32];
+#pragma unroll
+        for (int k = 0; k < kMixZ32; ++k) off[k] = as_const(jo)[a * 8 + k];
+#pragma unroll
+        for (int k = 0; k < kMixZ32; ++k) {
+            if (k < c32) {
+                double s[3];
+#pragma unroll
+                for (int e = 0; e < 3; ++e) s[e] = V.fe(e, q32[k]);
+                const cptr<double> J = as_const(M->e23.mats) + uniform(off[k]);
+#pragma unroll
+                for (int d = 0; d < 8; ++d)
+#pragma unroll
+                    for (int e = 0; e < 3; ++e) H[d] = fma(J[e * 8 + d], s[e], H[d]);
+                SLOT_FENCE();
+            }
+        }
+        return;
+    }
 #pragma unroll
     for (int k = 0; k < kMixZ32; ++k) {
         if (k < c32) {
