@@ -35,13 +35,6 @@ namespace csl {
 
 namespace {
 
-// End of one bond slot's products: the scheduler keeps the next slot's
-// matrix loads (scalar, 9-64 doubles) behind it, so at most one slot's
-// matrix is held in SGPRs.
-#ifndef SLOT_FENCE
-#define SLOT_FENCE() __builtin_amdgcn_sched_barrier(0)
-#endif
-
 // a double of lane l of this wave, in every lane
 __device__ __forceinline__ double lane_double(double v, int l) {
     const int lo = __builtin_amdgcn_readlane(__double2loint(v), l);
@@ -111,7 +104,6 @@ __device__ __forceinline__ void fe_field(const MixDev* Mg, const MixStageView& V
             for (int d = 0; d < 3; ++d)
 #pragma unroll
                 for (int e = 0; e < 3; ++e) H[d] = fma(J[d * 3 + e], s[e], H[d]);
-            SLOT_FENCE();
         }
     }
 #pragma unroll
@@ -125,7 +117,6 @@ __device__ __forceinline__ void fe_field(const MixDev* Mg, const MixStageView& V
             for (int d = 0; d < 3; ++d)
 #pragma unroll
                 for (int e = 0; e < 8; ++e) H[d] = fma(J[d * 8 + e], s[e], H[d]);
-            SLOT_FENCE();
         }
     }
 }
@@ -172,7 +163,6 @@ __device__ __forceinline__ void tm_field(const MixDev* Mg, const MixStageView& V
                 for (int d = 0; d < 8; ++d)
 #pragma unroll
                     for (int e = 0; e < 8; ++e) H[d] = fma(J[d * 8 + e], s[e], H[d]);
-                SLOT_FENCE();
             }
         }
     }
@@ -195,7 +185,6 @@ __device__ __forceinline__ void tm_field(const MixDev* Mg, const MixStageView& V
                 for (int d = 0; d < 8; ++d)
 #pragma unroll
                     for (int e = 0; e < 3; ++e) H[d] = fma(J[e * 8 + d], s[e], H[d]);
-                SLOT_FENCE();
             }
         }
         return;
@@ -211,7 +200,6 @@ __device__ __forceinline__ void tm_field(const MixDev* Mg, const MixStageView& V
             for (int d = 0; d < 8; ++d)
 #pragma unroll
                 for (int e = 0; e < 3; ++e) H[d] = fma(J[d * 3 + e], s[e], H[d]);
-            SLOT_FENCE();
         }
     }
 }
