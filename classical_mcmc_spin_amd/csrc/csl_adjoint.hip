@@ -106,6 +106,45 @@ struct AdjSites {
     }
 };
 
+// The 9 magnetisation sums [M_anti, M_local, M_global] of the state x of the
+// thread's sites, summed over the wave into red[wave][9] (DPP, the total in
+// lane 63); record_put sums the waves in order after a barrier.  Shared by the
+// forward and the records kernel, so both produce the same bits.
+template <int SPT, int ZMAX>
+__device__ __forceinline__ void adj_record_sum(const LatDev& L, const AdjSites<SPT, ZMAX>& st,
+                                               const double (&x)[SPT][3], int wave, int lane, double* red) {
+    double part[9];
+#pragma unroll
+    for (int j = 0; j < 9; ++j) part[j] = 0.0;
+#pragma unroll
+    for (int m = 0; m < SPT; ++m)
+        if (st.act[m]) {
+            cptr<double> R = as_const(L.frames) + st.atm[m] * 9;
+            const double sg = as_const(L.afm)[st.atm[m]];
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+                const double g = R[d * 3] * x[m][0] + R[d * 3 + 1] * x[m][1] + R[d * 3 + 2] * x[m][2];
+                part[d] += sg * g;
+                part[3 + d] += x[m][d];
+                part[6 + d] += g;
+            }
+        }
+#pragma unroll
+    for (int j = 0; j < 9; ++j) {
+        const double v = wave_sum_lane63(part[j]);
+        if (lane == kWave - 1) red[wave * 9 + j] = v;
+    }
+}
+
+__device__ __forceinline__ void adj_record_put(const AdjArgs& A, int b, int ri, int tid, int nwaves,
+                                               const double* red, double inv_n) {
+    if (tid < 9) {
+        double v = 0.0;
+        for (int w = 0; w < nwaves; ++w) v += red[w * 9 + tid];
+        A.rec[(size_t(ri) * A.B + b) * 9 + tid] = v * inv_n;
+    }
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -164,38 +203,10 @@ __global__ void __launch_bounds__(adjoint_max_block<SPT>()) k_adj_fwd(LatDev Lk,
     };
 
     double x[SPT][3];
-    // wave sums of the 9 magnetisations of x -> red (DPP, total in lane 63);
-    // the sums over the waves are taken by record_put after the next barrier
-    auto record_sum = [&]() {
-        double part[9];
-#pragma unroll
-        for (int j = 0; j < 9; ++j) part[j] = 0.0;
-#pragma unroll
-        for (int m = 0; m < SPT; ++m)
-            if (st.act[m]) {
-                cptr<double> R = as_const(L.frames) + st.atm[m] * 9;
-                const double sg = as_const(L.afm)[st.atm[m]];
-#pragma unroll
-                for (int d = 0; d < 3; ++d) {
-                    const double g = R[d * 3] * x[m][0] + R[d * 3 + 1] * x[m][1] + R[d * 3 + 2] * x[m][2];
-                    part[d] += sg * g;
-                    part[3 + d] += x[m][d];
-                    part[6 + d] += g;
-                }
-            }
-#pragma unroll
-        for (int j = 0; j < 9; ++j) {
-            const double v = wave_sum_lane63(part[j]);
-            if (lane == kWave - 1) red[wave * 9 + j] = v;
-        }
-    };
-    auto record_put = [&](int ri) {
-        if (tid < 9) {
-            double v = 0.0;
-            for (int w = 0; w < nwaves; ++w) v += red[w * 9 + tid];
-            A.rec[(size_t(ri) * A.B + b) * 9 + tid] = v * inv_n;
-        }
-    };
+    // wave sums of the 9 magnetisations of x -> red; the sums over the waves
+    // are taken by record_put after the next barrier
+    auto record_sum = [&]() { adj_record_sum<SPT, ZMAX>(L, st, x, wave, lane, red); };
+    auto record_put = [&](int ri) { adj_record_put(A, b, ri, tid, nwaves, red, inv_n); };
     auto store_own = [&](double* dst, const double (&v)[SPT][3]) {
 #pragma unroll
         for (int m = 0; m < SPT; ++m) {
@@ -295,7 +306,7 @@ __global__ void __launch_bounds__(adjoint_max_block<SPT>()) k_adj_fwd(LatDev Lk,
 #pragma unroll
         for (int q = 0; q < 6; ++q) fcur[q] = fnext[q];
         load_fac(n + 1, fnext);
-        const bool rec = A.record != 0;
+        const bool rec = A.record == 1;  // 2: the final record only (the rest from Y by k_adj_rec)
         if (rec) record_sum();
         stage(std::integral_constant<int, 0>{}, n, rec);
         stage(std::integral_constant<int, 1>{}, n, rec);
@@ -307,6 +318,42 @@ __global__ void __launch_bounds__(adjoint_max_block<SPT>()) k_adj_fwd(LatDev Lk,
         __syncthreads();
         record_put(A.n_steps);
     }
+}
+
+// Records m_n of the steps n0 .. n1-1 from the stored stage inputs (the
+// stage-0 input of step n is x_n), one workgroup per (step, trajectory) with
+// the forward's thread layout and summation order: the forward then leaves
+// the per-step reductions (9 wave sums and a barrier-bound write every step,
+// on its sequential path) to this kernel, which runs on the whole GPU.
+template <int SPT, int ZMAX>
+__global__ void __launch_bounds__(adjoint_max_block<SPT>()) k_adj_rec(LatDev L, AdjArgs A) {
+    __shared__ double red[(adjoint_max_block<SPT>() / kWave) * 9];
+    const int n = A.n0 + int(blockIdx.x);
+    const int b = blockIdx.y;
+    const int T = blockDim.x;
+    const int tid = threadIdx.x;
+    const int wave = uniform(tid >> 6);
+    const int lane = tid & (kWave - 1);
+    const int nwaves = T >> 6;
+    const int NP = L.NP;
+    const size_t SZ = size_t(3) * NP;
+    AdjSites<SPT, ZMAX> st;
+    st.init(L, tid, T, wave);
+    const double* y = A.Y + (size_t(b) * A.seg_len * 4 + size_t(n - A.y0) * 4) * SZ;
+    double x[SPT][3];
+#pragma unroll
+    for (int m = 0; m < SPT; ++m) {
+        const int p = tid + m * T;
+        x[m][0] = x[m][1] = x[m][2] = 0.0;
+        if (st.act[m]) {
+            x[m][0] = y[p];
+            x[m][1] = y[NP + p];
+            x[m][2] = y[2 * NP + p];
+        }
+    }
+    adj_record_sum<SPT, ZMAX>(L, st, x, wave, lane, red);
+    __syncthreads();
+    adj_record_put(A, b, n, tid, nwaves, red, 1.0 / double(L.N));
 }
 
 // ---------------------------------------------------------------------------
@@ -767,6 +814,21 @@ int adjoint_sites_per_thread(const LatDev& L) {
 
 hipError_t launch_adjoint_forward(const LatDev& L, const AdjArgs& A, hipStream_t st) {
     return launch_adj_spt(L, A, true, st);
+}
+
+hipError_t launch_adjoint_records(const LatDev& L, const AdjArgs& A, hipStream_t st) {
+    const int spt = adjoint_sites_per_thread(L);
+    if (A.n1 <= A.n0) return hipSuccess;
+    const int block = adj_block(L, spt > 0 ? spt : 1);
+    const dim3 grid(unsigned(A.n1 - A.n0), unsigned(A.B));
+    if (spt == 1 && L.zmax <= 3) hipLaunchKernelGGL((k_adj_rec<1, 3>), grid, dim3(block), 0, st, L, A);
+    else if (spt == 1 && L.zmax <= 6) hipLaunchKernelGGL((k_adj_rec<1, 6>), grid, dim3(block), 0, st, L, A);
+    else if (spt == 2 && L.zmax <= 3) hipLaunchKernelGGL((k_adj_rec<2, 3>), grid, dim3(block), 0, st, L, A);
+    else if (spt == 2 && L.zmax <= 6) hipLaunchKernelGGL((k_adj_rec<2, 6>), grid, dim3(block), 0, st, L, A);
+    else if (spt == 4 && L.zmax <= 3) hipLaunchKernelGGL((k_adj_rec<4, 3>), grid, dim3(block), 0, st, L, A);
+    else if (spt == 4 && L.zmax <= 6) hipLaunchKernelGGL((k_adj_rec<4, 6>), grid, dim3(block), 0, st, L, A);
+    else return hipErrorInvalidValue;
+    return hipGetLastError();
 }
 
 hipError_t launch_adjoint_backward(const LatDev& L, const AdjArgs& A, hipStream_t st) {

@@ -2350,12 +2350,18 @@ int csl_adjoint_gradient(csl_lattice* h, size_t B, const double* init, int init_
             F.n0 = 0;
             F.n1 = int(n_steps);
             F.ckpt_src = -1;
-            F.record = 1;
+            // with every stage input stored, the per-step records come from
+            // them on the whole GPU (same arithmetic, same bits)
+            F.record = full ? 2 : 1;
             F.ckpt_interval = full ? 0 : int(C);
             F.store_stages = full ? 1 : 0;
             F.y0 = 0;
             hip_check(csl::launch_adjoint_forward(L, F, s), "k_adj_fwd launch");
             ++launches;
+            if (full) {
+                hip_check(csl::launch_adjoint_records(L, F, s), "k_adj_rec launch");
+                ++launches;
+            }
         }
         if (full) {
             // the backward sweep in segments; the parameter terms of a

@@ -131,7 +131,7 @@ struct AdjArgs {
     int seg_len;               // steps held per trajectory in Y and U
     // forward launches
     int ckpt_src;              // >= 0: start from checkpoint ckpt_src (state of step n0); -1: from init
-    int record;                // write the records of steps n0 .. n1
+    int record;                // 1: write the records of steps n0 .. n1; 2: only the final one
     int ckpt_interval;         // store x_n every ckpt_interval steps (0: none)
     int n_ckpt;
     double* ckpt;              // [B][n_ckpt][3*NP] checkpoint states (SoA)
@@ -152,6 +152,8 @@ struct AdjArgs {
 size_t adjoint_lds_bytes(const LatDev& L, int block);
 int adjoint_sites_per_thread(const LatDev& L);
 hipError_t launch_adjoint_forward(const LatDev& L, const AdjArgs& A, hipStream_t st);
+// records of steps A.n0 .. A.n1-1 from the stored stage inputs (A.Y)
+hipError_t launch_adjoint_records(const LatDev& L, const AdjArgs& A, hipStream_t st);
 hipError_t launch_adjoint_backward(const LatDev& L, const AdjArgs& A, hipStream_t st);
 // blocks_per_traj: workgroups per trajectory (each loops over its share of
 // the segment's stage slots)
