@@ -106,6 +106,20 @@ struct AdjSites {
     }
 };
 
+// One stored stage input in Y (3 NP doubles, 16-byte aligned): the (x, y)
+// pairs of all sites, then the z plane -- one 16-byte and one 8-byte access
+// per site for the forward's stores and the sweeps' loads.
+__device__ __forceinline__ void ys_put(double* y, int NP, int p, const double S[3]) {
+    reinterpret_cast<double2*>(y)[p] = make_double2(S[0], S[1]);
+    y[2 * NP + p] = S[2];
+}
+__device__ __forceinline__ void ys_get(const double* y, int NP, int p, double S[3]) {
+    const double2 v = reinterpret_cast<const double2*>(y)[p];
+    S[0] = v.x;
+    S[1] = v.y;
+    S[2] = y[2 * NP + p];
+}
+
 // The 9 magnetisation sums [M_anti, M_local, M_global] of the state x of the
 // thread's sites, summed over the wave into red[wave][9] (DPP, the total in
 // lane 63); record_put sums the waves in order after a barrier.  Shared by the
@@ -278,11 +292,7 @@ __global__ void __launch_bounds__(adjoint_max_block<SPT>()) k_adj_fwd(LatDev Lk,
             const int p = fresh(tid + m * T);
             double S[3] = {0.0, 0.0, 0.0};
             if (p < NP) cur.load(p, S);
-            if (ys && p < NP) {
-                ys[p] = S[0];
-                ys[NP + p] = S[1];
-                ys[2 * NP + p] = S[2];
-            }
+            if (ys && p < NP) ys_put(ys, NP, p, S);
             if (st.act[m]) {
                 double k[3], y[3];
                 rhs(cur, m, S, f1, f2, k);
@@ -345,11 +355,7 @@ __global__ void __launch_bounds__(adjoint_max_block<SPT>()) k_adj_rec(LatDev L, 
     for (int m = 0; m < SPT; ++m) {
         const int p = tid + m * T;
         x[m][0] = x[m][1] = x[m][2] = 0.0;
-        if (st.act[m]) {
-            x[m][0] = y[p];
-            x[m][1] = y[NP + p];
-            x[m][2] = y[2 * NP + p];
-        }
+        if (st.act[m]) ys_get(y, NP, p, x[m]);
     }
     adj_record_sum<SPT, ZMAX>(L, st, x, wave, lane, red);
     __syncthreads();
@@ -543,11 +549,7 @@ __global__ void __launch_bounds__(adjoint_max_block<SPT>()) k_adj_bwd(LatDev Lk,
         for (int m = 0; m < SPT; ++m) {
             const int p = fresh(tid + m * T);
             pre[m][0] = pre[m][1] = pre[m][2] = 0.0;
-            if (p < NP) {
-                pre[m][0] = src[p];
-                pre[m][1] = src[NP + p];
-                pre[m][2] = src[2 * NP + p];
-            }
+            if (p < NP) ys_get(src, NP, p, pre[m]);
         }
     };
     auto put = [&](const Lds3& dst) {
@@ -697,9 +699,7 @@ __global__ void __launch_bounds__(256) k_adj_par(LatDev L, AdjArgs A) {
             for (int z = 0; z < ZMAX; ++z)
                 if (z < cnt) {
                     const int q = L.col[size_t(z) * NP + p];
-                    Sp[z][0] = y[q];
-                    Sp[z][1] = y[NP + q];
-                    Sp[z][2] = y[2 * NP + q];
+                    ys_get(y, NP, q, Sp[z]);
                 }
 #pragma unroll
             for (int q = 0; q < kAdjMaxP; ++q) {

@@ -136,7 +136,7 @@ struct AdjArgs {
     int n_ckpt;
     double* ckpt;              // [B][n_ckpt][3*NP] checkpoint states (SoA)
     int store_stages;          // store the stage inputs of every step into Y
-    double* Y;                 // [B][seg_len][4][3*NP] stage inputs y_1..y_4 of steps n0.. (SoA)
+    double* Y;                 // [B][seg_len][4][3*NP] stage inputs y_1..y_4 of steps n0.. ((x,y) pairs, z plane)
     double* U;                 // [B][seg_len][4][3*NP] cotangents u of the fields at those stages
     // backward launches
     double* lam;               // [B][3*NP] adjoint state handed between backward launches (SoA)
