@@ -492,9 +492,7 @@ __global__ void __launch_bounds__(adjoint_max_block<SPT>()) k_adj_bwd(LatDev Lk,
             }
             if (p < NP) {
                 ub.store(p, u);
-                ug[p] = u[0];
-                ug[NP + p] = u[1];
-                ug[2 * NP + p] = u[2];
+                ys_put(ug, NP, p, u);
             }
         }
     };
@@ -693,7 +691,8 @@ __global__ void __launch_bounds__(256) k_adj_par(LatDev L, AdjArgs A) {
             const int atom = uniform(pw < NP ? pw / L.npad : 0);
             if (!active_site(L, p)) continue;
             const int cnt = uniform(as_const(L.cnt_atom)[atom]);
-            const double uu[3] = {u[p], u[NP + p], u[2 * NP + p]};
+            double uu[3];
+            ys_get(u, NP, p, uu);
             double Sp[ZMAX][3];
 #pragma unroll
             for (int z = 0; z < ZMAX; ++z)

@@ -137,7 +137,7 @@ struct AdjArgs {
     double* ckpt;              // [B][n_ckpt][3*NP] checkpoint states (SoA)
     int store_stages;          // store the stage inputs of every step into Y
     double* Y;                 // [B][seg_len][4][3*NP] stage inputs y_1..y_4 of steps n0.. ((x,y) pairs, z plane)
-    double* U;                 // [B][seg_len][4][3*NP] cotangents u of the fields at those stages
+    double* U;                 // [B][seg_len][4][3*NP] cotangents u of the fields at those stages (as Y)
     // backward launches
     double* lam;               // [B][3*NP] adjoint state handed between backward launches (SoA)
     int first;                 // the launch holds the last step: lambda starts from dl/dx_N
