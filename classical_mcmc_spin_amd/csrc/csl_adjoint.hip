@@ -764,7 +764,29 @@ __global__ void __launch_bounds__(256) k_adj_final(AdjArgs A) {
 
 namespace {
 
-int adj_block(const LatDev& L, int spt) { return ((L.NP + spt - 1) / spt + kWave - 1) / kWave * kWave; }
+// Block size for spt sites per thread: the multiple of 64 between the
+// smallest that covers NP and the kernels' bound whose busiest SIMD holds the
+// fewest wave-sites (a workgroup's waves are dealt to the 4 SIMDs in turn; a
+// wave-site is one wave's 64 lanes of one of its spt sites, counted when any
+// lane is a site), the smaller block on ties.  NP = 1152, spt 2: 576 threads
+// load the SIMDs 6, 4, 4, 4 wave-sites, 640 threads 5, 5, 4, 4.
+int adj_block(const LatDev& L, int spt) {
+    const int lo = ((L.NP + spt - 1) / spt + kWave - 1) / kWave * kWave;
+    const int hi = spt == 1 ? adjoint_max_block<1>() : adjoint_max_block<2>();
+    int best = lo, best_load = 1 << 30;
+    for (int T = lo; T <= hi; T += kWave) {
+        int simd[4] = {0, 0, 0, 0};
+        for (int w = 0; w < T / kWave; ++w)
+            for (int m = 0; m < spt; ++m)
+                if (w * kWave + m * T < L.NP) ++simd[w % 4];
+        const int load = std::max(std::max(simd[0], simd[1]), std::max(simd[2], simd[3]));
+        if (load < best_load) {
+            best_load = load;
+            best = T;
+        }
+    }
+    return best;
+}
 
 template <int SPT, int ZMAX>
 hipError_t launch_adj2(const LatDev& L, const AdjArgs& A, bool fwd, hipStream_t st) {
