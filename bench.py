@@ -645,10 +645,10 @@ def bench_mixed(args):
     + 2048 Tm SU(3) sites), pulses A = 0.5, w = 0.5 along x (Fe) and
     lambda_2 (Tm) at 0 and tau = 0, 0.05, ..., 6.35 (128 delays):
     M0 + 128 M1 + 128 M01 = 257 trajectories x 8192 RK4 steps (dt = 0.005),
-    records every step.  Default path: the persistent kernel, one workgroup
-    per trajectory on the co-resident workgroups, the one trajectory beyond
-    them on the stage kernels; the grid-wide stage kernels alone are timed
-    beside it.  Then d loss / d (Fe-Tm scale) for every trajectory.  Error:
+    records every step.  Default path: the persistent kernel, one launch,
+    the 257 trajectories' steps shared equally by the co-resident workgroups
+    (whole trajectories plus one split with a neighbour); the grid-wide stage
+    kernels alone are timed beside it.  Then d loss / d (Fe-Tm scale) for every trajectory.  Error:
     two trajectories over 200 steps against the mixed oracle."""
     from classical_mcmc_spin_amd import gpu_mixed as GM
     muc, ml, g, taus, B, tp, sc, f2, f3 = _mixed_setup()
@@ -673,8 +673,8 @@ def bench_mixed(args):
         one_step()
     el, res, paths, kernel_ms = timed(args.steps)
     path = paths[-1]
-    kms = float(np.mean([p["kernel_ms"] for p in paths]))   # the persistent launch alone
-    kms_all = float(np.mean([k[0] for k in kernel_ms]))      # with the tail on the stage kernels
+    kms = float(np.mean([p["kernel_ms"] for p in paths]))   # the persistent launch
+    kms_all = float(np.mean([k[0] for k in kernel_ms]))      # the whole integration
     if args.profile:
         gm.close()
         return {"profile": True, "workload": "mixed", "kernel_ms": kms, "integration_ms": kms_all,
@@ -708,9 +708,8 @@ def bench_mixed(args):
         err["state"] = max(err["state"], float(np.max(np.abs(pr["final"][j] - ref["final"]))))
     traj_s = B * args.steps / el
     n_sites = ml.N2 + ml.N3
-    n_main = B - path["tail"]
-    roof = fp64_roofline("k_mixed_persist", MIX_FLOPS_PAIR_STEP * ml.N2 * n_main * MIX_STEPS, kms,
-                         algorithmic_flops_per_pair_step=MIX_FLOPS_PAIR_STEP, trajectories_in_launch=n_main)
+    roof = fp64_roofline("k_mixed_persist", MIX_FLOPS_PAIR_STEP * ml.N2 * B * MIX_STEPS, kms,
+                         algorithmic_flops_per_pair_step=MIX_FLOPS_PAIR_STEP, trajectories_in_launch=B)
     roof["traffic"] = profile_number("traffic_k_mixed_persist_tmfeo3_8x8x8.txt", "traffic_bytes_per_launch")
     bytes_stage = B * (ml.N2 * 3 + ml.N3 * 8) * 2 * 8   # one stage: read + write of every component
     stage_roof = hbm_roofline("k_mixed_stage", bytes_stage, kms_st)
@@ -719,8 +718,8 @@ def bench_mixed(args):
         "value": traj_s, "unit": "trajectories/s", "ms_per_step": 1e3 * el / args.steps,
         "config": {"workload": f"2dcs_tmfeo3_mixed_8x8x8_257traj_{MIX_STEPS}steps_rk4_dt0.005",
                    "sites_su2": ml.N2, "sites_su3": ml.N3, "trajectories": B, "rk4_steps": MIX_STEPS,
-                   "record_every": 1, "path": "persistent kernel (one workgroup per trajectory)",
-                   "workgroups": path["workgroups"], "tail_on_stage_kernels": path["tail"]},
+                   "record_every": 1, "path": "persistent kernel (trajectory-steps shared by the workgroups)",
+                   "workgroups": path["workgroups"], "split_trajectories": path["split"]},
         "site_steps_per_s": traj_s * n_sites * MIX_STEPS,
         "max_abs_err_vs_oracle_200_steps": err,
         "roofline": roof,

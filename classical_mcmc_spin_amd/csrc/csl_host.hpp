@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstring>
 #include <limits>
@@ -104,6 +105,44 @@ inline std::vector<double> make_grid(double T0, double T1, double dt, int policy
 }
 
 inline size_t records_for(size_t n_steps, size_t si) { return (n_steps + si - 1) / si + 1; }
+
+// Work split of B trajectories of n steps over G workgroups (B > G):
+// workgroup g takes the range [g*W, (g+1)*W) of the trajectory-major step
+// order, W = ceil(B*n/G) >= n, as pieces (trajectory, first step, end step,
+// slot to load, slot to save): first the head of the trajectory it shares
+// with g+1 (saved to slot g), then its whole trajectories, last the rest of
+// the trajectory begun by g-1 (loaded from slot g-1).  Rows [G][max_p][5],
+// unused pieces -1.
+inline std::vector<int> split_schedule(size_t B, size_t G, size_t n, size_t& max_p) {
+    const size_t W = B * n, per = (W + G - 1) / G;
+    std::vector<std::vector<std::array<int, 5>>> pc(G);
+    max_p = 1;
+    for (size_t g = 0; g < G; ++g) {
+        const size_t a = g * per, e = std::min(W, (g + 1) * per);
+        std::vector<std::array<int, 5>> lay;
+        for (size_t u = a; u < e;) {
+            const size_t j = u / n, s0 = u - j * n, s1 = std::min(n, e - j * n);
+            lay.push_back({int(j), int(s0), int(s1), -1, -1});
+            u = j * n + s1;
+        }
+        auto& out = pc[g];
+        if (!lay.empty() && lay.back()[1] == 0 && size_t(lay.back()[2]) < n) {
+            lay.back()[4] = int(g);
+            out.push_back(lay.back());
+            lay.pop_back();
+        }
+        const bool tail = !lay.empty() && lay.front()[1] > 0;
+        if (tail) lay.front()[3] = int(g) - 1;
+        for (size_t i = tail ? 1 : 0; i < lay.size(); ++i) out.push_back(lay[i]);
+        if (tail) out.push_back(lay.front());
+        max_p = std::max(max_p, out.size());
+    }
+    std::vector<int> rows(G * max_p * 5, -1);
+    for (size_t g = 0; g < G; ++g)
+        for (size_t i = 0; i < pc[g].size(); ++i)
+            for (int k = 0; k < 5; ++k) rows[(g * max_p + i) * 5 + k] = pc[g][i][k];
+    return rows;
+}
 
 inline bool method_is_rk4(const char* m) { return m == nullptr || std::strcmp(m, "rk4") == 0; }
 

@@ -1453,12 +1453,14 @@ struct csl_mixed {
     // kernels, n > 0 at most n co-resident workgroups
     int persist_mode = 0;
     int last_persist = 0, last_persist_grid = 0;
-    size_t last_tail = 0;
+    size_t last_split = 0;  // trajectories the persistent kernel ran in two pieces
     double last_persist_ms = 0.0;  // the persistent launch alone
     DBuf<double> pG, ptgrid;
     DBuf<int> j32off;  // see MixDev::j32off
     DBuf<csl::MixDev> pM;  // device copy of M for the persistent kernel
-    DBuf<unsigned> coop_ctl;  // cooperative tail: barrier counter, status
+    DBuf<unsigned> coop_ctl;  // cooperative kernel: barrier counter, status
+    DBuf<int> psched;         // work split of the persistent kernel
+    DBuf<unsigned> pflags;    // its seam flags [grid] and status
     int zmax[4] = {0, 0, 0, 0};  // bonds per site of kinds 22, 23, 33, 32 (0: none)
     // batched stage-kernel launch sequence replayed from a hipGraph
     int stage_graph = 0;
@@ -1781,11 +1783,13 @@ csl::MixPersistShape mixed_persist_shape(const csl_mixed* h) {
 }
 
 // The batch on the persistent kernel (states in h->X, records into h->recs):
-// one workgroup per trajectory, as many as are co-resident, each running
-// trajectories w, w + grid, ...  When the last round would hold only a few
-// trajectories (at most 1/16 of the grid), they run on the stage kernels
-// after the persistent launch instead of costing a whole round.  False (and
-// nothing launched) when the kernel cannot be resident.
+// one workgroup per trajectory, as many as are co-resident.  With more
+// trajectories than workgroups, workgroup g runs the range [g W, (g+1) W) of
+// the trajectory-major step order (W = ceil(B n / grid)): the head of the
+// trajectory it shares with g + 1 first, then whole trajectories, last the
+// rest of the trajectory begun by g - 1, from the state that workgroup left
+// in X (a seam flag per workgroup).  False (and nothing launched) when the
+// kernel cannot be resident.
 bool run_mixed_persist(csl_mixed* h, size_t B, const double* d_tp, const double* d_tp3, const double* d_scale,
                        const std::vector<double>& grid, double dt, size_t si) {
     const csl::MixPersistShape shape = mixed_persist_shape(h);
@@ -1805,19 +1809,24 @@ bool run_mixed_persist(csl_mixed* h, size_t B, const double* d_tp, const double*
     size_t co = size_t(occ) * size_t(n_cu);
     if (h->persist_mode > 0) co = std::min(co, size_t(h->persist_mode));
     const size_t grid_wg = std::min(B, co);
-    size_t n_main = B, tail = 0;
-    if (B > grid_wg) {
-        const size_t r = B % grid_wg;
-        if (r > 0 && r * 16 <= grid_wg) {
-            n_main = B - r;
-            tail = r;
-        }
+    size_t n_split = 0;
+    if (B > grid_wg && n_steps > 0) {
+        size_t max_p = 1;
+        const std::vector<int> sched = split_schedule(B, grid_wg, n_steps, max_p);
+        h->psched.upload(sched.data(), sched.size(), s);
+        h->pflags.ensure(grid_wg + 1);
+        hip_check(csl::launch_fill_u32(h->pflags.p, 0u, grid_wg + 1, s), "fill");
+        for (size_t r = 0; r < sched.size(); r += 5) n_split += sched[r + 4] >= 0;
+        A.sched = h->psched.p;
+        A.max_pieces = int(max_p);
+        A.flags = h->pflags.p;
+        A.status = h->pflags.p + grid_wg;
     }
     h->recs.ensure(n_rec * B * CSL_MIXED_RECORD);
     h->pG.ensure(grid_wg * 4 * size_t(h->M.NP3));
     h->ptgrid.upload(grid.data(), grid.size(), s);
     A.b0 = 0;
-    A.B = int(n_main);
+    A.B = int(B);
     A.B_rec = int(B);
     A.n_steps = int(n_steps);
     A.save_interval = int(si);
@@ -1834,43 +1843,28 @@ bool run_mixed_persist(csl_mixed* h, size_t B, const double* d_tp, const double*
     A.inv2 = 1.0 / double(h->N2);
     A.inv3 = 1.0 / double(h->N3);
     A.invE = 1.0 / double(h->N2 + h->N3);
-    hipEvent_t e0, em, e1;
+    hipEvent_t e0, e1;
     hip_check(hipEventCreate(&e0), "event");
-    hip_check(hipEventCreate(&em), "event");
     hip_check(hipEventCreate(&e1), "event");
     hip_check(hipEventRecord(e0, s), "event record");
     hip_check(csl::launch_mixed_persist(h->M, h->pM.p, A, shape, int(grid_wg), s, nullptr), "k_mixed_persist");
-    hip_check(hipEventRecord(em, s), "event record");
-    bool coop = false;
-    if (tail) {
-        const size_t o = 2 * n_main;
-        const double* tp2 = d_tp ? d_tp + o : nullptr;
-        const double* tp3 = d_tp3 ? d_tp3 + o : nullptr;
-        const double* sc = d_scale ? d_scale + o : nullptr;
-        double* X0 = h->X.p + n_main * h->sz();
-        double* R0 = h->recs.p + n_main * CSL_MIXED_RECORD;
-        coop = run_mixed_coop(h, tail, tp2, tp3, sc, grid, dt, si, X0, R0, B);
-        if (!coop) run_mixed(h, tail, tp2, tp3, sc, grid, dt, si, nullptr, false, false, X0, R0, B);
-    }
     hip_check(hipEventRecord(e1, s), "event record");
     hip_check(hipStreamSynchronize(s), "mixed persistent integration");
-    if (coop) {
+    if (A.sched) {
         unsigned st = 0;
-        hip_check(hipMemcpy(&st, h->coop_ctl.p + 1, sizeof(unsigned), hipMemcpyDeviceToHost), "status");
-        if (st) fail(CSL_ERR_HIP, "cooperative stage kernel: a grid barrier wait timed out");
+        hip_check(hipMemcpy(&st, A.status, sizeof(unsigned), hipMemcpyDeviceToHost), "status");
+        if (st) fail(CSL_ERR_HIP, "mixed persistent kernel: a wait for a trajectory's head piece timed out");
     }
-    float ms = 0.f, ms_main = 0.f;
+    float ms = 0.f;
     hip_check(hipEventElapsedTime(&ms, e0, e1), "elapsed");
-    hip_check(hipEventElapsedTime(&ms_main, e0, em), "elapsed");
     (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(em);
     (void)hipEventDestroy(e1);
     h->last_ms = ms;
-    h->last_persist_ms = ms_main;
+    h->last_persist_ms = ms;
     h->last_launches = 1;
     h->last_persist = 1;
     h->last_persist_grid = int(grid_wg);
-    h->last_tail = tail;
+    h->last_split = n_split;
     return true;
 }
 
@@ -2342,7 +2336,7 @@ int csl_mixed_integrate_batched(csl_mixed* h, size_t B, const double* init, int 
         to_soa(h, h->init.p, init_per_replica ? nf : 0, h->X.p, B);
         h->last_persist = 0;
         h->last_persist_grid = 0;
-        h->last_tail = 0;
+        h->last_split = 0;
         h->last_persist_ms = 0.0;
         const bool persistent = h->persist_mode >= 0 && !h->stage_graph &&
                                 run_mixed_persist(h, B, h->tp.p, h->tp.p, h->scale.p, grid, dt, save_interval);
@@ -2644,12 +2638,12 @@ int csl_mixed_set_persistent(csl_mixed* h, int mode) {
     });
 }
 
-int csl_mixed_last_path(csl_mixed* h, int* persistent, int* workgroups, size_t* tail, double* kernel_ms) {
+int csl_mixed_last_path(csl_mixed* h, int* persistent, int* workgroups, size_t* split, double* kernel_ms) {
     return guarded([&] {
         check_mixed(h);
         if (persistent) *persistent = h->last_persist;
         if (workgroups) *workgroups = h->last_persist_grid;
-        if (tail) *tail = h->last_tail;
+        if (split) *split = h->last_split;
         if (kernel_ms) *kernel_ms = h->last_persist ? h->last_persist_ms : 0.0;
     });
 }

@@ -91,6 +91,15 @@ struct MixPersistArgs {
     const double* scale;       // [B_rec][2] pulse scales, or null (1)
     double* recs;              // [n_rec][B_rec][34]
     double inv2, inv3, invE;   // record normalisations (1/N2, 1/N3, 1/(N2+N3))
+    // work split of more trajectories than workgroups (null: workgroup w
+    // runs trajectories w, w + grid, ...): rows [grid][max_pieces][5] of
+    // (trajectory, first step, end step, flag to wait for, flag to set),
+    // unused rows -1; a piece that continues a trajectory starts from the
+    // state the previous piece left in X
+    const int* sched;
+    int max_pieces;
+    unsigned* flags;           // [grid] set once a trajectory's head piece has stored its state
+    unsigned* status;          // set when a wait for a flag gives up
 };
 
 // Bonds per site of each kind the persistent kernel unrolls (larger tables

@@ -297,7 +297,22 @@ __global__ void __launch_bounds__(T) k_mixed_persist(const MixDev* __restrict__ 
         if (p < NP3 && p - at3[j] * M0->npad < M0->n_cells) act3 |= 1u << j;
     }
 
-    for (int b = A.b0 + int(blockIdx.x); b < A.b0 + A.B; b += int(gridDim.x)) {
+    for (int piece = 0;; ++piece) {
+        // this workgroup's next trajectory (steps [s_begin, s_end))
+        int b = int(blockIdx.x) + piece * int(gridDim.x), s_begin = 0, s_end = A.n_steps, wait = -1, post = -1;
+        if (A.sched) {
+            if (piece >= A.max_pieces) break;
+            const int* r = A.sched + (size_t(blockIdx.x) * A.max_pieces + piece) * 5;
+            b = r[0];
+            s_begin = r[1];
+            s_end = r[2];
+            wait = r[3];
+            post = r[4];
+            if (b < 0) break;  // unused rows come last
+        } else if (b >= A.B) {
+            break;
+        }
+        b += A.b0;
         double* X = A.X + size_t(b) * SZ;
         const bool drive = A.tp != nullptr;
         const double tpa = drive ? A.tp[2 * size_t(b)] : 0.0, tpb = drive ? A.tp[2 * size_t(b) + 1] : 0.0;
@@ -305,6 +320,22 @@ __global__ void __launch_bounds__(T) k_mixed_persist(const MixDev* __restrict__ 
         const double sca = (drive && A.scale) ? A.scale[2 * size_t(b)] : 1.0;
         const double scb = (drive && A.scale) ? A.scale[2 * size_t(b) + 1] : 1.0;
 
+        if (wait >= 0) {
+            // the state at s_begin, stored in X by the workgroup that ran
+            // the trajectory's first steps as its first piece; a bounded
+            // wait that gives up flags the launch
+            if (tid == 0) {
+                unsigned ok = 0;
+                for (long it = 0; it < (1L << 24); ++it) {
+                    ok = __hip_atomic_load(A.flags + wait, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+                    if (ok) break;
+                    __builtin_amdgcn_s_sleep(8);
+                }
+                if (!ok) __hip_atomic_store(A.status, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            }
+            __syncthreads();
+            __atomic_thread_fence(__ATOMIC_ACQUIRE);
+        }
         // initial state: the stage values of stage 0 (LDS and G[0])
         __syncthreads();  // the previous trajectory is done with LDS and G
 #pragma unroll
@@ -323,7 +354,7 @@ __global__ void __launch_bounds__(T) k_mixed_persist(const MixDev* __restrict__ 
         __syncthreads();
 
         double acc2[SPT][3], acc3[SPT][8];
-        for (int n = 0; n < A.n_steps; ++n) {
+        for (int n = s_begin; n < s_end; ++n) {
             const bool rec = (n % A.save_interval) == 0;
             const int ri = n / A.save_interval;
             const double tn = A.tgrid[n];
@@ -493,9 +524,16 @@ __global__ void __launch_bounds__(T) k_mixed_persist(const MixDev* __restrict__ 
                 __syncthreads();  // y_{s+1} complete (LDS and G)
             }
         }
+        if (post >= 0) {
+            // hand the state at s_end (in X since the last step's stage 3)
+            // to the workgroup that continues the trajectory
+            __threadfence();
+            __syncthreads();
+            if (tid == 0) __hip_atomic_store(A.flags + post, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        }
         // final record: the state after the last step (the stage values of
         // the next step's stage 0: LDS and G[0])
-        {
+        if (s_end == A.n_steps) {
             const MP M = fresh_mix(Mg);
             const MixStageView V{F, TL, Gw, NP2, NP3};
             observe<SPT, T>(M, V, tid, lane, wave, at2, at3, act2, act3, red);

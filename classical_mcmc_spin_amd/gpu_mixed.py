@@ -139,11 +139,12 @@ class GPUMixedLattice:
 
     def last_path(self):
         """csl_mixed_last_path of the last integrate_batched:
-        {"persistent": bool, "workgroups": int, "tail": int, "kernel_ms": float}
-        (kernel_ms: the persistent launch alone)."""
+        {"persistent": bool, "workgroups": int, "split": int, "kernel_ms": float}
+        (split: trajectories run in two pieces by two workgroups; kernel_ms:
+        the persistent launch)."""
         p, w, t, ms = C.c_int(), C.c_int(), C.c_size_t(), C.c_double()
         check(lib().csl_mixed_last_path(self.h, C.byref(p), C.byref(w), C.byref(t), C.byref(ms)))
-        return {"persistent": bool(p.value), "workgroups": w.value, "tail": t.value, "kernel_ms": ms.value}
+        return {"persistent": bool(p.value), "workgroups": w.value, "split": t.value, "kernel_ms": ms.value}
 
     def normalize(self, len2=1.0, len3=1.0):
         check(lib().csl_mixed_normalize(self.h, len2, len3))

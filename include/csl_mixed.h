@@ -173,21 +173,22 @@ int csl_mixed_last_kernel_ms(csl_mixed* h, double* ms, size_t* launches);
  * persistent kernel when the lattice qualifies (translation-invariant bond
  * tables of all four kinds, per-sublattice fields and onsite matrices, at
  * most 2048 sites of each species) — one workgroup integrates one trajectory
- * over the whole time grid, trajectories beyond the co-resident workgroups
- * run in further rounds, and a last round of at most 1/16 of the workgroups
- * runs on the stage kernels instead; -1: always the grid-wide stage kernels
- * (one launch per RK stage, as mixed_lattice_gpu.cu:278-347); n > 0: the
- * persistent kernel with at most n workgroups (tests of the trajectory loop
- * and of the tail).  The stage-graph mode (csl_mixed_set_stage_graph) keeps
- * the stage kernels.  No counterpart in the reference. */
+ * over the whole time grid; with more trajectories than co-resident
+ * workgroups, each workgroup runs an equal share of the trajectory-steps
+ * (whole trajectories plus the head or the rest of a trajectory split with
+ * a neighbouring workgroup); -1: always the grid-wide stage kernels (one
+ * launch per RK stage, as mixed_lattice_gpu.cu:278-347); n > 0: the
+ * persistent kernel with at most n workgroups (tests of the work split).
+ * The stage-graph mode (csl_mixed_set_stage_graph) keeps the stage kernels.
+ * No counterpart in the reference. */
 int csl_mixed_set_persistent(csl_mixed* h, int mode);
 
 /* Path of the last csl_mixed_integrate_batched: *persistent = 1 when the
- * persistent kernel ran, *workgroups its grid, *tail the trajectories that
- * ran on the stage kernels after it, *kernel_ms the device time of the
- * persistent launch alone (HIP events).  Null outputs are skipped.  No
+ * persistent kernel ran, *workgroups its grid, *split the trajectories run
+ * in two pieces by two workgroups, *kernel_ms the device time of the
+ * persistent launch (HIP events).  Null outputs are skipped.  No
  * counterpart in the reference. */
-int csl_mixed_last_path(csl_mixed* h, int* persistent, int* workgroups, size_t* tail, double* kernel_ms);
+int csl_mixed_last_path(csl_mixed* h, int* persistent, int* workgroups, size_t* split, double* kernel_ms);
 
 /* With enable != 0, csl_mixed_integrate_batched issues its launch sequence
  * (4 stage launches and one record reduction per step) from a hipGraph

@@ -7,8 +7,8 @@ Bar: records and final states against the mixed oracle <= 1e-10 over <= 300
 steps with every term of the Hamiltonian, Gilbert and Bloch damping and both
 pulses on; against the grid-wide stage kernels on the same batch (same
 arithmetic per site, records summed in another order) <= 1e-12; a trajectory
-alone equals the same trajectory in a batch bit for bit; the trajectory loop
-(more trajectories than workgroups) and the tail on the stage kernels.
+alone equals the same trajectory in a batch bit for bit, also when the batch
+has more trajectories than workgroups and the trajectory ran in two pieces.
 """
 import numpy as np
 import pytest
@@ -60,7 +60,7 @@ def test_rich_cell_damped_driven_vs_oracle(po):
         GM._set_pulses(gm, d2a, d2b, d3a, d3b, 0.0, 0.0, 0.5, 0.3, 2.0, 0.4, 0.35, 1.1)
         res = gm.integrate_batched(inits, tp, sc, T0, _t1(T0, n, dt), dt, want_final=True)
         path = gm.last_path()
-    assert path["persistent"] and path["tail"] == 0, path
+    assert path["persistent"] and path["split"] == 0, path
     for b in range(B):
         o = po.MixedOracle(muc, *dims)
         o.set_damping(0.02, gamma, eq)
@@ -115,12 +115,13 @@ def test_full_size_equals_stage_kernels():
     assert np.max(np.abs(got["final"] - ref["final"])) <= 1e-12
 
 
-def test_trajectory_rounds_tail_and_alone():
-    """More trajectories than workgroups: with at most 8 workgroups, 17
-    trajectories take three rounds; with 32 workgroups, 33 trajectories run
-    32 on the persistent kernel and the last one on the stage kernels.  Both
-    equal the stage kernels on the same batch; a trajectory alone equals the
-    batched one bit for bit."""
+def test_work_split_and_alone():
+    """More trajectories than workgroups: with at most 8 workgroups, 16
+    trajectories run as two whole trajectories per workgroup and 17 as equal
+    shares of the trajectory-steps (seven trajectories run in two pieces by
+    two workgroups, the second piece starting from the state the first left);
+    with 32 workgroups, 33 trajectories (30 split).  All equal the stage kernels on the
+    same batch; a trajectory alone equals the batched one bit for bit."""
     muc = mm.build_tmfeo3_mixed(seed=9)
     ml = mm.MixedLattice(muc, 4, 4, 4)
     rng = np.random.default_rng(12)
@@ -128,7 +129,7 @@ def test_trajectory_rounds_tail_and_alone():
     T0, dt, n = -0.2, 0.005, 150
     with GM.GPUMixedLattice(ml) as gm:
         GM._set_pulses(gm, f2, f2, f3, f3, 0.0, 0.0, 0.5, 0.5, 0.0, 0.5, 0.5, 0.0)
-        for cap, B, tail in ((8, 17, 0), (32, 33, 1)):
+        for cap, B, split in ((8, 16, 0), (8, 17, 7), (32, 33, 30)):
             inits = np.stack([random_flat(ml, 200 + b) for b in range(B)])
             tp = rng.uniform(0.0, 0.4, size=(B, 2))
             sc = rng.uniform(0.5, 1.5, size=(B, 2))
@@ -136,14 +137,14 @@ def test_trajectory_rounds_tail_and_alone():
             gm.set_persistent(cap)
             got = gm.integrate_batched(inits, tp, sc, T0, _t1(T0, n, dt), dt, want_final=True)
             path = gm.last_path()
-            assert (path["persistent"], path["workgroups"], path["tail"]) == (True, cap, tail), path
+            assert (path["persistent"], path["workgroups"], path["split"]) == (True, cap, split), path
             assert path["kernel_ms"] > 0.0
             gm.set_persistent(-1)
             ref = gm.integrate_batched(inits, tp, sc, T0, _t1(T0, n, dt), dt, want_final=True)
             assert np.max(np.abs(got["records"] - ref["records"])) <= 1e-12, (cap, B)
             assert np.max(np.abs(got["final"] - ref["final"])) <= 1e-12, (cap, B)
             gm.set_persistent(0)
-            for b in (0, B - 2):
+            for b in (0, 2, B - 2, B - 1):
                 one = gm.integrate_batched(inits[b], tp[b:b + 1], sc[b:b + 1], T0, _t1(T0, n, dt), dt,
                                            want_final=True)
                 assert gm.last_path()["persistent"]
