@@ -512,8 +512,8 @@ __global__ void __launch_bounds__(256) k_mixed_stage(MixDev M, MixStageArgs A) {
     mixed_stage_wg<TL, false>(M, A, bx, b, fac, red);
 }
 
-// Cooperative form of the stage kernels for a small batch (the trajectories
-// a persistent launch leaves over): the nblk x B workgroups of one stage
+// Cooperative form of the stage kernels for a small batch of a large
+// lattice (csl_mixed_set_persistent mode -2): the nblk x B workgroups of one stage
 // launch, all co-resident, run every stage of every step in ONE launch with
 // a grid-wide barrier between stages instead of one launch per stage.  The
 // values other workgroups gather are written and read at agent scope; a
@@ -1710,8 +1710,8 @@ void run_mixed(csl_mixed* h, size_t B, const double* d_tp, const double* d_tp3, 
     h->last_launches = launches;
 }
 
-// The trajectories a persistent launch leaves over (states at X0, records to
-// recs_out with row stride rec_stride) on the cooperative stage kernel: one
+// B trajectories (states at X0, records to recs_out with row stride
+// rec_stride) on the cooperative stage kernel: one
 // launch for the whole time grid.  False (nothing launched) when its
 // workgroups cannot all be resident; the caller checks h->coop_ctl[1] after
 // synchronising.
@@ -1865,6 +1865,39 @@ bool run_mixed_persist(csl_mixed* h, size_t B, const double* d_tp, const double*
     h->last_persist = 1;
     h->last_persist_grid = int(grid_wg);
     h->last_split = n_split;
+    return true;
+}
+
+// The whole batch on the cooperative stage kernel (states in h->X, records
+// into h->recs): one launch for the time grid.  False (nothing launched)
+// when its workgroups cannot all be resident.
+bool run_mixed_coop_batch(csl_mixed* h, size_t B, const double* d_tp, const double* d_tp3, const double* d_scale,
+                          const std::vector<double>& grid, double dt, size_t si) {
+    const size_t n_rec = records_for(grid.size() - 1, si);
+    hipStream_t s = h->stream;
+    h->recs.ensure(n_rec * B * CSL_MIXED_RECORD);
+    h->ptgrid.upload(grid.data(), grid.size(), s);
+    hipEvent_t e0, e1;
+    hip_check(hipEventCreate(&e0), "event");
+    hip_check(hipEventCreate(&e1), "event");
+    hip_check(hipEventRecord(e0, s), "event record");
+    const bool ran = run_mixed_coop(h, B, d_tp, d_tp3, d_scale, grid, dt, si, h->X.p, h->recs.p, B);
+    hip_check(hipEventRecord(e1, s), "event record");
+    hip_check(hipStreamSynchronize(s), "cooperative integration");
+    float ms = 0.f;
+    hip_check(hipEventElapsedTime(&ms, e0, e1), "elapsed");
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (!ran) return false;
+    unsigned st = 0;
+    hip_check(hipMemcpy(&st, h->coop_ctl.p + 1, sizeof(unsigned), hipMemcpyDeviceToHost), "status");
+    if (st) fail(CSL_ERR_HIP, "cooperative stage kernel: a grid barrier wait timed out");
+    h->last_ms = ms;
+    h->last_persist_ms = ms;
+    h->last_launches = 1;
+    h->last_persist = 2;
+    h->last_persist_grid = int(B) * (blocks_of(h->M.NP2) + blocks_of(h->M.NP3));
+    h->last_split = 0;
     return true;
 }
 
@@ -2338,8 +2371,11 @@ int csl_mixed_integrate_batched(csl_mixed* h, size_t B, const double* init, int 
         h->last_persist_grid = 0;
         h->last_split = 0;
         h->last_persist_ms = 0.0;
-        const bool persistent = h->persist_mode >= 0 && !h->stage_graph &&
-                                run_mixed_persist(h, B, h->tp.p, h->tp.p, h->scale.p, grid, dt, save_interval);
+        bool persistent = false;
+        if (h->persist_mode >= 0 && !h->stage_graph)
+            persistent = run_mixed_persist(h, B, h->tp.p, h->tp.p, h->scale.p, grid, dt, save_interval);
+        else if (h->persist_mode == -2 && !h->stage_graph)
+            persistent = run_mixed_coop_batch(h, B, h->tp.p, h->tp.p, h->scale.p, grid, dt, save_interval);
         if (!persistent)
             run_mixed(h, B, h->tp.p, h->tp.p, h->scale.p, grid, dt, save_interval, nullptr, false, h->stage_graph != 0);
         copy_back(recs, h->recs.p, n_rec * B * CSL_MIXED_RECORD, s);
@@ -2633,7 +2669,7 @@ int csl_mixed_set_stage_graph(csl_mixed* h, int enable) {
 int csl_mixed_set_persistent(csl_mixed* h, int mode) {
     return guarded([&] {
         check_mixed(h);
-        if (mode < -1) fail(CSL_ERR_ARG, "mode must be -1, 0 or a positive workgroup cap");
+        if (mode < -2) fail(CSL_ERR_ARG, "mode must be -2, -1, 0 or a positive workgroup cap");
         h->persist_mode = mode;
     });
 }

@@ -133,18 +133,21 @@ class GPUMixedLattice:
 
     def set_persistent(self, mode: int):
         """csl_mixed_set_persistent: 0 the persistent batched kernel when the
-        lattice qualifies (default), -1 the stage kernels, n > 0 the
-        persistent kernel with at most n workgroups."""
+        lattice qualifies (default), -1 the stage kernels, -2 the cooperative
+        stage kernel (the whole batch's stage workgroups in one launch over
+        the time grid), n > 0 the persistent kernel with at most n
+        workgroups."""
         check(lib().csl_mixed_set_persistent(self.h, int(mode)))
 
     def last_path(self):
         """csl_mixed_last_path of the last integrate_batched:
-        {"persistent": bool, "workgroups": int, "split": int, "kernel_ms": float}
-        (split: trajectories run in two pieces by two workgroups; kernel_ms:
-        the persistent launch)."""
+        {"persistent": bool, "cooperative": bool, "workgroups": int, "split": int,
+        "kernel_ms": float} (split: trajectories run in two pieces by two
+        workgroups; kernel_ms: the persistent or cooperative launch)."""
         p, w, t, ms = C.c_int(), C.c_int(), C.c_size_t(), C.c_double()
         check(lib().csl_mixed_last_path(self.h, C.byref(p), C.byref(w), C.byref(t), C.byref(ms)))
-        return {"persistent": bool(p.value), "workgroups": w.value, "split": t.value, "kernel_ms": ms.value}
+        return {"persistent": p.value == 1, "cooperative": p.value == 2, "workgroups": w.value,
+                "split": t.value, "kernel_ms": ms.value}
 
     def normalize(self, len2=1.0, len3=1.0):
         check(lib().csl_mixed_normalize(self.h, len2, len3))

@@ -177,14 +177,19 @@ int csl_mixed_last_kernel_ms(csl_mixed* h, double* ms, size_t* launches);
  * workgroups, each workgroup runs an equal share of the trajectory-steps
  * (whole trajectories plus the head or the rest of a trajectory split with
  * a neighbouring workgroup); -1: always the grid-wide stage kernels (one
- * launch per RK stage, as mixed_lattice_gpu.cu:278-347); n > 0: the
- * persistent kernel with at most n workgroups (tests of the work split).
+ * launch per RK stage, as mixed_lattice_gpu.cu:278-347); -2: the stage
+ * kernels' workgroups of the whole batch in one cooperative launch over the
+ * time grid (a grid-wide barrier per stage; for a few trajectories of a
+ * large lattice, where one workgroup per trajectory is slower), the stage
+ * kernels when they cannot all be resident; n > 0: the persistent kernel
+ * with at most n workgroups (tests of the work split).
  * The stage-graph mode (csl_mixed_set_stage_graph) keeps the stage kernels.
  * No counterpart in the reference. */
 int csl_mixed_set_persistent(csl_mixed* h, int mode);
 
 /* Path of the last csl_mixed_integrate_batched: *persistent = 1 when the
- * persistent kernel ran, *workgroups its grid, *split the trajectories run
+ * persistent kernel ran, 2 the cooperative one, 0 the stage kernels,
+ * *workgroups its grid, *split the trajectories run
  * in two pieces by two workgroups, *kernel_ms the device time of the
  * persistent launch (HIP events).  Null outputs are skipped.  No
  * counterpart in the reference. */

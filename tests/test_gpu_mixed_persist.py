@@ -150,3 +150,30 @@ def test_work_split_and_alone():
                 assert gm.last_path()["persistent"]
                 assert np.array_equal(one["records"][:, 0], got["records"][:, b]), (cap, b)
                 assert np.array_equal(one["final"][0], got["final"][b]), (cap, b)
+
+
+def test_cooperative_small_batch_equals_stage_kernels():
+    """Mode -2: two 8x8x8 trajectories on the cooperative stage kernel (16
+    workgroups each, one launch, a grid-wide barrier per stage) against the
+    stage kernels on the same batch."""
+    muc = mm.build_tmfeo3_mixed(seed=10)
+    ml = mm.MixedLattice(muc, 8, 8, 8)
+    B = 2
+    inits = np.stack([random_flat(ml, 300 + b) for b in range(B)])
+    tp = np.array([[0.0, 0.2], [0.1, 0.0]])
+    sc = np.array([[1.0, 0.5], [1.0, 1.0]])
+    f2, f3 = _drive(3, 1), _drive(8, 2)
+    T0, dt, n = -0.3, 0.005, 300
+    with GM.GPUMixedLattice(ml) as gm:
+        GM._set_pulses(gm, f2, f2, f3, f3, 0.0, 0.0, 0.5, 0.5, 0.0, 0.5, 0.5, 0.0)
+        gm.set_persistent(-2)
+        print("[mixed-coop] 8x8x8, B 2", flush=True)
+        got = gm.integrate_batched(inits, tp, sc, T0, _t1(T0, n, dt), dt, save_interval=3, want_final=True)
+        path = gm.last_path()
+        assert path["cooperative"] and not path["persistent"] and path["workgroups"] == 32, path
+        gm.set_persistent(-1)
+        ref = gm.integrate_batched(inits, tp, sc, T0, _t1(T0, n, dt), dt, save_interval=3, want_final=True)
+        assert not (gm.last_path()["persistent"] or gm.last_path()["cooperative"])
+    assert got["records"].shape == ref["records"].shape
+    assert np.max(np.abs(got["records"] - ref["records"])) <= 1e-12
+    assert np.max(np.abs(got["final"] - ref["final"])) <= 1e-12
