@@ -343,6 +343,33 @@ std::vector<TuneVal> persistent_candidates(const csl_lattice* h) {
             if (csl::persistent_lds_bytes(h->L, block, fl & csl::kPersistDoubleBuffer) <= h->lds_limit)
                 c.push_back({block, spt, fl});
     }
+    // atom-major shapes with more waves than the exact tiling when that
+    // lowers the busiest SIMD's share (the units of a sublattice dealt
+    // unevenly over its waves): for each sites-per-thread value the block
+    // with the lowest share, after the shapes above
+    const int fl = csl::kPersistDoubleBuffer | csl::kPersistUnrollStages;
+    const int n_atoms = h->L.n_atoms;
+    if (h->L.npad % 64 == 0 && n_atoms > 0) {
+        const int U = h->L.npad / 64;
+        for (int spt : {1, 2, 3, 4, 6}) {
+            int best = 0, best_load = 1 << 30;
+            for (int wpa = (U + spt - 1) / spt; wpa <= U; ++wpa) {
+                const int block = n_atoms * wpa * 64;
+                if (block > csl::persistent_block_limit(spt)) break;
+                if (!(csl::persistent_variant(h->L, block, spt, fl) & csl::kVariantAtomMajor)) continue;
+                if (csl::persistent_lds_bytes(h->L, block, true) > h->lds_limit) continue;
+                const int load = csl::atom_major_max_load(block / 64, n_atoms, U);
+                if (load < best_load) {
+                    best_load = load;
+                    best = block;
+                }
+            }
+            if (best == 0) continue;
+            bool seen = false;
+            for (const auto& v : c) seen = seen || (v.block == best && v.spt == spt);
+            if (!seen) c.push_back({best, spt, fl});
+        }
+    }
     return c;
 }
 

@@ -2,6 +2,8 @@
 // csl_kernels.hip.
 #pragma once
 
+#include <algorithm>
+
 #include "csl_internal.hpp"
 
 namespace csl {
@@ -255,14 +257,61 @@ constexpr int kVariantFieldOnly = 1;
 constexpr int kVariantAtomMajor = 2;
 constexpr int kVariantPartitioned = 4;  // reported for launches of the partitioned kernel
 inline int zmax_bucket(int zmax) { return zmax <= 3 ? 3 : (zmax <= 6 ? 6 : 12); }
+
+// Atom-major tiles: the npad / 64 wave-sized units of each sublattice are
+// dealt to its nwaves / n_atoms waves (at most spt per wave) in order, every
+// wave taking the floor share and the remainder going one unit at a time to
+// the wave of the sublattice whose SIMD (wave % 4: waves are dealt to the 4
+// SIMDs in turn) carries the fewest units so far (ties: the lower wave), so
+// that the busiest SIMD, which sets the time of a stage, carries as few as
+// possible.  Wave `wave` owns units [start, start + units) of its
+// sublattice.  With npad = waves per atom * 64 * spt every wave takes spt.
+__host__ __device__ inline void atom_major_tile(int nwaves, int n_atoms, int U, int wave, int& start, int& units) {
+    const int wpa = nwaves / n_atoms;
+    const int q = U / wpa, r = U % wpa;
+    const int a_w = wave / wpa;
+    int load[4] = {0, 0, 0, 0};
+    unsigned extra = 0;  // waves of sublattice a_w given q + 1 units
+    for (int a = 0; a <= a_w; ++a) {
+        for (int i = 0; i < wpa; ++i) load[(a * wpa + i) & 3] += q;
+        unsigned ex = 0;
+        for (int e = 0; e < r; ++e) {
+            int best = -1;
+            for (int i = 0; i < wpa; ++i)
+                if (!((ex >> i) & 1u) && (best < 0 || load[(a * wpa + i) & 3] < load[(a * wpa + best) & 3])) best = i;
+            ex |= 1u << best;
+            ++load[(a * wpa + best) & 3];
+        }
+        extra = ex;
+    }
+    const int i_w = wave - a_w * wpa;
+    start = 0;
+    for (int i = 0; i < i_w; ++i) start += q + int((extra >> i) & 1u);
+    units = q + int((extra >> i_w) & 1u);
+}
+
 inline int persistent_variant(const LatDev& L, int block, int spt, int flags) {
     if (flags & kPersistMulti) return 0;
     const bool dbuf = flags & kPersistDoubleBuffer, unroll = flags & kPersistUnrollStages;
     if (!(unroll && dbuf && L.uniform && L.site_uniform && !L.has_onsite && L.full_coord &&
           L.zmax == zmax_bucket(L.zmax)))
         return 0;
-    const bool am = L.npad % (kWave * spt) == 0 && block == L.n_atoms * (L.npad / (kWave * spt)) * kWave;
+    // atom-major: whole waves per sublattice, each with 1..spt units
+    const int U = L.npad / kWave, wpa = block / (L.n_atoms * kWave);
+    const bool am = L.npad % kWave == 0 && block % (L.n_atoms * kWave) == 0 && wpa * spt >= U && wpa <= U &&
+                    wpa <= 32;
     return kVariantFieldOnly | (am ? kVariantAtomMajor : 0);
+}
+
+// Busiest SIMD's units of an atom-major launch shape
+inline int atom_major_max_load(int nwaves, int n_atoms, int U) {
+    int load[4] = {0, 0, 0, 0};
+    for (int w = 0; w < nwaves; ++w) {
+        int s0 = 0, u = 0;
+        atom_major_tile(nwaves, n_atoms, U, w, s0, u);
+        load[w & 3] += u;
+    }
+    return std::max(std::max(load[0], load[1]), std::max(load[2], load[3]));
 }
 
 // Launches the persistent kernel, or with occupancy != null only stores its

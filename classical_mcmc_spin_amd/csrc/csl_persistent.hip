@@ -96,9 +96,17 @@ __global__ void __launch_bounds__(persistent_max_block<SPT>()) k_persistent(LatD
 
     // per-thread ownership: sites p = tid + m*T, or the atom-major mapping
     // (the same for every trajectory)
-    const int wpa = ATOM_MAJOR ? Lk.npad / (kWave * SPT) : 1;  // waves per atom
+    const int wpa = ATOM_MAJOR ? nwaves / Lk.n_atoms : 1;  // waves per atom
     const int atom_w = ATOM_MAJOR ? wave / wpa : 0;
-    const int pbase = ATOM_MAJOR ? atom_w * Lk.npad + (wave - atom_w * wpa) * (SPT * kWave) + lane : tid;
+    // atom-major: this wave owns units [wstart, wstart + wunits) of its
+    // sublattice (slots m >= wunits own nothing)
+    int wstart = 0, wunits = SPT;
+    if constexpr (ATOM_MAJOR) {
+        atom_major_tile(nwaves, Lk.n_atoms, Lk.npad / kWave, wave, wstart, wunits);
+        wstart = uniform(wstart);
+        wunits = uniform(wunits);
+    }
+    const int pbase = ATOM_MAJOR ? atom_w * Lk.npad + wstart * kWave + lane : tid;
     auto site = [&](int m) { return ATOM_MAJOR ? pbase + m * kWave : tid + m * T; };
     double x[SPT][3], acc[SPT][3];
     int nb[SPT][ZMAX];
@@ -110,7 +118,7 @@ __global__ void __launch_bounds__(persistent_max_block<SPT>()) k_persistent(LatD
             const int p = site(m);
             const int pw = ATOM_MAJOR ? atom_w * Lo.npad : wave * kWave + m * T;  // wave-uniform
             atm[m] = ATOM_MAJOR ? atom_w : uniform(pw < NP ? pw / Lo.npad : 0);
-            act[m] = active_site(Lo, p);
+            act[m] = m < wunits && active_site(Lo, p);
             const int c = UNI ? uniform(pw < NP ? as_const(Lo.cnt_atom)[atm[m]] : 0) : (p < NP ? Lo.cnt[p] : 0);
             cnt[m] = c;
 #pragma unroll
@@ -342,6 +350,8 @@ __global__ void __launch_bounds__(persistent_max_block<SPT>()) k_persistent(LatD
             double e_obs[SPT];
 #pragma unroll
             for (int m = 0; m < SPT; ++m) {
+                e_obs[m] = 0.0;
+                if (m >= wunits) continue;  // wave-uniform: the slot owns no sites
                 double S[3], H[3];
                 // stage 0 reads x itself (the LDS copy holds the same bits)
                 if (s == 0) {
@@ -461,7 +471,7 @@ __global__ void __launch_bounds__(persistent_max_block<SPT>()) k_persistent(LatD
 #pragma unroll
         for (int m = 0; m < SPT; ++m) {
             const int p = site(m);
-            if (p < NP) {
+            if (p < NP && m < wunits) {
 #pragma unroll
                 for (int d = 0; d < 3; ++d) sv[3 * p + d] = act[m] ? x[m][d] : 0.0;
             }
