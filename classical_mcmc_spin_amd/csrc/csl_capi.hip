@@ -330,6 +330,24 @@ csl_lattice* new_handle() {
     return h;
 }
 
+// Wave-sized units of sites (64 site slots of which at least one is a real
+// site) on the busiest of a CU's four SIMDs for a persistent launch shape
+// (the waves of a workgroup are dealt to the SIMDs in turn).
+int simd_share(const csl::LatDev& L, int block, int spt, int flags) {
+    const int nw = block / 64;
+    if (csl::persistent_variant(L, block, spt, flags) & csl::kVariantAtomMajor)
+        return csl::atom_major_max_load(nw, L.n_atoms, L.npad / 64);
+    int load[4] = {0, 0, 0, 0};
+    for (int w = 0; w < nw; ++w)
+        for (int m = 0; m < spt; ++m) {
+            bool any = false;
+            for (int p = w * 64 + m * block; p < std::min(w * 64 + m * block + 64, L.NP) && !any; ++p)
+                any = (p % L.npad) < L.n_cells;
+            load[w & 3] += any ? 1 : 0;
+        }
+    return std::max(std::max(load[0], load[1]), std::max(load[2], load[3]));
+}
+
 // Candidate persistent launch shapes for this lattice (block multiple of 64).
 std::vector<TuneVal> persistent_candidates(const csl_lattice* h) {
     std::vector<TuneVal> c;
@@ -384,10 +402,23 @@ TuneVal pick_persistent(const csl_lattice* h, size_t B, int block_req) {
     auto it = h->tune.find(batch_class(B));
     if (it != h->tune.end()) return it->second;
     // heuristic: two LDS buffers (one barrier per stage) and compile-time
-    // stages with the fewest sites per thread that fit
-    for (int want : {csl::kPersistDoubleBuffer | csl::kPersistUnrollStages, csl::kPersistDoubleBuffer})
-        for (auto& c : cands)
-            if (c.flags == want) return c;
+    // stages, the shape whose busiest SIMD carries the fewest wave-sized
+    // units of sites (a stage ends when that SIMD is done), ties: the fewest
+    // sites per thread
+    const int fl = csl::kPersistDoubleBuffer | csl::kPersistUnrollStages;
+    const TuneVal* best = nullptr;
+    int best_share = 0;
+    for (const auto& c : cands) {
+        if (c.flags != fl) continue;
+        const int sh = simd_share(h->L, c.block, c.spt, c.flags);
+        if (!best || sh < best_share || (sh == best_share && c.spt < best->spt)) {
+            best = &c;
+            best_share = sh;
+        }
+    }
+    if (best) return *best;
+    for (auto& c : cands)
+        if (c.flags == csl::kPersistDoubleBuffer) return c;
     return cands.front();
 }
 
