@@ -132,3 +132,24 @@ def test_split_bit_identical_every_candidate(name):
             assert np.array_equal(a["mags"], b["mags"]), tag
             assert np.array_equal(a["energy"], b["energy"]), tag
             assert np.array_equal(a["final"], b["final"]), tag
+
+
+def test_default_shape_balances_simds(oracle):
+    """Without a tuned entry the launcher takes the shape whose busiest SIMD
+    carries the fewest 64-site units: for one 1152-site trajectory the
+    atom-major 640 x 2 shape (9 units per sublattice dealt 2, 2, 2, 2, 1 over
+    its 5 waves: SIMD shares 5, 5, 4, 4; the exact tilings carry 6), and its
+    records equal the oracle's."""
+    uc, lat = _lattice("kitaev24_h")
+    s = random_state(lat, 56)
+    T1 = (N_STEPS + 0.5) * DT
+    o = oracle.OracleLattice(uc, lat.dim1, lat.dim2, 1)
+    ref = o.integrate_md(s.reshape(-1), 0.0, T1, DT)
+    with GPULattice(lat) as gl:
+        gl.set_spins(s)
+        print("[shape] kitaev24_h default plain persistent", flush=True)
+        res = gl.integrate(0.0, T1, DT, 1, tgrid="boost", engine=2)
+        got = gl.last_launch_config()
+    assert (got["block"], got["sites_per_thread"], got["atom_major"]) == (640, 2, True), got
+    assert np.max(np.abs(res["mags"] - ref["mags"])) <= TOL
+    assert np.max(np.abs(res["energy"] - ref["energy"])) <= TOL
