@@ -43,11 +43,12 @@ __device__ __forceinline__ double site_energy_atom(const LatDev& L, int atom, co
 }
 }  // namespace
 
-// ATOM_MAJOR (FIELD_ONLY shapes with npad = waves_per_atom * 64 * SPT and
-// n_atoms * waves_per_atom waves): every wave owns SPT * 64 cells of one
-// sublattice, site m of lane l of wave w at atom(w) * npad + (w_in_atom *
-// SPT + m) * 64 + l, so all owned sites share one atom and its bond matrices
-// are loaded once per bond for all of them (instead of once per site slot).
+// ATOM_MAJOR (FIELD_ONLY shapes of n_atoms * waves_per_atom waves whose
+// waves can hold the npad / 64 units of 64 cells of a sublattice, at most
+// SPT each): wave w owns units [start, start + units) of sublattice atom(w)
+// (atom_major_tile), site m < units of lane l at atom(w) * npad + (start +
+// m) * 64 + l, so all owned sites share one atom and its bond matrices are
+// loaded once per bond for all of them (instead of once per site slot).
 // MULTI: every trajectory carries its own lattice (A.lats[A.lat_of[b]]),
 // so the LDS layout and the site ownership are set up per piece.
 template <int SPT, int ZMAX, bool UNI, bool DBUF, bool UNROLL, bool FIELD_ONLY, bool ATOM_MAJOR = false,
