@@ -339,21 +339,38 @@ __global__ void __launch_bounds__(persistent_max_block<SPT>()) k_persistent(LatD
 #pragma unroll
                 for (int d = 0; d < 3; ++d) D[d] = d0[d] * f1 + d1[d] * f2;
             }
-            // the wave's bond matrices, read once per stage before any
-            // gather (scalar loads: one wait, then only LDS traffic)
-            double Jr[ZMAX][9];
+            // bond by bond: one bond matrix (scalar loads) live at a time and
+            // the gathers of the wave's sites for that bond in flight
+            // together; every site's field is still -B + J_0 S_0 + J_1 S_1
+            // + ... in this order (one FMA chain per component)
+            double H[SPT][3];
+#pragma unroll
+            for (int m = 0; m < SPT; ++m)
+#pragma unroll
+                for (int d = 0; d < 3; ++d) H[m][d] = -Bu[d];
 #pragma unroll
             for (int k = 0; k < ZMAX; ++k) {
                 cptr<double> J = as_const(L.mats) + (atom_w * L.zmax + k) * 9;
+                double Jk[9];
 #pragma unroll
-                for (int q = 0; q < 9; ++q) Jr[k][q] = J[q];
+                for (int q = 0; q < 9; ++q) Jk[q] = J[q];
+#pragma unroll
+                for (int m = 0; m < SPT; ++m) {
+                    if (m >= wunits) continue;  // wave-uniform: the slot owns no sites
+                    double Sp[3];
+                    lds_ld(cur, nb[m][k], Sp);
+#pragma unroll
+                    for (int d = 0; d < 3; ++d)
+#pragma unroll
+                        for (int e = 0; e < 3; ++e) H[m][d] = fma(Jk[d * 3 + e], Sp[e], H[m][d]);
+                }
             }
             double e_obs[SPT];
 #pragma unroll
             for (int m = 0; m < SPT; ++m) {
                 e_obs[m] = 0.0;
                 if (m >= wunits) continue;  // wave-uniform: the slot owns no sites
-                double S[3], H[3];
+                double S[3];
                 // stage 0 reads x itself (the LDS copy holds the same bits)
                 if (s == 0) {
 #pragma unroll
@@ -361,24 +378,13 @@ __global__ void __launch_bounds__(persistent_max_block<SPT>()) k_persistent(LatD
                 } else {
                     lds_ld(cur, site(m), S);
                 }
-#pragma unroll
-                for (int d = 0; d < 3; ++d) H[d] = -Bu[d];
-#pragma unroll
-                for (int k = 0; k < ZMAX; ++k) {
-                    double Sp[3];
-                    lds_ld(cur, nb[m][k], Sp);
-#pragma unroll
-                    for (int d = 0; d < 3; ++d)
-#pragma unroll
-                        for (int e = 0; e < 3; ++e) H[d] = fma(Jr[k][d * 3 + e], Sp[e], H[d]);
-                }
-                e_obs[m] = (obs && want_e) ? site_energy_atom(L, atom_w, S, H) : 0.0;
+                e_obs[m] = (obs && want_e) ? site_energy_atom(L, atom_w, S, H[m]) : 0.0;
                 if (drive_now) {
 #pragma unroll
-                    for (int d = 0; d < 3; ++d) H[d] -= D[d];
+                    for (int d = 0; d < 3; ++d) H[m][d] -= D[d];
                 }
                 double k[3];
-                ll_su2_guarded(H, S, alpha, k);
+                ll_su2_guarded(H[m], S, alpha, k);
                 double y[3];
 #pragma unroll
                 for (int d = 0; d < 3; ++d) {
